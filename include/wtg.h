@@ -1,0 +1,178 @@
+/*
+ * wtg.h -- C ABI of the MI355X (gfx950) wavelet-tree / rank / select / FM path.
+ *
+ * Plain C types only: device buffers are `void*` / `uint64_t*` obtained from
+ * wtg_malloc (or any hipMalloc'd pointer on the context's device), host buffers
+ * are ordinary pointers.  Every function returns WTG_OK (0) or a negative
+ * status; wtg_last_error() gives the message of the last failure on the
+ * calling thread.  Nothing throws across this boundary.
+ *
+ * Bit layout everywhere: bit j of 64-bit word k is bit position 64k+j; bits
+ * past the end of a vector are zero.
+ *
+ * Each entry point names the reference interface it replaces (file:line in
+ * the reference repository jlabeit/wavelet-suffix-fm-index).
+ */
+#ifndef WTG_H
+#define WTG_H
+#include <stdint.h>
+#include <stddef.h>
+
+#ifdef __cplusplus
+extern "C" {
+#endif
+
+#define WTG_OK 0
+#define WTG_ERR_ARG (-1)          /* invalid argument (null pointer, size, width) */
+#define WTG_ERR_HIP (-2)          /* a HIP runtime call failed */
+#define WTG_ERR_NOMEM (-3)        /* device allocation failed */
+#define WTG_ERR_UNSUPPORTED (-4)  /* valid input outside what this build handles */
+#define WTG_ERR_ABSENT (-5)       /* select(i,c): c does not occur i times */
+#define WTG_ERR_DEVICE (-6)       /* a kernel's bounded wait expired (device-side error flag) */
+
+#define WTG_NONE UINT64_MAX       /* absent node / child (reference: UINT_T_MAX) */
+
+typedef struct wtg_ctx wtg_ctx;
+
+/* ------------------------------------------------------------ context */
+const char* wtg_version(void);
+const char* wtg_last_error(void);
+int wtg_device_count(int* out);
+/* One context = one device + one HIP stream + a growable device workspace.
+ * Contexts are independent; a context is used by one host thread at a time. */
+int wtg_ctx_create(int device, wtg_ctx** out);
+int wtg_ctx_destroy(wtg_ctx* ctx);
+/* Wait for all work queued on the context's stream; reports device-side errors. */
+int wtg_ctx_sync(wtg_ctx* ctx);
+/* 1 (default): every entry point returns after its work finished.
+ * 0: entry points only enqueue; errors surface at the next wtg_ctx_sync. */
+int wtg_ctx_set_blocking(wtg_ctx* ctx, int blocking);
+/* The HIP stream (hipStream_t) the context launches on, as an opaque pointer. */
+void* wtg_ctx_stream(wtg_ctx* ctx);
+
+int wtg_malloc(wtg_ctx* ctx, void** dptr, uint64_t bytes);
+int wtg_free(wtg_ctx* ctx, void* dptr);
+int wtg_memcpy_h2d(wtg_ctx* ctx, void* dst, const void* src, uint64_t bytes);
+int wtg_memcpy_d2h(wtg_ctx* ctx, void* dst, const void* src, uint64_t bytes);
+int wtg_memcpy_d2d(wtg_ctx* ctx, void* dst, const void* src, uint64_t bytes);
+int wtg_memset_zero(wtg_ctx* ctx, void* dst, uint64_t bytes); /* fill kernel, no hipMemset */
+
+/* Stream timing with HIP events on the context's stream. */
+int wtg_timer_start(wtg_ctx* ctx);
+int wtg_timer_stop(wtg_ctx* ctx, float* ms);
+/* Per-kernel timing: when enabled, each launch of a library kernel is
+ * bracketed by HIP events on the context's stream.  wtg_kernel_time reports the
+ * summed milliseconds and number of launches of kernel `name` since the last
+ * reset (it synchronizes the stream). */
+int wtg_kernel_timing(wtg_ctx* ctx, int enable);
+int wtg_kernel_time(wtg_ctx* ctx, const char* name, double* total_ms, uint64_t* launches);
+int wtg_kernel_timing_reset(wtg_ctx* ctx);
+
+/* ------------------------------------------------ (a) level-wise wavelet tree */
+typedef struct {
+  uint64_t bitmapPtr, length, parent, leftChild, rightChild;
+} wtg_node;
+
+/* Number of 64-bit words of an n*levels-bit tree. */
+uint64_t wtg_levels_words(uint64_t n, uint32_t levels);
+/* Level-wise stable bit split.  Replaces
+ *   pair<WTnode*,long*> WT(symbol* s, uintT n, uintT sigma)
+ *   (wavelet/levelWT/WT.h:43, wavelet/levelWT/WT.C:43-227; same output as
+ *   serialWT/ddWT/recWT WT.C) with levels = max(1, log2Up(sigma)), and the tree
+ *   bit_vector of sdsl::wt_int (wt_int/wt_int_par.hpp:166-244,368-406) with
+ *   levels = max_level.
+ * d_sym: n device symbols of sym_bytes (1, 2, 4 or 8), each < 2^levels.
+ * d_words: wtg_levels_words(n, levels) device words (fully overwritten).
+ * d_nodes: NULL, or node_slots device records indexed by heap id (root 0,
+ *   children 2v+1 / 2v+2); reached nodes as in the reference, other slots
+ *   {0, 0, WTG_NONE, WTG_NONE, WTG_NONE}.
+ * Limits: 1 <= levels <= 24 (dense per-level node tables), n < 2^32. */
+int wtg_levels_build(wtg_ctx* ctx, const void* d_sym, int sym_bytes, uint64_t n,
+                     uint32_t levels, uint64_t* d_words, wtg_node* d_nodes,
+                     uint64_t node_slots);
+/* Dense alphabet remap of the driver (wavelet/levelWT/WTTime.C:80-94), in place
+ * on the device; *sigma = number of distinct values. */
+int wtg_remap(wtg_ctx* ctx, void* d_sym, int sym_bytes, uint64_t n, uint64_t* sigma);
+/* Host-buffer convenience used by the C++ WT() drop-in: copies h_sym to the
+ * device, builds with levels = max(1, log2Up(sigma)), copies back
+ * wtg_levels_words(n, levels) words and 2*sigma nodes (h_nodes may be NULL). */
+int wtg_wt_host(wtg_ctx* ctx, const void* h_sym, int sym_bytes, uint64_t n, uint64_t sigma,
+                uint64_t* h_words, wtg_node* h_nodes);
+
+/* ------------------------------------------------ (b) rank directory */
+/* Number of 64-bit words of the rank_support_v<1,1> directory:
+ * 2*((capacity>>9)+1), capacity = 64*ceil(nbits/64) (2 for nbits = 0). */
+uint64_t wtg_rank9_words(uint64_t nbits);
+/* Builds the interleaved superblock/block directory of
+ *   sdsl::rank_support_v<1,1>(const bit_vector*)  (rank/rank_support_v_par.hpp:69-123). */
+int wtg_rank9_build(wtg_ctx* ctx, const uint64_t* d_words, uint64_t nbits, uint64_t* d_dir);
+/* rank(idx) for a batch of idx (0 <= idx <= nbits; larger idx -> WTG_NONE)
+ * (rank_support_v_par.hpp:131-142). */
+int wtg_rank9_batch(wtg_ctx* ctx, const uint64_t* d_words, const uint64_t* d_dir,
+                    uint64_t nbits, const uint64_t* d_idx, uint64_t nq, uint64_t* d_out);
+
+/* ------------------------------------------------ (b') select samples */
+/* Content of sdsl::select_support_mcl<bit,1>  (select/select_support_mcl_par.hpp:72-117,
+ * :227-276 init_slow, :483-561 init_fast).  All arrays are device arrays owned
+ * by the structure (release with wtg_select_free). */
+typedef struct {
+  uint64_t nbits, arg_cnt, sb, n_long;
+  uint32_t logn, logn4;
+  int32_t bit;
+  int32_t _pad;
+  uint64_t* superblock;   /* sb: position of argument 4096k (width logn) */
+  uint32_t* mini;         /* sb*64: pos(4096k+64t) - superblock[k]; 0 past the end / for long */
+  uint8_t* mini_width;    /* sb: hi(end-start)+1, 0 for a long superblock */
+  int64_t* long_index;    /* sb: row of long_pool, -1 for a mini superblock */
+  uint8_t* long_width;    /* sb: hi(end)+1, 0 for a mini superblock */
+  uint64_t* long_pool;    /* n_long*4096 argument positions (0 past the end), NULL if none */
+} wtg_select;
+/* Builds select1 (bit=1) or select0 (bit=0) samples.  d_dir: the rank9
+ * directory of the same words (wtg_rank9_build), used to locate samples. */
+int wtg_select_build(wtg_ctx* ctx, const uint64_t* d_words, const uint64_t* d_dir,
+                     uint64_t nbits, int bit, wtg_select* out);
+/* rank9 directory + select1 + select0 in one pass over the words. */
+int wtg_rank9_select_build(wtg_ctx* ctx, const uint64_t* d_words, uint64_t nbits,
+                           uint64_t* d_dir, wtg_select* sel1, wtg_select* sel0);
+int wtg_select_free(wtg_ctx* ctx, wtg_select* s);
+/* select(i), 1 <= i <= arg_cnt, the sampled scan of select_support_mcl_par.hpp:564-611. */
+int wtg_select_batch(wtg_ctx* ctx, const wtg_select* s, const uint64_t* d_words,
+                     const uint64_t* d_i, uint64_t nq, uint64_t* d_out);
+
+/* ------------------------------------------------ wt_int (integer wavelet tree) */
+typedef struct wtg_wtint wtg_wtint;
+/* sdsl::wt_int<>(int_vector<w>& buf, size, max_level)  (wt_int/wt_int_par.hpp:368-406):
+ * max_level = 0 -> hi(max(1, max value))+1.  Builds the tree, rank9, select1, select0. */
+int wtg_wtint_build(wtg_ctx* ctx, const void* d_sym, int sym_bytes, uint64_t n,
+                    uint32_t max_level, wtg_wtint** out);
+int wtg_wtint_destroy(wtg_ctx* ctx, wtg_wtint* wt);
+/* size, max_level, sigma (#distinct values), device tree words and rank directory */
+int wtg_wtint_info(const wtg_wtint* wt, uint64_t* n, uint32_t* levels, uint64_t* sigma,
+                   const uint64_t** d_words, const uint64_t** d_dir);
+const wtg_select* wtg_wtint_select1(const wtg_wtint* wt);
+const wtg_select* wtg_wtint_select0(const wtg_wtint* wt);
+/* rank(i,c) = #c in [0,i)  (wt_int_par.hpp:553-577) */
+int wtg_wtint_rank_batch(wtg_ctx* ctx, const wtg_wtint* wt, const uint64_t* d_i,
+                         const uint64_t* d_c, uint64_t nq, uint64_t* d_out);
+/* operator[](i)  (wt_int_par.hpp:519-541) */
+int wtg_wtint_access_batch(wtg_ctx* ctx, const wtg_wtint* wt, const uint64_t* d_i,
+                           uint64_t nq, uint64_t* d_out);
+/* select(i,c): position of the i-th c  (wt_int_par.hpp:622-660).  Where c does not
+ * occur i times the output is WTG_NONE (the reference throws std::logic_error). */
+int wtg_wtint_select_batch(wtg_ctx* ctx, const wtg_wtint* wt, const uint64_t* d_i,
+                           const uint64_t* d_c, uint64_t nq, uint64_t* d_out);
+/* inverse_select(i) = (rank(i, wt[i]), wt[i])  (wt_int_par.hpp:588-611) */
+int wtg_wtint_inverse_select_batch(wtg_ctx* ctx, const wtg_wtint* wt, const uint64_t* d_i,
+                                   uint64_t nq, uint64_t* d_rank, uint64_t* d_sym);
+
+/* ------------------------------------------------ synthetic workloads (host) */
+/* Seeded generators of the benchmark inputs (splitmix64 streams). */
+uint64_t wtg_splitmix64(uint64_t* state);
+int wtg_gen_uniform(uint64_t seed, uint64_t n, uint64_t range, int sym_bytes, void* out);
+int wtg_gen_geometric(uint64_t seed, uint64_t n, double p, uint64_t cap, int sym_bytes, void* out);
+int wtg_gen_zipf_bytes(uint64_t seed, uint64_t n, double s, uint8_t* out);
+
+#ifdef __cplusplus
+}
+#endif
+#endif /* WTG_H */

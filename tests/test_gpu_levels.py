@@ -1,0 +1,116 @@
+"""GPU parity of (a), the level-wise wavelet tree (wavelet/levelWT/WT.C:43-227),
+against the oracle (itself pinned to the compiled reference) and the golden
+fixtures written from the compiled reference.  Bit-exact words and nodes."""
+import os
+
+import numpy as np
+import pytest
+
+pytestmark = pytest.mark.gpu
+GOLD = os.path.join(os.path.dirname(os.path.abspath(__file__)), "golden", "levels_ref.npz")
+
+
+def say(*a):
+    print(*a, flush=True)
+
+
+def golden_cases():
+    z = np.load(GOLD, allow_pickle=False)
+    names = sorted({k.split("__")[0] for k in z.files})
+    return [(nm, z[nm + "__s"], int(z[nm + "__sigma"][0]), z[nm + "__words"], z[nm + "__reach"],
+             z[nm + "__nodes"]) for nm in names]
+
+
+@pytest.mark.parametrize("case", golden_cases(), ids=lambda c: c[0])
+def test_wt_drop_in_matches_reference_golden(ctx, wsfm, case):
+    """wtg_wt_host = the WT(s, n, sigma) drop-in (host buffers in and out)."""
+    name, s, sigma, words, reach, nodes = case
+    say("golden", name, s.size, sigma)
+    w, nd = ctx.wt_host(s, sigma)
+    assert np.array_equal(w, words)
+    r = reach.astype(np.int64)
+    assert np.array_equal(nd[r], nodes)
+    # unreached slots are {0,0,NONE,NONE,NONE}
+    mask = np.ones(nd.shape[0], bool)
+    mask[r] = False
+    if mask.any():
+        assert (nd[mask, 0] == 0).all() and (nd[mask, 1] == 0).all()
+        assert (nd[mask, 2:] == wsfm.NONE).all()
+
+
+CASES = []
+for dt in (np.uint8, np.uint16, np.uint32, np.uint64):
+    for n in (1, 63, 64, 65, 1000, 4095, 4096, 4097, 16384, 100003):
+        CASES.append((dt, n))
+
+
+@pytest.mark.parametrize("dt,n", CASES, ids=lambda x: str(x))
+def test_levels_vs_oracle_sizes(ctx, orc, dt, n):
+    rng = np.random.default_rng(n)
+    bits = np.dtype(dt).itemsize * 8
+    for L in sorted({1, 2, 3, 7, min(bits, 8), min(bits, 13), min(bits, 20)}):
+        s = rng.integers(0, 1 << L, n, dtype=np.uint64).astype(dt)
+        say("levels", np.dtype(dt).name, n, L)
+        w, nd = ctx.levels(s, L, with_nodes=L <= 16)
+        ow, ond = orc.levels_serial(s, L, with_nodes=L <= 16)
+        assert np.array_equal(w, ow), (n, L)
+        if L <= 16:
+            reached = np.nonzero(ond[:, 5])[0]
+            assert np.array_equal(nd[reached], ond[reached, :5])
+
+
+@pytest.mark.parametrize("seed", range(6))
+def test_levels_skewed_and_sparse_alphabets(ctx, orc, seed):
+    rng = np.random.default_rng(100 + seed)
+    n = int(rng.integers(20000, 300000))
+    L = int(rng.integers(10, 25))
+    kind = seed % 3
+    if kind == 0:
+        s = np.minimum(rng.geometric(1e-3, n) - 1, (1 << L) - 1)
+    elif kind == 1:
+        s = rng.choice(rng.integers(0, 1 << L, 37), n)
+    else:
+        s = np.where(rng.random(n) < 0.9, 5, rng.integers(0, 1 << L, n))
+    s = s.astype(np.uint32)
+    say("skewed", seed, n, L)
+    w, _ = ctx.levels(s, L)
+    ow, _ = orc.levels_serial(s, L, with_nodes=False)
+    assert np.array_equal(w, ow)
+
+
+def test_levels_empty_input(ctx, wsfm):
+    w, nd = ctx.wt_host(np.zeros(0, np.uint32), 5)
+    assert w.size == 0
+    assert nd[0].tolist() == [0, 0, wsfm.NONE, wsfm.NONE, wsfm.NONE]
+
+
+def test_remap_matches_driver(ctx, orc):
+    rng = np.random.default_rng(3)
+    s = rng.choice(np.array([5, 17, 1000, 65535, 3], np.uint32), 50000)
+    d = ctx.to_device(s)
+    sigma = ctx.remap(d)
+    exp, esig = orc.remap(s)
+    assert sigma == esig == 5
+    assert np.array_equal(d.get(), exp)
+    d.free()
+
+
+def test_bad_symbol_reports_error(ctx, wsfm):
+    s = np.array([1, 2, 300], np.uint32)  # 300 >= 2^8
+    with pytest.raises(wsfm.WtgError):
+        ctx.levels(s, 8)
+    ctx.sync()
+
+
+def test_levels_large_bit_exact(ctx, wsfm, orc):
+    """n = 2^24 uint32, L = 20 (the wt_int shape at 1/16 of its full size):
+    bit-exact against the oracle, plus per-level one counts."""
+    n, L = 1 << 24, 20
+    s = wsfm.gen_uniform(0x5EED2026, n, 1 << L)
+    say("large", n, L)
+    w, _ = ctx.levels(s, L)
+    ow, _ = orc.levels_serial(s, L, with_nodes=False)
+    assert np.array_equal(w, ow)
+    bits = np.unpackbits(w.view(np.uint8), bitorder="little")[:n * L].reshape(L, n)
+    for l in range(L):
+        assert bits[l].sum() == int(((s >> (L - 1 - l)) & 1).sum())

@@ -1,0 +1,405 @@
+"""Python front end of libwtg.so (the gfx950 wavelet-tree / rank / select C ABI).
+
+This is plumbing for tests and benchmarks: every call goes straight through
+ctypes to the C ABI declared in include/wtg.h.  There is no CPU fallback: if
+libwtg.so is missing or no gfx950 device is present the calls raise.
+"""
+import ctypes as C
+import os
+
+import numpy as np
+
+HERE = os.path.dirname(os.path.abspath(__file__))
+LIB_PATH = os.path.join(HERE, "libwtg.so")
+NONE = (1 << 64) - 1
+
+_lib = None
+
+vp, u64, u32, i32 = C.c_void_p, C.c_uint64, C.c_uint32, C.c_int
+U64P = C.POINTER(C.c_uint64)
+
+
+class WtgError(RuntimeError):
+    def __init__(self, rc, msg):
+        super().__init__("wtg error %d: %s" % (rc, msg))
+        self.rc = rc
+
+
+class Node(C.Structure):
+    _fields_ = [("bitmapPtr", u64), ("length", u64), ("parent", u64),
+                ("leftChild", u64), ("rightChild", u64)]
+
+
+class Select(C.Structure):
+    _fields_ = [("nbits", u64), ("arg_cnt", u64), ("sb", u64), ("n_long", u64),
+                ("logn", u32), ("logn4", u32), ("bit", C.c_int32), ("_pad", C.c_int32),
+                ("superblock", vp), ("mini", vp), ("mini_width", vp), ("long_index", vp),
+                ("long_width", vp), ("long_pool", vp)]
+
+
+# name -> (restype, argtypes); every symbol here is declared in include/wtg.h
+SIGNATURES = {
+    "wtg_version": (C.c_char_p, []),
+    "wtg_last_error": (C.c_char_p, []),
+    "wtg_device_count": (i32, [C.POINTER(i32)]),
+    "wtg_ctx_create": (i32, [i32, C.POINTER(vp)]),
+    "wtg_ctx_destroy": (i32, [vp]),
+    "wtg_ctx_sync": (i32, [vp]),
+    "wtg_ctx_set_blocking": (i32, [vp, i32]),
+    "wtg_ctx_stream": (vp, [vp]),
+    "wtg_malloc": (i32, [vp, C.POINTER(vp), u64]),
+    "wtg_free": (i32, [vp, vp]),
+    "wtg_memcpy_h2d": (i32, [vp, vp, vp, u64]),
+    "wtg_memcpy_d2h": (i32, [vp, vp, vp, u64]),
+    "wtg_memcpy_d2d": (i32, [vp, vp, vp, u64]),
+    "wtg_memset_zero": (i32, [vp, vp, u64]),
+    "wtg_timer_start": (i32, [vp]),
+    "wtg_timer_stop": (i32, [vp, C.POINTER(C.c_float)]),
+    "wtg_kernel_timing": (i32, [vp, i32]),
+    "wtg_kernel_time": (i32, [vp, C.c_char_p, C.POINTER(C.c_double), U64P]),
+    "wtg_kernel_timing_reset": (i32, [vp]),
+    "wtg_levels_words": (u64, [u64, u32]),
+    "wtg_levels_build": (i32, [vp, vp, i32, u64, u32, vp, vp, u64]),
+    "wtg_remap": (i32, [vp, vp, i32, u64, U64P]),
+    "wtg_wt_host": (i32, [vp, vp, i32, u64, u64, vp, vp]),
+    "wtg_rank9_words": (u64, [u64]),
+    "wtg_rank9_build": (i32, [vp, vp, u64, vp]),
+    "wtg_rank9_batch": (i32, [vp, vp, vp, u64, vp, u64, vp]),
+    "wtg_select_build": (i32, [vp, vp, vp, u64, i32, C.POINTER(Select)]),
+    "wtg_rank9_select_build": (i32, [vp, vp, u64, vp, C.POINTER(Select), C.POINTER(Select)]),
+    "wtg_select_free": (i32, [vp, C.POINTER(Select)]),
+    "wtg_select_batch": (i32, [vp, C.POINTER(Select), vp, vp, u64, vp]),
+    "wtg_wtint_build": (i32, [vp, vp, i32, u64, u32, C.POINTER(vp)]),
+    "wtg_wtint_destroy": (i32, [vp, vp]),
+    "wtg_wtint_info": (i32, [vp, U64P, C.POINTER(u32), U64P, C.POINTER(vp), C.POINTER(vp)]),
+    "wtg_wtint_select1": (C.POINTER(Select), [vp]),
+    "wtg_wtint_select0": (C.POINTER(Select), [vp]),
+    "wtg_wtint_rank_batch": (i32, [vp, vp, vp, vp, u64, vp]),
+    "wtg_wtint_access_batch": (i32, [vp, vp, vp, u64, vp]),
+    "wtg_wtint_select_batch": (i32, [vp, vp, vp, vp, u64, vp]),
+    "wtg_wtint_inverse_select_batch": (i32, [vp, vp, vp, u64, vp, vp]),
+    "wtg_splitmix64": (u64, [U64P]),
+    "wtg_gen_uniform": (i32, [u64, u64, u64, i32, vp]),
+    "wtg_gen_geometric": (i32, [u64, u64, C.c_double, u64, i32, vp]),
+    "wtg_gen_zipf_bytes": (i32, [u64, u64, C.c_double, vp]),
+}
+
+
+def lib():
+    """Load libwtg.so (raises if it has not been built)."""
+    global _lib
+    if _lib is None:
+        if not os.path.exists(LIB_PATH):
+            raise ImportError("libwtg.so not built: run wavelet-suffix-fm-index_amd/build.py")
+        L = C.CDLL(LIB_PATH)
+        for name, (res, args) in SIGNATURES.items():
+            f = getattr(L, name)
+            f.restype = res
+            f.argtypes = args
+        _lib = L
+    return _lib
+
+
+def _check(rc):
+    if rc != 0:
+        raise WtgError(rc, lib().wtg_last_error().decode())
+    return rc
+
+
+def _p(a):
+    return C.c_void_p(a.ctypes.data)
+
+
+def device_count():
+    n = C.c_int(0)
+    _check(lib().wtg_device_count(C.byref(n)))
+    return n.value
+
+
+def levels_of_sigma(sigma):
+    """L = max(1, log2Up(sigma))  (wavelet/levelWT/WT.C:53)."""
+    sigma = int(sigma)
+    return max(1, (sigma - 1).bit_length()) if sigma > 1 else 1
+
+
+# ------------------------------------------------------------ generators
+def gen_uniform(seed, n, rng, dtype=np.uint32):
+    out = np.empty(n, dtype=dtype)
+    _check(lib().wtg_gen_uniform(seed, n, rng, out.itemsize, _p(out)))
+    return out
+
+
+def gen_geometric(seed, n, p, cap, dtype=np.uint32):
+    out = np.empty(n, dtype=dtype)
+    _check(lib().wtg_gen_geometric(seed, n, p, cap, out.itemsize, _p(out)))
+    return out
+
+
+def gen_zipf_bytes(seed, n, s=1.1):
+    out = np.empty(n, dtype=np.uint8)
+    _check(lib().wtg_gen_zipf_bytes(seed, n, s, _p(out)))
+    return out
+
+
+# ------------------------------------------------------------ device side
+class DeviceArray:
+    """A device allocation with a numpy dtype/shape (owned by a Context)."""
+
+    def __init__(self, ctx, nbytes, dtype=np.uint64, shape=None):
+        self.ctx = ctx
+        self.nbytes = int(nbytes)
+        self.dtype = np.dtype(dtype)
+        self.shape = shape if shape is not None else (self.nbytes // self.dtype.itemsize,)
+        p = C.c_void_p()
+        _check(lib().wtg_malloc(ctx.h, C.byref(p), max(1, self.nbytes)))
+        self.ptr = p
+
+    @property
+    def size(self):
+        return int(np.prod(self.shape))
+
+    def free(self):
+        if self.ptr is not None and self.ptr.value:
+            _check(lib().wtg_free(self.ctx.h, self.ptr))
+        self.ptr = None
+
+    def get(self):
+        out = np.empty(self.shape, dtype=self.dtype)
+        if self.nbytes:
+            _check(lib().wtg_memcpy_d2h(self.ctx.h, _p(out), self.ptr, out.nbytes))
+        return out
+
+    def zero(self):
+        _check(lib().wtg_memset_zero(self.ctx.h, self.ptr, self.nbytes))
+
+
+class SelectHandle:
+    """Owns a wtg_select built on the device."""
+
+    def __init__(self, ctx, st, owned=True):
+        self.ctx, self.st, self.owned = ctx, st, owned
+
+    @property
+    def arg_cnt(self):
+        return int(self.st.arg_cnt)
+
+    def content(self):
+        """Copy the structure back: dict of numpy arrays (sizes exact)."""
+        s, ctx = self.st, self.ctx
+        sb, nl = int(s.sb), int(s.n_long)
+        out = {"arg_cnt": int(s.arg_cnt), "sb": sb, "n_long": nl, "logn": int(s.logn),
+               "logn4": int(s.logn4), "bit": int(s.bit)}
+
+        def cp(ptr, n, dt):
+            a = np.empty(n, dtype=dt)
+            if n:
+                _check(lib().wtg_memcpy_d2h(ctx.h, _p(a), C.c_void_p(ptr), a.nbytes))
+            return a
+        out["superblock"] = cp(s.superblock, sb, np.uint64)
+        out["mini"] = cp(s.mini, sb * 64, np.uint32)
+        out["mini_width"] = cp(s.mini_width, sb, np.uint8)
+        out["long_index"] = cp(s.long_index, sb, np.int64)
+        out["long_width"] = cp(s.long_width, sb, np.uint8)
+        out["long_pool"] = cp(s.long_pool, nl * 4096, np.uint64) if nl else np.zeros(0, np.uint64)
+        return out
+
+    def free(self):
+        if self.owned and self.st is not None:
+            _check(lib().wtg_select_free(self.ctx.h, C.byref(self.st)))
+            self.st = None
+
+
+class Context:
+    def __init__(self, device=0):
+        h = C.c_void_p()
+        _check(lib().wtg_ctx_create(device, C.byref(h)))
+        self.h = h
+
+    def close(self):
+        if self.h is not None:
+            lib().wtg_ctx_destroy(self.h)
+            self.h = None
+
+    def __enter__(self):
+        return self
+
+    def __exit__(self, *a):
+        self.close()
+
+    def sync(self):
+        _check(lib().wtg_ctx_sync(self.h))
+
+    def set_blocking(self, b):
+        _check(lib().wtg_ctx_set_blocking(self.h, 1 if b else 0))
+
+    # ---- memory
+    def empty(self, n, dtype=np.uint64):
+        dt = np.dtype(dtype)
+        return DeviceArray(self, int(n) * dt.itemsize, dt, (int(n),))
+
+    def to_device(self, a):
+        a = np.ascontiguousarray(a)
+        d = DeviceArray(self, a.nbytes, a.dtype, a.shape)
+        if a.nbytes:
+            _check(lib().wtg_memcpy_h2d(self.h, d.ptr, _p(a), a.nbytes))
+        return d
+
+    # ---- timing
+    def timer_start(self):
+        _check(lib().wtg_timer_start(self.h))
+
+    def timer_stop(self):
+        ms = C.c_float(0)
+        _check(lib().wtg_timer_stop(self.h, C.byref(ms)))
+        return ms.value
+
+    def kernel_timing(self, enable):
+        _check(lib().wtg_kernel_timing(self.h, 1 if enable else 0))
+
+    def kernel_time(self, name):
+        ms, n = C.c_double(0), C.c_uint64(0)
+        _check(lib().wtg_kernel_time(self.h, name.encode(), C.byref(ms), C.byref(n)))
+        return ms.value, n.value
+
+    def kernel_timing_reset(self):
+        _check(lib().wtg_kernel_timing_reset(self.h))
+
+    # ---- (a) levels
+    def levels_build(self, d_sym, n, L, d_words, d_nodes=None, slots=0):
+        _check(lib().wtg_levels_build(self.h, d_sym.ptr, d_sym.dtype.itemsize, int(n), int(L),
+                                      d_words.ptr, d_nodes.ptr if d_nodes is not None else None,
+                                      int(slots)))
+
+    def levels(self, s, L, with_nodes=False):
+        """Host convenience: returns (words, nodes[slots,5] or None)."""
+        s = np.ascontiguousarray(s)
+        n = s.size
+        d = self.to_device(s)
+        nw = lib().wtg_levels_words(n, L)
+        dw = self.empty(max(nw, 1), np.uint64)
+        dn = None
+        slots = (1 << L) if with_nodes else 0
+        if with_nodes:
+            dn = DeviceArray(self, slots * 40, np.uint64, (slots, 5))
+        try:
+            self.levels_build(d, n, L, dw, dn, slots)
+            words = dw.get()[:nw]
+            nodes = dn.get() if with_nodes else None
+        finally:
+            d.free(); dw.free()
+            if dn is not None:
+                dn.free()
+        return words, nodes
+
+    def remap(self, d_sym):
+        sig = C.c_uint64(0)
+        _check(lib().wtg_remap(self.h, d_sym.ptr, d_sym.dtype.itemsize, d_sym.size, C.byref(sig)))
+        return sig.value
+
+    def wt_host(self, s, sigma):
+        """The WT() drop-in path: host in, host out (levels = max(1, log2Up(sigma)))."""
+        s = np.ascontiguousarray(s)
+        L = levels_of_sigma(sigma)
+        nw = lib().wtg_levels_words(s.size, L)
+        words = np.zeros(max(nw, 1), np.uint64)
+        nodes = np.zeros((2 * int(sigma), 5), np.uint64)
+        _check(lib().wtg_wt_host(self.h, _p(s), s.itemsize, s.size, int(sigma), _p(words), _p(nodes)))
+        return words[:nw], nodes
+
+    # ---- (b) rank
+    def rank9_build(self, d_words, nbits):
+        D = self.empty(lib().wtg_rank9_words(int(nbits)), np.uint64)
+        _check(lib().wtg_rank9_build(self.h, d_words.ptr, int(nbits), D.ptr))
+        return D
+
+    def rank9_batch(self, d_words, D, nbits, d_idx, d_out=None):
+        nq = d_idx.size
+        out = d_out if d_out is not None else self.empty(nq, np.uint64)
+        _check(lib().wtg_rank9_batch(self.h, d_words.ptr, D.ptr, int(nbits), d_idx.ptr, nq, out.ptr))
+        return out
+
+    # ---- (b') select
+    def rank9_select_build(self, d_words, nbits):
+        D = self.empty(lib().wtg_rank9_words(int(nbits)), np.uint64)
+        s1, s0 = Select(), Select()
+        _check(lib().wtg_rank9_select_build(self.h, d_words.ptr, int(nbits), D.ptr,
+                                            C.byref(s1), C.byref(s0)))
+        return D, SelectHandle(self, s1), SelectHandle(self, s0)
+
+    def select_build(self, d_words, D, nbits, bit):
+        s = Select()
+        _check(lib().wtg_select_build(self.h, d_words.ptr, D.ptr, int(nbits), int(bit), C.byref(s)))
+        return SelectHandle(self, s)
+
+    def select_batch(self, sel, d_words, d_i, d_out=None):
+        nq = d_i.size
+        out = d_out if d_out is not None else self.empty(nq, np.uint64)
+        _check(lib().wtg_select_batch(self.h, C.byref(sel.st), d_words.ptr, d_i.ptr, nq, out.ptr))
+        return out
+
+    # ---- wt_int
+    def wtint_build(self, d_sym, n, max_level=0):
+        h = C.c_void_p()
+        _check(lib().wtg_wtint_build(self.h, d_sym.ptr, d_sym.dtype.itemsize, int(n),
+                                     int(max_level), C.byref(h)))
+        return WtInt(self, h)
+
+
+class WtInt:
+    def __init__(self, ctx, h):
+        self.ctx, self.h = ctx, h
+        n, L, sg = C.c_uint64(), C.c_uint32(), C.c_uint64()
+        w, d = C.c_void_p(), C.c_void_p()
+        _check(lib().wtg_wtint_info(h, C.byref(n), C.byref(L), C.byref(sg), C.byref(w), C.byref(d)))
+        self.n, self.L, self.sigma = n.value, L.value, sg.value
+        self.words_ptr, self.dir_ptr = w, d
+        self.nbits = self.n * self.L
+
+    def words(self):
+        nw = (self.nbits + 63) // 64
+        a = np.empty(nw, np.uint64)
+        if nw:
+            _check(lib().wtg_memcpy_d2h(self.ctx.h, _p(a), self.words_ptr, a.nbytes))
+        return a
+
+    def directory(self):
+        nd = lib().wtg_rank9_words(self.nbits)
+        a = np.empty(nd, np.uint64)
+        _check(lib().wtg_memcpy_d2h(self.ctx.h, _p(a), self.dir_ptr, a.nbytes))
+        return a
+
+    def select1(self):
+        return SelectHandle(self.ctx, lib().wtg_wtint_select1(self.h).contents, owned=False)
+
+    def select0(self):
+        return SelectHandle(self.ctx, lib().wtg_wtint_select0(self.h).contents, owned=False)
+
+    def _q(self, fn, *arrs, n_out=1):
+        ctx = self.ctx
+        nq = arrs[0].size
+        d_in = [ctx.to_device(np.ascontiguousarray(a, dtype=np.uint64)) for a in arrs]
+        d_out = [ctx.empty(nq, np.uint64) for _ in range(n_out)]
+        try:
+            _check(fn(ctx.h, self.h, *[d.ptr for d in d_in], nq, *[d.ptr for d in d_out]))
+            res = [d.get() for d in d_out]
+        finally:
+            for d in d_in + d_out:
+                d.free()
+        return res[0] if n_out == 1 else res
+
+    def rank(self, i, c):
+        return self._q(lib().wtg_wtint_rank_batch, i, c)
+
+    def access(self, i):
+        return self._q(lib().wtg_wtint_access_batch, i)
+
+    def select(self, i, c):
+        return self._q(lib().wtg_wtint_select_batch, i, c)
+
+    def inverse_select(self, i):
+        return self._q(lib().wtg_wtint_inverse_select_batch, i, n_out=2)
+
+    def free(self):
+        if self.h is not None:
+            _check(lib().wtg_wtint_destroy(self.ctx.h, self.h))
+            self.h = None

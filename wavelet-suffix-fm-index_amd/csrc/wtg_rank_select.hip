@@ -1,0 +1,493 @@
+// wtg_rank_select.hip -- (b) rank_support_v<1,1> directory and (b') the
+// select_support_mcl<b,1> samples, built in one pass over the bit words.
+//
+// Directory (rank/rank_support_v_par.hpp:78-113): entry b covers words
+// [8b, 8b+8): D[2b] = ones in words [0,8b); D[2b+1] = sum_j c_j << (63-9j) for
+// j = 1..min(7, m-1), plus c_m << (63-9m) when the entry has m < 8 words.
+//
+// Select content (select/select_support_mcl_par.hpp:227-276 / :483-561): for
+// superblock k of 4096 arguments, a = pos(4096k), e = pos(min(4096k+4095,m-1));
+// e-a > logn^4 -> "long" (all 4096 positions) else "mini" (pos(4096k+64t)-a).
+//
+// One thread per directory entry (512 bits).  A block of 256 entries is one
+// look-back tile.  The same thread emits, for each argument rank j in its
+// entry with j%64==0, the low 32 bits of pos(j) straight into slot j/64 of the
+// final mini array (mini offsets are < logn^4 < 2^32, so subtracting the low
+// 32 bits of the superblock start later gives the exact offset), and the
+// superblock starts/ends.  A second kernel classifies long/mini superblocks
+// and rebases the mini offsets; a third fills the rare long superblocks.
+#include "wtg_internal.hpp"
+#include <cstring>
+#include <vector>
+
+#define FLAG_LOOKBACK 1u
+
+struct SelOut {
+  uint32_t* mini;        // worst-case sized: ceil(nbits/4096)*64 entries
+  uint64_t* sbstart;     // ceil(nbits/4096)+1
+  uint64_t* sbend;       // ceil(nbits/4096)+1
+  unsigned long long* last;  // 1 + position of the last argument (0 = none)
+};
+
+struct RankSelArgs {
+  const uint64_t* words;
+  uint64_t nbits;
+  uint64_t W;            // words
+  uint64_t S;            // capacity>>9 (entries 0..S)
+  uint64_t* D;           // out (BUILD) or in (FROM_D)
+  uint64_t* states;
+  uint32_t* counter;
+  uint32_t* err;
+  unsigned long long* ones_total;  // out: total ones
+  SelOut sel[2];         // [0] = select0, [1] = select1
+  int want[2];
+};
+
+__device__ __forceinline__ void emit_samples(const uint64_t* wk, const uint32_t* cum, uint64_t pre,
+                                             uint32_t cnt, uint64_t wbase, bool inv,
+                                             const uint64_t* vmask, const SelOut& o) {
+  // argument ranks j in [pre, pre+cnt) of this entry
+  if (cnt == 0) return;
+  const uint64_t end = pre + cnt;
+  for (uint64_t j = (pre + 63) & ~63ull; j < end; j += 64) {
+    uint32_t r = (uint32_t)(j - pre);
+    int k = 0;
+#pragma unroll
+    for (int q = 1; q < 8; q++) k += (cum[q] <= r) ? 1 : 0;
+    uint64_t x = inv ? (~wk[k] & vmask[k]) : wk[k];
+    uint64_t pos = (wbase + k) * 64 + select_in_word(x, r - cum[k]);
+    o.mini[j >> 6] = (uint32_t)pos;
+    if ((j & 4095) == 0) o.sbstart[j >> 12] = pos;
+  }
+  // superblock end: rank j with j%4096 == 4095
+  uint64_t je = ((pre >> 12) << 12) + 4095;
+  if (je < pre) je += 4096;
+  if (je < end) {
+    uint32_t r = (uint32_t)(je - pre);
+    int k = 0;
+#pragma unroll
+    for (int q = 1; q < 8; q++) k += (cum[q] <= r) ? 1 : 0;
+    uint64_t x = inv ? (~wk[k] & vmask[k]) : wk[k];
+    o.sbend[je >> 12] = (wbase + k) * 64 + select_in_word(x, r - cum[k]);
+  }
+}
+
+template <bool BUILD>
+__global__ __launch_bounds__(256) void k_rank_select(RankSelArgs a) {
+  __shared__ uint32_t s_wave[4];
+  __shared__ uint32_t s_tile;
+  __shared__ uint64_t s_pre;
+  __shared__ unsigned long long s_last[2];
+  uint32_t tile;
+  if (BUILD) {
+    if (threadIdx.x == 0) s_tile = atomicAdd(a.counter, 1u);
+    if (threadIdx.x < 2) s_last[threadIdx.x] = 0;
+    __syncthreads();
+    tile = s_tile;
+  } else {
+    if (threadIdx.x < 2) s_last[threadIdx.x] = 0;
+    __syncthreads();
+    tile = blockIdx.x;
+  }
+  const uint64_t b = (uint64_t)tile * 256 + threadIdx.x;
+  const bool act = b <= a.S;
+  uint64_t wk[8], vm[8];
+  uint32_t c1[9];
+  c1[0] = 0;
+  const uint64_t wb = b * 8;
+#pragma unroll
+  for (int k = 0; k < 8; k++) {
+    const uint64_t wi = wb + k;
+    wk[k] = (act && wi < a.W) ? a.words[wi] : 0ull;
+    // valid-bit mask of the word (positions < nbits)
+    uint64_t lo = wi * 64;
+    vm[k] = (lo >= a.nbits) ? 0ull : ((a.nbits - lo >= 64) ? ~0ull : ((1ull << (a.nbits - lo)) - 1));
+    c1[k + 1] = c1[k] + popc64(wk[k]);
+  }
+  const uint32_t tot1 = act ? c1[8] : 0;
+  uint64_t pre1;
+  if (BUILD) {
+    uint32_t btot;
+    uint32_t ex = 0;
+    {
+      const int w = threadIdx.x >> 6, l = lane_id();
+      uint32_t incl = wave_incl_scan(tot1);
+      if (l == 63) s_wave[w] = incl;
+      __syncthreads();
+      uint32_t off = 0, tt = 0;
+#pragma unroll
+      for (int q = 0; q < 4; q++) { uint32_t t = s_wave[q]; if (q < w) off += t; tt += t; }
+      btot = tt;
+      ex = incl - tot1 + off;
+    }
+    if (threadIdx.x < 64) {
+      uint64_t p = lb_exclusive(a.states, tile, btot, a.err);
+      if (threadIdx.x == 0) s_pre = p;
+    }
+    __syncthreads();
+    pre1 = s_pre + ex;
+    if (act) {
+      // fields: m = valid words of this entry (0 for the padding entry)
+      const uint64_t m = (a.W > wb) ? ((a.W - wb < 8) ? a.W - wb : 8) : 0;
+      uint64_t f = 0;
+      uint64_t i;
+      for (i = 1; i < m; i++) f |= (uint64_t)c1[i] << (63 - 9 * i);
+      if (m == 0) i = 1;
+      if ((i & 7) != 0) f |= (uint64_t)c1[i] << (63 - 9 * (i & 7));
+      a.D[2 * b] = pre1;
+      a.D[2 * b + 1] = f;
+      if (b == a.S) *a.ones_total = pre1 + tot1;
+    }
+  } else {
+    pre1 = act ? a.D[2 * b] : 0;
+  }
+  if (act && a.want[1]) {
+    emit_samples(wk, c1, pre1, tot1, wb, false, vm, a.sel[1]);
+    if (tot1) {
+      int k = 7;
+      while (wk[k] == 0) k--;
+      unsigned long long lp = (wb + k) * 64 + (63 - __clzll(wk[k])) + 1;
+      atomicMax(&s_last[1], lp);
+    }
+  }
+  if (act && a.want[0]) {
+    uint32_t c0[9];
+    c0[0] = 0;
+#pragma unroll
+    for (int k = 0; k < 8; k++) c0[k + 1] = c0[k] + popc64(~wk[k] & vm[k]);
+    const uint64_t vbits_before = (b * 512 < a.nbits) ? b * 512 : a.nbits;
+    const uint64_t pre0 = vbits_before - pre1;
+    emit_samples(wk, c0, pre0, c0[8], wb, true, vm, a.sel[0]);
+    if (c0[8]) {
+      int k = 7;
+      while ((~wk[k] & vm[k]) == 0) k--;
+      uint64_t z = ~wk[k] & vm[k];
+      unsigned long long lp = (wb + k) * 64 + (63 - __clzll(z)) + 1;
+      atomicMax(&s_last[0], lp);
+    }
+  }
+  __syncthreads();
+  if (threadIdx.x < 2 && a.want[threadIdx.x] && s_last[threadIdx.x])
+    atomicMax(a.sel[threadIdx.x].last, s_last[threadIdx.x]);
+}
+
+// ----------------------------------------------------- select assembly
+struct AsmArgs {
+  uint32_t* mini;
+  uint8_t* mini_width;
+  uint8_t* long_width;
+  uint32_t* long_flag;
+  const uint64_t* sbstart;
+  const uint64_t* sbend;
+  const unsigned long long* last;
+  uint64_t m;          // arguments
+  uint64_t sb;
+  uint32_t logn4;
+};
+
+__device__ __forceinline__ uint32_t hi_dev(uint64_t x) { return x ? 63 - __clzll(x) : 0; }
+
+__global__ void k_select_assemble(AsmArgs a) {
+  const uint64_t idx = (uint64_t)blockIdx.x * blockDim.x + threadIdx.x;
+  if (idx >= a.sb * 64) return;
+  const uint64_t k = idx >> 6, t = idx & 63;
+  const uint64_t st = a.sbstart[k];
+  const uint64_t en = (k + 1 < a.sb) ? a.sbend[k] : (uint64_t)(*a.last) - 1;
+  const bool lng = (en - st) > (uint64_t)a.logn4;
+  const uint64_t nsamp = (a.m + 63) / 64;
+  uint32_t v = 0;
+  if (!lng && idx < nsamp) v = a.mini[idx] - (uint32_t)st;
+  a.mini[idx] = v;
+  if (t == 0) {
+    a.mini_width[k] = lng ? 0 : (uint8_t)(hi_dev(en - st) + 1);
+    a.long_width[k] = lng ? (uint8_t)(hi_dev(en) + 1) : 0;
+    a.long_flag[k] = lng ? 1u : 0u;
+  }
+}
+
+__global__ void k_long_index(const uint32_t* __restrict__ flag, const uint32_t* __restrict__ ex,
+                             int64_t* __restrict__ li, uint64_t sb) {
+  const uint64_t k = (uint64_t)blockIdx.x * blockDim.x + threadIdx.x;
+  if (k < sb) li[k] = flag[k] ? (int64_t)ex[k] : -1;
+}
+
+// one block per long superblock: enumerate the arguments in [a, e]
+__global__ __launch_bounds__(256) void k_long_fill(const uint64_t* __restrict__ words, uint64_t nbits,
+                                                   int bit, const uint64_t* __restrict__ sbstart,
+                                                   const uint64_t* __restrict__ sbend,
+                                                   const unsigned long long* last,
+                                                   const uint32_t* __restrict__ flag,
+                                                   const uint32_t* __restrict__ ex,
+                                                   uint64_t sb, uint64_t* __restrict__ pool) {
+  __shared__ uint32_t s_wave[4];
+  const uint64_t k = blockIdx.x;
+  if (k >= sb || !flag[k]) return;
+  const uint64_t st = sbstart[k];
+  const uint64_t en = (k + 1 < sb) ? sbend[k] : (uint64_t)(*last) - 1;
+  uint64_t* out = pool + (uint64_t)ex[k] * 4096;
+  uint32_t run = 0;
+  const uint64_t w0 = st >> 6, w1 = en >> 6;
+  for (uint64_t base = w0; base <= w1; base += 256) {
+    const uint64_t wi = base + threadIdx.x;
+    uint64_t x = 0;
+    if (wi <= w1) {
+      x = bit ? words[wi] : ~words[wi];
+      uint64_t lo = wi * 64;
+      if (lo < st) x &= ~0ull << (st - lo);
+      if (lo + 63 > en) x &= (en - lo == 63) ? ~0ull : ((1ull << (en - lo + 1)) - 1);
+      if (lo + 64 > nbits) x &= (nbits > lo) ? ((nbits - lo >= 64) ? ~0ull : ((1ull << (nbits - lo)) - 1)) : 0ull;
+    }
+    const uint32_t c = popc64(x);
+    const int w = threadIdx.x >> 6, l = lane_id();
+    uint32_t incl = wave_incl_scan(c);
+    if (l == 63) s_wave[w] = incl;
+    __syncthreads();
+    uint32_t off = 0, tt = 0;
+#pragma unroll
+    for (int q = 0; q < 4; q++) { uint32_t t = s_wave[q]; if (q < w) off += t; tt += t; }
+    __syncthreads();
+    uint32_t r = run + incl - c + off;
+    while (x) {
+      if (r < 4096) out[r] = wi * 64 + (uint64_t)__builtin_ctzll(x);
+      r++;
+      x &= x - 1;
+    }
+    run += tt;
+  }
+}
+
+// ------------------------------------------------------------- queries
+__global__ void k_rank9_query(const uint64_t* __restrict__ words, const uint64_t* __restrict__ D,
+                              uint64_t nbits, const uint64_t* __restrict__ idx, uint64_t nq,
+                              uint64_t* __restrict__ out) {
+  const uint64_t q = (uint64_t)blockIdx.x * blockDim.x + threadIdx.x;
+  if (q >= nq) return;
+  const uint64_t i = idx[q];
+  if (i > nbits) { out[q] = WTG_NONE; return; }
+  const uint64_t b = i >> 9;
+  uint64_t r = D[2 * b] + ((D[2 * b + 1] >> (63 - 9 * ((i & 511) >> 6))) & 0x1FF);
+  if (i & 63) r += popc64(words[i >> 6] & ((1ull << (i & 63)) - 1));
+  out[q] = r;
+}
+
+__global__ void k_select_query(SelQ s, const uint64_t* __restrict__ iv, uint64_t nq, uint64_t m,
+                               uint64_t* __restrict__ out) {
+  const uint64_t q = (uint64_t)blockIdx.x * blockDim.x + threadIdx.x;
+  if (q >= nq) return;
+  const uint64_t i = iv[q];
+  out[q] = (i >= 1 && i <= m) ? select_plain(s, i) : WTG_NONE;
+}
+
+// ================================================================== host
+extern "C" uint64_t wtg_rank9_words(uint64_t nbits) {
+  if (nbits == 0) return 2;
+  const uint64_t cap = (nbits + 63) / 64 * 64;
+  return ((cap >> 9) + 1) * 2;
+}
+
+namespace {
+struct SelTmp {
+  uint64_t* sbend = nullptr;
+  unsigned long long* last = nullptr;
+};
+
+int sel_alloc_worst(wtg_ctx* ctx, uint64_t nbits, int bit, wtg_select* s) {
+  std::memset(s, 0, sizeof(*s));
+  s->nbits = nbits;
+  s->bit = bit;
+  const uint64_t sbmax = wtg_ceil_div(nbits, 4096) + 1;
+  WTG_TRY(wtg_malloc(ctx, (void**)&s->mini, sbmax * 64 * 4));
+  WTG_TRY(wtg_malloc(ctx, (void**)&s->superblock, sbmax * 8));
+  return WTG_OK;
+}
+
+// finish one select structure once the arguments are counted
+int sel_finish(wtg_ctx* ctx, const uint64_t* d_words, uint64_t nbits, wtg_select* s, uint64_t m,
+               const uint64_t* sbend, const unsigned long long* last, char* scratch) {
+  const uint64_t cap = (nbits + 63) / 64 * 64;
+  s->arg_cnt = m;
+  s->logn = (uint32_t)wtg_hi_host(cap) + 1;
+  s->logn4 = (s->logn * s->logn) * (s->logn * s->logn);
+  s->sb = wtg_ceil_div(m, 4096);
+  s->n_long = 0;
+  const uint64_t sb = s->sb;
+  WTG_TRY(wtg_malloc(ctx, (void**)&s->mini_width, sb ? sb : 1));
+  WTG_TRY(wtg_malloc(ctx, (void**)&s->long_width, sb ? sb : 1));
+  WTG_TRY(wtg_malloc(ctx, (void**)&s->long_index, (sb ? sb : 1) * 8));
+  if (sb == 0) return WTG_OK;
+  uint32_t* flag = (uint32_t*)scratch;
+  uint32_t* ex = (uint32_t*)(scratch + wtg_align256(sb * 4));
+  uint64_t* st = (uint64_t*)(scratch + wtg_align256(sb * 4) + wtg_align256((sb + 1) * 4));
+  uint32_t* ctr = (uint32_t*)((char*)st + wtg_align256((wtg_ceil_div(sb, 4096) + 1) * 8));
+  WTG_TRY(wtg_fill_bytes_zero(ctx, st, (char*)ctr + 256 - (char*)st));
+  AsmArgs aa{};
+  aa.mini = s->mini; aa.mini_width = s->mini_width; aa.long_width = s->long_width;
+  aa.long_flag = flag; aa.sbstart = s->superblock; aa.sbend = sbend; aa.last = last;
+  aa.m = m; aa.sb = sb; aa.logn4 = s->logn4;
+  WTG_LAUNCH(ctx, "select_assemble", k_select_assemble, dim3((unsigned)wtg_ceil_div(sb * 64, 256)),
+             dim3(256), 0, aa);
+  WTG_LAUNCH_CHECK();
+  WTG_LAUNCH(ctx, "select_scan", k_scan_u32<0>, dim3((unsigned)wtg_ceil_div(sb, 4096)), dim3(256), 0,
+             flag, ex, sb, st, ctr, ctx->d_flag);
+  WTG_LAUNCH_CHECK();
+  WTG_LAUNCH(ctx, "select_longidx", k_long_index, dim3((unsigned)wtg_ceil_div(sb, 256)), dim3(256), 0,
+             flag, ex, s->long_index, sb);
+  WTG_LAUNCH_CHECK();
+  uint32_t nl = 0;
+  WTG_CHECK_HIP(hipMemcpyAsync(&nl, ex + sb, 4, hipMemcpyDeviceToHost, ctx->stream));
+  WTG_CHECK_HIP(hipStreamSynchronize(ctx->stream));
+  s->n_long = nl;
+  if (nl) {
+    WTG_TRY(wtg_malloc(ctx, (void**)&s->long_pool, (uint64_t)nl * 4096 * 8));
+    WTG_TRY(wtg_fill_u64(ctx, s->long_pool, (uint64_t)nl * 4096, 0));
+    WTG_LAUNCH(ctx, "select_longfill", k_long_fill, dim3((unsigned)sb), dim3(256), 0, d_words, nbits,
+               s->bit, s->superblock, sbend, last, flag, ex, sb, s->long_pool);
+    WTG_LAUNCH_CHECK();
+  }
+  return WTG_OK;
+}
+
+}  // namespace
+
+int wtg_rank_select_run(wtg_ctx* ctx, const uint64_t* d_words, uint64_t nbits, uint64_t* d_dir,
+                        bool build, wtg_select* sel1, wtg_select* sel0) {
+  const uint64_t cap = (nbits + 63) / 64 * 64;
+  const uint64_t W = cap >> 6, S = cap >> 9;
+  const uint64_t entries = S + 1;
+  const uint64_t tiles = wtg_ceil_div(entries, 256);
+  const uint64_t sbmax = wtg_ceil_div(nbits, 4096) + 1;
+  // workspace: states | counter | ones_total | last[2] | sbend[2] | finish scratch
+  size_t o = 0;
+  const size_t o_states = o; o = wtg_align256(o + (tiles + 1) * 8);
+  const size_t o_ctr = o; o = wtg_align256(o + 256);
+  const size_t o_end0 = o; o = wtg_align256(o + sbmax * 8);
+  const size_t o_end1 = o; o = wtg_align256(o + sbmax * 8);
+  const size_t o_fin = o; o = wtg_align256(o + 4 * wtg_align256((sbmax + 1) * 8) + 4096);
+  WTG_TRY(wtg_ws_reserve(ctx, o));
+  char* ws = (char*)ctx->ws;
+  WTG_TRY(wtg_fill_bytes_zero(ctx, ws, o_end0));
+  uint32_t* ctr = (uint32_t*)(ws + o_ctr);
+  unsigned long long* ones_total = (unsigned long long*)(ws + o_ctr + 64);
+  unsigned long long* last = (unsigned long long*)(ws + o_ctr + 128);  // [0]=zeros, [1]=ones
+  wtg_select* outs[2] = {sel0, sel1};
+  RankSelArgs ra{};
+  ra.words = d_words; ra.nbits = nbits; ra.W = W; ra.S = S; ra.D = d_dir;
+  ra.states = (uint64_t*)(ws + o_states); ra.counter = ctr; ra.err = ctx->d_flag;
+  ra.ones_total = ones_total;
+  for (int t = 0; t < 2; t++) {
+    ra.want[t] = outs[t] != nullptr && nbits > 0;
+    if (ra.want[t]) {
+      WTG_TRY(sel_alloc_worst(ctx, nbits, t, outs[t]));
+      ra.sel[t].mini = outs[t]->mini;
+      ra.sel[t].sbstart = outs[t]->superblock;
+      ra.sel[t].sbend = (uint64_t*)(ws + (t ? o_end1 : o_end0));
+      ra.sel[t].last = last + t;
+    } else if (outs[t]) {
+      std::memset(outs[t], 0, sizeof(wtg_select));
+      outs[t]->bit = t;
+    }
+  }
+  if (nbits == 0) {
+    if (build) WTG_TRY(wtg_fill_u64(ctx, d_dir, 2, 0));
+    for (int t = 0; t < 2; t++)
+      if (outs[t]) {
+        WTG_TRY(wtg_malloc(ctx, (void**)&outs[t]->mini_width, 1));
+        WTG_TRY(wtg_malloc(ctx, (void**)&outs[t]->long_width, 1));
+        WTG_TRY(wtg_malloc(ctx, (void**)&outs[t]->long_index, 8));
+        outs[t]->logn = 1;
+        outs[t]->logn4 = 1;
+      }
+    return WTG_OK;
+  }
+  if (build) {
+    WTG_LAUNCH(ctx, "rank_select", k_rank_select<true>, dim3((unsigned)tiles), dim3(256), 0, ra);
+  } else {
+    WTG_LAUNCH(ctx, "rank_select", k_rank_select<false>, dim3((unsigned)tiles), dim3(256), 0, ra);
+  }
+  WTG_LAUNCH_CHECK();
+  uint64_t m1 = 0;
+  if (build) {
+    unsigned long long t = 0;
+    WTG_CHECK_HIP(hipMemcpyAsync(&t, ones_total, 8, hipMemcpyDeviceToHost, ctx->stream));
+    WTG_CHECK_HIP(hipStreamSynchronize(ctx->stream));
+    m1 = t;
+  } else {
+    // ones = D[2S] + ones in the last entry
+    uint64_t e[2];
+    WTG_CHECK_HIP(hipMemcpyAsync(e, d_dir + 2 * S, 16, hipMemcpyDeviceToHost, ctx->stream));
+    std::vector<uint64_t> tailw(8, 0);
+    const uint64_t wb = 8 * S;
+    const uint64_t nt = W > wb ? W - wb : 0;
+    if (nt) WTG_CHECK_HIP(hipMemcpyAsync(tailw.data(), d_words + wb, nt * 8, hipMemcpyDeviceToHost, ctx->stream));
+    WTG_CHECK_HIP(hipStreamSynchronize(ctx->stream));
+    m1 = e[0];
+    for (uint64_t k = 0; k < nt; k++) m1 += __builtin_popcountll(tailw[k]);
+  }
+  const uint64_t m[2] = {nbits - m1, m1};
+  for (int t = 0; t < 2; t++) {
+    if (!ra.want[t]) continue;
+    WTG_TRY(sel_finish(ctx, d_words, nbits, outs[t], m[t], ra.sel[t].sbend, last + t, ws + o_fin));
+  }
+  return WTG_OK;
+}
+
+extern "C" int wtg_rank9_build(wtg_ctx* ctx, const uint64_t* d_words, uint64_t nbits, uint64_t* d_dir) {
+  WTG_REQUIRE(ctx && d_dir && (nbits == 0 || d_words), "wtg_rank9_build: null argument");
+  WTG_TRY(wtg_set_device(ctx));
+  WTG_TRY(wtg_rank_select_run(ctx, d_words, nbits, d_dir, true, nullptr, nullptr));
+  return wtg_finish(ctx);
+}
+
+extern "C" int wtg_rank9_select_build(wtg_ctx* ctx, const uint64_t* d_words, uint64_t nbits,
+                                      uint64_t* d_dir, wtg_select* sel1, wtg_select* sel0) {
+  WTG_REQUIRE(ctx && d_dir && (nbits == 0 || d_words), "wtg_rank9_select_build: null argument");
+  WTG_TRY(wtg_set_device(ctx));
+  WTG_TRY(wtg_rank_select_run(ctx, d_words, nbits, d_dir, true, sel1, sel0));
+  return wtg_finish(ctx);
+}
+
+extern "C" int wtg_select_build(wtg_ctx* ctx, const uint64_t* d_words, const uint64_t* d_dir,
+                                uint64_t nbits, int bit, wtg_select* out) {
+  WTG_REQUIRE(ctx && out && d_dir && (nbits == 0 || d_words), "wtg_select_build: null argument");
+  WTG_REQUIRE(bit == 0 || bit == 1, "bit must be 0 or 1");
+  WTG_TRY(wtg_set_device(ctx));
+  WTG_TRY(wtg_rank_select_run(ctx, d_words, nbits, (uint64_t*)d_dir, false, bit ? out : nullptr,
+                          bit ? nullptr : out));
+  return wtg_finish(ctx);
+}
+
+extern "C" int wtg_select_free(wtg_ctx* ctx, wtg_select* s) {
+  WTG_REQUIRE(ctx && s, "null argument");
+  wtg_free(ctx, s->superblock);
+  wtg_free(ctx, s->mini);
+  wtg_free(ctx, s->mini_width);
+  wtg_free(ctx, s->long_index);
+  wtg_free(ctx, s->long_width);
+  wtg_free(ctx, s->long_pool);
+  std::memset(s, 0, sizeof(*s));
+  return WTG_OK;
+}
+
+extern "C" int wtg_rank9_batch(wtg_ctx* ctx, const uint64_t* d_words, const uint64_t* d_dir,
+                               uint64_t nbits, const uint64_t* d_idx, uint64_t nq, uint64_t* d_out) {
+  WTG_REQUIRE(ctx && d_dir && (nq == 0 || (d_idx && d_out)), "wtg_rank9_batch: null argument");
+  if (nq == 0) return WTG_OK;
+  WTG_TRY(wtg_set_device(ctx));
+  WTG_LAUNCH(ctx, "rank9_query", k_rank9_query, dim3((unsigned)wtg_ceil_div(nq, 256)), dim3(256), 0,
+             d_words, d_dir, nbits, d_idx, nq, d_out);
+  WTG_LAUNCH_CHECK();
+  return wtg_finish(ctx);
+}
+
+extern "C" int wtg_select_batch(wtg_ctx* ctx, const wtg_select* s, const uint64_t* d_words,
+                                const uint64_t* d_i, uint64_t nq, uint64_t* d_out) {
+  WTG_REQUIRE(ctx && s && (nq == 0 || (d_i && d_out)), "wtg_select_batch: null argument");
+  if (nq == 0) return WTG_OK;
+  WTG_TRY(wtg_set_device(ctx));
+  SelQ q{s->superblock, s->mini, s->long_index, s->long_pool, d_words,
+         (s->nbits + 63) / 64, s->bit};
+  WTG_LAUNCH(ctx, "select_query", k_select_query, dim3((unsigned)wtg_ceil_div(nq, 256)), dim3(256), 0,
+             q, d_i, nq, s->arg_cnt, d_out);
+  WTG_LAUNCH_CHECK();
+  return wtg_finish(ctx);
+}
