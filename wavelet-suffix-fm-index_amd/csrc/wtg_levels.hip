@@ -4,56 +4,76 @@
 // level l holds P_l = the stable sort of the symbols by x >> (L-l); the bit at
 // global position l*n+i is (P_l[i] >> (L-1-l)) & 1.
 //
-// GPU pipeline for one build (all integer, HBM bound):
-//   1. k_hist       histogram of symbol values (2^L bins)
-//   2. k_scan_u32   C = exclusive prefix of the histogram (node starts)
-//   3. k_nodetab    per level l < L-1 and node prefix p: A(p) = S(p)-Zs(p),
-//                   B(p) = nz(p)+Zs(p) (S = node start, nz = zeros of the node,
-//                   Zs = zeros of all earlier nodes of the level)
-//   4. k_nodes      WTnode records (optional)
-//   5. k_level<l>   per level: 16-byte vector loads, the level's bit words
-//                   formed from wave ballots, zero counts scanned with mbcnt +
-//                   LDS + a decoupled look-back, and the stable scatter
-//                   dest = bit ? i - Z(i) + B(p) : Z(i) + A(p)
-//                   into the next ping-pong buffer (last level: bits only).
+// GPU pipeline for one build (all integer, HBM bound).  CNT_j[q] = number of
+// symbols whose top j bits equal q (kept for every j in one heap-ordered array).
+//   1. k_prehist      LDS-privatised histogram of the top h = min(L,13) bits
+//                     (one partial row per block) + k_prehist_reduce + k_cnt_up
+//                     -> CNT_0..CNT_h
+//   2. per level k:
+//      k_nodetab      node starts S(p) and, for scatter levels, the per-node
+//                     constants A(p) = S(p)-Zs(p), B(p) = nz(p)+Zs(p) (nz =
+//                     zeros of node p = CNT_{k+1}[2p], Zs = zeros of the earlier
+//                     nodes of the level); for k+1 > h it also derives
+//                     CNT_{k+1} from CNT_k and the counts produced by level k-1.
+//      k_level        16-byte vector loads; the level's bit words from wave
+//                     ballots; tile zero ranks from mbcnt + LDS; tile stable
+//                     0/1 partition staged in LDS; a decoupled look-back for the
+//                     global zero prefix; lane-consecutive stores of
+//                     dest = slot < Z_t ? Z_prev + slot + A(p)
+//                                       : O_prev + slot - Z_t + B(p)
+//                     into the next ping-pong buffer (last level: bits only).
+//                     Levels whose next counts are not covered by the pre-pass
+//                     also count, per (k+1)-prefix run of the staged tile, the
+//                     elements whose next bit is 1 (two atomics per run).
+//   3. k_nodes        WTnode records (optional).
 #include "wtg_internal.hpp"
 
 #define FLAG_LOOKBACK 1u
 #define FLAG_BADSYM 2u
+#define PRE_BITS 13       // top bits histogrammed by the pre-pass
+typedef uint32_t v4u32 __attribute__((ext_vector_type(4)));
+#define PRE_BLOCKS 512    // partial histogram rows
 
-// ------------------------------------------------------------ histogram
+static __host__ __device__ __forceinline__ uint64_t heap_off(uint32_t j) { return (1ull << j) - 1; }
+
+// ------------------------------------------------------ pre-pass histogram
 template <typename T>
-__global__ __launch_bounds__(256) void k_hist_lds(const T* __restrict__ s, uint64_t n,
-                                                  uint32_t* __restrict__ hist, uint32_t nbins,
-                                                  uint32_t* err) {
+__global__ __launch_bounds__(256) void k_prehist(const T* __restrict__ s, uint64_t n, uint32_t shift,
+                                                 uint32_t nbins, uint64_t chunk,
+                                                 uint32_t* __restrict__ partial, uint32_t* err) {
   extern __shared__ uint32_t sh[];
-  for (uint32_t i = threadIdx.x; i < nbins; i += blockDim.x) sh[i] = 0;
+  for (uint32_t i = threadIdx.x; i < nbins; i += 256) sh[i] = 0;
   __syncthreads();
-  uint64_t i = (uint64_t)blockIdx.x * blockDim.x + threadIdx.x;
-  uint64_t stride = (uint64_t)gridDim.x * blockDim.x;
+  const uint64_t a = (uint64_t)blockIdx.x * chunk;
+  const uint64_t b = (a + chunk < n) ? a + chunk : n;
   bool bad = false;
-  for (; i < n; i += stride) {
-    uint64_t x = (uint64_t)s[i];
-    if (x < nbins) atomicAdd(&sh[x], 1u); else bad = true;
+  for (uint64_t i = a + threadIdx.x; i < b; i += 256) {
+    const uint64_t q = (uint64_t)s[i] >> shift;
+    if (q < nbins) atomicAdd(&sh[q], 1u); else bad = true;
   }
   if (bad) atomicOr(err, FLAG_BADSYM);
   __syncthreads();
-  for (uint32_t b = threadIdx.x; b < nbins; b += blockDim.x)
-    if (sh[b]) atomicAdd(&hist[b], sh[b]);
+  uint32_t* row = partial + (uint64_t)blockIdx.x * nbins;
+  for (uint32_t i = threadIdx.x; i < nbins; i += 256) row[i] = sh[i];
 }
 
-template <typename T>
-__global__ __launch_bounds__(256) void k_hist_global(const T* __restrict__ s, uint64_t n,
-                                                     uint32_t* __restrict__ hist, uint32_t nbins,
-                                                     uint32_t* err) {
-  uint64_t i = (uint64_t)blockIdx.x * blockDim.x + threadIdx.x;
-  uint64_t stride = (uint64_t)gridDim.x * blockDim.x;
-  bool bad = false;
-  for (; i < n; i += stride) {
-    uint64_t x = (uint64_t)s[i];
-    if (x < nbins) atomicAdd(&hist[x], 1u); else bad = true;
+__global__ void k_prehist_reduce(const uint32_t* __restrict__ partial, uint32_t rows, uint32_t nbins,
+                                 uint32_t* __restrict__ out) {
+  const uint32_t q = blockIdx.x * blockDim.x + threadIdx.x;
+  if (q >= nbins) return;
+  uint32_t acc = 0;
+  for (uint32_t r = 0; r < rows; r++) acc += partial[(uint64_t)r * nbins + q];
+  out[q] = acc;
+}
+
+// CNT_j[q] = CNT_{j+1}[2q] + CNT_{j+1}[2q+1] for j = h-1 .. 0 (one block)
+__global__ __launch_bounds__(256) void k_cnt_up(uint32_t* __restrict__ cnt, uint32_t h) {
+  for (int j = (int)h - 1; j >= 0; j--) {
+    uint32_t* lo = cnt + heap_off(j);
+    const uint32_t* hi = cnt + heap_off(j + 1);
+    for (uint32_t q = threadIdx.x; q < (1u << j); q += 256) lo[q] = hi[2 * q] + hi[2 * q + 1];
+    __syncthreads();
   }
-  if (bad) atomicOr(err, FLAG_BADSYM);
 }
 
 __global__ void k_count_nonzero(const uint32_t* __restrict__ h, uint64_t n,
@@ -68,77 +88,97 @@ __global__ void k_count_nonzero(const uint32_t* __restrict__ h, uint64_t n,
 
 // ------------------------------------------------------------ node tables
 struct NodeTabArgs {
-  const uint32_t* C;     // exclusive prefix of the 2^L-bin histogram, C[2^L] = n
-  uint2* tab;            // level l (< L-1) at offset 2^l - 1
-  uint64_t* states;      // per tile (tiles of one level are contiguous)
+  uint32_t* cnt;         // heap CNT (levels 0..L)
+  const uint32_t* c1;    // heap, level k: #elements of k-prefix p whose next bit is 1
+  uint32_t* start;       // heap START (levels 0..L-1)
+  uint2* tab;            // level k table (A,B) or nullptr
+  uint64_t* st_s;        // look-back states, scan of CNT_k
+  uint64_t* st_z;        // look-back states, scan of nz
   uint32_t* counter;
   uint32_t* err;
-  uint32_t L;
-  uint32_t tile_base[25];  // first tile of level l; tile_base[L-1] = total tiles
+  unsigned long long* distinct;  // optional: += #nonzero CNT_{k+1}
+  uint32_t k;
+  uint32_t derive;       // 1: CNT_{k+1} = [CNT_k - c1, c1] (not covered by the pre-pass)
 };
 
 __global__ __launch_bounds__(256) void k_nodetab(NodeTabArgs a) {
   __shared__ uint32_t s_wave[4];
   __shared__ uint32_t s_tile;
-  __shared__ uint64_t s_pre;
+  __shared__ uint64_t s_pre[2];
   if (threadIdx.x == 0) s_tile = atomicAdd(a.counter, 1u);
   __syncthreads();
-  const uint32_t t = s_tile;
-  uint32_t l = 0;
-  while (l + 1 < a.L - 1 && a.tile_base[l + 1] <= t) l++;
-  const uint32_t tile_in_level = t - a.tile_base[l];
-  const uint64_t nnodes = 1ull << l;
-  const uint32_t R = 1u << (a.L - l);
-  const uint64_t p0 = (uint64_t)tile_in_level * 4096 + (uint64_t)threadIdx.x * 16;
-  uint32_t S[16], nz[16];
-  uint32_t sum = 0;
+  const uint32_t tile = s_tile;
+  const uint64_t nnodes = 1ull << a.k;
+  const uint32_t* ck = a.cnt + heap_off(a.k);
+  uint32_t* ck1 = a.cnt + heap_off(a.k + 1);
+  const uint32_t* c1 = a.c1 + heap_off(a.k);
+  const uint64_t p0 = (uint64_t)tile * 4096 + (uint64_t)threadIdx.x * 16;
+  uint32_t cv[16], nz[16];
+  uint32_t sum_c = 0, sum_z = 0, nzc = 0;
 #pragma unroll
-  for (int k = 0; k < 16; k++) {
-    uint64_t p = p0 + k;
+  for (int j = 0; j < 16; j++) {
+    const uint64_t p = p0 + j;
+    uint32_t c = 0, z = 0;
     if (p < nnodes) {
-      uint32_t st = a.C[p * R], mid = a.C[p * R + R / 2];
-      S[k] = st; nz[k] = mid - st;
-    } else { S[k] = 0; nz[k] = 0; }
-    sum += nz[k];
+      c = ck[p];
+      if (a.derive) {
+        const uint32_t one = c1[p];
+        z = c - one;
+        ck1[2 * p] = z;
+        ck1[2 * p + 1] = one;
+        nzc += (z != 0) + (one != 0);
+      } else {
+        z = ck1[2 * p];
+        nzc += (z != 0) + (ck1[2 * p + 1] != 0);
+      }
+    }
+    cv[j] = c; nz[j] = z;
+    sum_c += c; sum_z += z;
   }
-  uint32_t tot;
-  uint32_t ex = block_excl_scan_u32(sum, s_wave, tot);
+  uint32_t tot_c, tot_z;
+  const uint32_t ex_c = block_excl_scan_u32(sum_c, s_wave, tot_c);
+  const uint32_t ex_z = block_excl_scan_u32(sum_z, s_wave, tot_z);
   if (threadIdx.x < 64) {
-    uint64_t pre = lb_exclusive(a.states + a.tile_base[l], tile_in_level, tot, a.err);
-    if (threadIdx.x == 0) s_pre = pre;
+    const uint64_t ps = lb_exclusive(a.st_s, tile, tot_c, a.err);
+    const uint64_t pz = lb_exclusive(a.st_z, tile, tot_z, a.err);
+    if (threadIdx.x == 0) { s_pre[0] = ps; s_pre[1] = pz; }
   }
   __syncthreads();
-  uint32_t zs = (uint32_t)(s_pre + ex);
-  uint2* tab = a.tab + (nnodes - 1);
+  uint32_t S = (uint32_t)(s_pre[0] + ex_c), Z = (uint32_t)(s_pre[1] + ex_z);
+  uint32_t* st = a.start + heap_off(a.k);
 #pragma unroll
-  for (int k = 0; k < 16; k++) {
-    uint64_t p = p0 + k;
-    if (p < nnodes) tab[p] = make_uint2(S[k] - zs, nz[k] + zs);
-    zs += nz[k];
+  for (int j = 0; j < 16; j++) {
+    const uint64_t p = p0 + j;
+    if (p < nnodes) {
+      st[p] = S;
+      if (a.tab) a.tab[p] = make_uint2(S - Z, nz[j] + Z);
+    }
+    S += cv[j];
+    Z += nz[j];
+  }
+  if (a.distinct) {
+    uint64_t d = wave_sum((uint64_t)nzc);
+    if (lane_id() == 0 && d) atomicAdd(a.distinct, (unsigned long long)d);
   }
 }
 
 // ---------------------------------------------------------------- WTnodes
-__global__ void k_nodes(const uint32_t* __restrict__ C, uint32_t L, uint64_t n,
-                        wtg_node* __restrict__ nodes, uint64_t slots) {
-  uint64_t v = (uint64_t)blockIdx.x * blockDim.x + threadIdx.x;
+__global__ void k_nodes(const uint32_t* __restrict__ cnt, const uint32_t* __restrict__ start,
+                        uint32_t L, uint64_t n, wtg_node* __restrict__ nodes, uint64_t slots) {
+  const uint64_t v = (uint64_t)blockIdx.x * blockDim.x + threadIdx.x;
   if (v >= slots) return;
   wtg_node r;
   r.bitmapPtr = 0; r.length = 0; r.parent = WTG_NONE; r.leftChild = WTG_NONE; r.rightChild = WTG_NONE;
-  uint64_t heap_end = (1ull << L) - 1;
-  if (v < heap_end) {
-    uint32_t l = 63 - __clzll(v + 1);
-    uint64_t p = v + 1 - (1ull << l);
-    uint64_t R = 1ull << (L - l);
-    uint64_t st = C[p * R], en = C[(p + 1) * R];
-    if (en > st || v == 0) {
-      r.bitmapPtr = (uint64_t)l * n + st;
-      r.length = en - st;
+  if (v < heap_off(L)) {
+    const uint32_t l = 63 - __clzll(v + 1);
+    const uint64_t c = cnt[v];
+    if (c > 0 || v == 0) {
+      r.bitmapPtr = (uint64_t)l * n + start[v];
+      r.length = c;
       r.parent = v ? (v - 1) / 2 : WTG_NONE;
       if (l + 1 < L) {
-        uint64_t mid = C[p * R + R / 2];
-        if (mid > st) r.leftChild = 2 * v + 1;
-        if (en > mid) r.rightChild = 2 * v + 2;
+        if (cnt[2 * v + 1] > 0) r.leftChild = 2 * v + 1;
+        if (cnt[2 * v + 2] > 0) r.rightChild = 2 * v + 2;
       }
     }
   }
@@ -185,6 +225,7 @@ struct LevelArgs {
   uint32_t node_shift;   // L-l (node prefix p = x >> node_shift)
   uint32_t nnodes;       // 2^l
   const uint2* tab;      // this level's (A,B) table
+  uint32_t* c1;          // counts of next-bit-1 per (l+1)-prefix (heap level l+1)
   uint64_t* states;
   uint32_t* counter;
   uint32_t* err;
@@ -199,15 +240,18 @@ template <> struct VecOf<uint64_t> { static constexpr int V = 2; };
 
 #define LV_STEPS 4
 
-template <typename T, bool SCATTER, bool VEC>
+template <typename T, bool SCATTER, bool VEC, bool COUNT>
 __global__ __launch_bounds__(256) void k_level(LevelArgs a) {
   constexpr int V = VecOf<T>::V;
   constexpr int G = 64 / V;              // lanes per bit word
   constexpr int TILE = 256 * LV_STEPS * V;
   constexpr int NW = TILE / 64;          // words per tile
+  constexpr int ROUNDS = TILE / 256;     // store rounds (one element per lane)
   constexpr int NPL = (V == 16) ? 5 : (V == 8) ? 4 : (V == 4) ? 3 : 2;  // bit-planes of a count <= V
   __shared__ uint64_t s_words[NW + 1];
   __shared__ uint32_t s_wz[LV_STEPS * 4];
+  __shared__ T s_stage[SCATTER ? TILE : 1];
+  __shared__ uint32_t s_cc[COUNT ? ROUNDS * 4 : 1];
   __shared__ uint32_t s_tile;
   __shared__ uint64_t s_pre;
 
@@ -231,7 +275,7 @@ __global__ __launch_bounds__(256) void k_level(LevelArgs a) {
   for (int s = 0; s < LV_STEPS; s++) {
     const uint64_t e0 = tile_start + (uint64_t)(s * 256 + tid) * V;
     if (VEC && full) {
-      const uint4 q = *(const uint4*)(src + e0);
+      const v4u32 q = __builtin_nontemporal_load((const v4u32*)(src + e0));
       const T* qp = (const T*)&q;
 #pragma unroll
       for (int v = 0; v < V; v++) x[s][v] = qp[v];
@@ -241,7 +285,7 @@ __global__ __launch_bounds__(256) void k_level(LevelArgs a) {
     }
   }
 
-  uint32_t mk[LV_STEPS], vmk[LV_STEPS], pre[LV_STEPS];
+  uint32_t mk[LV_STEPS], pre[LV_STEPS];
 #pragma unroll
   for (int s = 0; s < LV_STEPS; s++) {
     const uint64_t e0 = tile_start + (uint64_t)(s * 256 + tid) * V;
@@ -252,8 +296,9 @@ __global__ __launch_bounds__(256) void k_level(LevelArgs a) {
       vm |= (uint32_t)(e0 + v < n) << v;
     }
     m &= vm;
-    mk[s] = m;
-    vmk[s] = vm;
+    // invalid elements are marked as ones past the valid range: they never
+    // count as zeros and are never stored
+    mk[s] = m | ~vm;
     // bit words of this (step, wave): lane j < V assembles word j from V ballots
     uint64_t word = 0;
     const int j = lane & (V - 1);
@@ -298,48 +343,103 @@ __global__ __launch_bounds__(256) void k_level(LevelArgs a) {
       else if (val) atomicOr((unsigned long long*)&a.words[q], (unsigned long long)val);
     }
   }
-  if (!SCATTER) return;
-
-  // ---- tile zero total and look-back for the global zero prefix
-  uint32_t woff[LV_STEPS];
-  uint32_t ttot = 0;
-  {
-    uint32_t run = 0;
+  if constexpr (!SCATTER) {
+    return;
+  } else {
+    // ---- tile zero total and each (step, wave) exclusive offset
+    uint32_t woff[LV_STEPS];
+    uint32_t zt = 0;
 #pragma unroll
     for (int k = 0; k < LV_STEPS * 4; k++) {
       const uint32_t t = s_wz[k];
 #pragma unroll
       for (int s = 0; s < LV_STEPS; s++)
-        if (k == s * 4 + w) woff[s] = run;
-      run += t;
+        if (k == s * 4 + w) woff[s] = zt;
+      zt += t;
     }
-    ttot = run;
-  }
-  if (w == 0) {
-    uint64_t p = lb_exclusive(a.states, tile, ttot, a.err);
-    if (lane == 0) s_pre = p;
-  }
-  __syncthreads();
-  const uint64_t zprev = s_pre;
-
-  // ---- stable scatter into the next level's order
-  T* dst = (T*)a.dst;
+    // ---- stable 0/1 partition of the tile into LDS
 #pragma unroll
-  for (int s = 0; s < LV_STEPS; s++) {
-    const uint64_t e0 = tile_start + (uint64_t)(s * 256 + tid) * V;
-    const uint32_t m = mk[s], vm = vmk[s];
-    const uint32_t zbase = woff[s] + pre[s];
+    for (int s = 0; s < LV_STEPS; s++) {
+      const uint32_t m = mk[s];
+      const uint32_t zb = woff[s] + pre[s];
+      const uint32_t j0 = (uint32_t)(s * 256 + tid) * V;
 #pragma unroll
-    for (int v = 0; v < V; v++) {
-      if (!((vm >> v) & 1u)) continue;
-      const uint64_t i = e0 + v;
-      const uint64_t Z = zprev + zbase + (uint32_t)__popc(~m & vm & ((1u << v) - 1u));
-      uint64_t pnode = (uint64_t)x[s][v] >> a.node_shift;
-      if (pnode >= a.nnodes) { atomicOr(a.err, FLAG_BADSYM); continue; }
-      const uint2 ab = a.tab[pnode];
-      const uint64_t dest = ((m >> v) & 1u) ? (i - Z + ab.y) : (Z + ab.x);
-      if (dest < n) dst[dest] = x[s][v];
-      else atomicOr(a.err, FLAG_BADSYM);
+      for (int v = 0; v < V; v++) {
+        const uint32_t zl = zb + (uint32_t)__popc(~m & ((1u << v) - 1u));
+        const uint32_t j = j0 + v;
+        if (j < valid) {
+          const uint32_t slot = ((m >> v) & 1u) ? zt + (j - zl) : zl;
+          s_stage[slot] = x[s][v];
+        }
+      }
+    }
+    if (w == 0) {
+      const uint64_t p = lb_exclusive(a.states, tile, zt, a.err);
+      if (lane == 0) s_pre = p;
+    }
+    __syncthreads();
+    const uint64_t zprev = s_pre;
+    const uint64_t oprev = tile_start - zprev;
+    T* dst = (T*)a.dst;
+#pragma unroll 4
+    for (int r = 0; r < ROUNDS; r++) {
+      const uint32_t q = (uint32_t)(r * 256 + tid);
+      if (q < valid) {
+        const T xv = s_stage[q];
+        const uint64_t pnode = (uint64_t)xv >> a.node_shift;
+        if (pnode < a.nnodes) {
+          const uint2 ab = a.tab[pnode];
+          const uint64_t dest = (q < zt) ? zprev + q + ab.x : oprev + (q - zt) + ab.y;
+          if (dest < n) __builtin_nontemporal_store(xv, dst + dest);
+          else atomicOr(a.err, FLAG_BADSYM);
+        } else {
+          atomicOr(a.err, FLAG_BADSYM);
+        }
+      }
+    }
+    if constexpr (COUNT) {
+      // per run of equal (l+1)-prefix g in the staged tile: += #elements whose
+      // next bit is 1, as (+inclusive count at the run's tail, -exclusive count
+      // at its head); c = bit (bitpos-1), g = x >> bitpos
+      const uint32_t cb = a.bitpos - 1;
+#pragma unroll 4
+      for (int r = 0; r < ROUNDS; r++) {
+        const uint32_t q = (uint32_t)(r * 256 + tid);
+        const bool c = (q < valid) && ((((uint64_t)s_stage[q]) >> cb) & 1u);
+        const uint64_t B = wave_ballot(c);
+        if (lane == 0) s_cc[r * 4 + w] = popc64(B);
+      }
+      __syncthreads();
+      if (w == 0) {
+        // exclusive scan of the ROUNDS*4 wave counts, in slot order
+        uint32_t run = 0;
+        for (int base = 0; base < ROUNDS * 4; base += 64) {
+          const int k = base + lane;
+          const uint32_t v = (k < ROUNDS * 4) ? s_cc[k] : 0u;
+          const uint32_t inc = wave_incl_scan(v);
+          if (k < ROUNDS * 4) s_cc[k] = run + inc - v;
+          run += __shfl(inc, 63);
+        }
+      }
+      __syncthreads();
+#pragma unroll 2
+      for (int r = 0; r < ROUNDS; r++) {
+        const uint32_t q = (uint32_t)(r * 256 + tid);
+        const bool in = q < valid;
+        const uint64_t xv = in ? (uint64_t)s_stage[q] : 0;
+        const bool c = in && ((xv >> cb) & 1u);
+        const uint64_t B = wave_ballot(c);
+        if (in) {
+          const uint32_t ce = s_cc[r * 4 + w] + mbcnt64(B);
+          const uint64_t g = xv >> a.bitpos;
+          const bool head = (q == 0) || ((((uint64_t)s_stage[q - 1]) >> a.bitpos) != g);
+          const bool tail = (q + 1 == valid) || ((((uint64_t)s_stage[q + 1]) >> a.bitpos) != g);
+          if (g < 2ull * a.nnodes) {
+            if (head) atomicAdd(&a.c1[g], 0u - ce);
+            if (tail) atomicAdd(&a.c1[g], ce + (c ? 1u : 0u));
+          }
+        }
+      }
     }
   }
 }
@@ -347,15 +447,17 @@ __global__ __launch_bounds__(256) void k_level(LevelArgs a) {
 // ================================================================== host
 namespace {
 struct LevelPlan {
-  uint32_t L;
+  uint32_t L, h;
   uint64_t n;
   int w;
   uint64_t tiles;       // tiles per level
   uint32_t tile_elems;
-  size_t off_hist, off_C, off_tab, off_scanst, off_ntst, off_states, off_ctr, off_buf0, off_buf1, total;
-  uint64_t scan_tiles;
-  uint32_t nt_tile_base[25];
-  uint32_t nt_tiles;
+  uint32_t pre_rows;
+  uint64_t pre_chunk;
+  size_t off_cnt, off_c1, off_start, off_ntst, off_states, off_ctr, off_tab, off_partial,
+      off_buf0, off_buf1, total;
+  uint64_t nt_tiles[26];   // node-table tiles of level k
+  uint64_t nt_base[26];    // state offset of level k (two arrays: s and z)
 };
 
 int tile_elems_for(int w) { return 256 * LV_STEPS * (16 / w); }
@@ -363,59 +465,48 @@ int tile_elems_for(int w) { return 256 * LV_STEPS * (16 / w); }
 LevelPlan plan_levels(uint64_t n, uint32_t L, int w) {
   LevelPlan P{};
   P.L = L; P.n = n; P.w = w;
+  P.h = L < PRE_BITS ? L : PRE_BITS;
   P.tile_elems = tile_elems_for(w);
   P.tiles = wtg_ceil_div(n, P.tile_elems);
-  const uint64_t nb = 1ull << L;
-  P.scan_tiles = wtg_ceil_div(nb, 4096);
-  // node-table tiles: level l < L-1 has ceil(2^l/4096) tiles
-  uint32_t t = 0;
-  for (uint32_t l = 0; l + 1 < L; l++) {
-    P.nt_tile_base[l] = t;
-    t += (uint32_t)wtg_ceil_div(1ull << l, 4096);
+  P.pre_rows = (uint32_t)std::max<uint64_t>(1, std::min<uint64_t>(PRE_BLOCKS, wtg_ceil_div(n, 16384)));
+  P.pre_chunk = wtg_ceil_div(n, P.pre_rows);
+  uint64_t t = 0;
+  for (uint32_t k = 0; k < L; k++) {
+    P.nt_tiles[k] = wtg_ceil_div(1ull << k, 4096);
+    P.nt_base[k] = t;
+    t += 2 * (P.nt_tiles[k] + 1);
   }
-  for (uint32_t l = (L > 1 ? L - 1 : 0); l < 25; l++) P.nt_tile_base[l] = t;
-  P.nt_tiles = t;
   size_t o = 0;
-  // everything from off_hist up to off_buf0 is cleared at the start of a build
-  P.off_hist = o; o = wtg_align256(o + nb * 4);
-  P.off_C = o; o = wtg_align256(o + (nb + 1) * 4);
-  P.off_scanst = o; o = wtg_align256(o + (P.scan_tiles + 1) * 8);
-  P.off_ntst = o; o = wtg_align256(o + ((size_t)t + 1) * 8);
+  // [off_cnt, off_tab) is cleared at the start of a build
+  P.off_cnt = o; o = wtg_align256(o + ((2ull << L) + 2) * 4);
+  P.off_c1 = o; o = wtg_align256(o + ((2ull << L) + 2) * 4);
+  P.off_ntst = o; o = wtg_align256(o + (t + 1) * 8);
   P.off_states = o; o = wtg_align256(o + (size_t)L * (P.tiles + 1) * 8);
-  P.off_ctr = o; o = wtg_align256(o + 64 * 4);
-  P.off_tab = o; o = wtg_align256(o + (nb / 2 + 1) * 8);
+  P.off_ctr = o; o = wtg_align256(o + 128 * 4);
+  P.off_start = o; o = wtg_align256(o + ((1ull << L) + 1) * 4);
+  P.off_tab = o; o = wtg_align256(o + ((1ull << (L - 1)) + 1) * 8);
+  P.off_partial = o; o = wtg_align256(o + (size_t)P.pre_rows * (1ull << P.h) * 4);
   P.off_buf0 = o; o = wtg_align256(o + n * w);
   P.off_buf1 = o; o = wtg_align256(o + n * w);
   P.total = o;
   return P;
 }
 
-template <typename T>
-int launch_hist(wtg_ctx* ctx, const T* s, uint64_t n, uint32_t* hist, uint32_t nb) {
-  if (n == 0) return WTG_OK;
-  uint64_t blocks = wtg_ceil_div(n, 256 * 16);
-  if (blocks > 4096) blocks = 4096;
-  if (nb <= 16384) {
-    WTG_LAUNCH(ctx, "hist", k_hist_lds<T>, dim3((unsigned)blocks), dim3(256), nb * 4, s, n, hist, nb,
-               ctx->d_flag);
-  } else {
-    WTG_LAUNCH(ctx, "hist", k_hist_global<T>, dim3((unsigned)blocks), dim3(256), 0, s, n, hist, nb,
-               ctx->d_flag);
-  }
-  WTG_LAUNCH_CHECK();
-  return WTG_OK;
+template <typename T, bool SCATTER, bool VEC>
+void pick_level(bool count, void (*&k)(LevelArgs)) {
+  if (count) k = k_level<T, SCATTER, VEC, SCATTER>;
+  else k = k_level<T, SCATTER, VEC, false>;
 }
 
 template <typename T>
-int launch_level(wtg_ctx* ctx, const LevelArgs& la, uint64_t tiles, bool scatter, bool vec) {
-  dim3 g((unsigned)tiles), b(256);
+int launch_level(wtg_ctx* ctx, const LevelArgs& la, uint64_t tiles, bool scatter, bool vec, bool count) {
+  void (*k)(LevelArgs) = nullptr;
   if (scatter) {
-    if (vec) WTG_LAUNCH(ctx, "level_split", (k_level<T, true, true>), g, b, 0, la);
-    else WTG_LAUNCH(ctx, "level_split", (k_level<T, true, false>), g, b, 0, la);
+    if (vec) pick_level<T, true, true>(count, k); else pick_level<T, true, false>(count, k);
   } else {
-    if (vec) WTG_LAUNCH(ctx, "level_bits", (k_level<T, false, true>), g, b, 0, la);
-    else WTG_LAUNCH(ctx, "level_bits", (k_level<T, false, false>), g, b, 0, la);
+    if (vec) pick_level<T, false, true>(false, k); else pick_level<T, false, false>(false, k);
   }
+  WTG_LAUNCH(ctx, scatter ? "level_split" : "level_bits", k, dim3((unsigned)tiles), dim3(256), 0, la);
   WTG_LAUNCH_CHECK();
   return WTG_OK;
 }
@@ -427,64 +518,89 @@ int levels_build_t(wtg_ctx* ctx, const T* d_sym, uint64_t n, uint32_t L, uint64_
   LevelPlan P = plan_levels(n, L, w);
   WTG_TRY(wtg_ws_reserve(ctx, P.total));
   char* ws = (char*)ctx->ws;
-  uint32_t* hist = (uint32_t*)(ws + P.off_hist);
-  uint32_t* C = (uint32_t*)(ws + P.off_C);
+  uint32_t* cnt = (uint32_t*)(ws + P.off_cnt);
+  uint32_t* c1 = (uint32_t*)(ws + P.off_c1);
+  uint32_t* start = (uint32_t*)(ws + P.off_start);
   uint2* tab = (uint2*)(ws + P.off_tab);
-  uint64_t* scanst = (uint64_t*)(ws + P.off_scanst);
   uint64_t* ntst = (uint64_t*)(ws + P.off_ntst);
   uint64_t* states = (uint64_t*)(ws + P.off_states);
   uint32_t* ctr = (uint32_t*)(ws + P.off_ctr);
+  uint32_t* partial = (uint32_t*)(ws + P.off_partial);
   void* buf[2] = {ws + P.off_buf0, ws + P.off_buf1};
-  const uint64_t nb = 1ull << L;
   const uint64_t nwords = wtg_levels_words(n, L);
+  const bool want_nodes = d_nodes && node_slots;
 
-  // clear histogram, scan/look-back states and counters (one contiguous range)
-  WTG_TRY(wtg_fill_bytes_zero(ctx, ws + P.off_hist, P.off_tab - P.off_hist));
+  // clear counts, look-back states and counters (one contiguous range)
+  WTG_TRY(wtg_fill_bytes_zero(ctx, ws + P.off_cnt, P.off_start - P.off_cnt));
   WTG_TRY(wtg_fill_bytes_zero(ctx, d_words, nwords * 8));
+  if (d_distinct) WTG_TRY(wtg_fill_u64(ctx, d_distinct, 1, 0));
 
-  WTG_TRY(launch_hist<T>(ctx, d_sym, n, hist, (uint32_t)nb));
-  if (d_distinct) {
-    WTG_TRY(wtg_fill_u64(ctx, d_distinct, 1, 0));
-    WTG_LAUNCH(ctx, "distinct", k_count_nonzero, dim3((unsigned)std::min<uint64_t>(wtg_ceil_div(nb, 256), 1024)),
-               dim3(256), 0, hist, nb, d_distinct);
+  // 1. pre-pass: CNT_h, then CNT_0..CNT_{h-1}
+  const uint32_t nb = 1u << P.h;
+  if (n > 0) {
+    WTG_LAUNCH(ctx, "prehist", k_prehist<T>, dim3(P.pre_rows), dim3(256), nb * 4, d_sym, n, L - P.h,
+               nb, P.pre_chunk, partial, ctx->d_flag);
+    WTG_LAUNCH_CHECK();
+    WTG_LAUNCH(ctx, "prehist_reduce", k_prehist_reduce, dim3((unsigned)wtg_ceil_div(nb, 256)), dim3(256), 0,
+               partial, P.pre_rows, nb, cnt + heap_off(P.h));
     WTG_LAUNCH_CHECK();
   }
-  WTG_LAUNCH(ctx, "scan_hist", k_scan_u32<0>, dim3((unsigned)P.scan_tiles), dim3(256), 0, hist, C, nb,
-             scanst, ctr + 0, ctx->d_flag);
+  WTG_LAUNCH(ctx, "cnt_up", k_cnt_up, dim3(1), dim3(256), 0, cnt, P.h);
   WTG_LAUNCH_CHECK();
-  if (L > 1) {
-    NodeTabArgs na{};
-    na.C = C; na.tab = tab; na.states = ntst; na.counter = ctr + 1; na.err = ctx->d_flag;
-    na.L = L;
-    for (int l = 0; l < 25; l++) na.tile_base[l] = P.nt_tile_base[l];
-    WTG_LAUNCH(ctx, "nodetab", k_nodetab, dim3(P.nt_tiles), dim3(256), 0, na);
+  if (d_distinct && L <= P.h) {
+    WTG_LAUNCH(ctx, "distinct", k_count_nonzero, dim3((unsigned)std::min<uint64_t>(wtg_ceil_div(nb, 256), 1024)),
+               dim3(256), 0, cnt + heap_off(L), (uint64_t)nb, d_distinct);
     WTG_LAUNCH_CHECK();
   }
-  if (d_nodes && node_slots) {
-    WTG_LAUNCH(ctx, "nodes", k_nodes, dim3((unsigned)wtg_ceil_div(node_slots, 256)), dim3(256), 0,
-               C, L, n, d_nodes, node_slots);
-    WTG_LAUNCH_CHECK();
-  }
-  if (n == 0) return WTG_OK;
+
   const bool vec = ((uintptr_t)d_sym & 15) == 0;
   const void* src = d_sym;
-  for (uint32_t l = 0; l < L; l++) {
+  for (uint32_t k = 0; k < L; k++) {
+    const bool scatter = k + 1 < L;
+    // 2a. node table of level k (node starts always: cheap, needed by WTnodes)
+    {
+      NodeTabArgs na{};
+      na.cnt = cnt; na.c1 = c1; na.start = start;
+      na.tab = scatter ? tab + heap_off(k) : nullptr;
+      na.st_s = ntst + P.nt_base[k];
+      na.st_z = ntst + P.nt_base[k] + P.nt_tiles[k] + 1;
+      na.counter = ctr + 64 + k;
+      na.err = ctx->d_flag;
+      na.k = k;
+      na.derive = (k + 1 > P.h) ? 1u : 0u;
+      na.distinct = (d_distinct && k + 1 == L && L > P.h) ? d_distinct : nullptr;
+      if (scatter || want_nodes || na.distinct) {
+        WTG_LAUNCH(ctx, "nodetab", k_nodetab, dim3((unsigned)P.nt_tiles[k]), dim3(256), 0, na);
+        WTG_LAUNCH_CHECK();
+      }
+    }
+    if (n == 0) continue;
+    // 2b. the level pass
     LevelArgs la{};
     la.src = src;
-    la.dst = buf[l & 1];
+    la.dst = buf[k & 1];
     la.words = d_words;
-    la.bit_base = (uint64_t)l * n;
+    la.bit_base = (uint64_t)k * n;
     la.n = (uint32_t)n;
-    la.bitpos = L - 1 - l;
-    la.node_shift = L - l;
-    la.nnodes = 1u << l;
-    la.tab = tab + ((1u << l) - 1);
-    la.states = states + (size_t)l * (P.tiles + 1);
-    la.counter = ctr + 2 + l;
+    la.bitpos = L - 1 - k;
+    la.node_shift = L - k;
+    la.nnodes = 1u << k;
+    la.tab = tab + heap_off(k);
+    la.c1 = c1 + heap_off(k + 1);
+    la.states = states + (size_t)k * (P.tiles + 1);
+    la.counter = ctr + k;
     la.err = ctx->d_flag;
-    const bool scatter = (l + 1 < L);
-    WTG_TRY(launch_level<T>(ctx, la, P.tiles, scatter, l == 0 ? vec : true));
-    src = buf[l & 1];
+    // counts of the (k+2)-prefixes are needed by level k+1's table when the
+    // pre-pass does not cover them, and for the distinct count at the last level
+    const bool count = scatter && (k + 2 > P.h) &&
+                       ((k + 2 <= L - 1) || (d_distinct && k + 2 == L));
+    WTG_TRY(launch_level<T>(ctx, la, P.tiles, scatter, k == 0 ? vec : true, count));
+    src = buf[k & 1];
+  }
+  if (want_nodes) {
+    WTG_LAUNCH(ctx, "nodes", k_nodes, dim3((unsigned)wtg_ceil_div(node_slots, 256)), dim3(256), 0,
+               cnt, start, L, n, d_nodes, node_slots);
+    WTG_LAUNCH_CHECK();
   }
   return WTG_OK;
 }
@@ -531,6 +647,7 @@ extern "C" int wtg_levels_build(wtg_ctx* ctx, const void* d_sym, int sym_bytes, 
                                  nullptr));
   return wtg_finish(ctx);
 }
+
 
 // ------------------------------------------------------------------ remap
 template <typename T>
