@@ -147,18 +147,20 @@ __device__ __forceinline__ uint64_t lb_load(uint64_t* p) {
   return __hip_atomic_load(p, __ATOMIC_RELAXED, __HIP_MEMORY_SCOPE_AGENT);
 }
 
-// Called by ONE full wave (all 64 lanes).  Publishes `agg` for `tile`, walks
-// back over predecessors and returns the exclusive prefix (same in all lanes).
-// Every wait is bounded: on expiry the device error flag is raised and the
-// partial sum is used, so the kernel always drains.
-__device__ __forceinline__ uint64_t lb_exclusive(uint64_t* states, uint64_t tile, uint64_t agg,
-                                                 uint32_t* err) {
+// Publishes the aggregate of `tile` (one lane).  Tile 0 publishes its
+// inclusive prefix directly.
+__device__ __forceinline__ void lb_publish(uint64_t* states, uint64_t tile, uint64_t agg) {
+  lb_store(&states[tile], tile == 0 ? LB_PRE : LB_AGG, agg);
+}
+
+// Called by ONE full wave (all 64 lanes) after lb_publish: walks back over the
+// predecessors, publishes the inclusive prefix and returns the exclusive
+// prefix (same in all lanes).  Every wait is bounded: on expiry the device
+// error flag is raised and the partial sum is used, so the kernel always drains.
+__device__ __forceinline__ uint64_t lb_wait(uint64_t* states, uint64_t tile, uint64_t agg,
+                                            uint32_t* err) {
   const int l = lane_id();
-  if (tile == 0) {
-    if (l == 0) lb_store(&states[0], LB_PRE, agg);
-    return 0;
-  }
-  if (l == 0) lb_store(&states[tile], LB_AGG, agg);
+  if (tile == 0) return 0;
   uint64_t excl = 0;
   int64_t base = (int64_t)tile - 1;
   uint32_t spins = 0;
@@ -185,6 +187,13 @@ __device__ __forceinline__ uint64_t lb_exclusive(uint64_t* states, uint64_t tile
   }
   if (l == 0) lb_store(&states[tile], LB_PRE, excl + agg);
   return excl;
+}
+
+// publish + wait in one call (one full wave)
+__device__ __forceinline__ uint64_t lb_exclusive(uint64_t* states, uint64_t tile, uint64_t agg,
+                                                 uint32_t* err) {
+  if (lane_id() == 0) lb_publish(states, tile, agg);
+  return lb_wait(states, tile, agg, err);
 }
 
 // ------------------------------------------------------ block scan helper

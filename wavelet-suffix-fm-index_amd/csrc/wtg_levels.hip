@@ -5,76 +5,51 @@
 // global position l*n+i is (P_l[i] >> (L-1-l)) & 1.
 //
 // GPU pipeline for one build (all integer, HBM bound).  CNT_j[q] = number of
-// symbols whose top j bits equal q (kept for every j in one heap-ordered array).
-//   1. k_prehist      LDS-privatised histogram of the top h = min(L,13) bits
-//                     (one partial row per block) + k_prehist_reduce + k_cnt_up
-//                     -> CNT_0..CNT_h
-//   2. per level k:
-//      k_nodetab      node starts S(p) and, for scatter levels, the per-node
+// symbols whose top j bits equal q, kept for every j in one heap-ordered array.
+//   per level k:
+//      k_level<bits>  16-byte vector loads; the level's bit words from wave
+//                     ballots; the tile's zero count (level 0: also CNT_1).
+//      k_scan_u32     exclusive prefix of the per-tile zero counts.
+//      k_nodetab      node starts S(p) and, for split levels, the per-node
 //                     constants A(p) = S(p)-Zs(p), B(p) = nz(p)+Zs(p) (nz =
 //                     zeros of node p = CNT_{k+1}[2p], Zs = zeros of the earlier
-//                     nodes of the level); for k+1 > h it also derives
-//                     CNT_{k+1} from CNT_k and the counts produced by level k-1.
-//      k_level        16-byte vector loads; the level's bit words from wave
-//                     ballots; tile zero ranks from mbcnt + LDS; tile stable
-//                     0/1 partition staged in LDS; a decoupled look-back for the
-//                     global zero prefix; lane-consecutive stores of
-//                     dest = slot < Z_t ? Z_prev + slot + A(p)
-//                                       : O_prev + slot - Z_t + B(p)
-//                     into the next ping-pong buffer (last level: bits only).
-//                     Levels whose next counts are not covered by the pre-pass
-//                     also count, per (k+1)-prefix run of the staged tile, the
-//                     elements whose next bit is 1 (two atomics per run).
-//   3. k_nodes        WTnode records (optional).
+//                     nodes of the level); for k >= 1 it first derives CNT_{k+1}
+//                     from CNT_k and the next-bit counts made by level k-1.
+//      k_level<split> tile zero ranks from bit-plane ballots + mbcnt + LDS;
+//                     the tile's stable 0/1 partition staged in LDS and stored
+//                     lane-consecutively to
+//                       dest = slot < Z_t ? Z_prev + slot + A(p)
+//                                         : O_prev + slot - Z_t + B(p)
+//                     in the next ping-pong buffer (last level: bits only).
+//                     In the store loop it also counts, per (k+1)-prefix group,
+//                     the elements whose next bit is 1: each wave splits its 64
+//                     consecutive slots into runs of one group (head-flag
+//                     ballot), adds each run's popcount to an LDS counter and the
+//                     tile flushes one atomic per group into counters replicated
+//                     across tiles for shallow levels -> level k+1's table.
+//   k_nodes           WTnode records (optional).
+// The per-tile zero prefix comes from the bits pass, so the split pass needs no
+// cross-tile wait: a single-pass decoupled look-back measured ~4x slower here
+// because its device-coherent state loads sit on the critical path.
 #include "wtg_internal.hpp"
 
 #define FLAG_LOOKBACK 1u
 #define FLAG_BADSYM 2u
-#define PRE_BITS 13       // top bits histogrammed by the pre-pass
+#define TAB_CACHE 256     // node-table entries cached in LDS per tile
 typedef uint32_t v4u32 __attribute__((ext_vector_type(4)));
-#define PRE_BLOCKS 512    // partial histogram rows
 
 static __host__ __device__ __forceinline__ uint64_t heap_off(uint32_t j) { return (1ull << j) - 1; }
+// counter replicas of the next-bit counts of the 2^j groups of level j
+static __host__ __device__ __forceinline__ uint32_t c1_reps(uint32_t j) {
+  return j >= 12 ? 1u : ((4096u >> j) > 32u ? 32u : (4096u >> j));
+}
 
-// ------------------------------------------------------ pre-pass histogram
 template <typename T>
-__global__ __launch_bounds__(256) void k_prehist(const T* __restrict__ s, uint64_t n, uint32_t shift,
-                                                 uint32_t nbins, uint64_t chunk,
-                                                 uint32_t* __restrict__ partial, uint32_t* err) {
-  extern __shared__ uint32_t sh[];
-  for (uint32_t i = threadIdx.x; i < nbins; i += 256) sh[i] = 0;
-  __syncthreads();
-  const uint64_t a = (uint64_t)blockIdx.x * chunk;
-  const uint64_t b = (a + chunk < n) ? a + chunk : n;
-  bool bad = false;
-  for (uint64_t i = a + threadIdx.x; i < b; i += 256) {
-    const uint64_t q = (uint64_t)s[i] >> shift;
-    if (q < nbins) atomicAdd(&sh[q], 1u); else bad = true;
-  }
-  if (bad) atomicOr(err, FLAG_BADSYM);
-  __syncthreads();
-  uint32_t* row = partial + (uint64_t)blockIdx.x * nbins;
-  for (uint32_t i = threadIdx.x; i < nbins; i += 256) row[i] = sh[i];
-}
-
-__global__ void k_prehist_reduce(const uint32_t* __restrict__ partial, uint32_t rows, uint32_t nbins,
-                                 uint32_t* __restrict__ out) {
-  const uint32_t q = blockIdx.x * blockDim.x + threadIdx.x;
-  if (q >= nbins) return;
-  uint32_t acc = 0;
-  for (uint32_t r = 0; r < rows; r++) acc += partial[(uint64_t)r * nbins + q];
-  out[q] = acc;
-}
-
-// CNT_j[q] = CNT_{j+1}[2q] + CNT_{j+1}[2q+1] for j = h-1 .. 0 (one block)
-__global__ __launch_bounds__(256) void k_cnt_up(uint32_t* __restrict__ cnt, uint32_t h) {
-  for (int j = (int)h - 1; j >= 0; j--) {
-    uint32_t* lo = cnt + heap_off(j);
-    const uint32_t* hi = cnt + heap_off(j + 1);
-    for (uint32_t q = threadIdx.x; q < (1u << j); q += 256) lo[q] = hi[2 * q] + hi[2 * q + 1];
-    __syncthreads();
-  }
-}
+struct VecOf;
+template <> struct VecOf<uint8_t> { static constexpr int V = 16; };
+template <> struct VecOf<uint16_t> { static constexpr int V = 8; };
+template <> struct VecOf<uint32_t> { static constexpr int V = 4; };
+template <> struct VecOf<uint64_t> { static constexpr int V = 2; };
 
 __global__ void k_count_nonzero(const uint32_t* __restrict__ h, uint64_t n,
                                 unsigned long long* out) {
@@ -89,7 +64,8 @@ __global__ void k_count_nonzero(const uint32_t* __restrict__ h, uint64_t n,
 // ------------------------------------------------------------ node tables
 struct NodeTabArgs {
   uint32_t* cnt;         // heap CNT (levels 0..L)
-  const uint32_t* c1;    // heap, level k: #elements of k-prefix p whose next bit is 1
+  const uint32_t* c1;    // level k's next-bit counts (c1_reps(k) replicas of 2^k)
+  uint32_t reps;
   uint32_t* start;       // heap START (levels 0..L-1)
   uint2* tab;            // level k table (A,B) or nullptr
   uint64_t* st_s;        // look-back states, scan of CNT_k
@@ -98,7 +74,7 @@ struct NodeTabArgs {
   uint32_t* err;
   unsigned long long* distinct;  // optional: += #nonzero CNT_{k+1}
   uint32_t k;
-  uint32_t derive;       // 1: CNT_{k+1} = [CNT_k - c1, c1] (not covered by the pre-pass)
+  uint32_t derive;       // 1: CNT_{k+1} = [CNT_k - c1, c1]
 };
 
 __global__ __launch_bounds__(256) void k_nodetab(NodeTabArgs a) {
@@ -111,7 +87,6 @@ __global__ __launch_bounds__(256) void k_nodetab(NodeTabArgs a) {
   const uint64_t nnodes = 1ull << a.k;
   const uint32_t* ck = a.cnt + heap_off(a.k);
   uint32_t* ck1 = a.cnt + heap_off(a.k + 1);
-  const uint32_t* c1 = a.c1 + heap_off(a.k);
   const uint64_t p0 = (uint64_t)tile * 4096 + (uint64_t)threadIdx.x * 16;
   uint32_t cv[16], nz[16];
   uint32_t sum_c = 0, sum_z = 0, nzc = 0;
@@ -122,7 +97,9 @@ __global__ __launch_bounds__(256) void k_nodetab(NodeTabArgs a) {
     if (p < nnodes) {
       c = ck[p];
       if (a.derive) {
-        const uint32_t one = c1[p];
+        uint32_t one = 0;
+#pragma unroll 8
+        for (uint32_t r = 0; r < a.reps; r++) one += a.c1[(uint64_t)r * nnodes + p];
         z = c - one;
         ck1[2 * p] = z;
         ck1[2 * p + 1] = one;
@@ -215,6 +192,26 @@ __device__ __forceinline__ uint64_t spread_bits(uint64_t x) {
   return x;
 }
 
+// wave-exclusive prefix and wave total of a per-lane count <= V (bit planes)
+template <int NPL>
+__device__ __forceinline__ uint32_t plane_scan(uint32_t c, uint32_t& tot) {
+  uint32_t p = 0, t = 0;
+#pragma unroll
+  for (int b = 0; b < NPL; b++) {
+    const uint64_t B = wave_ballot((c >> b) & 1u);
+    p += mbcnt64(B) << b;
+    t += popc64(B) << b;
+  }
+  tot = t;
+  return p;
+}
+
+template <typename T>
+__device__ __forceinline__ T shfl_t(T v, int src) {
+  if constexpr (sizeof(T) == 8) return (T)__shfl((unsigned long long)v, src);
+  else return (T)__shfl((unsigned int)v, src);
+}
+
 struct LevelArgs {
   const void* src;
   void* dst;
@@ -224,51 +221,46 @@ struct LevelArgs {
   uint32_t bitpos;       // L-1-l
   uint32_t node_shift;   // L-l (node prefix p = x >> node_shift)
   uint32_t nnodes;       // 2^l
+  uint32_t check_shift;  // symbols must satisfy x >> check_shift == 0 (64: no check)
   const uint2* tab;      // this level's (A,B) table
-  uint32_t* c1;          // counts of next-bit-1 per (l+1)-prefix (heap level l+1)
-  uint64_t* states;
-  uint32_t* counter;
+  uint32_t* c1;          // next-bit counts of the 2^(l+1) groups of level l+1 (replicated)
+  uint32_t reps;         // replicas of c1 (power of two)
+  uint32_t* zc;          // bits pass: zeros per tile (out)
+  const uint32_t* zpre;  // split pass: exclusive prefix of zc (in)
   uint32_t* err;
 };
 
-template <typename T>
-struct VecOf;
-template <> struct VecOf<uint8_t> { static constexpr int V = 16; };
-template <> struct VecOf<uint16_t> { static constexpr int V = 8; };
-template <> struct VecOf<uint32_t> { static constexpr int V = 4; };
-template <> struct VecOf<uint64_t> { static constexpr int V = 2; };
-
 #define LV_STEPS 4
 
-template <typename T, bool SCATTER, bool VEC, bool COUNT>
+// MODE 0 ("bits"): the level's bit words + the tile's zero count.
+// MODE 1 ("split"): the stable scatter of the tile into the next level, with
+//                   the tile's zero prefix taken from the scanned counts.
+template <typename T, int MODE, bool VEC, bool COUNT>
 __global__ __launch_bounds__(256) void k_level(LevelArgs a) {
+  constexpr bool SCATTER = MODE == 1;
   constexpr int V = VecOf<T>::V;
   constexpr int G = 64 / V;              // lanes per bit word
   constexpr int TILE = 256 * LV_STEPS * V;
   constexpr int NW = TILE / 64;          // words per tile
   constexpr int ROUNDS = TILE / 256;     // store rounds (one element per lane)
-  constexpr int NPL = (V == 16) ? 5 : (V == 8) ? 4 : (V == 4) ? 3 : 2;  // bit-planes of a count <= V
-  __shared__ uint64_t s_words[NW + 1];
-  __shared__ uint32_t s_wz[LV_STEPS * 4];
+  constexpr int NPL = (V == 16) ? 5 : (V == 8) ? 4 : (V == 4) ? 3 : 2;  // bit planes of a count <= V
+  constexpr int NCH = LV_STEPS * 4;      // (step, wave) chunks
+  __shared__ uint64_t s_words[SCATTER ? 1 : NW + 1];
+  __shared__ uint32_t s_wz[NCH];
+  __shared__ uint32_t s_cnt[COUNT ? 2 * TAB_CACHE : 1];
+  __shared__ T s_edge[2][SCATTER ? NCH : 1];
   __shared__ T s_stage[SCATTER ? TILE : 1];
-  __shared__ uint32_t s_cc[COUNT ? ROUNDS * 4 : 1];
-  __shared__ uint32_t s_tile;
-  __shared__ uint64_t s_pre;
+  __shared__ uint2 s_tab[SCATTER ? TAB_CACHE : 1];
 
   const int tid = threadIdx.x, w = tid >> 6, lane = lane_id();
-  uint32_t tile;
-  if (SCATTER) {
-    if (tid == 0) s_tile = atomicAdd(a.counter, 1u);
-    __syncthreads();
-    tile = s_tile;
-  } else {
-    tile = blockIdx.x;
-  }
+  const uint32_t tile = blockIdx.x;
   const uint64_t tile_start = (uint64_t)tile * TILE;
   const uint64_t n = a.n;
   const uint64_t valid = (n - tile_start < (uint64_t)TILE) ? n - tile_start : (uint64_t)TILE;
   const bool full = valid == (uint64_t)TILE;
   const T* src = (const T*)a.src;
+  uint64_t zprev = 0;
+  if (SCATTER) zprev = a.zpre[tile];
 
   T x[LV_STEPS][V];
 #pragma unroll
@@ -284,8 +276,12 @@ __global__ __launch_bounds__(256) void k_level(LevelArgs a) {
       for (int v = 0; v < V; v++) x[s][v] = (e0 + v < n) ? src[e0 + v] : (T)0;
     }
   }
+  if (COUNT) {
+    for (int i = tid; i < 2 * TAB_CACHE; i += 256) s_cnt[i] = 0;
+  }
 
   uint32_t mk[LV_STEPS], pre[LV_STEPS];
+  bool bad = false;
 #pragma unroll
   for (int s = 0; s < LV_STEPS; s++) {
     const uint64_t e0 = tile_start + (uint64_t)(s * 256 + tid) * V;
@@ -294,36 +290,47 @@ __global__ __launch_bounds__(256) void k_level(LevelArgs a) {
     for (int v = 0; v < V; v++) {
       m |= (uint32_t)(((uint64_t)x[s][v] >> a.bitpos) & 1u) << v;
       vm |= (uint32_t)(e0 + v < n) << v;
+      if (!SCATTER && a.check_shift < 64) bad |= (((uint64_t)x[s][v]) >> a.check_shift) != 0;
     }
     m &= vm;
-    // invalid elements are marked as ones past the valid range: they never
-    // count as zeros and are never stored
+    // invalid elements are marked as ones: they never count as zeros and sit
+    // after every valid element of the tile, so they are never staged
     mk[s] = m | ~vm;
-    // bit words of this (step, wave): lane j < V assembles word j from V ballots
-    uint64_t word = 0;
-    const int j = lane & (V - 1);
+    if (!SCATTER) {
+      // bit words of this (step, wave): lane j < V assembles word j from V ballots
+      uint64_t word = 0;
+      const int j = lane & (V - 1);
 #pragma unroll
-    for (int v = 0; v < V; v++) {
-      uint64_t B = wave_ballot((m >> v) & 1u);
-      word |= spread_bits<V>(B >> (j * G)) << v;
+      for (int v = 0; v < V; v++) {
+        const uint64_t B = wave_ballot((m >> v) & 1u);
+        word |= spread_bits<V>(B >> (j * G)) << v;
+      }
+      if (lane < V) s_words[(s * 4 + w) * V + lane] = word;
     }
-    if (lane < V) s_words[(s * 4 + w) * V + lane] = word;
-    // wave-exclusive prefix of per-lane zero counts via bit-plane ballots
-    const uint32_t zc = (uint32_t)__popc(~m & vm);
-    uint32_t p = 0, tot = 0;
-#pragma unroll
-    for (int b = 0; b < NPL; b++) {
-      uint64_t Bp = wave_ballot((zc >> b) & 1u);
-      p += mbcnt64(Bp) << b;
-      tot += popc64(Bp) << b;
-    }
-    pre[s] = p;
+    uint32_t tot;
+    pre[s] = plane_scan<NPL>((uint32_t)__popc(~m & vm), tot);
     if (lane == 0) s_wz[s * 4 + w] = tot;
+    if (SCATTER) {
+      if (lane == 0) s_edge[0][s * 4 + w] = x[s][0];
+      if (lane == 63) s_edge[1][s * 4 + w] = x[s][V - 1];
+    }
   }
+  if (bad) atomicOr(a.err, FLAG_BADSYM);
   __syncthreads();
 
-  // ---- bit words out (funnel shift when the level base is not 64-aligned)
-  {
+  uint32_t woff[LV_STEPS];
+  uint32_t zt = 0;
+#pragma unroll
+  for (int k = 0; k < NCH; k++) {
+#pragma unroll
+    for (int s = 0; s < LV_STEPS; s++)
+      if (k == s * 4 + w) woff[s] = zt;
+    zt += s_wz[k];
+  }
+
+  if constexpr (!SCATTER) {
+    if (tid == 0) a.zc[tile] = zt;
+    // bit words out (funnel shift when the level base is not 64-aligned)
     const uint64_t gb = a.bit_base + tile_start;
     const uint32_t sh = (uint32_t)(gb & 63);
     const uint64_t q0 = gb >> 6;
@@ -342,21 +349,12 @@ __global__ __launch_bounds__(256) void k_level(LevelArgs a) {
       if (fullword) a.words[q] = val;
       else if (val) atomicOr((unsigned long long*)&a.words[q], (unsigned long long)val);
     }
-  }
-  if constexpr (!SCATTER) {
-    return;
   } else {
-    // ---- tile zero total and each (step, wave) exclusive offset
-    uint32_t woff[LV_STEPS];
-    uint32_t zt = 0;
-#pragma unroll
-    for (int k = 0; k < LV_STEPS * 4; k++) {
-      const uint32_t t = s_wz[k];
-#pragma unroll
-      for (int s = 0; s < LV_STEPS; s++)
-        if (k == s * 4 + w) woff[s] = zt;
-      zt += t;
-    }
+    // ---- node-table entries of the tile's node range into LDS
+    const uint64_t pfirst = (uint64_t)s_edge[0][0] >> a.node_shift;
+    const uint64_t plast = (uint64_t)s_edge[1][NCH - 1] >> a.node_shift;
+    const bool cached = full && plast >= pfirst && plast - pfirst < TAB_CACHE && plast < a.nnodes;
+    if (cached && (uint64_t)tid <= plast - pfirst) s_tab[tid] = a.tab[pfirst + tid];
     // ---- stable 0/1 partition of the tile into LDS
 #pragma unroll
     for (int s = 0; s < LV_STEPS; s++) {
@@ -373,89 +371,81 @@ __global__ __launch_bounds__(256) void k_level(LevelArgs a) {
         }
       }
     }
-    if (w == 0) {
-      const uint64_t p = lb_exclusive(a.states, tile, zt, a.err);
-      if (lane == 0) s_pre = p;
-    }
     __syncthreads();
-    const uint64_t zprev = s_pre;
     const uint64_t oprev = tile_start - zprev;
     T* dst = (T*)a.dst;
+    uint32_t* c1 = COUNT ? a.c1 + (uint64_t)(tile & (a.reps - 1)) * (2ull * a.nnodes) : nullptr;
+    const uint64_t gfirst = 2 * pfirst;
 #pragma unroll 4
     for (int r = 0; r < ROUNDS; r++) {
       const uint32_t q = (uint32_t)(r * 256 + tid);
-      if (q < valid) {
-        const T xv = s_stage[q];
+      const bool in = q < valid;
+      const T xv = in ? s_stage[q] : (T)0;
+      if (in) {
         const uint64_t pnode = (uint64_t)xv >> a.node_shift;
-        if (pnode < a.nnodes) {
-          const uint2 ab = a.tab[pnode];
-          const uint64_t dest = (q < zt) ? zprev + q + ab.x : oprev + (q - zt) + ab.y;
-          if (dest < n) __builtin_nontemporal_store(xv, dst + dest);
-          else atomicOr(a.err, FLAG_BADSYM);
-        } else {
-          atomicOr(a.err, FLAG_BADSYM);
+        uint2 ab;
+        bool ok = true;
+        if (cached && pnode - pfirst < TAB_CACHE) ab = s_tab[pnode - pfirst];
+        else if (pnode < a.nnodes) ab = a.tab[pnode];
+        else ok = false;
+        const uint64_t dest = (q < zt) ? zprev + q + ab.x : oprev + (q - zt) + ab.y;
+        if (ok && dest < n) __builtin_nontemporal_store(xv, dst + dest);
+        else atomicOr(a.err, FLAG_BADSYM);
+      }
+      if constexpr (COUNT) {
+        // slots of one group g = x >> bitpos ((l+1)-prefix) are contiguous:
+        // split this wave's 64 slots into pieces of equal g and add each
+        // piece's number of next-bit-1 elements to the group's counter
+        const uint64_t g = (uint64_t)xv >> a.bitpos;
+        const bool c = in && ((((uint64_t)xv) >> (a.bitpos - 1)) & 1u);
+        const uint64_t gp = (uint64_t)shfl_t<T>(xv, (lane + 63) & 63) >> a.bitpos;
+        const bool head = in && (lane == 0 || gp != g);
+        const uint64_t H = wave_ballot(head);
+        const uint64_t C = wave_ballot(c);
+        if (head) {
+          const uint64_t above = (lane == 63) ? 0ull : (H >> (lane + 1)) << (lane + 1);
+          const uint64_t nxt = above ? (above & (0ull - above)) : 0ull;  // lowest head above
+          const uint64_t span = (nxt ? nxt - 1 : ~0ull) & (~0ull << lane);
+          const uint32_t cnt = popc64(C & span);
+          if (cnt && g < 2ull * a.nnodes) {
+            if (cached && g - gfirst < 2 * TAB_CACHE) atomicAdd(&s_cnt[g - gfirst], cnt);
+            else atomicAdd(&c1[g], cnt);
+          }
         }
       }
     }
     if constexpr (COUNT) {
-      // per run of equal (l+1)-prefix g in the staged tile: += #elements whose
-      // next bit is 1, as (+inclusive count at the run's tail, -exclusive count
-      // at its head); c = bit (bitpos-1), g = x >> bitpos
-      const uint32_t cb = a.bitpos - 1;
-#pragma unroll 4
-      for (int r = 0; r < ROUNDS; r++) {
-        const uint32_t q = (uint32_t)(r * 256 + tid);
-        const bool c = (q < valid) && ((((uint64_t)s_stage[q]) >> cb) & 1u);
-        const uint64_t B = wave_ballot(c);
-        if (lane == 0) s_cc[r * 4 + w] = popc64(B);
-      }
-      __syncthreads();
-      if (w == 0) {
-        // exclusive scan of the ROUNDS*4 wave counts, in slot order
-        uint32_t run = 0;
-        for (int base = 0; base < ROUNDS * 4; base += 64) {
-          const int k = base + lane;
-          const uint32_t v = (k < ROUNDS * 4) ? s_cc[k] : 0u;
-          const uint32_t inc = wave_incl_scan(v);
-          if (k < ROUNDS * 4) s_cc[k] = run + inc - v;
-          run += __shfl(inc, 63);
-        }
-      }
-      __syncthreads();
-#pragma unroll 2
-      for (int r = 0; r < ROUNDS; r++) {
-        const uint32_t q = (uint32_t)(r * 256 + tid);
-        const bool in = q < valid;
-        const uint64_t xv = in ? (uint64_t)s_stage[q] : 0;
-        const bool c = in && ((xv >> cb) & 1u);
-        const uint64_t B = wave_ballot(c);
-        if (in) {
-          const uint32_t ce = s_cc[r * 4 + w] + mbcnt64(B);
-          const uint64_t g = xv >> a.bitpos;
-          const bool head = (q == 0) || ((((uint64_t)s_stage[q - 1]) >> a.bitpos) != g);
-          const bool tail = (q + 1 == valid) || ((((uint64_t)s_stage[q + 1]) >> a.bitpos) != g);
-          if (g < 2ull * a.nnodes) {
-            if (head) atomicAdd(&a.c1[g], 0u - ce);
-            if (tail) atomicAdd(&a.c1[g], ce + (c ? 1u : 0u));
-          }
+      if (cached) {
+        __syncthreads();
+        for (int i = tid; i < 2 * TAB_CACHE; i += 256) {
+          const uint32_t v = s_cnt[i];
+          if (v) atomicAdd(&c1[gfirst + i], v);
         }
       }
     }
   }
 }
 
+__global__ void k_cnt_from_zeros(uint32_t* __restrict__ cnt, const uint32_t* zeros, uint64_t n) {
+  if (threadIdx.x == 0) {
+    cnt[0] = (uint32_t)n;
+    cnt[1] = *zeros;
+    cnt[2] = (uint32_t)(n - *zeros);
+  }
+}
+
 // ================================================================== host
 namespace {
 struct LevelPlan {
-  uint32_t L, h;
+  uint32_t L;
   uint64_t n;
   int w;
   uint64_t tiles;       // tiles per level
   uint32_t tile_elems;
-  uint32_t pre_rows;
-  uint64_t pre_chunk;
-  size_t off_cnt, off_c1, off_start, off_ntst, off_states, off_ctr, off_tab, off_partial,
+  uint64_t scan_tiles;  // tiles of the scan over the per-tile zero counts
+  size_t off_cnt, off_c1, off_ntst, off_scst, off_ctr, off_start, off_tab, off_zc, off_zpre,
       off_buf0, off_buf1, total;
+  uint64_t c1off[26];      // c1 region of level j (next-bit counts of its 2^j groups)
   uint64_t nt_tiles[26];   // node-table tiles of level k
   uint64_t nt_base[26];    // state offset of level k (two arrays: s and z)
 };
@@ -465,48 +455,51 @@ int tile_elems_for(int w) { return 256 * LV_STEPS * (16 / w); }
 LevelPlan plan_levels(uint64_t n, uint32_t L, int w) {
   LevelPlan P{};
   P.L = L; P.n = n; P.w = w;
-  P.h = L < PRE_BITS ? L : PRE_BITS;
   P.tile_elems = tile_elems_for(w);
   P.tiles = wtg_ceil_div(n, P.tile_elems);
-  P.pre_rows = (uint32_t)std::max<uint64_t>(1, std::min<uint64_t>(PRE_BLOCKS, wtg_ceil_div(n, 16384)));
-  P.pre_chunk = wtg_ceil_div(n, P.pre_rows);
-  uint64_t t = 0;
-  for (uint32_t k = 0; k < L; k++) {
-    P.nt_tiles[k] = wtg_ceil_div(1ull << k, 4096);
-    P.nt_base[k] = t;
-    t += 2 * (P.nt_tiles[k] + 1);
+  P.scan_tiles = wtg_ceil_div(P.tiles, 4096);
+  uint64_t t = 0, c = 0;
+  for (uint32_t k = 0; k <= L; k++) {
+    P.c1off[k] = c;
+    c += (uint64_t)c1_reps(k) << k;
+    if (k < L) {
+      P.nt_tiles[k] = wtg_ceil_div(1ull << k, 4096);
+      P.nt_base[k] = t;
+      t += 2 * (P.nt_tiles[k] + 1);
+    }
   }
   size_t o = 0;
-  // [off_cnt, off_tab) is cleared at the start of a build
+  // [off_cnt, off_start) is cleared at the start of a build
   P.off_cnt = o; o = wtg_align256(o + ((2ull << L) + 2) * 4);
-  P.off_c1 = o; o = wtg_align256(o + ((2ull << L) + 2) * 4);
+  P.off_c1 = o; o = wtg_align256(o + (c + 1) * 4);
   P.off_ntst = o; o = wtg_align256(o + (t + 1) * 8);
-  P.off_states = o; o = wtg_align256(o + (size_t)L * (P.tiles + 1) * 8);
+  P.off_scst = o; o = wtg_align256(o + (size_t)L * (P.scan_tiles + 1) * 8);
   P.off_ctr = o; o = wtg_align256(o + 128 * 4);
   P.off_start = o; o = wtg_align256(o + ((1ull << L) + 1) * 4);
   P.off_tab = o; o = wtg_align256(o + ((1ull << (L - 1)) + 1) * 8);
-  P.off_partial = o; o = wtg_align256(o + (size_t)P.pre_rows * (1ull << P.h) * 4);
+  P.off_zc = o; o = wtg_align256(o + (P.tiles + 1) * 4);
+  P.off_zpre = o; o = wtg_align256(o + (P.tiles + 1) * 4);
   P.off_buf0 = o; o = wtg_align256(o + n * w);
   P.off_buf1 = o; o = wtg_align256(o + n * w);
   P.total = o;
   return P;
 }
 
-template <typename T, bool SCATTER, bool VEC>
+template <typename T, int MODE, bool VEC>
 void pick_level(bool count, void (*&k)(LevelArgs)) {
-  if (count) k = k_level<T, SCATTER, VEC, SCATTER>;
-  else k = k_level<T, SCATTER, VEC, false>;
+  if (count && MODE == 1) k = k_level<T, MODE, VEC, MODE == 1>;
+  else k = k_level<T, MODE, VEC, false>;
 }
 
 template <typename T>
-int launch_level(wtg_ctx* ctx, const LevelArgs& la, uint64_t tiles, bool scatter, bool vec, bool count) {
+int launch_level(wtg_ctx* ctx, const LevelArgs& la, uint64_t tiles, int mode, bool vec, bool count) {
   void (*k)(LevelArgs) = nullptr;
-  if (scatter) {
-    if (vec) pick_level<T, true, true>(count, k); else pick_level<T, true, false>(count, k);
+  if (mode == 1) {
+    if (vec) pick_level<T, 1, true>(count, k); else pick_level<T, 1, false>(count, k);
   } else {
-    if (vec) pick_level<T, false, true>(false, k); else pick_level<T, false, false>(false, k);
+    if (vec) pick_level<T, 0, true>(false, k); else pick_level<T, 0, false>(false, k);
   }
-  WTG_LAUNCH(ctx, scatter ? "level_split" : "level_bits", k, dim3((unsigned)tiles), dim3(256), 0, la);
+  WTG_LAUNCH(ctx, mode == 1 ? "level_split" : "level_bits", k, dim3((unsigned)tiles), dim3(256), 0, la);
   WTG_LAUNCH_CHECK();
   return WTG_OK;
 }
@@ -523,59 +516,23 @@ int levels_build_t(wtg_ctx* ctx, const T* d_sym, uint64_t n, uint32_t L, uint64_
   uint32_t* start = (uint32_t*)(ws + P.off_start);
   uint2* tab = (uint2*)(ws + P.off_tab);
   uint64_t* ntst = (uint64_t*)(ws + P.off_ntst);
-  uint64_t* states = (uint64_t*)(ws + P.off_states);
+  uint64_t* scst = (uint64_t*)(ws + P.off_scst);
   uint32_t* ctr = (uint32_t*)(ws + P.off_ctr);
-  uint32_t* partial = (uint32_t*)(ws + P.off_partial);
+  uint32_t* zc = (uint32_t*)(ws + P.off_zc);
+  uint32_t* zpre = (uint32_t*)(ws + P.off_zpre);
   void* buf[2] = {ws + P.off_buf0, ws + P.off_buf1};
   const uint64_t nwords = wtg_levels_words(n, L);
   const bool want_nodes = d_nodes && node_slots;
 
-  // clear counts, look-back states and counters (one contiguous range)
+  // clear counts, scan states and counters (one contiguous range)
   WTG_TRY(wtg_fill_bytes_zero(ctx, ws + P.off_cnt, P.off_start - P.off_cnt));
   WTG_TRY(wtg_fill_bytes_zero(ctx, d_words, nwords * 8));
   if (d_distinct) WTG_TRY(wtg_fill_u64(ctx, d_distinct, 1, 0));
 
-  // 1. pre-pass: CNT_h, then CNT_0..CNT_{h-1}
-  const uint32_t nb = 1u << P.h;
-  if (n > 0) {
-    WTG_LAUNCH(ctx, "prehist", k_prehist<T>, dim3(P.pre_rows), dim3(256), nb * 4, d_sym, n, L - P.h,
-               nb, P.pre_chunk, partial, ctx->d_flag);
-    WTG_LAUNCH_CHECK();
-    WTG_LAUNCH(ctx, "prehist_reduce", k_prehist_reduce, dim3((unsigned)wtg_ceil_div(nb, 256)), dim3(256), 0,
-               partial, P.pre_rows, nb, cnt + heap_off(P.h));
-    WTG_LAUNCH_CHECK();
-  }
-  WTG_LAUNCH(ctx, "cnt_up", k_cnt_up, dim3(1), dim3(256), 0, cnt, P.h);
-  WTG_LAUNCH_CHECK();
-  if (d_distinct && L <= P.h) {
-    WTG_LAUNCH(ctx, "distinct", k_count_nonzero, dim3((unsigned)std::min<uint64_t>(wtg_ceil_div(nb, 256), 1024)),
-               dim3(256), 0, cnt + heap_off(L), (uint64_t)nb, d_distinct);
-    WTG_LAUNCH_CHECK();
-  }
-
   const bool vec = ((uintptr_t)d_sym & 15) == 0;
   const void* src = d_sym;
   for (uint32_t k = 0; k < L; k++) {
-    const bool scatter = k + 1 < L;
-    // 2a. node table of level k (node starts always: cheap, needed by WTnodes)
-    {
-      NodeTabArgs na{};
-      na.cnt = cnt; na.c1 = c1; na.start = start;
-      na.tab = scatter ? tab + heap_off(k) : nullptr;
-      na.st_s = ntst + P.nt_base[k];
-      na.st_z = ntst + P.nt_base[k] + P.nt_tiles[k] + 1;
-      na.counter = ctr + 64 + k;
-      na.err = ctx->d_flag;
-      na.k = k;
-      na.derive = (k + 1 > P.h) ? 1u : 0u;
-      na.distinct = (d_distinct && k + 1 == L && L > P.h) ? d_distinct : nullptr;
-      if (scatter || want_nodes || na.distinct) {
-        WTG_LAUNCH(ctx, "nodetab", k_nodetab, dim3((unsigned)P.nt_tiles[k]), dim3(256), 0, na);
-        WTG_LAUNCH_CHECK();
-      }
-    }
-    if (n == 0) continue;
-    // 2b. the level pass
+    const bool split = k + 1 < L;
     LevelArgs la{};
     la.src = src;
     la.dst = buf[k & 1];
@@ -585,16 +542,59 @@ int levels_build_t(wtg_ctx* ctx, const T* d_sym, uint64_t n, uint32_t L, uint64_
     la.bitpos = L - 1 - k;
     la.node_shift = L - k;
     la.nnodes = 1u << k;
+    la.check_shift = (k == 0 && L < 64) ? L : 64;
     la.tab = tab + heap_off(k);
-    la.c1 = c1 + heap_off(k + 1);
-    la.states = states + (size_t)k * (P.tiles + 1);
-    la.counter = ctr + k;
+    la.c1 = c1 + P.c1off[k + 1];
+    la.reps = c1_reps(k + 1);
+    la.zc = zc;
+    la.zpre = zpre;
     la.err = ctx->d_flag;
-    // counts of the (k+2)-prefixes are needed by level k+1's table when the
-    // pre-pass does not cover them, and for the distinct count at the last level
-    const bool count = scatter && (k + 2 > P.h) &&
-                       ((k + 2 <= L - 1) || (d_distinct && k + 2 == L));
-    WTG_TRY(launch_level<T>(ctx, la, P.tiles, scatter, k == 0 ? vec : true, count));
+    const bool kvec = k == 0 ? vec : true;
+    // 1. bits pass: this level's bit words + zeros per tile; exclusive scan
+    if (n > 0) {
+      WTG_TRY(launch_level<T>(ctx, la, P.tiles, 0, kvec, false));
+      if (split || k == 0) {
+        WTG_LAUNCH(ctx, "scan_tiles", k_scan_u32<0>, dim3((unsigned)P.scan_tiles), dim3(256), 0, zc, zpre,
+                   P.tiles, scst + (size_t)k * (P.scan_tiles + 1), ctr + k, ctx->d_flag);
+        WTG_LAUNCH_CHECK();
+      }
+      if (k == 0) {
+        // CNT_0 = n, CNT_1 = (zeros, ones) of the top bit
+        WTG_LAUNCH(ctx, "cnt_init", k_cnt_from_zeros, dim3(1), dim3(64), 0, cnt, zpre + P.tiles, n);
+        WTG_LAUNCH_CHECK();
+      }
+    }
+    if (k == 0 && d_distinct && L == 1) {
+      WTG_LAUNCH(ctx, "distinct", k_count_nonzero, dim3(1), dim3(256), 0, cnt + heap_off(1), (uint64_t)2,
+                 d_distinct);
+      WTG_LAUNCH_CHECK();
+    }
+    // 2. node table of level k (node starts always: cheap, needed by WTnodes)
+    {
+      NodeTabArgs na{};
+      na.cnt = cnt;
+      na.c1 = c1 + P.c1off[k];
+      na.reps = c1_reps(k);
+      na.start = start;
+      na.tab = split ? tab + heap_off(k) : nullptr;
+      na.st_s = ntst + P.nt_base[k];
+      na.st_z = ntst + P.nt_base[k] + P.nt_tiles[k] + 1;
+      na.counter = ctr + 64 + k;
+      na.err = ctx->d_flag;
+      na.k = k;
+      na.derive = k >= 1 ? 1u : 0u;
+      na.distinct = (d_distinct && k + 1 == L && L > 1) ? d_distinct : nullptr;
+      if (split || want_nodes || na.distinct) {
+        WTG_LAUNCH(ctx, "nodetab", k_nodetab, dim3((unsigned)P.nt_tiles[k]), dim3(256), 0, na);
+        WTG_LAUNCH_CHECK();
+      }
+    }
+    if (n == 0 || !split) continue;
+    // 3. split pass: stable scatter into the next level's order; the counts of
+    // the next bit feed level k+1's table and, at the last split, the
+    // distinct-symbol count
+    const bool count = (k + 2 <= L - 1) || (d_distinct && k + 2 == L);
+    WTG_TRY(launch_level<T>(ctx, la, P.tiles, 1, kvec, count));
     src = buf[k & 1];
   }
   if (want_nodes) {
@@ -647,7 +647,6 @@ extern "C" int wtg_levels_build(wtg_ctx* ctx, const void* d_sym, int sym_bytes, 
                                  nullptr));
   return wtg_finish(ctx);
 }
-
 
 // ------------------------------------------------------------------ remap
 template <typename T>
