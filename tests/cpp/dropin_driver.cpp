@@ -1,0 +1,95 @@
+// dropin_driver.cpp -- exercises the C++ drop-in headers the way reference
+// code does (WT() as called by wavelet/levelWT/WTTime.C; sdsl-shaped
+// rank_support_v / select_support_mcl / wt_int as in rank/, select/, wt_int/
+// WTTime.C) and writes every answer to binary files that the GPU test compares
+// with the oracle.  Inputs come from the library's seeded generators, which the
+// test calls with the same seeds.  Build variants: byte symbols, -DINT,
+// -DINT -DLONG.
+#include <cstdio>
+#include <cstring>
+#include <string>
+#include <vector>
+#include "WT.h"
+#include "rank_support_v_par.hpp"
+#include "select_support_mcl_par.hpp"
+#include "wt_int_par.hpp"
+
+static void dump(const std::string& path, const void* p, size_t bytes) {
+  FILE* f = std::fopen(path.c_str(), "wb");
+  if (!f || std::fwrite(p, 1, bytes, f) != bytes) { std::perror(path.c_str()); std::exit(2); }
+  std::fclose(f);
+}
+
+int main(int argc, char** argv) {
+  if (argc < 2) { std::fprintf(stderr, "usage: %s outdir\n", argv[0]); return 2; }
+  const std::string out = argv[1];
+  // ---- WT(s, n, sigma)
+  {
+    const uint64_t n = 100003, sigma = 1000;
+    std::vector<symbol> s(n);
+    wtg::check(wtg_gen_uniform(11, n, sizeof(symbol) == 1 ? 256 : sigma, (int)sizeof(symbol), s.data()));
+    const uint64_t sg = sizeof(symbol) == 1 ? 256 : sigma;
+    pair<WTnode*, long*> R = WT(s.data(), (uintT)n, (uintT)sg);
+    int L = 0;
+    for (uint64_t b = sg - 1; b > 0; b >>= 1) L++;
+    dump(out + "/wt_words.bin", R.second, (n * L + 63) / 64 * 8);
+    std::vector<uint64_t> nd(2 * sg * 5);
+    for (uint64_t v = 0; v < 2 * sg; v++) {
+      const WTnode& x = R.first[v];
+      uint64_t f[5] = {(uint64_t)x.bitmapPtr, (uint64_t)x.length, (uint64_t)x.parent,
+                       (uint64_t)x.leftChild, (uint64_t)x.rightChild};
+      for (int k = 2; k < 5; k++) if (f[k] == (uint64_t)UINT_T_MAX) f[k] = UINT64_MAX;
+      std::memcpy(&nd[v * 5], f, 40);
+    }
+    dump(out + "/wt_nodes.bin", nd.data(), nd.size() * 8);
+    free(R.first);
+    free(R.second);
+  }
+  // ---- rank_support_v<1,1>, select_support_mcl<1,1>, select_support_mcl<0,1>
+  {
+    const uint64_t nbits = 1000003, nw = (nbits + 63) / 64;
+    std::vector<uint64_t> w(nw);
+    wtg::check(wtg_gen_uniform(12, nw, UINT64_MAX, 8, w.data()));
+    w[nw - 1] &= (1ull << (nbits & 63)) - 1;
+    sdsl::bit_vector bv(nbits);
+    std::memcpy(bv.data(), w.data(), nw * 8);
+    sdsl::rank_support_v<1, 1> rs(&bv);
+    std::vector<uint64_t> r;
+    for (uint64_t i = 0; i <= nbits; i += 7) r.push_back(rs(i));
+    r.push_back(rs(nbits));
+    dump(out + "/rank.bin", r.data(), r.size() * 8);
+    sdsl::select_support_mcl<1, 1> s1(&bv);
+    sdsl::select_support_mcl<0, 1> s0(&bv);
+    std::vector<uint64_t> a, b;
+    for (uint64_t i = 1; i <= s1.arg_cnt(); i += 5) a.push_back(s1(i));
+    for (uint64_t i = 1; i <= s0.arg_cnt(); i += 5) b.push_back(s0.select(i));
+    dump(out + "/sel1.bin", a.data(), a.size() * 8);
+    dump(out + "/sel0.bin", b.data(), b.size() * 8);
+  }
+  // ---- wt_int<>
+  {
+    const uint64_t n = 50000;
+    std::vector<uint32_t> s(n);
+    wtg::check(wtg_gen_uniform(13, n, 5000, 4, s.data()));
+    sdsl::int_vector<32> iv(n);
+    for (uint64_t i = 0; i < n; i++) iv[i] = s[i];
+    sdsl::wt_int<> wt(iv, n);
+    std::vector<uint64_t> res;
+    res.push_back(wt.max_level);
+    res.push_back(wt.sigma);
+    for (uint64_t i = 0; i < n; i += 97) {
+      const uint64_t c = s[(i * 31) % n];
+      res.push_back(wt[i]);
+      res.push_back(wt.rank(i, c));
+      auto is = wt.inverse_select(i);
+      res.push_back(is.first);
+      res.push_back(is.second);
+      uint64_t sel;
+      try { sel = wt.select(1 + (i % 7), c); } catch (const std::logic_error&) { sel = UINT64_MAX; }
+      res.push_back(sel);
+    }
+    dump(out + "/wtint.bin", res.data(), res.size() * 8);
+  }
+  std::printf("dropin ok\n");
+  return 0;
+}

@@ -1,0 +1,74 @@
+"""The C++ drop-in layer, driven the way reference code calls it: WT() from
+wavelet/levelWT/WT.h (byte, INT and INT+LONG builds), sdsl-shaped
+rank_support_v / select_support_mcl<1> / <0> and wt_int<>.  The driver
+(tests/cpp/dropin_driver.cpp, built by build()) writes its answers; they must
+equal the oracle's and brute force bit for bit."""
+import os
+import subprocess
+import tempfile
+
+import numpy as np
+import pytest
+
+pytestmark = pytest.mark.gpu
+ROOT = os.path.dirname(os.path.dirname(os.path.abspath(__file__)))
+BUILD = os.path.join(ROOT, "tests", "cpp", "_build")
+
+
+def run_driver(variant, outdir):
+    exe = os.path.join(BUILD, "dropin_" + variant)
+    if not os.path.exists(exe):
+        pytest.fail("drop-in driver not built: run __graft_entry__.build()")
+    print("driver", variant, flush=True)
+    r = subprocess.run([exe, outdir], capture_output=True, text=True, timeout=300)
+    assert r.returncode == 0, r.stderr
+    assert "dropin ok" in r.stdout
+
+
+def load(path, dt=np.uint64):
+    return np.fromfile(path, dtype=dt)
+
+
+@pytest.mark.parametrize("variant,dt", [("u8", np.uint8), ("u32", np.uint32), ("u64", np.uint64)])
+def test_dropin_driver(wsfm, orc, variant, dt):
+    with tempfile.TemporaryDirectory() as d:
+        run_driver(variant, d)
+        # WT()
+        n = 100003
+        sigma = 256 if variant == "u8" else 1000
+        s = wsfm.gen_uniform(11, n, sigma, dt)
+        L = orc.levels_of_sigma(sigma)
+        ow, ond = orc.levels_serial(s, L)
+        assert np.array_equal(load(d + "/wt_words.bin"), ow)
+        nodes = load(d + "/wt_nodes.bin").reshape(2 * sigma, 5)
+        reached = np.nonzero(ond[:, 5])[0]
+        assert np.array_equal(nodes[reached], ond[reached, :5])
+        # rank / select
+        nbits = 1000003
+        nw = (nbits + 63) // 64
+        w = wsfm.gen_uniform(12, nw, (1 << 64) - 1, np.uint64)
+        w[-1] &= np.uint64((1 << (nbits & 63)) - 1)
+        bits = np.unpackbits(w.view(np.uint8), bitorder="little")[:nbits].astype(np.int64)
+        pref = np.concatenate([[0], np.cumsum(bits)])
+        idx = np.concatenate([np.arange(0, nbits + 1, 7), [nbits]])
+        assert np.array_equal(load(d + "/rank.bin"), pref[idx].astype(np.uint64))
+        p1 = np.nonzero(bits == 1)[0]
+        p0 = np.nonzero(bits == 0)[0]
+        assert np.array_equal(load(d + "/sel1.bin"), p1[::5].astype(np.uint64))
+        assert np.array_equal(load(d + "/sel0.bin"), p0[::5].astype(np.uint64))
+        # wt_int
+        m = 50000
+        sv = wsfm.gen_uniform(13, m, 5000, np.uint32)
+        res = load(d + "/wtint.bin")
+        Lw = int(max(1, int(sv.max()))).bit_length()
+        assert res[0] == Lw and res[1] == len(np.unique(sv))
+        rows = res[2:].reshape(-1, 5)
+        for k, i in enumerate(range(0, m, 97)):
+            c = int(sv[(i * 31) % m])
+            acc, rk, isr, isc, sel = (int(x) for x in rows[k])
+            assert acc == sv[i]
+            assert rk == int((sv[:i] == c).sum())
+            assert isc == sv[i] and isr == int((sv[:i] == sv[i]).sum())
+            occ = np.nonzero(sv == c)[0]
+            j = 1 + (i % 7)
+            assert sel == (occ[j - 1] if j <= occ.size else orc.NONE)

@@ -66,6 +66,30 @@ def build(verbose=False, jobs=None):
     return LIB
 
 
+
+
+def build_dropin_drivers(verbose=False):
+    """g++ builds of tests/cpp/dropin_driver.cpp against the C++ drop-in headers
+    (byte, -DINT and -DINT -DLONG symbol types), linked to libwtg.so."""
+    src = os.path.join(ROOT, "tests", "cpp", "dropin_driver.cpp")
+    outdir = os.path.join(ROOT, "tests", "cpp", "_build")
+    os.makedirs(outdir, exist_ok=True)
+    outs = []
+    for name, defs in (("u8", []), ("u32", ["-DINT"]), ("u64", ["-DINT", "-DLONG"])):
+        exe = os.path.join(outdir, "dropin_" + name)
+        cmd = ["g++", "-O2", "-std=c++17", "-Wall", "-Wno-unused-variable"] + defs + [
+            "-I" + os.path.join(ROOT, "include"), "-I" + os.path.join(HERE, "host"), src,
+            "-o", exe, "-L" + HERE, "-lwtg", "-Wl,-rpath," + HERE, "-Wl,-rpath,/opt/rocm/lib"]
+        r = subprocess.run(cmd, capture_output=True, text=True)
+        if r.returncode != 0:
+            raise RuntimeError("driver build failed: %s\n%s" % (" ".join(cmd), r.stderr))
+        outs.append(exe)
+    if verbose:
+        print("built", ", ".join(outs))
+    return outs
+
+
 if __name__ == "__main__":
     build(verbose=True)
+    build_dropin_drivers(verbose=True)
     sys.exit(0)

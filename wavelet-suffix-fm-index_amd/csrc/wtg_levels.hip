@@ -6,10 +6,11 @@
 //
 // GPU pipeline for one build (all integer, HBM bound).  CNT_j[q] = number of
 // symbols whose top j bits equal q, kept for every j in one heap-ordered array.
-//   per level k:
+//   level 0:
 //      k_level<bits>  16-byte vector loads; the level's bit words from wave
-//                     ballots; the tile's zero count (level 0: also CNT_1).
-//      k_scan_u32     exclusive prefix of the per-tile zero counts.
+//                     ballots; the tile's zero count; total zeros -> CNT_1.
+//   per level k:
+//      k_scan_u32     exclusive prefix of the level's per-tile zero counts.
 //      k_nodetab      node starts S(p) and, for split levels, the per-node
 //                     constants A(p) = S(p)-Zs(p), B(p) = nz(p)+Zs(p) (nz =
 //                     zeros of node p = CNT_{k+1}[2p], Zs = zeros of the earlier
@@ -20,13 +21,14 @@
 //                     lane-consecutively to
 //                       dest = slot < Z_t ? Z_prev + slot + A(p)
 //                                         : O_prev + slot - Z_t + B(p)
-//                     in the next ping-pong buffer (last level: bits only).
-//                     In the store loop it also counts, per (k+1)-prefix group,
-//                     the elements whose next bit is 1: each wave splits its 64
-//                     consecutive slots into runs of one group (head-flag
-//                     ballot), adds each run's popcount to an LDS counter and the
-//                     tile flushes one atomic per group into counters replicated
-//                     across tiles for shallow levels -> level k+1's table.
+//                     in the next ping-pong buffer.  The slots of one group
+//                     ((k+1)-prefix) are contiguous and land on contiguous
+//                     destinations, so the pass also emits, per group, level
+//                     k+1's bit words (assembled in slot order in LDS from
+//                     ballots; interior words stored, the two edge words OR-ed),
+//                     level k+1's zeros per tile and the group's next-bit ones
+//                     (level k+1's table; counters replicated across tiles for
+//                     shallow levels).  Level L-1 needs no pass of its own.
 //   k_nodes           WTnode records (optional).
 // The per-tile zero prefix comes from the bits pass, so the split pass needs no
 // cross-tile wait: a single-pass decoupled look-back measured ~4x slower here
@@ -39,9 +41,14 @@
 typedef uint32_t v4u32 __attribute__((ext_vector_type(4)));
 
 static __host__ __device__ __forceinline__ uint64_t heap_off(uint32_t j) { return (1ull << j) - 1; }
-// counter replicas of the next-bit counts of the 2^j groups of level j
-static __host__ __device__ __forceinline__ uint32_t c1_reps(uint32_t j) {
-  return j >= 12 ? 1u : ((4096u >> j) > 32u ? 32u : (4096u >> j));
+// Counter replicas for the next-bit counts of the 2^j groups of level j: a
+// group receives about 2*tiles/2^j tile contributions; replicas keep that
+// near 64 per counter (power of two, at most 1024).
+static uint32_t c1_reps(uint32_t j, uint64_t tiles) {
+  uint64_t per = (2 * tiles) >> j;
+  uint32_t r = 1;
+  while (r < 1024 && (uint64_t)r * 64 < per) r <<= 1;
+  return r;
 }
 
 template <typename T>
@@ -64,7 +71,7 @@ __global__ void k_count_nonzero(const uint32_t* __restrict__ h, uint64_t n,
 // ------------------------------------------------------------ node tables
 struct NodeTabArgs {
   uint32_t* cnt;         // heap CNT (levels 0..L)
-  const uint32_t* c1;    // level k's next-bit counts (c1_reps(k) replicas of 2^k)
+  const uint32_t* c1;    // level k's next-bit counts (2^k, replicas already summed)
   uint32_t reps;
   uint32_t* start;       // heap START (levels 0..L-1)
   uint2* tab;            // level k table (A,B) or nullptr
@@ -226,6 +233,8 @@ struct LevelArgs {
   uint32_t* c1;          // next-bit counts of the 2^(l+1) groups of level l+1 (replicated)
   uint32_t reps;         // replicas of c1 (power of two)
   uint32_t* zc;          // bits pass: zeros per tile (out)
+  uint32_t* zc_next;     // split pass: zeros per tile of level l+1 (atomically accumulated)
+  uint64_t next_base;    // split pass: (l+1)*n, first bit of level l+1
   const uint32_t* zpre;  // split pass: exclusive prefix of zc (in)
   uint32_t* err;
 };
@@ -247,7 +256,8 @@ __global__ __launch_bounds__(256) void k_level(LevelArgs a) {
   constexpr int NCH = LV_STEPS * 4;      // (step, wave) chunks
   __shared__ uint64_t s_words[SCATTER ? 1 : NW + 1];
   __shared__ uint32_t s_wz[NCH];
-  __shared__ uint32_t s_cnt[COUNT ? 2 * TAB_CACHE : 1];
+  __shared__ uint64_t s_nb[SCATTER ? NW + 1 : 1];
+  __shared__ uint64_t s_head[SCATTER ? NW : 1];
   __shared__ T s_edge[2][SCATTER ? NCH : 1];
   __shared__ T s_stage[SCATTER ? TILE : 1];
   __shared__ uint2 s_tab[SCATTER ? TAB_CACHE : 1];
@@ -275,9 +285,6 @@ __global__ __launch_bounds__(256) void k_level(LevelArgs a) {
 #pragma unroll
       for (int v = 0; v < V; v++) x[s][v] = (e0 + v < n) ? src[e0 + v] : (T)0;
     }
-  }
-  if (COUNT) {
-    for (int i = tid; i < 2 * TAB_CACHE; i += 256) s_cnt[i] = 0;
   }
 
   uint32_t mk[LV_STEPS], pre[LV_STEPS];
@@ -374,14 +381,30 @@ __global__ __launch_bounds__(256) void k_level(LevelArgs a) {
     __syncthreads();
     const uint64_t oprev = tile_start - zprev;
     T* dst = (T*)a.dst;
-    uint32_t* c1 = COUNT ? a.c1 + (uint64_t)(tile & (a.reps - 1)) * (2ull * a.nnodes) : nullptr;
-    const uint64_t gfirst = 2 * pfirst;
-#pragma unroll 4
+    const uint32_t nb = a.bitpos - 1;   // the next level's bit
+    // ---- slot-order bit arrays of the next level's bit (s_nb) and of group
+    // starts (s_head, group = (l+1)-prefix); LDS only, so the barrier below
+    // does not wait on any global store
     for (int r = 0; r < ROUNDS; r++) {
       const uint32_t q = (uint32_t)(r * 256 + tid);
       const bool in = q < valid;
       const T xv = in ? s_stage[q] : (T)0;
-      if (in) {
+      const uint64_t g = (uint64_t)xv >> a.bitpos;
+      T pv = shfl_t<T>(xv, (lane + 63) & 63);
+      if (lane == 0 && q > 0 && in) pv = s_stage[q - 1];
+      const bool head = in && (q == 0 || ((uint64_t)pv >> a.bitpos) != g);
+      const uint64_t H = wave_ballot(head);
+      const uint64_t C = wave_ballot(in && ((((uint64_t)xv) >> nb) & 1u));
+      if (lane == 0) { s_nb[q >> 6] = C; s_head[q >> 6] = H; }
+    }
+    if (tid == 0) s_nb[NW] = 0;
+    __syncthreads();
+    // ---- lane-consecutive stores of the staged tile
+#pragma unroll 4
+    for (int r = 0; r < ROUNDS; r++) {
+      const uint32_t q = (uint32_t)(r * 256 + tid);
+      if (q < valid) {
+        const T xv = s_stage[q];
         const uint64_t pnode = (uint64_t)xv >> a.node_shift;
         uint2 ab;
         bool ok = true;
@@ -392,38 +415,107 @@ __global__ __launch_bounds__(256) void k_level(LevelArgs a) {
         if (ok && dest < n) __builtin_nontemporal_store(xv, dst + dest);
         else atomicOr(a.err, FLAG_BADSYM);
       }
-      if constexpr (COUNT) {
-        // slots of one group g = x >> bitpos ((l+1)-prefix) are contiguous:
-        // split this wave's 64 slots into pieces of equal g and add each
-        // piece's number of next-bit-1 elements to the group's counter
-        const uint64_t g = (uint64_t)xv >> a.bitpos;
-        const bool c = in && ((((uint64_t)xv) >> (a.bitpos - 1)) & 1u);
-        const uint64_t gp = (uint64_t)shfl_t<T>(xv, (lane + 63) & 63) >> a.bitpos;
-        const bool head = in && (lane == 0 || gp != g);
-        const uint64_t H = wave_ballot(head);
-        const uint64_t C = wave_ballot(c);
-        if (head) {
-          const uint64_t above = (lane == 63) ? 0ull : (H >> (lane + 1)) << (lane + 1);
-          const uint64_t nxt = above ? (above & (0ull - above)) : 0ull;  // lowest head above
-          const uint64_t span = (nxt ? nxt - 1 : ~0ull) & (~0ull << lane);
-          const uint32_t cnt = popc64(C & span);
-          if (cnt && g < 2ull * a.nnodes) {
-            if (cached && g - gfirst < 2 * TAB_CACHE) atomicAdd(&s_cnt[g - gfirst], cnt);
-            else atomicAdd(&c1[g], cnt);
-          }
+    }
+    // ---- per group (contiguous slots [sG, eG) -> contiguous dests [dG, dG+len)):
+    // one wave per group, one output word per lane: level l+1's bit words
+    // (interior words stored, the two edge words OR-ed), the group's next-bit
+    // ones (level l+1's table) and its zeros per destination tile of level l+1
+    uint32_t* c1 = COUNT ? a.c1 + (uint64_t)(tile & (a.reps - 1)) * (2ull * a.nnodes) : nullptr;
+    constexpr int HPL = (NW + 63) / 64;  // head words per lane
+    uint64_t hwl[HPL];
+    uint32_t hc = 0;
+#pragma unroll
+    for (int j = 0; j < HPL; j++) {
+      const int idx = lane * HPL + j;
+      hwl[j] = (idx < NW) ? s_head[idx] : 0ull;
+      hc += popc64(hwl[j]);
+    }
+    const uint32_t hincl = wave_incl_scan(hc);
+    const uint32_t hex = hincl - hc;
+    const uint32_t ng = __shfl(hincl, 63);
+    auto group_start = [&](uint32_t gi) -> uint32_t {
+      const uint64_t own = wave_ballot(hex <= gi && gi < hincl);
+      const int ol = (int)__builtin_ctzll(own);
+      uint32_t slot = 0;
+      if (lane == ol) {
+        uint32_t r = gi - hex;
+#pragma unroll
+        for (int j = 0; j < HPL; j++) {
+          const uint32_t c = popc64(hwl[j]);
+          if (r < c) { slot = (uint32_t)(lane * HPL + j) * 64 + select_in_word(hwl[j], r); break; }
+          r -= c;
         }
       }
-    }
-    if constexpr (COUNT) {
-      if (cached) {
-        __syncthreads();
-        for (int i = tid; i < 2 * TAB_CACHE; i += 256) {
-          const uint32_t v = s_cnt[i];
-          if (v) atomicAdd(&c1[gfirst + i], v);
+      return __shfl(slot, ol);
+    };
+    for (uint32_t gi = (uint32_t)w; gi < ng; gi += 4) {
+      const uint32_t sG = group_start(gi);
+      const uint32_t eG = (gi + 1 < ng) ? group_start(gi + 1) : (uint32_t)valid;
+      const uint32_t len = eG - sG;
+      const uint64_t x0 = (uint64_t)s_stage[sG];
+      const uint64_t pnode = x0 >> a.node_shift;
+      const uint64_t g = x0 >> a.bitpos;
+      uint2 ab;
+      if (cached && pnode - pfirst < TAB_CACHE) ab = s_tab[pnode - pfirst];
+      else if (pnode < a.nnodes) ab = a.tab[pnode];
+      else { if (lane == 0) atomicOr(a.err, FLAG_BADSYM); continue; }
+      const uint64_t dG = (sG < zt) ? zprev + sG + ab.x : oprev + (sG - zt) + ab.y;
+      if (dG + len > n || len == 0) { if (lane == 0) atomicOr(a.err, FLAG_BADSYM); continue; }
+      const uint64_t b0 = (dG / TILE + 1) * TILE;  // first destination-tile boundary
+      const int64_t rel0 = (dG + len > b0) ? (int64_t)(b0 - dG) : (int64_t)len;
+      const uint64_t GB0 = a.next_base + dG;
+      const uint64_t k0 = GB0 >> 6, k1 = (GB0 + len - 1) >> 6;
+      uint32_t o_lo = 0, o_hi = 0;
+      for (uint64_t k = k0 + lane; k <= k1; k += 64) {
+        const int64_t off = (int64_t)(k * 64) - (int64_t)GB0;  // group bit at word bit 0
+        const int64_t P = (int64_t)sG + off;                   // its slot
+        uint64_t val;
+        if (P >= 0) {
+          const uint32_t w0 = (uint32_t)(P >> 6), sh = (uint32_t)(P & 63);
+          val = s_nb[w0] >> sh;
+          if (sh) val |= s_nb[w0 + 1] << (64 - sh);
+        } else {
+          val = s_nb[0] << (uint32_t)(-P);
+        }
+        const int64_t lo = off < 0 ? -off : 0;
+        const int64_t hi = ((int64_t)len - off < 64) ? (int64_t)len - off : 64;
+        const uint64_t msk = (hi >= 64 ? ~0ull : ((1ull << hi) - 1)) & (~0ull << lo);
+        val &= msk;
+        if (lo == 0 && hi == 64) a.words[k] = val;
+        else if (val) atomicOr((unsigned long long*)&a.words[k], (unsigned long long)val);
+        const int64_t cut = rel0 - off;  // bits below `cut` belong to the first destination tile
+        const uint64_t mlo = cut <= 0 ? 0ull : (cut >= 64 ? ~0ull : ((1ull << cut) - 1));
+        o_lo += popc64(val & mlo);
+        o_hi += popc64(val & ~mlo);
+      }
+      o_lo = wave_sum(o_lo);
+      o_hi = wave_sum(o_hi);
+      if (lane == 0) {
+        const uint32_t ones = o_lo + o_hi;
+        if (COUNT && ones && g < 2ull * a.nnodes) atomicAdd(&c1[g], ones);
+        const uint64_t D0 = dG / TILE;
+        const uint32_t z_lo = (uint32_t)rel0 - o_lo;
+        if (z_lo) atomicAdd(&a.zc_next[D0], z_lo);
+        if ((int64_t)len > rel0) {
+          const uint32_t z_hi = (uint32_t)(len - rel0) - o_hi;
+          if (z_hi) atomicAdd(&a.zc_next[D0 + 1], z_hi);
         }
       }
     }
   }
+}
+
+// sums the replicas of a level's next-bit counters: out[g] = sum_r c1[r*ng + g]
+__global__ __launch_bounds__(256) void k_c1_reduce(const uint32_t* __restrict__ c1, uint32_t reps,
+                                                   uint64_t ngroups, uint32_t* __restrict__ out) {
+  __shared__ uint32_t s_wave[4];
+  const uint64_t g = blockIdx.x;
+  uint32_t acc = 0;
+  for (uint32_t r = threadIdx.x; r < reps; r += 256) acc += c1[(uint64_t)r * ngroups + g];
+  acc = wave_sum(acc);
+  if (lane_id() == 0) s_wave[threadIdx.x >> 6] = acc;
+  __syncthreads();
+  if (threadIdx.x == 0) out[g] = s_wave[0] + s_wave[1] + s_wave[2] + s_wave[3];
 }
 
 __global__ void k_cnt_from_zeros(uint32_t* __restrict__ cnt, const uint32_t* zeros, uint64_t n) {
@@ -446,6 +538,7 @@ struct LevelPlan {
   size_t off_cnt, off_c1, off_ntst, off_scst, off_ctr, off_start, off_tab, off_zc, off_zpre,
       off_buf0, off_buf1, total;
   uint64_t c1off[26];      // c1 region of level j (next-bit counts of its 2^j groups)
+  uint32_t reps[26];       // replicas of level j's counters
   uint64_t nt_tiles[26];   // node-table tiles of level k
   uint64_t nt_base[26];    // state offset of level k (two arrays: s and z)
 };
@@ -461,7 +554,8 @@ LevelPlan plan_levels(uint64_t n, uint32_t L, int w) {
   uint64_t t = 0, c = 0;
   for (uint32_t k = 0; k <= L; k++) {
     P.c1off[k] = c;
-    c += (uint64_t)c1_reps(k) << k;
+    P.reps[k] = c1_reps(k, P.tiles);
+    c += (uint64_t)P.reps[k] << k;
     if (k < L) {
       P.nt_tiles[k] = wtg_ceil_div(1ull << k, 4096);
       P.nt_base[k] = t;
@@ -475,9 +569,9 @@ LevelPlan plan_levels(uint64_t n, uint32_t L, int w) {
   P.off_ntst = o; o = wtg_align256(o + (t + 1) * 8);
   P.off_scst = o; o = wtg_align256(o + (size_t)L * (P.scan_tiles + 1) * 8);
   P.off_ctr = o; o = wtg_align256(o + 128 * 4);
+  P.off_zc = o; o = wtg_align256(o + (size_t)L * (P.tiles + 1) * 4);
   P.off_start = o; o = wtg_align256(o + ((1ull << L) + 1) * 4);
   P.off_tab = o; o = wtg_align256(o + ((1ull << (L - 1)) + 1) * 8);
-  P.off_zc = o; o = wtg_align256(o + (P.tiles + 1) * 4);
   P.off_zpre = o; o = wtg_align256(o + (P.tiles + 1) * 4);
   P.off_buf0 = o; o = wtg_align256(o + n * w);
   P.off_buf1 = o; o = wtg_align256(o + n * w);
@@ -533,6 +627,7 @@ int levels_build_t(wtg_ctx* ctx, const T* d_sym, uint64_t n, uint32_t L, uint64_
   const void* src = d_sym;
   for (uint32_t k = 0; k < L; k++) {
     const bool split = k + 1 < L;
+    uint32_t* zck = zc + (size_t)k * (P.tiles + 1);
     LevelArgs la{};
     la.src = src;
     la.dst = buf[k & 1];
@@ -545,16 +640,19 @@ int levels_build_t(wtg_ctx* ctx, const T* d_sym, uint64_t n, uint32_t L, uint64_
     la.check_shift = (k == 0 && L < 64) ? L : 64;
     la.tab = tab + heap_off(k);
     la.c1 = c1 + P.c1off[k + 1];
-    la.reps = c1_reps(k + 1);
-    la.zc = zc;
+    la.reps = P.reps[k + 1];
+    la.zc = zck;
+    la.zc_next = split ? zc + (size_t)(k + 1) * (P.tiles + 1) : nullptr;
+    la.next_base = (uint64_t)(k + 1) * n;
     la.zpre = zpre;
     la.err = ctx->d_flag;
     const bool kvec = k == 0 ? vec : true;
-    // 1. bits pass: this level's bit words + zeros per tile; exclusive scan
     if (n > 0) {
-      WTG_TRY(launch_level<T>(ctx, la, P.tiles, 0, kvec, false));
+      // level 0: bits pass (its bit words + zeros per tile); every later level's
+      // bits and tile zeros were produced by the previous split
+      if (k == 0) WTG_TRY(launch_level<T>(ctx, la, P.tiles, 0, kvec, false));
       if (split || k == 0) {
-        WTG_LAUNCH(ctx, "scan_tiles", k_scan_u32<0>, dim3((unsigned)P.scan_tiles), dim3(256), 0, zc, zpre,
+        WTG_LAUNCH(ctx, "scan_tiles", k_scan_u32<0>, dim3((unsigned)P.scan_tiles), dim3(256), 0, zck, zpre,
                    P.tiles, scst + (size_t)k * (P.scan_tiles + 1), ctr + k, ctx->d_flag);
         WTG_LAUNCH_CHECK();
       }
@@ -569,12 +667,17 @@ int levels_build_t(wtg_ctx* ctx, const T* d_sym, uint64_t n, uint32_t L, uint64_
                  d_distinct);
       WTG_LAUNCH_CHECK();
     }
-    // 2. node table of level k (node starts always: cheap, needed by WTnodes)
+    // node table of level k (node starts always: cheap, needed by WTnodes)
+    if (k >= 1 && n > 0 && P.reps[k] > 1) {
+      WTG_LAUNCH(ctx, "c1_reduce", k_c1_reduce, dim3((unsigned)(1u << k)), dim3(256), 0, c1 + P.c1off[k],
+                 P.reps[k], (uint64_t)(1ull << k), c1 + P.c1off[k]);
+      WTG_LAUNCH_CHECK();
+    }
     {
       NodeTabArgs na{};
       na.cnt = cnt;
       na.c1 = c1 + P.c1off[k];
-      na.reps = c1_reps(k);
+      na.reps = 1;
       na.start = start;
       na.tab = split ? tab + heap_off(k) : nullptr;
       na.st_s = ntst + P.nt_base[k];
@@ -590,9 +693,9 @@ int levels_build_t(wtg_ctx* ctx, const T* d_sym, uint64_t n, uint32_t L, uint64_
       }
     }
     if (n == 0 || !split) continue;
-    // 3. split pass: stable scatter into the next level's order; the counts of
-    // the next bit feed level k+1's table and, at the last split, the
-    // distinct-symbol count
+    // split pass: stable scatter into the next level's order, plus the next
+    // level's bit words, its zeros per tile and the next-bit counts that feed
+    // level k+1's table (and, at the last split, the distinct-symbol count)
     const bool count = (k + 2 <= L - 1) || (d_distinct && k + 2 == L);
     WTG_TRY(launch_level<T>(ctx, la, P.tiles, 1, kvec, count));
     src = buf[k & 1];

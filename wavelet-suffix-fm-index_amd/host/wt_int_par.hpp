@@ -1,0 +1,201 @@
+// wt_int_par.hpp -- drop-in for sdsl::wt_int<> of the reference
+// (wt_int/wt_int_par.hpp:57-1086): same include guard, constructor
+// wt_int(int_vector<w>& buf, size, max_level=0) and the query surface size /
+// empty / sigma / tree / max_level / operator[] / rank / inverse_select /
+// select.  The tree bits, rank_support_v and select_support_mcl (for 1 and 0)
+// are built on the GPU (wtg_wtint_build) and copied into host supports, so the
+// scalar queries run the reference's host algorithms; *_batch() run batches on
+// the device against the device copy.  select() throws std::logic_error when
+// c does not occur i times, as the reference (:644-646).
+#ifndef INCLUDED_SDSL_INT_WAVELET_TREE_PAR
+#define INCLUDED_SDSL_INT_WAVELET_TREE_PAR
+#include <stdexcept>
+#include <string>
+#include <utility>
+#include <vector>
+#include "rank_support_v_par.hpp"
+#include "select_support_mcl_par.hpp"
+#include "sdsl_standin.hpp"
+#include "wtg_runtime.hpp"
+
+namespace sdsl {
+
+template <class t_bitvector = bit_vector, class t_rank = rank_support_v<1, 1>,
+          class t_select = select_support_mcl<1, 1>, class t_select_zero = select_support_mcl<0, 1>>
+class wt_int {
+ public:
+  typedef uint64_t size_type;
+  typedef uint64_t value_type;
+  typedef t_bitvector bit_vector_type;
+  typedef t_rank rank_1_type;
+  typedef t_select select_1_type;
+  typedef t_select_zero select_0_type;
+  enum { lex_ordered = 1 };
+
+  bit_vector_type m_tree;
+
+ protected:
+  size_type m_size = 0;
+  size_type m_sigma = 0;
+  rank_1_type m_tree_rank;
+  select_1_type m_tree_select1;
+  select_0_type m_tree_select0;
+  uint32_t m_max_level = 0;
+  wtg_wtint* m_dev = nullptr;  // device copy for the batched queries
+
+ public:
+  const size_type& sigma = m_sigma;
+  const bit_vector_type& tree = m_tree;
+  const uint32_t& max_level = m_max_level;
+
+  wt_int() = default;
+
+  template <uint8_t int_width>
+  wt_int(int_vector<int_width>& buf, size_type size, uint32_t max_level = 0) : m_size(size) {  // :368-406
+    if (0 == m_size) return;
+    if (buf.size() < m_size)
+      throw std::logic_error("n=" + std::to_string(buf.size()) + " < " + std::to_string(m_size) + "=m_size");
+    // the reference requires whole-word symbols (:371); copy them packed
+    const int w = buf.width() <= 8 ? 1 : buf.width() <= 16 ? 2 : buf.width() <= 32 ? 4 : 8;
+    std::vector<unsigned char> packed(m_size * w);
+    for (size_type i = 0; i < m_size; i++) {
+      const uint64_t v = buf[i];
+      for (int b = 0; b < w; b++) packed[i * w + b] = (unsigned char)(v >> (8 * b));
+    }
+    wtg::dbuf ds(m_size * w);
+    ds.upload(packed.data(), m_size * w);
+    wtg::check(wtg_wtint_build(wtg::default_context(), ds.get(), w, m_size, max_level, &m_dev));
+    fetch();
+  }
+  wt_int(const wt_int& o) { copy(o); }
+  wt_int& operator=(const wt_int& o) {
+    if (this != &o) copy(o);
+    return *this;
+  }
+  ~wt_int() {
+    if (m_dev) wtg_wtint_destroy(wtg::default_context(), m_dev);
+  }
+
+  size_type size() const { return m_size; }
+  bool empty() const { return m_size == 0; }
+
+  value_type operator[](size_type i) const {  // :519-541
+    size_type offset = 0, node_size = m_size;
+    value_type res = 0;
+    for (uint32_t k = 0; k < m_max_level; ++k) {
+      res <<= 1;
+      const size_type o = m_tree_rank(offset), oi = m_tree_rank(offset + i) - o,
+                      oe = m_tree_rank(offset + node_size) - o;
+      if (m_tree[offset + i]) { offset += node_size - oe; node_size = oe; i = oi; res |= 1; }
+      else { node_size = node_size - oe; i = i - oi; }
+      offset += m_size;
+    }
+    return res;
+  }
+
+  size_type rank(size_type i, value_type c) const {  // :553-577
+    if (((1ULL) << m_max_level) <= c) return 0;
+    size_type offset = 0, node_size = m_size;
+    uint64_t mask = 1ULL << (m_max_level - 1);
+    for (uint32_t k = 0; k < m_max_level && i; ++k) {
+      const size_type o = m_tree_rank(offset), oi = m_tree_rank(offset + i) - o,
+                      oe = m_tree_rank(offset + node_size) - o;
+      if (c & mask) { offset += node_size - oe; node_size = oe; i = oi; }
+      else { node_size = node_size - oe; i = i - oi; }
+      offset += m_size;
+      mask >>= 1;
+    }
+    return i;
+  }
+
+  std::pair<size_type, value_type> inverse_select(size_type i) const {  // :588-611
+    value_type c = 0;
+    size_type node_size = m_size, offset = 0;
+    for (uint32_t k = 0; k < m_max_level; ++k) {
+      const size_type o = m_tree_rank(offset), oi = m_tree_rank(offset + i) - o,
+                      oe = m_tree_rank(offset + node_size) - o;
+      c <<= 1;
+      if (m_tree[offset + i]) { offset += node_size - oe; node_size = oe; i = oi; c |= 1; }
+      else { node_size = node_size - oe; i = i - oi; }
+      offset += m_size;
+    }
+    return std::make_pair(i, c);
+  }
+
+  size_type select(size_type i, value_type c) const {  // :622-660
+    std::vector<size_type> path_off(m_max_level + 1), path_rank(m_max_level + 1);
+    size_type offset = 0, node_size = m_size;
+    uint64_t mask = 1ULL << (m_max_level - 1);
+    path_off[0] = path_rank[0] = 0;
+    for (uint32_t k = 0; k < m_max_level && node_size; ++k) {
+      const size_type o = m_tree_rank(offset);
+      path_rank[k] = o;
+      const size_type oe = m_tree_rank(offset + node_size) - o;
+      if (c & mask) { offset += node_size - oe; node_size = oe; }
+      else node_size = node_size - oe;
+      offset += m_size;
+      path_off[k + 1] = offset;
+      mask >>= 1;
+    }
+    if (0ULL == node_size || node_size < i)
+      throw std::logic_error("select(" + std::to_string(i) + "," + std::to_string(c) +
+                             "): c does not occur i times in the WT");
+    mask = 1ULL;
+    for (uint32_t k = m_max_level; k > 0; --k) {
+      offset = path_off[k - 1];
+      const size_type o = path_rank[k - 1];
+      if (c & mask) i = m_tree_select1(o + i) - offset + 1;
+      else i = m_tree_select0(offset - o + i) - offset + 1;
+      mask <<= 1;
+    }
+    return i - 1;
+  }
+
+  // ---- batched device queries (device buffers of nq uint64)
+  void rank_batch(const uint64_t* d_i, const uint64_t* d_c, uint64_t nq, uint64_t* d_out) const {
+    wtg::check(wtg_wtint_rank_batch(wtg::default_context(), m_dev, d_i, d_c, nq, d_out));
+  }
+  void select_batch(const uint64_t* d_i, const uint64_t* d_c, uint64_t nq, uint64_t* d_out) const {
+    wtg::check(wtg_wtint_select_batch(wtg::default_context(), m_dev, d_i, d_c, nq, d_out));
+  }
+  void access_batch(const uint64_t* d_i, uint64_t nq, uint64_t* d_out) const {
+    wtg::check(wtg_wtint_access_batch(wtg::default_context(), m_dev, d_i, nq, d_out));
+  }
+  const wtg_wtint* device() const { return m_dev; }
+
+ private:
+  void fetch() {
+    uint64_t n = 0, sg = 0;
+    uint32_t L = 0;
+    const uint64_t *dw = nullptr, *dd = nullptr;
+    wtg::check(wtg_wtint_info(m_dev, &n, &L, &sg, &dw, &dd));
+    m_max_level = L;
+    m_sigma = sg;
+    const uint64_t nbits = n * L;
+    m_tree = bit_vector_type(nbits);
+    wtg::check(wtg_memcpy_d2h(wtg::default_context(), m_tree.data(), dw, (nbits + 63) / 64 * 8));
+    m_tree_rank = rank_1_type();
+    m_tree_rank.assign_directory(dd, nbits);
+    m_tree_rank.set_vector(&m_tree);
+    m_tree_select1.assign(*wtg_wtint_select1(m_dev));
+    m_tree_select1.set_vector(&m_tree);
+    m_tree_select0.assign(*wtg_wtint_select0(m_dev));
+    m_tree_select0.set_vector(&m_tree);
+  }
+  void copy(const wt_int& o) {
+    m_size = o.m_size;
+    m_sigma = o.m_sigma;
+    m_tree = o.m_tree;
+    m_max_level = o.m_max_level;
+    m_tree_rank = o.m_tree_rank;
+    m_tree_rank.set_vector(&m_tree);
+    m_tree_select1 = o.m_tree_select1;
+    m_tree_select1.set_vector(&m_tree);
+    m_tree_select0 = o.m_tree_select0;
+    m_tree_select0.set_vector(&m_tree);
+    m_dev = nullptr;  // the copy answers host queries; batches need the original
+  }
+};
+
+}  // namespace sdsl
+#endif
