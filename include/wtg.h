@@ -165,12 +165,80 @@ int wtg_wtint_select_batch(wtg_ctx* ctx, const wtg_wtint* wt, const uint64_t* d_
 int wtg_wtint_inverse_select_batch(wtg_ctx* ctx, const wtg_wtint* wt, const uint64_t* d_i,
                                    uint64_t nq, uint64_t* d_rank, uint64_t* d_sym);
 
+/* ------------------------------------- wt_huff (Huffman-shaped WT over bytes) */
+/* Tree shape of sdsl::wt_huff<> (wt_huff/wt_huff_par.hpp:92-117: Huffman merge of
+ * (freq, id) minima, child[0] = first pop; nodes numbered breadth-first from the
+ * root, child[0] before child[1]).  Inner nodes hold bits at bv_pos[v] (BFS order);
+ * a symbol's code bit l is the branch taken at depth l. */
+#define WTG_HUFF_MAX_NODES 511
+typedef struct {
+  uint64_t n;                     /* text length */
+  uint64_t bv_size;               /* total bits over all inner nodes */
+  uint32_t sigma;                 /* symbols with nonzero count */
+  uint32_t n_nodes;               /* 2*sigma-1 (0 for an empty text) */
+  uint32_t n_inner;               /* sigma-1 */
+  uint32_t max_depth;             /* longest code */
+  uint64_t C[257];                /* C[c] = #bytes < c in the text */
+  uint64_t bv_pos[WTG_HUFF_MAX_NODES];
+  uint64_t bv_pos_rank[WTG_HUFF_MAX_NODES]; /* rank1 of the tree bits at bv_pos (inner) */
+  int32_t child[WTG_HUFF_MAX_NODES][2];     /* -1 at leaves */
+  int32_t parent[WTG_HUFF_MAX_NODES];       /* -1 at the root */
+  int32_t sym[WTG_HUFF_MAX_NODES];          /* leaf symbol, -1 at inner nodes */
+  int32_t c_to_leaf[256];                   /* -1 for absent symbols */
+  uint64_t path[256];                       /* code bits, bit l = branch at depth l */
+  uint32_t path_len[256];
+} wtg_huff_shape;
+
+typedef struct wtg_huff wtg_huff;
+/* sdsl::wt_huff<>(int_vector<8>& text, size)  (wt_huff/wt_pc_par.hpp:321-361, tree
+ * bits by build_recursive :193-257): counts, Huffman shape, all tree bits, rank9,
+ * select1, select0.  d_text: n bytes in device memory. */
+int wtg_huff_build(wtg_ctx* ctx, const uint8_t* d_text, uint64_t n, wtg_huff** out);
+int wtg_huff_destroy(wtg_ctx* ctx, wtg_huff* wt);
+int wtg_huff_get_shape(const wtg_huff* wt, wtg_huff_shape* out);
+/* device tree words (ceil(bv_size/64)+1, last = 0) and rank9 directory */
+int wtg_huff_data(const wtg_huff* wt, const uint64_t** d_words, const uint64_t** d_dir);
+const wtg_select* wtg_huff_select1(const wtg_huff* wt);
+const wtg_select* wtg_huff_select0(const wtg_huff* wt);
+/* rank(i,c) = #c in [0,i)  (wt_pc_par.hpp:462-485); i > n -> WTG_NONE */
+int wtg_huff_rank_batch(wtg_ctx* ctx, const wtg_huff* wt, const uint64_t* d_i,
+                        const uint32_t* d_c, uint64_t nq, uint64_t* d_out);
+/* operator[](i) (:430-448) and inverse_select(i) (:498-516); i >= n -> WTG_NONE.
+ * d_rank may be null (access only). */
+int wtg_huff_access_batch(wtg_ctx* ctx, const wtg_huff* wt, const uint64_t* d_i,
+                          uint64_t nq, uint64_t* d_sym, uint64_t* d_rank);
+/* select(i,c) = position of the i-th c (:527-554); WTG_NONE when c does not occur
+ * i times or i = 0 (the reference asserts 1 <= i <= rank(n, c)) */
+int wtg_huff_select_batch(wtg_ctx* ctx, const wtg_huff* wt, const uint64_t* d_i,
+                          const uint32_t* d_c, uint64_t nq, uint64_t* d_out);
+
+/* ------------------------------------------------ FM-index over a BWT */
+/* Suffix array of a byte text (the reference calls divsufsort,
+ * fm-index/par/WT.C:35-36): suffixes in lexicographic order, a proper prefix
+ * before its extensions.  Host memory, n < 2^40. */
+int wtg_suffix_array(const uint8_t* text, uint64_t n, int64_t* sa);
+/* bwt[i] = text[sa[i]-1], or text[n-1] where sa[i] = 0  (fm-index/par/WT.C:38-41) */
+int wtg_bwt_from_sa(const uint8_t* text, const int64_t* sa, uint64_t n, uint8_t* bwt);
+/* Backward-search count of npat patterns of m bytes each (d_pats, pattern q at
+ * q*m) over the wt_huff of a BWT (the index fm-index/par/WT.C:33-45 builds; the
+ * reference stops after construction).  out[q] = occurrences of pattern q in
+ * the text, exact when the text ends with a unique smallest byte. */
+int wtg_fm_count_batch(wtg_ctx* ctx, const wtg_huff* bwt_wt, const uint8_t* d_pats,
+                       uint32_t m, uint64_t npat, uint64_t* d_out);
+
 /* ------------------------------------------------ synthetic workloads (host) */
 /* Seeded generators of the benchmark inputs (splitmix64 streams). */
 uint64_t wtg_splitmix64(uint64_t* state);
 int wtg_gen_uniform(uint64_t seed, uint64_t n, uint64_t range, int sym_bytes, void* out);
 int wtg_gen_geometric(uint64_t seed, uint64_t n, double p, uint64_t cap, int sym_bytes, void* out);
 int wtg_gen_zipf_bytes(uint64_t seed, uint64_t n, double s, uint8_t* out);
+/* n bytes: order-5 Markov text over ACGT (Dirichlet(0.5) transition rows), 5% of
+ * positions overwritten by copies of 1-10 kb segments, 0.1% 'N', text[n-1] = 0 */
+int wtg_gen_dna(uint64_t seed, uint64_t n, uint8_t* out);
+/* npat patterns of m bytes from text[0, n-1): even q exact substrings, odd q with
+ * 1-2 substitutions by ACGT */
+int wtg_gen_patterns(uint64_t seed, const uint8_t* text, uint64_t n, uint32_t m,
+                     uint64_t npat, uint8_t* out);
 
 #ifdef __cplusplus
 }

@@ -13,6 +13,8 @@
 #include "rank_support_v_par.hpp"
 #include "select_support_mcl_par.hpp"
 #include "wt_int_par.hpp"
+#include "wt_huff_par.hpp"
+#include "fm_index_par.hpp"
 
 static void dump(const std::string& path, const void* p, size_t bytes) {
   FILE* f = std::fopen(path.c_str(), "wb");
@@ -89,6 +91,42 @@ int main(int argc, char** argv) {
       res.push_back(sel);
     }
     dump(out + "/wtint.bin", res.data(), res.size() * 8);
+  }
+  // ---- FMIndex() + wt_huff<> over the BWT (fm-index/par/WT.C:33-45)
+  {
+    const uint64_t n = 200003;
+    std::vector<unsigned char> t(n);
+    wtg::check(wtg_gen_dna(14, n, t.data()));
+    sdsl::int_vector<8> bwt(n);
+    FMIndex(t.data(), (long)n, bwt);
+    std::vector<unsigned char> b(n);
+    for (uint64_t i = 0; i < n; i++) b[i] = (unsigned char)bwt[i];
+    dump(out + "/fm_bwt.bin", b.data(), n);
+    sdsl::wt_huff<> wt(bwt, n);
+    const unsigned char cs[6] = {'A', 'C', 'G', 'T', 'N', 0};
+    std::vector<uint64_t> res;
+    res.push_back(wt.sigma);
+    for (uint64_t i = 0; i < n; i += 101) {
+      const unsigned char c = cs[(i / 101) % 6];
+      res.push_back(wt[i]);
+      res.push_back(wt.rank(i, c));
+      auto is = wt.inverse_select(i);
+      res.push_back(is.first);
+      res.push_back(is.second);
+      uint64_t sel;
+      try { sel = wt.select(1 + (i % 5000), c); } catch (const std::logic_error&) { sel = UINT64_MAX; }
+      res.push_back(sel);
+    }
+    dump(out + "/huff.bin", res.data(), res.size() * 8);
+    // host backward-search counts through the fm_index wrapper
+    wtg::fm_index idx(t.data(), n);
+    const uint32_t m = 12;
+    const uint64_t np = 300;
+    std::vector<unsigned char> pats(np * m);
+    wtg::check(wtg_gen_patterns(15, t.data(), n, m, np, pats.data()));
+    std::vector<uint64_t> cnt(np);
+    for (uint64_t q = 0; q < np; q++) cnt[q] = idx.count(pats.data() + q * m, m);
+    dump(out + "/fm_count.bin", cnt.data(), np * 8);
   }
   std::printf("dropin ok\n");
   return 0;

@@ -72,3 +72,25 @@ def test_dropin_driver(wsfm, orc, variant, dt):
             occ = np.nonzero(sv == c)[0]
             j = 1 + (i % 7)
             assert sel == (occ[j - 1] if j <= occ.size else orc.NONE)
+        # FMIndex() + wt_huff<>
+        n = 200003
+        t = wsfm.gen_dna(14, n)
+        sa = wsfm.suffix_array(t)
+        bwt = orc.bwt_from_sa(t, sa)
+        assert np.array_equal(load(d + "/fm_bwt.bin", np.uint8), bwt)
+        res = load(d + "/huff.bin")
+        assert res[0] == len(np.unique(bwt))
+        rows = res[1:].reshape(-1, 5)
+        cs = [65, 67, 71, 84, 78, 0]
+        for k, i in enumerate(range(0, n, 101)):
+            c = cs[k % 6]
+            acc, rk, isr, isc, sel = (int(x) for x in rows[k])
+            assert acc == bwt[i] and isc == bwt[i]
+            if k % 40 == 0:
+                assert rk == int((bwt[:i] == c).sum())
+                assert isr == int((bwt[:i] == bwt[i]).sum())
+                occ = np.nonzero(bwt == c)[0]
+                j = 1 + (i % 5000)
+                assert sel == (occ[j - 1] if j <= occ.size else orc.NONE)
+        pats = wsfm.gen_patterns(15, t, 12, 300)
+        assert np.array_equal(load(d + "/fm_count.bin"), orc.sa_count(t, sa, pats))

@@ -37,6 +37,15 @@ class Select(C.Structure):
                 ("long_width", vp), ("long_pool", vp)]
 
 
+class HuffShape(C.Structure):
+    _fields_ = [("n", u64), ("bv_size", u64), ("sigma", u32), ("n_nodes", u32),
+                ("n_inner", u32), ("max_depth", u32), ("C", u64 * 257),
+                ("bv_pos", u64 * 511), ("bv_pos_rank", u64 * 511),
+                ("child", (C.c_int32 * 2) * 511), ("parent", C.c_int32 * 511),
+                ("sym", C.c_int32 * 511), ("c_to_leaf", C.c_int32 * 256),
+                ("path", u64 * 256), ("path_len", u32 * 256)]
+
+
 # name -> (restype, argtypes); every symbol here is declared in include/wtg.h
 SIGNATURES = {
     "wtg_version": (C.c_char_p, []),
@@ -78,10 +87,24 @@ SIGNATURES = {
     "wtg_wtint_access_batch": (i32, [vp, vp, vp, u64, vp]),
     "wtg_wtint_select_batch": (i32, [vp, vp, vp, vp, u64, vp]),
     "wtg_wtint_inverse_select_batch": (i32, [vp, vp, vp, u64, vp, vp]),
+    "wtg_huff_build": (i32, [vp, vp, u64, C.POINTER(vp)]),
+    "wtg_huff_destroy": (i32, [vp, vp]),
+    "wtg_huff_get_shape": (i32, [vp, C.POINTER(HuffShape)]),
+    "wtg_huff_data": (i32, [vp, C.POINTER(vp), C.POINTER(vp)]),
+    "wtg_huff_select1": (C.POINTER(Select), [vp]),
+    "wtg_huff_select0": (C.POINTER(Select), [vp]),
+    "wtg_huff_rank_batch": (i32, [vp, vp, vp, vp, u64, vp]),
+    "wtg_huff_access_batch": (i32, [vp, vp, vp, u64, vp, vp]),
+    "wtg_huff_select_batch": (i32, [vp, vp, vp, vp, u64, vp]),
+    "wtg_suffix_array": (i32, [vp, u64, vp]),
+    "wtg_bwt_from_sa": (i32, [vp, vp, u64, vp]),
+    "wtg_fm_count_batch": (i32, [vp, vp, vp, u32, u64, vp]),
     "wtg_splitmix64": (u64, [U64P]),
     "wtg_gen_uniform": (i32, [u64, u64, u64, i32, vp]),
     "wtg_gen_geometric": (i32, [u64, u64, C.c_double, u64, i32, vp]),
     "wtg_gen_zipf_bytes": (i32, [u64, u64, C.c_double, vp]),
+    "wtg_gen_dna": (i32, [u64, u64, vp]),
+    "wtg_gen_patterns": (i32, [u64, vp, u64, u32, u64, vp]),
 }
 
 
@@ -138,6 +161,37 @@ def gen_geometric(seed, n, p, cap, dtype=np.uint32):
 def gen_zipf_bytes(seed, n, s=1.1):
     out = np.empty(n, dtype=np.uint8)
     _check(lib().wtg_gen_zipf_bytes(seed, n, s, _p(out)))
+    return out
+
+
+def gen_dna(seed, n):
+    """n bytes of order-5 Markov DNA with repeats and N's, text[n-1] = 0."""
+    out = np.empty(n, dtype=np.uint8)
+    _check(lib().wtg_gen_dna(seed, n, _p(out)))
+    return out
+
+
+def gen_patterns(seed, text, m, npat):
+    """npat x m patterns: even rows exact substrings, odd rows 1-2 substitutions."""
+    text = np.ascontiguousarray(text, dtype=np.uint8)
+    out = np.empty((npat, m), dtype=np.uint8)
+    _check(lib().wtg_gen_patterns(seed, _p(text), text.size, m, npat, _p(out)))
+    return out
+
+
+def suffix_array(text):
+    """Suffix array of a byte text (host, SA-IS)."""
+    text = np.ascontiguousarray(text, dtype=np.uint8)
+    sa = np.empty(text.size, dtype=np.int64)
+    _check(lib().wtg_suffix_array(_p(text), text.size, _p(sa)))
+    return sa
+
+
+def bwt_from_sa(text, sa):
+    text = np.ascontiguousarray(text, dtype=np.uint8)
+    sa = np.ascontiguousarray(sa, dtype=np.int64)
+    out = np.empty(text.size, dtype=np.uint8)
+    _check(lib().wtg_bwt_from_sa(_p(text), _p(sa), text.size, _p(out)))
     return out
 
 
@@ -343,6 +397,90 @@ class Context:
         _check(lib().wtg_wtint_build(self.h, d_sym.ptr, d_sym.dtype.itemsize, int(n),
                                      int(max_level), C.byref(h)))
         return WtInt(self, h)
+
+    # ---- wt_huff / FM
+    def huff_build(self, d_text, n=None):
+        h = C.c_void_p()
+        n = d_text.size if n is None else n
+        _check(lib().wtg_huff_build(self.h, d_text.ptr, int(n), C.byref(h)))
+        return WtHuff(self, h)
+
+
+class WtHuff:
+    def __init__(self, ctx, h):
+        self.ctx, self.h = ctx, h
+        self.shape = HuffShape()
+        _check(lib().wtg_huff_get_shape(h, C.byref(self.shape)))
+        w, d = C.c_void_p(), C.c_void_p()
+        _check(lib().wtg_huff_data(h, C.byref(w), C.byref(d)))
+        self.words_ptr, self.dir_ptr = w, d
+        self.n, self.nbits = int(self.shape.n), int(self.shape.bv_size)
+
+    def words(self):
+        nw = self.nbits // 64 + 1
+        a = np.empty(nw, np.uint64)
+        if self.words_ptr.value:
+            _check(lib().wtg_memcpy_d2h(self.ctx.h, _p(a), self.words_ptr, a.nbytes))
+        else:
+            a[:] = 0
+        return a
+
+    def directory(self):
+        nd = lib().wtg_rank9_words(self.nbits)
+        a = np.empty(nd, np.uint64)
+        _check(lib().wtg_memcpy_d2h(self.ctx.h, _p(a), self.dir_ptr, a.nbytes))
+        return a
+
+    def select1(self):
+        return SelectHandle(self.ctx, lib().wtg_huff_select1(self.h).contents, owned=False)
+
+    def select0(self):
+        return SelectHandle(self.ctx, lib().wtg_huff_select0(self.h).contents, owned=False)
+
+    def _q(self, fn, arrs, dts, n_out=1, null_out=False):
+        ctx = self.ctx
+        nq = arrs[0].size
+        d_in = [ctx.to_device(np.ascontiguousarray(a, dtype=dt)) for a, dt in zip(arrs, dts)]
+        d_out = [ctx.empty(nq, np.uint64) for _ in range(n_out)]
+        try:
+            outs = [d.ptr for d in d_out] + ([None] if null_out else [])
+            _check(fn(ctx.h, self.h, *[d.ptr for d in d_in], nq, *outs))
+            res = [d.get() for d in d_out]
+        finally:
+            for d in d_in + d_out:
+                d.free()
+        return res[0] if n_out == 1 else res
+
+    def rank(self, i, c):
+        return self._q(lib().wtg_huff_rank_batch, [i, c], [np.uint64, np.uint32])
+
+    def access(self, i):
+        return self._q(lib().wtg_huff_access_batch, [i], [np.uint64], null_out=True)
+
+    def inverse_select(self, i):
+        sym, rk = self._q(lib().wtg_huff_access_batch, [i], [np.uint64], n_out=2)
+        return rk, sym
+
+    def select(self, i, c):
+        return self._q(lib().wtg_huff_select_batch, [i, c], [np.uint64, np.uint32])
+
+    def fm_count(self, pats):
+        """Backward-search counts; pats: (npat, m) uint8."""
+        pats = np.ascontiguousarray(pats, dtype=np.uint8)
+        npat, m = pats.shape
+        ctx = self.ctx
+        dp = ctx.to_device(pats.reshape(-1))
+        do = ctx.empty(npat, np.uint64)
+        try:
+            _check(lib().wtg_fm_count_batch(ctx.h, self.h, dp.ptr, m, npat, do.ptr))
+            return do.get()
+        finally:
+            dp.free(); do.free()
+
+    def free(self):
+        if self.h is not None:
+            _check(lib().wtg_huff_destroy(self.ctx.h, self.h))
+            self.h = None
 
 
 class WtInt:

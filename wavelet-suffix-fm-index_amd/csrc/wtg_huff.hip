@@ -1,0 +1,785 @@
+// wtg_huff.hip -- sdsl::wt_huff<> over a byte text on the GPU, its batched
+// queries, and FM-index backward-search count over the wt_huff of a BWT.
+//
+// Construction (wt_huff/wt_pc_par.hpp:321-361; the reference fills the bits by
+// recursive stable partitions, build_recursive :193-257).  Here the text is read
+// three times and the bits written once, with no per-level data movement:
+//   1. k_huff_hist   byte counts -> host builds the Huffman shape (a few us)
+//   2. k_huff_count  per (inner node v, segment s) count of text positions whose
+//                    code passes through v.  Segment = one wave's S consecutive
+//                    bytes.  Table laid out [v][s] with inner nodes in BFS order,
+//                    so ONE flat exclusive scan of it gives each (v, s) run's
+//                    global bit position: bv_pos[v] is the sum of the inner
+//                    nodes before v, the prefix within row v the bits of v before
+//                    segment s.
+//   3. k_huff_emit   each wave walks its segment in 64-byte chunks, level by
+//                    level; per (chunk, level) the lanes are grouped by node with
+//                    ballots, each group's bits compacted into one word (lane
+//                    scatter with ds_permute + ballot) and appended to the node's
+//                    run in a per-wave LDS staging area; at the segment end every
+//                    run is written out as whole words (edge words OR-ed).
+// Then rank9 + select1 + select0 over the bits, and the per-node rank bases.
+#include "wtg_internal.hpp"
+#include <cstring>
+#include <queue>
+#include <vector>
+
+namespace {
+constexpr int HF_THREADS = 256;          // 4 waves per block
+constexpr int HF_PIECE = 1024;           // bytes per wave per piece (16 per lane)
+constexpr uint32_t HF_LEAF = 0x100u;     // child value >= 0x100: leaf, symbol = value - 0x100
+
+// Device copy of the shape.  Inner nodes are indexed 0..n_inner-1 in BFS
+// order (the order of their bv_pos).
+struct HuffTab {
+  uint64_t code[256];        // code bits of each byte, bit l = branch at depth l
+  uint64_t bvpos[256];       // per inner node
+  uint64_t rbase[256];       // rank1 at bvpos (after the rank directory is built)
+  uint64_t C[257];           // C[c] = #bytes < c
+  uint16_t child[256][2];    // per inner node: inner index or HF_LEAF + symbol
+  uint8_t len[256];          // code length, 0 = absent
+  uint32_t n_inner, max_depth, sigma, single;  // single: the symbol when sigma == 1
+};
+}  // namespace
+
+struct wtg_huff {
+  wtg_huff_shape shape{};
+  HuffTab tab{};
+  HuffTab* d_tab = nullptr;
+  uint64_t* words = nullptr;   // ceil(bv_size/64)+1 words
+  uint64_t* dir = nullptr;
+  wtg_select s1{}, s0{};
+  uint64_t* d_hist = nullptr;  // 256 counts
+};
+
+// ------------------------------------------------------------------ kernels
+namespace {
+
+// 16 text bytes at p (16-byte vector load when the text is 16-byte aligned)
+__device__ __forceinline__ uint4 load16(const uint8_t* __restrict__ text, uint64_t n, uint64_t p, bool aligned) {
+  uint4 v;
+  if (aligned && p + 16 <= n) {
+    v = *reinterpret_cast<const uint4*>(text + p);
+  } else {
+    uint8_t b[16];
+#pragma unroll
+    for (int k = 0; k < 16; k++) b[k] = p + k < n ? text[p + k] : 0;
+    memcpy(&v, b, 16);
+  }
+  return v;
+}
+
+__device__ __forceinline__ void wave_sync_lds() {
+  __builtin_amdgcn_fence(__ATOMIC_RELEASE, "wavefront");
+  __builtin_amdgcn_wave_barrier();
+  __builtin_amdgcn_fence(__ATOMIC_ACQUIRE, "wavefront");
+}
+
+// byte histogram: per chunk of 64 bytes, lanes with equal bytes are counted
+// together (8 rounds of ballots), the rest by LDS atomics
+__global__ __launch_bounds__(HF_THREADS) void k_huff_hist(const uint8_t* __restrict__ text, uint64_t n,
+                                                          unsigned long long* __restrict__ hist, bool aligned) {
+  __shared__ uint32_t s_h[256];
+  for (int k = threadIdx.x; k < 256; k += HF_THREADS) s_h[k] = 0;
+  __syncthreads();
+  const int lane = lane_id();
+  const uint64_t waves = (uint64_t)gridDim.x * (HF_THREADS / 64);
+  const uint64_t pieces = (n + HF_PIECE - 1) / HF_PIECE;
+  for (uint64_t pc = (uint64_t)blockIdx.x * (HF_THREADS / 64) + (threadIdx.x >> 6); pc < pieces; pc += waves) {
+    const uint64_t base = pc * HF_PIECE + (uint64_t)lane * 16;
+    uint8_t b[16];
+    const uint4 v = load16(text, n, base, aligned);
+    memcpy(b, &v, 16);
+#pragma unroll
+    for (int k = 0; k < 16; k++) {
+      const bool valid = base + k < n;
+      const uint32_t c = b[k];
+      uint64_t rem = wave_ballot(valid);
+      for (int it = 0; it < 8 && rem; it++) {
+        const int ld = __builtin_ctzll(rem);
+        const uint32_t s = __builtin_amdgcn_readlane(c, ld);
+        const uint64_t m = wave_ballot(valid && c == s);
+        rem &= ~m;
+        if (lane == ld) atomicAdd(&s_h[s], popc64(m));
+      }
+      if ((rem >> lane) & 1ull) atomicAdd(&s_h[c], 1u);
+    }
+  }
+  __syncthreads();
+  for (int k = threadIdx.x; k < 256; k += HF_THREADS)
+    if (s_h[k]) atomicAdd(&hist[k], (unsigned long long)s_h[k]);
+}
+
+// per-node values held across the wave: lane j keeps node j + 64q in a[q]
+struct NodeRegs {
+  uint32_t a[4];
+  __device__ __forceinline__ uint32_t get(uint32_t v) const {
+    const uint32_t q = v >> 6;
+    const uint32_t x = q == 0 ? a[0] : q == 1 ? a[1] : q == 2 ? a[2] : a[3];
+    return (uint32_t)__builtin_amdgcn_readlane((int)x, (int)(v & 63));
+  }
+  __device__ __forceinline__ void add(uint32_t v, uint32_t d, int lane) {
+    if ((uint32_t)lane == (v & 63)) {
+      const uint32_t q = v >> 6;
+      if (q == 0) a[0] += d; else if (q == 1) a[1] += d; else if (q == 2) a[2] += d; else a[3] += d;
+    }
+  }
+};
+
+struct HuffLds {
+  uint64_t code[256];
+  uint16_t child[256][2];
+  uint8_t len[256];
+};
+
+__device__ __forceinline__ void load_tab(HuffLds& L, const HuffTab* __restrict__ T) {
+  for (int k = threadIdx.x; k < 256; k += blockDim.x) {
+    L.code[k] = T->code[k];
+    L.len[k] = T->len[k];
+    L.child[k][0] = T->child[k][0];
+    L.child[k][1] = T->child[k][1];
+  }
+}
+
+// Walks one chunk of 64 bytes (lane = text order) down the tree.  For every
+// (level, node) group calls f(v, m, bits) with m = the group's lanes and bits =
+// ballot of their branch bits.
+template <typename F>
+__device__ __forceinline__ void walk_chunk(const HuffLds& L, uint32_t c, bool valid, F&& f) {
+  const uint32_t len = valid ? L.len[c] : 0u;
+  const uint64_t code = L.code[c];
+  uint32_t node = 0;
+  for (uint32_t l = 0;; l++) {
+    const bool act = l < len;
+    const uint64_t am = wave_ballot(act);
+    if (!am) break;
+    const uint32_t bit = (uint32_t)(code >> l) & 1u;
+    const uint64_t bb = wave_ballot(act && bit);
+    uint64_t rem = am;
+    while (rem) {
+      const int ld = __builtin_ctzll(rem);
+      const uint32_t v = (uint32_t)__builtin_amdgcn_readlane((int)node, ld);
+      const uint64_t m = (rem == am && l == 0) ? am : wave_ballot(act && node == v);
+      rem &= ~m;
+      f(v, m, bb, bit);
+    }
+    if (act) node = L.child[node][bit];   // leaves drop out at l+1 == len
+  }
+}
+
+struct SegArgs {
+  const uint8_t* text;
+  uint64_t n;
+  uint32_t S;            // bytes per segment (multiple of HF_PIECE)
+  uint64_t nseg;
+  uint32_t n_inner;
+  uint32_t stage_words;  // per wave
+  const HuffTab* tab;
+  uint32_t* cnt;         // [n_inner][nseg]
+  const uint64_t* pos;   // [n_inner][nseg] (+1): global bit of each run
+  uint64_t* words;       // tree bits (zeroed before emit)
+  bool aligned;          // text is 16-byte aligned
+};
+
+__device__ __forceinline__ void load_piece(uint8_t* s_txt, const uint8_t* __restrict__ text, uint64_t n,
+                                           uint64_t base, int lane, bool aligned) {
+  *reinterpret_cast<uint4*>(s_txt + lane * 16) = load16(text, n, base + (uint64_t)lane * 16, aligned);
+}
+
+// pass 2: cnt[v][s]
+__global__ __launch_bounds__(HF_THREADS) void k_huff_count(SegArgs a) {
+  __shared__ HuffLds L;
+  load_tab(L, a.tab);
+  __syncthreads();
+  const int lane = lane_id();
+  const uint64_t waves = (uint64_t)gridDim.x * (HF_THREADS / 64);
+  for (uint64_t s = (uint64_t)blockIdx.x * (HF_THREADS / 64) + (threadIdx.x >> 6); s < a.nseg; s += waves) {
+    NodeRegs cnt{{0, 0, 0, 0}};
+    for (uint32_t off = 0; off < a.S; off += HF_PIECE) {
+      const uint64_t base = s * a.S + off + (uint64_t)lane * 16;
+      if (base - (uint64_t)lane * 16 >= a.n) break;
+      const uint4 wv = load16(a.text, a.n, base, a.aligned);
+      // counting is order-free: chunk k = byte k of every lane
+#pragma unroll 1
+      for (int k = 0; k < 16; k++) {
+        const uint32_t q = (uint32_t)k >> 2;
+        const uint32_t word = q == 0 ? wv.x : q == 1 ? wv.y : q == 2 ? wv.z : wv.w;
+        const uint32_t c = (word >> (8 * (k & 3))) & 0xFFu;
+        walk_chunk(L, c, base + k < a.n, [&](uint32_t v, uint64_t m, uint64_t, uint32_t) {
+          cnt.add(v, popc64(m), lane);
+        });
+      }
+    }
+#pragma unroll
+    for (int q = 0; q < 4; q++) {
+      const uint32_t v = (uint32_t)(64 * q + lane);
+      if (v < a.n_inner) a.cnt[(uint64_t)v * a.nseg + s] = cnt.a[q];
+    }
+  }
+}
+
+// 64 bits of the staging area starting at local bit d (d may be negative: zeros)
+__device__ __forceinline__ uint64_t stage_bits(const uint64_t* stg, int64_t d) {
+  if (d < 0) return d <= -64 ? 0ull : stg[0] << (uint32_t)(-d);
+  const uint64_t i = (uint64_t)d >> 6;
+  const uint32_t sh = (uint32_t)(d & 63);
+  return sh ? (stg[i] >> sh) | (stg[i + 1] << (64 - sh)) : stg[i];
+}
+
+// pass 3: tree bits
+__global__ __launch_bounds__(HF_THREADS) void k_huff_emit(SegArgs a) {
+  __shared__ HuffLds L;
+  extern __shared__ uint64_t s_dyn[];
+  load_tab(L, a.tab);
+  __syncthreads();
+  const int lane = lane_id(), w = threadIdx.x >> 6;
+  uint64_t* stg = s_dyn + (size_t)w * (a.stage_words + HF_PIECE / 8);
+  uint8_t* s_txt = reinterpret_cast<uint8_t*>(stg + a.stage_words);
+  const uint64_t waves = (uint64_t)gridDim.x * (HF_THREADS / 64);
+  for (uint64_t s = (uint64_t)blockIdx.x * (HF_THREADS / 64) + w; s < a.nseg; s += waves) {
+    wave_sync_lds();
+    // local run starts: exclusive scan of this segment's node counts
+    NodeRegs lstart{{0, 0, 0, 0}}, run{{0, 0, 0, 0}};
+    uint32_t carry = 0;
+#pragma unroll
+    for (int q = 0; q < 4; q++) {
+      const uint32_t v = (uint32_t)(64 * q + lane);
+      const uint32_t c = v < a.n_inner ? a.cnt[(uint64_t)v * a.nseg + s] : 0u;
+      const uint32_t inc = wave_incl_scan(c);
+      lstart.a[q] = carry + inc - c;
+      carry += (uint32_t)__shfl(inc, 63);
+    }
+    const uint32_t used = (carry + 63) / 64 + 2;
+    for (uint32_t k = lane; k < used; k += 64) stg[k] = 0;
+    for (uint32_t off = 0; off < a.S; off += HF_PIECE) {
+      const uint64_t pbase = s * a.S + off;
+      if (pbase >= a.n) break;
+      wave_sync_lds();
+      load_piece(s_txt, a.text, a.n, pbase, lane, a.aligned);
+      wave_sync_lds();
+#pragma unroll 1
+      for (int k = 0; k < HF_PIECE / 64; k++) {
+        const uint64_t p = pbase + (uint64_t)k * 64 + lane;
+        const uint32_t c = s_txt[k * 64 + lane];
+        walk_chunk(L, c, p < a.n, [&](uint32_t v, uint64_t m, uint64_t bb, uint32_t bit) {
+          const uint32_t cm = popc64(m);
+          uint64_t W;
+          if (m == ~0ull) {
+            W = bb;
+          } else {
+            // lanes of m go to their rank, the others behind them: a permutation
+            const bool in = (m >> lane) & 1ull;
+            const uint32_t dst = in ? mbcnt64(m) : cm + mbcnt64(~m);
+            const int got = __builtin_amdgcn_ds_permute((int)(dst * 4), (int)bit);
+            W = wave_ballot((uint32_t)lane < cm && got);
+          }
+          const uint32_t at = lstart.get(v) + run.get(v);
+          run.add(v, cm, lane);
+          if (lane == 0) {
+            const uint32_t wi = at >> 6, sh = at & 63;
+            stg[wi] |= W << sh;
+            if (sh && sh + cm > 64) stg[wi + 1] |= W >> (64 - sh);
+          }
+        });
+      }
+    }
+    wave_sync_lds();
+    // write every run of the segment as whole words
+    for (uint32_t v = 0; v < a.n_inner; v++) {
+      const uint32_t len = run.get(v);
+      if (!len) continue;
+      const uint64_t g0 = a.pos[(uint64_t)v * a.nseg + s];
+      const int64_t ls = (int64_t)lstart.get(v);
+      const uint64_t w0 = g0 >> 6, w1 = (g0 + len - 1) >> 6;
+      for (uint64_t kw = w0 + lane; kw <= w1; kw += 64) {
+        const uint64_t lo = kw * 64 > g0 ? kw * 64 : g0;
+        const uint64_t hi = kw * 64 + 64 < g0 + len ? kw * 64 + 64 : g0 + len;
+        const uint32_t nb = (uint32_t)(hi - lo), at = (uint32_t)(lo - kw * 64);
+        const uint64_t mask = nb == 64 ? ~0ull : ((1ull << nb) - 1) << at;
+        const uint64_t val = stage_bits(stg, ls + (int64_t)(kw * 64) - (int64_t)g0) & mask;
+        if (nb == 64) a.words[kw] = val;
+        else if (val) atomicOr(reinterpret_cast<unsigned long long*>(&a.words[kw]), (unsigned long long)val);
+      }
+    }
+  }
+}
+
+__global__ void k_huff_rbase(HuffTab* tab, const uint64_t* __restrict__ D, const uint64_t* __restrict__ words) {
+  const uint32_t v = blockIdx.x * blockDim.x + threadIdx.x;
+  if (v < tab->n_inner) tab->rbase[v] = rank9_dev(D, words, tab->bvpos[v]);
+}
+
+// ------------------------------------------------------------------ queries
+struct QLds {
+  uint64_t code[256];
+  uint64_t bvpos[256];
+  uint64_t rbase[256];
+  uint16_t child[256][2];
+  uint8_t len[256];
+};
+__device__ __forceinline__ void load_q(QLds& Q, const HuffTab* __restrict__ T) {
+  for (int k = threadIdx.x; k < 256; k += blockDim.x) {
+    Q.code[k] = T->code[k];
+    Q.bvpos[k] = T->bvpos[k];
+    Q.rbase[k] = T->rbase[k];
+    Q.child[k][0] = T->child[k][0];
+    Q.child[k][1] = T->child[k][1];
+    Q.len[k] = T->len[k];
+  }
+}
+
+struct HQ {
+  const HuffTab* tab;
+  const uint64_t* words;
+  const uint64_t* D;
+  SelQ s1, s0;
+  uint64_t n;
+  uint32_t sigma, single;
+};
+
+// rank_c(i) by walking c's code (wt_pc_par.hpp:462-485)
+__device__ __forceinline__ uint64_t huff_rank(const QLds& Q, const HQ& h, uint64_t i, uint32_t c) {
+  const uint32_t len = Q.len[c];
+  if (!len) return h.sigma == 1 && c == h.single ? i : 0;
+  uint64_t code = Q.code[c];
+  uint32_t v = 0;
+  for (uint32_t l = 0; l < len && i; l++, code >>= 1) {
+    const uint64_t r = rank9_dev(h.D, h.words, Q.bvpos[v] + i) - Q.rbase[v];
+    const uint32_t b = (uint32_t)code & 1u;
+    i = b ? r : i - r;
+    v = Q.child[v][b];
+  }
+  return i;
+}
+
+__global__ __launch_bounds__(256) void k_huff_rank(HQ h, const uint64_t* __restrict__ qi,
+                                                   const uint32_t* __restrict__ qc, uint64_t nq,
+                                                   uint64_t* __restrict__ out) {
+  __shared__ QLds Q;
+  load_q(Q, h.tab);
+  __syncthreads();
+  for (uint64_t q = (uint64_t)blockIdx.x * blockDim.x + threadIdx.x; q < nq;
+       q += (uint64_t)gridDim.x * blockDim.x) {
+    const uint64_t i = qi[q];
+    const uint32_t c = qc[q];
+    out[q] = i > h.n ? WTG_NONE : c > 255 ? 0 : huff_rank(Q, h, i, c);
+  }
+}
+
+// operator[] / inverse_select (wt_pc_par.hpp:430-448, :498-516)
+__global__ __launch_bounds__(256) void k_huff_access(HQ h, const uint64_t* __restrict__ qi, uint64_t nq,
+                                                     uint64_t* __restrict__ out_sym,
+                                                     uint64_t* __restrict__ out_rank) {
+  __shared__ QLds Q;
+  load_q(Q, h.tab);
+  __syncthreads();
+  for (uint64_t q = (uint64_t)blockIdx.x * blockDim.x + threadIdx.x; q < nq;
+       q += (uint64_t)gridDim.x * blockDim.x) {
+    uint64_t i = qi[q];
+    uint64_t sym = WTG_NONE, rk = WTG_NONE;
+    if (i < h.n) {
+      if (h.sigma == 1) {
+        sym = h.single;
+        rk = i;
+      } else {
+        uint32_t v = 0;
+        for (;;) {
+          const uint64_t p = Q.bvpos[v] + i;
+          const uint64_t r = rank9_dev(h.D, h.words, p) - Q.rbase[v];
+          const uint32_t b = (uint32_t)((h.words[p >> 6] >> (p & 63)) & 1ull);
+          i = b ? r : i - r;
+          const uint32_t nx = Q.child[v][b];
+          if (nx >= HF_LEAF) { sym = nx - HF_LEAF; break; }
+          v = nx;
+        }
+        rk = i;
+      }
+    }
+    out_sym[q] = sym;
+    if (out_rank) out_rank[q] = rk;
+  }
+}
+
+// select(i, c): from the leaf up (wt_pc_par.hpp:527-554).  The inner node at
+// depth l of c's path is found by walking the code from the root.
+__global__ __launch_bounds__(256) void k_huff_select(HQ h, const uint64_t* __restrict__ qi,
+                                                     const uint32_t* __restrict__ qc, uint64_t nq,
+                                                     uint64_t* __restrict__ out) {
+  __shared__ QLds Q;
+  __shared__ uint64_t s_C[257];
+  load_q(Q, h.tab);
+  for (int k = threadIdx.x; k < 257; k += blockDim.x) s_C[k] = h.tab->C[k];
+  __syncthreads();
+  for (uint64_t q = (uint64_t)blockIdx.x * blockDim.x + threadIdx.x; q < nq;
+       q += (uint64_t)gridDim.x * blockDim.x) {
+    const uint64_t i = qi[q];
+    const uint32_t c = qc[q];
+    uint64_t res = WTG_NONE;
+    if (c <= 255 && i >= 1 && i <= s_C[c + 1] - s_C[c]) {
+      if (h.sigma == 1) {
+        res = i - 1;
+      } else {
+        const uint32_t len = Q.len[c];
+        const uint64_t code = Q.code[c];
+        res = i - 1;
+        for (int l = (int)len - 1; l >= 0; l--) {
+          uint32_t v = 0;
+          for (int t = 0; t < l; t++) v = Q.child[v][(code >> t) & 1u];
+          const uint64_t bp = Q.bvpos[v], br = Q.rbase[v];
+          if ((code >> l) & 1u) res = select_plain(h.s1, br + res + 1) - bp;
+          else res = select_plain(h.s0, bp - br + res + 1) - bp;
+        }
+      }
+    }
+    out[q] = res;
+  }
+}
+
+// FM-index count: backward search, [sp, ep) <- C[c] + rank_c([sp, ep)) from the
+// last pattern byte; rank_c of both ends walks the same nodes
+__global__ __launch_bounds__(256) void k_fm_count(HQ h, const uint8_t* __restrict__ pats, uint32_t m,
+                                                  uint64_t npat, uint64_t* __restrict__ out) {
+  __shared__ QLds Q;
+  __shared__ uint64_t s_C[257];
+  load_q(Q, h.tab);
+  for (int k = threadIdx.x; k < 257; k += blockDim.x) s_C[k] = h.tab->C[k];
+  __syncthreads();
+  for (uint64_t q = (uint64_t)blockIdx.x * blockDim.x + threadIdx.x; q < npat;
+       q += (uint64_t)gridDim.x * blockDim.x) {
+    const uint8_t* P = pats + q * m;
+    uint64_t sp = 0, ep = h.n;
+    for (int j = (int)m - 1; j >= 0 && sp < ep; j--) {
+      const uint32_t c = P[j];
+      const uint32_t len = Q.len[c];
+      if (!len && !(h.sigma == 1 && c == h.single)) { sp = ep = 0; break; }
+      uint64_t a = sp, b = ep;
+      uint64_t code = Q.code[c];
+      uint32_t v = 0;
+      for (uint32_t l = 0; l < len && a < b; l++, code >>= 1) {
+        const uint64_t bp = Q.bvpos[v], rb = Q.rbase[v];
+        const uint64_t ra = rank9_dev(h.D, h.words, bp + a) - rb;
+        const uint64_t rbv = rank9_dev(h.D, h.words, bp + b) - rb;
+        const uint32_t bit = (uint32_t)code & 1u;
+        if (bit) { a = ra; b = rbv; } else { a -= ra; b -= rbv; }
+        v = Q.child[v][bit];
+      }
+      if (a >= b) { sp = ep = 0; break; }
+      sp = s_C[c] + a;
+      ep = s_C[c] + b;
+    }
+    out[q] = ep > sp ? ep - sp : 0;
+  }
+}
+
+// --------------------------------------------------------------- host side
+// Huffman shape (wt_huff/wt_huff_par.hpp:92-117) and the BFS layout of the
+// byte tree; fills the public shape and the device table
+void build_shape(const uint64_t* cnt, uint64_t n, wtg_huff_shape& S, HuffTab& T) {
+  std::memset(&S, 0, sizeof(S));
+  std::memset(&T, 0, sizeof(T));
+  S.n = n;
+  for (int c = 0; c < 256; c++) S.C[c + 1] = S.C[c] + cnt[c];
+  for (int c = 0; c < 256; c++) { S.c_to_leaf[c] = -1; T.C[c] = S.C[c]; }
+  T.C[256] = S.C[256];
+  for (int v = 0; v < WTG_HUFF_MAX_NODES; v++) {
+    S.child[v][0] = S.child[v][1] = -1;
+    S.parent[v] = -1;
+    S.sym[v] = -1;
+  }
+  struct Tmp { uint64_t f; int sym, c0, c1; };
+  std::vector<Tmp> t;
+  typedef std::pair<uint64_t, uint64_t> PII;  // (freq, node id), smallest first
+  std::priority_queue<PII, std::vector<PII>, std::greater<PII>> pq;
+  for (int c = 0; c < 256; c++)
+    if (cnt[c]) { pq.push(PII(cnt[c], t.size())); t.push_back(Tmp{cnt[c], c, -1, -1}); }
+  S.sigma = (uint32_t)t.size();
+  if (t.empty()) return;
+  while (pq.size() > 1) {
+    const PII a = pq.top(); pq.pop();
+    const PII b = pq.top(); pq.pop();
+    pq.push(PII(a.first + b.first, t.size()));
+    t.push_back(Tmp{a.first + b.first, -1, (int)a.second, (int)b.second});
+  }
+  // breadth-first numbering from the root, child[0] first
+  std::vector<int> order(1, (int)t.size() - 1), id(t.size(), -1);
+  id[t.size() - 1] = 0;
+  for (size_t h = 0; h < order.size(); h++) {
+    const Tmp& x = t[order[h]];
+    if (x.c0 >= 0) {
+      id[x.c0] = (int)order.size(); order.push_back(x.c0);
+      id[x.c1] = (int)order.size(); order.push_back(x.c1);
+    }
+  }
+  S.n_nodes = (uint32_t)order.size();
+  uint64_t bv = 0;
+  std::vector<int> inner_of(order.size(), -1);
+  uint32_t ni = 0;
+  for (size_t k = 0; k < order.size(); k++) {
+    const Tmp& x = t[order[k]];
+    S.bv_pos[k] = bv;
+    S.sym[k] = x.sym;
+    if (x.c0 >= 0) {
+      S.child[k][0] = id[x.c0];
+      S.child[k][1] = id[x.c1];
+      S.parent[id[x.c0]] = (int)k;
+      S.parent[id[x.c1]] = (int)k;
+      T.bvpos[ni] = bv;
+      inner_of[k] = (int)ni++;
+      bv += x.f;
+    }
+  }
+  S.bv_size = bv;
+  S.n_inner = ni;
+  T.n_inner = ni;
+  T.sigma = S.sigma;
+  for (size_t k = 0; k < order.size(); k++) {
+    if (inner_of[k] < 0) continue;
+    for (int b = 0; b < 2; b++) {
+      const int ch = S.child[k][b];
+      T.child[inner_of[k]][b] = (uint16_t)(inner_of[ch] >= 0 ? (uint32_t)inner_of[ch]
+                                                              : HF_LEAF + (uint32_t)S.sym[ch]);
+    }
+  }
+  uint32_t maxd = 0;
+  for (size_t k = 0; k < order.size(); k++) {
+    if (S.sym[k] < 0) continue;
+    const int c = S.sym[k];
+    S.c_to_leaf[c] = (int)k;
+    uint64_t path = 0;
+    uint32_t len = 0;
+    for (int x = (int)k; S.parent[x] >= 0; x = S.parent[x]) {
+      const int p = S.parent[x];
+      path = (path << 1) | (S.child[p][1] == x ? 1u : 0u);
+      len++;
+    }
+    S.path[c] = path;
+    S.path_len[c] = len;
+    T.code[c] = path;
+    T.len[c] = (uint8_t)len;
+    if (len > maxd) maxd = len;
+  }
+  S.max_depth = maxd;
+  T.max_depth = maxd;
+  if (S.sigma == 1) T.single = (uint32_t)t[0].sym;
+}
+
+HQ make_hq(const wtg_huff* wt) {
+  HQ h{};
+  h.tab = wt->d_tab;
+  h.words = wt->words;
+  h.D = wt->dir;
+  const uint64_t W = wt->shape.bv_size / 64 + 1;
+  h.s1 = SelQ{wt->s1.superblock, wt->s1.mini, wt->s1.long_index, wt->s1.long_pool, wt->words, W, 1};
+  h.s0 = SelQ{wt->s0.superblock, wt->s0.mini, wt->s0.long_index, wt->s0.long_pool, wt->words, W, 0};
+  h.n = wt->shape.n;
+  h.sigma = wt->shape.sigma;
+  h.single = wt->tab.single;
+  return h;
+}
+
+unsigned query_grid(uint64_t nq) {
+  const uint64_t b = wtg_ceil_div(nq, 256);
+  return (unsigned)(b < 4096 ? (b ? b : 1) : 4096);
+}
+}  // namespace
+
+extern "C" int wtg_huff_destroy(wtg_ctx* ctx, wtg_huff* wt) {
+  if (!wt) return WTG_OK;
+  WTG_REQUIRE(ctx, "null ctx");
+  wtg_free(ctx, wt->d_tab);
+  wtg_free(ctx, wt->words);
+  wtg_free(ctx, wt->dir);
+  wtg_free(ctx, wt->d_hist);
+  wtg_select_free(ctx, &wt->s1);
+  wtg_select_free(ctx, &wt->s0);
+  delete wt;
+  return WTG_OK;
+}
+
+extern "C" int wtg_huff_build(wtg_ctx* ctx, const uint8_t* d_text, uint64_t n, wtg_huff** out) {
+  WTG_REQUIRE(ctx && out && (n == 0 || d_text), "wtg_huff_build: null argument");
+  WTG_REQUIRE(n < (1ull << 40), "wtg_huff_build: n >= 2^40");
+  WTG_TRY(wtg_set_device(ctx));
+  *out = nullptr;
+  wtg_huff* wt = new wtg_huff();
+  int rc = WTG_OK;
+  uint64_t hist[256] = {0};
+  wt->shape.n = n;
+  if (n == 0) {  // wt_pc_par.hpp:323-324: empty tree
+    for (int c = 0; c < 256; c++) wt->shape.c_to_leaf[c] = -1;
+    *out = wt;
+    return WTG_OK;
+  }
+  if ((rc = wtg_malloc(ctx, (void**)&wt->d_hist, 256 * 8)) != WTG_OK) goto fail;
+  if ((rc = wtg_malloc(ctx, (void**)&wt->d_tab, sizeof(HuffTab))) != WTG_OK) goto fail;
+  if ((rc = wtg_fill_u64(ctx, wt->d_hist, 256, 0)) != WTG_OK) goto fail;
+  {
+    const unsigned grid = (unsigned)std::min<uint64_t>(wtg_ceil_div(n, 4 * HF_PIECE), 2048);
+    WTG_LAUNCH(ctx, "huff_hist", k_huff_hist, dim3(grid), dim3(HF_THREADS), 0, d_text, n,
+               (unsigned long long*)wt->d_hist, ((uintptr_t)d_text & 15) == 0);
+    WTG_LAUNCH_CHECK();
+  }
+  if (hipMemcpyAsync(hist, wt->d_hist, sizeof(hist), hipMemcpyDeviceToHost, ctx->stream) != hipSuccess ||
+      hipStreamSynchronize(ctx->stream) != hipSuccess) {
+    wtg_set_error("wtg_huff_build: histogram copy failed");
+    rc = WTG_ERR_HIP;
+    goto fail;
+  }
+  build_shape(hist, n, wt->shape, wt->tab);
+  if (wt->shape.max_depth > 64) {
+    wtg_set_error("wtg_huff_build: Huffman code longer than 64 bits");
+    rc = WTG_ERR_UNSUPPORTED;
+    goto fail;
+  }
+  {
+    const wtg_huff_shape& S = wt->shape;
+    const uint64_t nw = S.bv_size / 64 + 1;  // + zero padding word
+    if ((rc = wtg_malloc(ctx, (void**)&wt->words, nw * 8)) != WTG_OK) goto fail;
+    if ((rc = wtg_malloc(ctx, (void**)&wt->dir, wtg_rank9_words(S.bv_size) * 8)) != WTG_OK) goto fail;
+    if ((rc = wtg_fill_u64(ctx, wt->words, nw, 0)) != WTG_OK) goto fail;
+    if (S.n_inner > 0) {
+      // segment: 1024..4096 bytes so the count table stays <= n/4 bytes
+      uint32_t S_bytes = HF_PIECE;
+      while (S_bytes < 4 * HF_PIECE && (uint64_t)S.n_inner * 16 > S_bytes) S_bytes *= 2;
+      const uint64_t nseg = wtg_ceil_div(n, S_bytes);
+      const uint64_t entries = (uint64_t)S.n_inner * nseg;
+      const uint64_t scan_tiles = wtg_ceil_div(entries, 4096);
+      const size_t o_cnt = 0;
+      const size_t o_pos = wtg_align256(o_cnt + entries * 4);
+      const size_t o_st = wtg_align256(o_pos + (entries + 1) * 8);
+      const size_t o_ctr = wtg_align256(o_st + scan_tiles * 8);
+      const size_t total = o_ctr + 256;
+      if ((rc = wtg_ws_reserve(ctx, total)) != WTG_OK) goto fail;
+      char* ws = (char*)ctx->ws;
+      if ((rc = wtg_fill_bytes_zero(ctx, ws + o_st, total - o_st)) != WTG_OK) goto fail;
+      if (hipMemcpyAsync(wt->d_tab, &wt->tab, sizeof(HuffTab), hipMemcpyHostToDevice, ctx->stream) !=
+          hipSuccess) {
+        wtg_set_error("wtg_huff_build: table copy failed");
+        rc = WTG_ERR_HIP;
+        goto fail;
+      }
+      SegArgs a{};
+      a.text = d_text;
+      a.n = n;
+      a.S = S_bytes;
+      a.nseg = nseg;
+      a.n_inner = S.n_inner;
+      a.stage_words = (uint32_t)(((uint64_t)S_bytes * S.max_depth + 63) / 64 + 2);
+      a.tab = wt->d_tab;
+      a.cnt = (uint32_t*)(ws + o_cnt);
+      a.pos = (const uint64_t*)(ws + o_pos);
+      a.words = wt->words;
+      a.aligned = ((uintptr_t)d_text & 15) == 0;
+      const unsigned grid = (unsigned)std::min<uint64_t>(wtg_ceil_div(nseg, 4), 8192);
+      WTG_LAUNCH(ctx, "huff_count", k_huff_count, dim3(grid), dim3(HF_THREADS), 0, a);
+      WTG_LAUNCH_CHECK();
+      WTG_LAUNCH(ctx, "huff_scan", k_scan_u32<uint64_t>, dim3((unsigned)scan_tiles), dim3(256), 0,
+                 (const uint32_t*)(ws + o_cnt), (uint64_t*)(ws + o_pos), entries, (uint64_t*)(ws + o_st),
+                 (uint32_t*)(ws + o_ctr), ctx->d_flag);
+      WTG_LAUNCH_CHECK();
+      const size_t lds = (size_t)4 * (a.stage_words + HF_PIECE / 8) * 8;
+      if (lds > 150 * 1024) {
+        wtg_set_error("wtg_huff_build: staging area exceeds LDS");
+        rc = WTG_ERR_UNSUPPORTED;
+        goto fail;
+      }
+      WTG_LAUNCH(ctx, "huff_emit", k_huff_emit, dim3(grid), dim3(HF_THREADS), lds, a);
+      WTG_LAUNCH_CHECK();
+    }
+    if ((rc = wtg_rank_select_run(ctx, wt->words, S.bv_size, wt->dir, true, &wt->s1, &wt->s0)) != WTG_OK)
+      goto fail;
+    if (S.n_inner > 0) {
+      WTG_LAUNCH(ctx, "huff_rbase", k_huff_rbase, dim3(1), dim3(256), 0, wt->d_tab, wt->dir, wt->words);
+      WTG_LAUNCH_CHECK();
+      uint64_t rb[256];
+      if (hipMemcpyAsync(rb, wt->d_tab->rbase, sizeof(rb), hipMemcpyDeviceToHost, ctx->stream) != hipSuccess ||
+          hipStreamSynchronize(ctx->stream) != hipSuccess) {
+        wtg_set_error("wtg_huff_build: rank base copy failed");
+        rc = WTG_ERR_HIP;
+        goto fail;
+      }
+      std::memcpy(wt->tab.rbase, rb, sizeof(rb));
+      uint32_t ni = 0;
+      for (uint32_t v = 0; v < wt->shape.n_nodes; v++)
+        if (wt->shape.child[v][0] >= 0) wt->shape.bv_pos_rank[v] = rb[ni++];
+    } else if (hipMemcpyAsync(wt->d_tab, &wt->tab, sizeof(HuffTab), hipMemcpyHostToDevice, ctx->stream) !=
+               hipSuccess) {
+      wtg_set_error("wtg_huff_build: table copy failed");
+      rc = WTG_ERR_HIP;
+      goto fail;
+    }
+  }
+  if ((rc = wtg_finish(ctx)) != WTG_OK) goto fail;
+  *out = wt;
+  return WTG_OK;
+fail:
+  wtg_huff_destroy(ctx, wt);
+  return rc;
+}
+
+extern "C" int wtg_huff_get_shape(const wtg_huff* wt, wtg_huff_shape* out) {
+  WTG_REQUIRE(wt && out, "null argument");
+  *out = wt->shape;
+  return WTG_OK;
+}
+extern "C" int wtg_huff_data(const wtg_huff* wt, const uint64_t** d_words, const uint64_t** d_dir) {
+  WTG_REQUIRE(wt, "null wt");
+  if (d_words) *d_words = wt->words;
+  if (d_dir) *d_dir = wt->dir;
+  return WTG_OK;
+}
+extern "C" const wtg_select* wtg_huff_select1(const wtg_huff* wt) { return wt ? &wt->s1 : nullptr; }
+extern "C" const wtg_select* wtg_huff_select0(const wtg_huff* wt) { return wt ? &wt->s0 : nullptr; }
+
+static int hq_check(wtg_ctx* ctx, const wtg_huff* wt, uint64_t nq, const void* a, const void* b) {
+  WTG_REQUIRE(ctx && wt && (nq == 0 || (a && b)), "null argument");
+  if (wt->shape.n == 0 && nq) {
+    wtg_set_error("query on an empty wt_huff");
+    return WTG_ERR_ARG;
+  }
+  return WTG_OK;
+}
+
+extern "C" int wtg_huff_rank_batch(wtg_ctx* ctx, const wtg_huff* wt, const uint64_t* d_i,
+                                   const uint32_t* d_c, uint64_t nq, uint64_t* d_out) {
+  WTG_TRY(hq_check(ctx, wt, nq, d_i, d_out));
+  WTG_REQUIRE(nq == 0 || d_c, "null d_c");
+  if (!nq) return WTG_OK;
+  WTG_TRY(wtg_set_device(ctx));
+  WTG_LAUNCH(ctx, "huff_rank", k_huff_rank, dim3(query_grid(nq)), dim3(256), 0, make_hq(wt), d_i, d_c, nq,
+             d_out);
+  WTG_LAUNCH_CHECK();
+  return wtg_finish(ctx);
+}
+extern "C" int wtg_huff_access_batch(wtg_ctx* ctx, const wtg_huff* wt, const uint64_t* d_i, uint64_t nq,
+                                     uint64_t* d_sym, uint64_t* d_rank) {
+  WTG_TRY(hq_check(ctx, wt, nq, d_i, d_sym));
+  if (!nq) return WTG_OK;
+  WTG_TRY(wtg_set_device(ctx));
+  WTG_LAUNCH(ctx, "huff_access", k_huff_access, dim3(query_grid(nq)), dim3(256), 0, make_hq(wt), d_i, nq,
+             d_sym, d_rank);
+  WTG_LAUNCH_CHECK();
+  return wtg_finish(ctx);
+}
+extern "C" int wtg_huff_select_batch(wtg_ctx* ctx, const wtg_huff* wt, const uint64_t* d_i,
+                                     const uint32_t* d_c, uint64_t nq, uint64_t* d_out) {
+  WTG_TRY(hq_check(ctx, wt, nq, d_i, d_out));
+  WTG_REQUIRE(nq == 0 || d_c, "null d_c");
+  if (!nq) return WTG_OK;
+  WTG_TRY(wtg_set_device(ctx));
+  WTG_LAUNCH(ctx, "huff_select", k_huff_select, dim3(query_grid(nq)), dim3(256), 0, make_hq(wt), d_i, d_c,
+             nq, d_out);
+  WTG_LAUNCH_CHECK();
+  return wtg_finish(ctx);
+}
+extern "C" int wtg_fm_count_batch(wtg_ctx* ctx, const wtg_huff* wt, const uint8_t* d_pats, uint32_t m,
+                                  uint64_t npat, uint64_t* d_out) {
+  WTG_TRY(hq_check(ctx, wt, npat, d_pats, d_out));
+  WTG_REQUIRE(m >= 1, "pattern length 0");
+  if (!npat) return WTG_OK;
+  WTG_TRY(wtg_set_device(ctx));
+  WTG_LAUNCH(ctx, "fm_count", k_fm_count, dim3(query_grid(npat)), dim3(256), 0, make_hq(wt), d_pats, m, npat,
+             d_out);
+  WTG_LAUNCH_CHECK();
+  return wtg_finish(ctx);
+}

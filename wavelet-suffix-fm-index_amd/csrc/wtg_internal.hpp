@@ -217,10 +217,11 @@ __device__ __forceinline__ uint32_t block_excl_scan_u32(uint32_t v, uint32_t* s_
 }
 
 // ------------------------------------------- exclusive scan of a u32 array
-// out[0..n] (out[n] = total); 256 threads x 16 items per tile
-template <int DUMMY>
+// out[0..n] (out[n] = total); 256 threads x 16 items per tile; OUT = uint32_t
+// or uint64_t (the running sum is 64-bit either way)
+template <typename OUT>
 __global__ __launch_bounds__(256) void k_scan_u32(const uint32_t* __restrict__ in,
-                                                  uint32_t* __restrict__ out, uint64_t n,
+                                                  OUT* __restrict__ out, uint64_t n,
                                                   uint64_t* states, uint32_t* counter,
                                                   uint32_t* err) {
   __shared__ uint32_t s_wave[4];
@@ -247,10 +248,10 @@ __global__ __launch_bounds__(256) void k_scan_u32(const uint32_t* __restrict__ i
   uint64_t run = s_pre + ex;
 #pragma unroll
   for (int k = 0; k < 16; k++) {
-    if (base + k < n) out[base + k] = (uint32_t)run;
+    if (base + k < n) out[base + k] = (OUT)run;
     run += v[k];
   }
-  if (base < n && base + 16 >= n) out[n] = (uint32_t)run;
+  if (base < n && base + 16 >= n) out[n] = (OUT)run;
   if (n == 0 && tile == 0 && threadIdx.x == 0) out[0] = 0;
 }
 

@@ -652,7 +652,7 @@ int levels_build_t(wtg_ctx* ctx, const T* d_sym, uint64_t n, uint32_t L, uint64_
       // bits and tile zeros were produced by the previous split
       if (k == 0) WTG_TRY(launch_level<T>(ctx, la, P.tiles, 0, kvec, false));
       if (split || k == 0) {
-        WTG_LAUNCH(ctx, "scan_tiles", k_scan_u32<0>, dim3((unsigned)P.scan_tiles), dim3(256), 0, zck, zpre,
+        WTG_LAUNCH(ctx, "scan_tiles", k_scan_u32<uint32_t>, dim3((unsigned)P.scan_tiles), dim3(256), 0, zck, zpre,
                    P.tiles, scst + (size_t)k * (P.scan_tiles + 1), ctr + k, ctx->d_flag);
         WTG_LAUNCH_CHECK();
       }
@@ -819,7 +819,7 @@ static int remap_t(wtg_ctx* ctx, T* s, uint64_t n, uint64_t* sigma) {
   WTG_TRY(wtg_fill_bytes_zero(ctx, ws, o_ctr + 256));
   WTG_LAUNCH(ctx, "remap_flag", k_flag<T>, dim3(grid), dim3(256), 0, s, n, A);
   WTG_LAUNCH_CHECK();
-  WTG_LAUNCH(ctx, "remap_scan", k_scan_u32<0>, dim3((unsigned)wtg_ceil_div(na, 4096)), dim3(256), 0, A, B,
+  WTG_LAUNCH(ctx, "remap_scan", k_scan_u32<uint32_t>, dim3((unsigned)wtg_ceil_div(na, 4096)), dim3(256), 0, A, B,
              na, (uint64_t*)(ws + o_st), (uint32_t*)(ws + o_ctr), ctx->d_flag);
   WTG_LAUNCH_CHECK();
   WTG_LAUNCH(ctx, "remap_apply", k_apply_map<T>, dim3(grid), dim3(256), 0, s, n, B);

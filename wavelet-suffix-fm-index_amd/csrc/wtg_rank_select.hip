@@ -327,7 +327,7 @@ int sel_finish(wtg_ctx* ctx, const uint64_t* d_words, uint64_t nbits, wtg_select
   WTG_LAUNCH(ctx, "select_assemble", k_select_assemble, dim3((unsigned)wtg_ceil_div(sb * 64, 256)),
              dim3(256), 0, aa);
   WTG_LAUNCH_CHECK();
-  WTG_LAUNCH(ctx, "select_scan", k_scan_u32<0>, dim3((unsigned)wtg_ceil_div(sb, 4096)), dim3(256), 0,
+  WTG_LAUNCH(ctx, "select_scan", k_scan_u32<uint32_t>, dim3((unsigned)wtg_ceil_div(sb, 4096)), dim3(256), 0,
              flag, ex, sb, st, ctr, ctx->d_flag);
   WTG_LAUNCH_CHECK();
   WTG_LAUNCH(ctx, "select_longidx", k_long_index, dim3((unsigned)wtg_ceil_div(sb, 256)), dim3(256), 0,

@@ -1,0 +1,189 @@
+// wt_huff_par.hpp -- drop-in for sdsl::wt_huff<> of the reference
+// (wt_huff/wt_huff_par.hpp:68-78 over wt_huff/wt_pc_par.hpp): same include
+// guard, the constructor wt_huff(int_vector<8>& text, size) (wt_pc_par.hpp:321-361)
+// and the query surface size / empty / sigma / operator[] / rank /
+// inverse_select / select.  Counts, Huffman shape, tree bits, rank_support_v
+// and select_support_mcl (1 and 0) are built on the GPU (wtg_huff_build) and
+// copied into host supports, so scalar queries run on the host with the
+// reference's results; rank_batch / select_batch / access_batch / count_batch
+// answer device-resident batches.  select() throws std::logic_error when c does
+// not occur i times (the reference asserts).
+#ifndef INCLUDED_SDSL_WT_HUFF
+#define INCLUDED_SDSL_WT_HUFF
+#include <stdexcept>
+#include <string>
+#include <utility>
+#include <vector>
+#include "rank_support_v_par.hpp"
+#include "select_support_mcl_par.hpp"
+#include "sdsl_standin.hpp"
+#include "wtg_runtime.hpp"
+
+namespace sdsl {
+
+struct byte_tree_standin {};
+
+template <class t_bitvector = bit_vector, class t_rank = rank_support_v<1, 1>,
+          class t_select = select_support_mcl<1, 1>, class t_select_zero = select_support_mcl<0, 1>,
+          class t_tree_strat = byte_tree_standin>
+class wt_huff {
+ public:
+  typedef uint64_t size_type;
+  typedef uint8_t value_type;
+  typedef t_bitvector bit_vector_type;
+  typedef t_rank rank_1_type;
+  typedef t_select select_1_type;
+  typedef t_select_zero select_0_type;
+  enum { lex_ordered = 0 };
+
+ protected:
+  size_type m_size = 0;
+  size_type m_sigma = 0;
+  bit_vector_type m_bv;
+  rank_1_type m_bv_rank;
+  select_1_type m_bv_select1;
+  select_0_type m_bv_select0;
+  wtg_huff_shape m_shape{};
+  wtg_huff* m_dev = nullptr;
+
+ public:
+  const size_type& sigma = m_sigma;
+
+  wt_huff() = default;
+
+  template <uint8_t int_width>
+  wt_huff(int_vector<int_width>& text, size_type size) : m_size(size) {  // wt_pc_par.hpp:321-361
+    if (0 == m_size) return;
+    if (text.size() < m_size) throw std::logic_error("Stream size is smaller than size!");
+    std::vector<unsigned char> bytes(m_size);
+    for (size_type i = 0; i < m_size; i++) bytes[i] = (unsigned char)text[i];
+    build(bytes.data());
+  }
+  // host byte array (the FMIndex() drop-in uses this)
+  wt_huff(const unsigned char* text, size_type size) : m_size(size) {
+    if (m_size) build(text);
+  }
+  wt_huff(const wt_huff& o) { copy(o); }
+  wt_huff(wt_huff&& o) noexcept { take(o); }
+  wt_huff& operator=(const wt_huff& o) {
+    if (this != &o) { release(); copy(o); }
+    return *this;
+  }
+  wt_huff& operator=(wt_huff&& o) noexcept {
+    if (this != &o) { release(); take(o); }
+    return *this;
+  }
+  ~wt_huff() { release(); }
+
+  size_type size() const { return m_size; }
+  bool empty() const { return m_size == 0; }
+  const wtg_huff_shape& shape() const { return m_shape; }
+  const bit_vector_type& bits() const { return m_bv; }
+
+  value_type operator[](size_type i) const { return inverse_select(i).second; }  // :430-448
+
+  size_type rank(size_type i, value_type c) const {  // :462-485
+    if (m_shape.c_to_leaf[c] < 0) return 0;
+    if (m_sigma == 1) return i;
+    uint64_t p = m_shape.path[c];
+    int v = 0;
+    for (uint32_t l = 0; l < m_shape.path_len[c] && i; ++l, p >>= 1) {
+      const size_type r = m_bv_rank(m_shape.bv_pos[v] + i) - m_shape.bv_pos_rank[v];
+      i = (p & 1) ? r : i - r;
+      v = m_shape.child[v][p & 1];
+    }
+    return i;
+  }
+
+  std::pair<size_type, value_type> inverse_select(size_type i) const {  // :498-516
+    if (m_sigma == 1) return std::make_pair(i, (value_type)m_shape.sym[0]);
+    int v = 0;
+    while (m_shape.child[v][0] >= 0) {
+      const size_type r = m_bv_rank(m_shape.bv_pos[v] + i) - m_shape.bv_pos_rank[v];
+      const int b = m_bv[m_shape.bv_pos[v] + i];
+      i = b ? r : i - r;
+      v = m_shape.child[v][b];
+    }
+    return std::make_pair(i, (value_type)m_shape.sym[v]);
+  }
+
+  size_type select(size_type i, value_type c) const {  // :527-554
+    const int leaf = m_shape.c_to_leaf[c];
+    if (leaf < 0 || i == 0 || i > m_shape.C[c + 1] - m_shape.C[c])
+      throw std::logic_error("select(" + std::to_string(i) + "," + std::to_string((int)c) +
+                             "): c does not occur i times");
+    if (m_sigma == 1) return i - 1;
+    size_type res = i - 1;
+    int v = leaf;
+    for (uint32_t l = 0; l < m_shape.path_len[c]; ++l) {
+      const int p = m_shape.parent[v];
+      const size_type bp = m_shape.bv_pos[p], br = m_shape.bv_pos_rank[p];
+      if (m_shape.child[p][1] == v) res = m_bv_select1(br + res + 1) - bp;
+      else res = m_bv_select0(bp - br + res + 1) - bp;
+      v = p;
+    }
+    return res;
+  }
+
+  // ---- batched device queries (device buffers of nq entries)
+  void rank_batch(const uint64_t* d_i, const uint32_t* d_c, uint64_t nq, uint64_t* d_out) const {
+    wtg::check(wtg_huff_rank_batch(wtg::default_context(), m_dev, d_i, d_c, nq, d_out));
+  }
+  void select_batch(const uint64_t* d_i, const uint32_t* d_c, uint64_t nq, uint64_t* d_out) const {
+    wtg::check(wtg_huff_select_batch(wtg::default_context(), m_dev, d_i, d_c, nq, d_out));
+  }
+  void access_batch(const uint64_t* d_i, uint64_t nq, uint64_t* d_sym) const {
+    wtg::check(wtg_huff_access_batch(wtg::default_context(), m_dev, d_i, nq, d_sym, nullptr));
+  }
+  // FM-index backward-search count when this tree was built over a BWT
+  void count_batch(const uint8_t* d_pats, uint32_t m, uint64_t npat, uint64_t* d_out) const {
+    wtg::check(wtg_fm_count_batch(wtg::default_context(), m_dev, d_pats, m, npat, d_out));
+  }
+  const wtg_huff* device() const { return m_dev; }
+
+ private:
+  void build(const unsigned char* host_text) {
+    wtg::dbuf dt(m_size);
+    dt.upload(host_text, m_size);
+    wtg::check(wtg_huff_build(wtg::default_context(), (const uint8_t*)dt.get(), m_size, &m_dev));
+    wtg::check(wtg_huff_get_shape(m_dev, &m_shape));
+    m_sigma = m_shape.sigma;
+    const uint64_t nbits = m_shape.bv_size;
+    const uint64_t *dw = nullptr, *dd = nullptr;
+    wtg::check(wtg_huff_data(m_dev, &dw, &dd));
+    m_bv = bit_vector_type(nbits);
+    if (nbits) wtg::check(wtg_memcpy_d2h(wtg::default_context(), m_bv.data(), dw, (nbits + 63) / 64 * 8));
+    m_bv_rank = rank_1_type();
+    m_bv_rank.assign_directory(dd, nbits);
+    m_bv_rank.set_vector(&m_bv);
+    m_bv_select1.assign(*wtg_huff_select1(m_dev));
+    m_bv_select1.set_vector(&m_bv);
+    m_bv_select0.assign(*wtg_huff_select0(m_dev));
+    m_bv_select0.set_vector(&m_bv);
+  }
+  void release() {
+    if (m_dev) wtg_huff_destroy(wtg::default_context(), m_dev);
+    m_dev = nullptr;
+  }
+  void take(wt_huff& o) {  // move: the device copy changes owner
+    copy(o);
+    m_dev = o.m_dev;
+    o.m_dev = nullptr;
+  }
+  void copy(const wt_huff& o) {
+    m_size = o.m_size;
+    m_sigma = o.m_sigma;
+    m_shape = o.m_shape;
+    m_bv = o.m_bv;
+    m_bv_rank = o.m_bv_rank;
+    m_bv_rank.set_vector(&m_bv);
+    m_bv_select1 = o.m_bv_select1;
+    m_bv_select1.set_vector(&m_bv);
+    m_bv_select0 = o.m_bv_select0;
+    m_bv_select0.set_vector(&m_bv);
+    m_dev = nullptr;  // the copy answers host queries; batches need the original
+  }
+};
+
+}  // namespace sdsl
+#endif
