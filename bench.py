@@ -66,6 +66,55 @@ def cpu_baseline(s, L, sample_n, rounds):
                       "same input, L=%d, best of %d rounds" % (sample_n, L, rounds)}
 
 
+def measured_traffic(log2n, L):
+    """HBM bytes per level-split launch from the committed rocprofv3 PMC summary
+    (FETCH_SIZE + WRITE_SIZE, each scaled by a 1 GiB copy), for the workload it
+    was measured on; None otherwise."""
+    path = os.path.join(ROOT, "profiles", "level_split_u32_n268M_L20.md")
+    if (log2n, L) != (28, 20) or not os.path.exists(path):
+        return None
+    for line in open(path):
+        if line.startswith("traffic_bytes_per_launch(level_split) = "):
+            return float(line.split("=")[1])
+    return None
+
+
+def query_bench(pkg, ctx, dw, nbits, nq):
+    """rank_support_v + select_support_mcl (1 and 0) built over all level bits,
+    then nq uniform rank(i) and nq uniform select1(j) queries (device batches)."""
+    rng = np.random.default_rng(SEED + 1)
+    ctx.set_blocking(True)
+    for _ in range(2):  # warm
+        D, s1, s0 = ctx.rank9_select_build(dw, nbits)
+        s1.free(); s0.free(); D.free()
+    ctx.timer_start()
+    D, s1, s0 = ctx.rank9_select_build(dw, nbits)
+    build_ms = ctx.timer_stop()
+    di = ctx.to_device(rng.integers(0, nbits + 1, nq, dtype=np.uint64))
+    dj = ctx.to_device(rng.integers(1, s1.arg_cnt + 1, nq, dtype=np.uint64))
+    out = ctx.empty(nq, np.uint64)
+    res = {}
+    for name, fn in (("rank", lambda: ctx.rank9_batch(dw, D, nbits, di, out)),
+                     ("select1", lambda: ctx.select_batch(s1, dw, dj, out))):
+        fn()
+        ctx.timer_start()
+        for _ in range(3):
+            fn()
+        t = ctx.timer_stop() / 3
+        res[name + "_Mq_s"] = round(nq / (t / 1e3) / 1e6, 1)
+    # spot parity on the last select batch: rank(select1(j)+1) == j
+    sel = out.get()[:1000]
+    dq = ctx.to_device(sel + 1)
+    r = ctx.rank9_batch(dw, D, nbits, dq).get()
+    ok = bool((r == dj.get()[:1000]).all())
+    for a in (di, dj, out, dq):
+        a.free()
+    s1.free(); s0.free(); D.free()
+    res.update({"rank9_select_build_ms": round(build_ms, 3), "nbits": nbits, "queries": nq,
+                "rank_select_consistent": ok})
+    return res
+
+
 def main():
     ap = argparse.ArgumentParser()
     ap.add_argument("--gpus", type=int, default=1)
@@ -75,6 +124,11 @@ def main():
     ap.add_argument("--levels", type=int, default=20)
     ap.add_argument("--cpu-log2n", type=int, default=26)
     ap.add_argument("--no-cpu", action="store_true")
+    ap.add_argument("--calibrate-copy", action="store_true",
+                    help="first copy 1 GiB device-to-device twice (a plain copy of known size, "
+                         "the reference point for HBM byte counters)")
+    ap.add_argument("--queries", type=int, default=10_000_000,
+                    help="rank and select queries on the built tree (0: skip)")
     args = ap.parse_args()
     if args.gpus != 1:
         print("bench.py measures one GPU only: the level build of this path runs on a single "
@@ -85,6 +139,13 @@ def main():
     log("generating n=%d uint32 symbols in [0,2^%d)" % (n, L))
     s = pkg.gen_uniform(SEED, n, 1 << L, np.uint32)
     ctx = pkg.Context(0)
+    if args.calibrate_copy:
+        a, b = ctx.empty(1 << 27, np.uint64), ctx.empty(1 << 27, np.uint64)
+        a.zero()
+        for _ in range(2):
+            pkg._check(pkg.lib().wtg_memcpy_d2d(ctx.h, b.ptr, a.ptr, 1 << 30))
+        ctx.sync()
+        a.free(); b.free()
     d = ctx.to_device(s)
     nw = pkg.lib().wtg_levels_words(n, L)
     dw = ctx.empty(nw, np.uint64)
@@ -109,6 +170,8 @@ def main():
     hist_ms, _ = ctx.kernel_time("hist")
     ctx.kernel_timing(False)
     ctx.set_blocking(True)
+
+    extras = query_bench(pkg, ctx, dw, n * L, args.queries) if args.queries else None
 
     # size-independent parity check of the timed output: ones per level
     words = dw.get()
@@ -138,7 +201,8 @@ def main():
                                "in [0,2^%d), max_level=%d" % (args.log2n, L, L),
                    "n": n, "levels": L, "sym_bytes": w, "seed": SEED},
         "roofline": {"bound": "hbm", "achieved": round(achieved, 1), "peak": HBM_PEAK_GBS,
-                     "unit": "GB/s", "frac": round(achieved / HBM_PEAK_GBS, 4), "traffic": None,
+                     "unit": "GB/s", "frac": round(achieved / HBM_PEAK_GBS, 4),
+                     "traffic": measured_traffic(args.log2n, L),
                      "kernel": "k_level<uint32,scatter> (level_split)",
                      "bytes_per_launch": split_bytes, "avg_launch_ms": round(split_avg_s * 1e3, 4),
                      "launches": split_n},
@@ -146,6 +210,7 @@ def main():
                            "level_bits_avg_ms": round(bits_ms / max(bits_n, 1), 4),
                            "hist_ms_per_step": round(hist_ms / args.steps, 4)},
         "parity": {"level_popcounts_match": ok},
+        "rank_select": extras,
         "wall_s": round(wall, 3),
     }
     if not args.no_cpu:

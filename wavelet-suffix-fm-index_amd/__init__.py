@@ -417,7 +417,8 @@ class WtHuff:
         self.n, self.nbits = int(self.shape.n), int(self.shape.bv_size)
 
     def words(self):
-        nw = self.nbits // 64 + 1
+        """Tree bits: ceil(bv_size/64) words and the zero padding word."""
+        nw = (self.nbits + 63) // 64 + 1
         a = np.empty(nw, np.uint64)
         if self.words_ptr.value:
             _check(lib().wtg_memcpy_d2h(self.ctx.h, _p(a), self.words_ptr, a.nbytes))

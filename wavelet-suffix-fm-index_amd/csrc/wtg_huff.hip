@@ -46,7 +46,7 @@ struct wtg_huff {
   wtg_huff_shape shape{};
   HuffTab tab{};
   HuffTab* d_tab = nullptr;
-  uint64_t* words = nullptr;   // ceil(bv_size/64)+1 words
+  uint64_t* words = nullptr;   // ceil(bv_size/64) words + one zero padding word
   uint64_t* dir = nullptr;
   wtg_select s1{}, s0{};
   uint64_t* d_hist = nullptr;  // 256 counts
@@ -568,7 +568,7 @@ HQ make_hq(const wtg_huff* wt) {
   h.tab = wt->d_tab;
   h.words = wt->words;
   h.D = wt->dir;
-  const uint64_t W = wt->shape.bv_size / 64 + 1;
+  const uint64_t W = (wt->shape.bv_size + 63) / 64 + 1;
   h.s1 = SelQ{wt->s1.superblock, wt->s1.mini, wt->s1.long_index, wt->s1.long_pool, wt->words, W, 1};
   h.s0 = SelQ{wt->s0.superblock, wt->s0.mini, wt->s0.long_index, wt->s0.long_pool, wt->words, W, 0};
   h.n = wt->shape.n;
@@ -633,7 +633,7 @@ extern "C" int wtg_huff_build(wtg_ctx* ctx, const uint8_t* d_text, uint64_t n, w
   }
   {
     const wtg_huff_shape& S = wt->shape;
-    const uint64_t nw = S.bv_size / 64 + 1;  // + zero padding word
+    const uint64_t nw = (S.bv_size + 63) / 64 + 1;  // + zero padding word
     if ((rc = wtg_malloc(ctx, (void**)&wt->words, nw * 8)) != WTG_OK) goto fail;
     if ((rc = wtg_malloc(ctx, (void**)&wt->dir, wtg_rank9_words(S.bv_size) * 8)) != WTG_OK) goto fail;
     if ((rc = wtg_fill_u64(ctx, wt->words, nw, 0)) != WTG_OK) goto fail;
