@@ -115,6 +115,63 @@ def query_bench(pkg, ctx, dw, nbits, nq):
     return res
 
 
+def bench_lsel(pkg, args):
+    """Learned vs plain sampled select: 64 bitvectors of 2^26 bits (densities
+    log-uniform in [0.01, 0.5], even Bernoulli, odd runs), uniform (vector, rank)
+    queries; both answer sets must be identical."""
+    nvec, nb, nq = 64, 1 << args.lsel_log2bits, args.lsel_queries
+    log("generating %d bitvectors of %d bits" % (nvec, nb))
+    w, dens = pkg.gen_select_vectors(SEED + 5, nvec, nb)
+    ctx = pkg.Context(0)
+    nw = w.shape[1]
+    dv = []
+    for b in range(nvec):
+        padded = np.zeros(nw + 1, np.uint64)
+        padded[:nw] = w[b]
+        dv.append(ctx.to_device(padded))
+    del w
+    log("build + train")
+    t0 = time.perf_counter()
+    ls = pkg.LearnedSelect(ctx, dv, [nb] * nvec)
+    build_s = time.perf_counter() - t0
+    infos = [ls.info(b) for b in range(nvec)]
+    rng = np.random.default_rng(SEED + 6)
+    vec = rng.integers(0, nvec, nq).astype(np.uint32)
+    ones = np.array([i["ones"] for i in infos], np.uint64)
+    j = (rng.random(nq) * ones[vec]).astype(np.uint64) + 1
+    dvq, djq = ctx.to_device(vec), ctx.to_device(j)
+    del vec, j
+    out_p, out_l = ctx.empty(nq, np.uint64), ctx.empty(nq, np.uint64)
+    res = {}
+    for name, learned, out in (("plain", False, out_p), ("learned", True, out_l)):
+        ls.query(dvq, djq, out, learned)
+        ctx.timer_start()
+        for _ in range(args.steps):
+            ls.query(dvq, djq, out, learned)
+        ms = ctx.timer_stop() / args.steps
+        res[name] = {"Mq_s": round(nq / (ms / 1e3) / 1e6, 1), "ms": round(ms, 3)}
+    same = bool(np.array_equal(out_p.get(), out_l.get()))
+    nl = sum(1 for i in infos if i["spacing"])
+    model_bytes = sum(i["nknots"] * 4 for i in infos if i["spacing"])
+    out = {"metric": "select Mq/s learned vs plain (bit-exact)", "value": res["learned"]["Mq_s"],
+           "unit": "Mq/s", "n_gpus": 1, "steps": args.steps, "higher_is_better": True,
+           "dtype": "u64", "data": "synthetic",
+           "config": {"workload": "learned select, 64 bitvectors x 2^%d bits, %d uniform queries"
+                                  % (args.lsel_log2bits, nq)},
+           "plain_Mq_s": res["plain"]["Mq_s"], "speedup": round(res["learned"]["Mq_s"] / res["plain"]["Mq_s"], 3),
+           "identical_answers": same, "learned_vectors": nl, "model_bytes": model_bytes,
+           "build_and_train_s": round(build_s, 3),
+           "eps_max_learned": max([i["eps"] for i in infos if i["spacing"]] or [0]),
+           "per_vector": [{"d": round(float(dens[b]), 4), "spacing": infos[b]["spacing"],
+                           "eps": infos[b]["eps"], "eps_init": infos[b]["eps_init"]} for b in range(nvec)]}
+    ls.free()
+    for a in dv + [dvq, djq, out_p, out_l]:
+        a.free()
+    ctx.close()
+    print(json.dumps(out), flush=True)
+    return 0 if same else 1
+
+
 def main():
     ap = argparse.ArgumentParser()
     ap.add_argument("--gpus", type=int, default=1)
@@ -129,12 +186,18 @@ def main():
                          "the reference point for HBM byte counters)")
     ap.add_argument("--queries", type=int, default=10_000_000,
                     help="rank and select queries on the built tree (0: skip)")
+    ap.add_argument("--workload", default="wt_int", choices=["wt_int", "lsel"],
+                    help="wt_int: the tree build (the headline line); lsel: learned vs plain select")
+    ap.add_argument("--lsel-log2bits", type=int, default=26)
+    ap.add_argument("--lsel-queries", type=int, default=100_000_000)
     args = ap.parse_args()
     if args.gpus != 1:
         print("bench.py measures one GPU only: the level build of this path runs on a single "
               "MI355X; --gpus %d is not supported" % args.gpus, file=sys.stderr)
         return 2
     pkg = load_pkg()
+    if args.workload == "lsel":
+        return bench_lsel(pkg, args)
     n, L, w = 1 << args.log2n, args.levels, 4
     log("generating n=%d uint32 symbols in [0,2^%d)" % (n, L))
     s = pkg.gen_uniform(SEED, n, 1 << L, np.uint32)

@@ -226,12 +226,61 @@ int wtg_bwt_from_sa(const uint8_t* text, const int64_t* sa, uint64_t n, uint8_t*
 int wtg_fm_count_batch(wtg_ctx* ctx, const wtg_huff* bwt_wt, const uint8_t* d_pats,
                        uint32_t m, uint64_t npat, uint64_t* d_out);
 
+/* ------------------------------------------------ learned select (added form) */
+/* No counterpart in the reference: select over a batch of bitvectors where a
+ * per-vector piecewise-linear model, trained on the GPU on the answers of the
+ * plain sampled select (select_support_mcl_par.hpp:564-611), predicts select(j)
+ * and rank9 (rank_support_v_par.hpp:131-142) plus a bounded word scan recovers
+ * the exact position.  Answers always equal the plain method's. */
+typedef struct {
+  uint32_t train_stride;  /* training samples select(T), select(2T), ...; power of two (16) */
+  uint32_t spacing_max;   /* largest knot spacing tried, in arguments; power of two (4096) */
+  uint32_t spacing_min;   /* smallest, >= 2*train_stride (64) */
+  uint32_t epochs;        /* Adam steps per spacing (200) */
+  float lr;               /* Adam step in bit positions (4) */
+  uint32_t eps_max;       /* accepted error window in bits (512) */
+} wtg_lsel_params;
+typedef struct {
+  uint64_t nbits, ones;
+  uint32_t spacing;       /* knot spacing kept; 0 = the vector answers with plain select */
+  uint32_t nknots;
+  uint64_t eps;           /* max |prediction - select(j)| over every j (exhaustive) */
+  uint64_t eps_init;      /* the same for the untrained knots at that spacing */
+  double loss_init, loss_final;  /* mean squared residual on the training samples */
+} wtg_lsel_vec_info;
+typedef struct wtg_lsel wtg_lsel;
+/* d_words[b]: device words of vector b (nbits[b] bits, zero padding word after
+ * them), 1 <= nvec <= 256; params NULL = defaults.  Builds rank9 + select1 per vector, trains. */
+int wtg_lsel_build(wtg_ctx* ctx, uint32_t nvec, const uint64_t* const* d_words,
+                   const uint64_t* nbits, const wtg_lsel_params* params, wtg_lsel** out);
+int wtg_lsel_destroy(wtg_ctx* ctx, wtg_lsel* ls);
+int wtg_lsel_info(const wtg_lsel* ls, uint32_t vec, wtg_lsel_vec_info* out);
+/* out[q] = select1 of vector d_vec[q] at rank d_j[q] (1-based); WTG_NONE when
+ * d_j[q] is 0 or above the vector's ones */
+int wtg_lsel_select_batch(wtg_ctx* ctx, const wtg_lsel* ls, const uint32_t* d_vec,
+                          const uint64_t* d_j, uint64_t nq, uint64_t* d_out);
+/* the same queries answered by the plain sampled select only */
+int wtg_lsel_plain_batch(wtg_ctx* ctx, const wtg_lsel* ls, const uint32_t* d_vec,
+                         const uint64_t* d_j, uint64_t nq, uint64_t* d_out);
+/* Training loss (mean squared residual) and its gradient for one model, by the
+ * training kernels: knots d_y[nknots] every `spacing` arguments, samples
+ * d_pos[t] = select((t+1)*stride), baseline alpha*j + beta. */
+int wtg_lsel_loss_grad(wtg_ctx* ctx, const float* d_y, uint32_t nknots, uint32_t spacing,
+                       uint32_t stride, double alpha, double beta, const uint64_t* d_pos,
+                       uint64_t ns, double* loss, float* d_grad);
+
 /* ------------------------------------------------ synthetic workloads (host) */
 /* Seeded generators of the benchmark inputs (splitmix64 streams). */
 uint64_t wtg_splitmix64(uint64_t* state);
 int wtg_gen_uniform(uint64_t seed, uint64_t n, uint64_t range, int sym_bytes, void* out);
 int wtg_gen_geometric(uint64_t seed, uint64_t n, double p, uint64_t cap, int sym_bytes, void* out);
 int wtg_gen_zipf_bytes(uint64_t seed, uint64_t n, double s, uint8_t* out);
+/* nvec bitvectors of nbits bits (vector b at words + b*ceil(nbits/64)), density
+ * d_b log-uniform in [0.01, 0.5] (density may be NULL): even b Bernoulli(d_b);
+ * odd b alternating runs, 0-runs Geometric(mean 64(1-d_b)/d_b), 1-runs
+ * Geometric(mean 64) */
+int wtg_gen_select_vectors(uint64_t seed, uint32_t nvec, uint64_t nbits, uint64_t* words,
+                           double* density);
 /* n bytes: order-5 Markov text over ACGT (Dirichlet(0.5) transition rows), 5% of
  * positions overwritten by copies of 1-10 kb segments, 0.1% 'N', text[n-1] = 0 */
 int wtg_gen_dna(uint64_t seed, uint64_t n, uint8_t* out);

@@ -323,3 +323,30 @@ class RefWT:
 
     def workers(self):
         return self.L.ref_workers()
+
+
+# ----------------------------------------------------------- learned select
+def spline_predict(y, spacing, alpha, beta, j):
+    """Piecewise-linear select model (float64): alpha*j + beta + knot interpolation."""
+    j = np.asarray(j, np.float64)
+    k = np.floor(j / spacing).astype(np.int64)
+    f = (j - k * spacing) / spacing
+    y = np.asarray(y, np.float64)
+    return alpha * j + beta + (1 - f) * y[k] + f * y[k + 1]
+
+
+def spline_loss_grad(y, spacing, stride, alpha, beta, pos):
+    """Mean squared residual of the model on samples pos[t] = select((t+1)*stride)
+    and its gradient with respect to the knots (float64, closed form)."""
+    y = np.asarray(y, np.float64)
+    pos = np.asarray(pos, np.float64)
+    ns = pos.size
+    j = (np.arange(ns, dtype=np.float64) + 1) * stride
+    k = np.floor(j / spacing).astype(np.int64)
+    f = (j - k * spacing) / spacing
+    e = alpha * j + beta + (1 - f) * y[k] + f * y[k + 1] - pos
+    loss = float((e * e).sum() / ns)
+    g = np.zeros_like(y)
+    np.add.at(g, k, 2.0 / ns * e * (1 - f))
+    np.add.at(g, k + 1, 2.0 / ns * e * f)
+    return loss, g

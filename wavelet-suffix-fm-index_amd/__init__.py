@@ -46,6 +46,17 @@ class HuffShape(C.Structure):
                 ("path", u64 * 256), ("path_len", u32 * 256)]
 
 
+class LselParams(C.Structure):
+    _fields_ = [("train_stride", u32), ("spacing_max", u32), ("spacing_min", u32),
+                ("epochs", u32), ("lr", C.c_float), ("eps_max", u32)]
+
+
+class LselInfo(C.Structure):
+    _fields_ = [("nbits", u64), ("ones", u64), ("spacing", u32), ("nknots", u32),
+                ("eps", u64), ("eps_init", u64), ("loss_init", C.c_double),
+                ("loss_final", C.c_double)]
+
+
 # name -> (restype, argtypes); every symbol here is declared in include/wtg.h
 SIGNATURES = {
     "wtg_version": (C.c_char_p, []),
@@ -99,6 +110,14 @@ SIGNATURES = {
     "wtg_suffix_array": (i32, [vp, u64, vp]),
     "wtg_bwt_from_sa": (i32, [vp, vp, u64, vp]),
     "wtg_fm_count_batch": (i32, [vp, vp, vp, u32, u64, vp]),
+    "wtg_lsel_build": (i32, [vp, u32, C.POINTER(vp), U64P, C.POINTER(LselParams), C.POINTER(vp)]),
+    "wtg_lsel_destroy": (i32, [vp, vp]),
+    "wtg_lsel_info": (i32, [vp, u32, C.POINTER(LselInfo)]),
+    "wtg_lsel_select_batch": (i32, [vp, vp, vp, vp, u64, vp]),
+    "wtg_lsel_plain_batch": (i32, [vp, vp, vp, vp, u64, vp]),
+    "wtg_lsel_loss_grad": (i32, [vp, vp, u32, u32, u32, C.c_double, C.c_double, vp, u64,
+                                 C.POINTER(C.c_double), vp]),
+    "wtg_gen_select_vectors": (i32, [u64, u32, u64, vp, vp]),
     "wtg_splitmix64": (u64, [U64P]),
     "wtg_gen_uniform": (i32, [u64, u64, u64, i32, vp]),
     "wtg_gen_geometric": (i32, [u64, u64, C.c_double, u64, i32, vp]),
@@ -162,6 +181,15 @@ def gen_zipf_bytes(seed, n, s=1.1):
     out = np.empty(n, dtype=np.uint8)
     _check(lib().wtg_gen_zipf_bytes(seed, n, s, _p(out)))
     return out
+
+
+def gen_select_vectors(seed, nvec, nbits):
+    """(words[nvec, ceil(nbits/64)], density[nvec]) of the learned-select workload."""
+    nw = (nbits + 63) // 64
+    w = np.empty((nvec, nw), np.uint64)
+    d = np.empty(nvec, np.float64)
+    _check(lib().wtg_gen_select_vectors(seed, nvec, nbits, _p(w), _p(d)))
+    return w, d
 
 
 def gen_dna(seed, n):
@@ -404,6 +432,60 @@ class Context:
         n = d_text.size if n is None else n
         _check(lib().wtg_huff_build(self.h, d_text.ptr, int(n), C.byref(h)))
         return WtHuff(self, h)
+
+
+class LearnedSelect:
+    """Learned select over device bitvectors (each with a zero padding word)."""
+
+    def __init__(self, ctx, d_vecs, nbits, params=None):
+        self.ctx = ctx
+        self.nvec = len(d_vecs)
+        ptrs = (vp * self.nvec)(*[d.ptr.value for d in d_vecs])
+        nb = (C.c_uint64 * self.nvec)(*[int(x) for x in nbits])
+        h = C.c_void_p()
+        pp = C.byref(params) if params is not None else None
+        _check(lib().wtg_lsel_build(ctx.h, self.nvec, ptrs, nb, pp, C.byref(h)))
+        self.h = h
+
+    def info(self, b):
+        i = LselInfo()
+        _check(lib().wtg_lsel_info(self.h, b, C.byref(i)))
+        return {k: getattr(i, k) for k, _ in LselInfo._fields_}
+
+    def query(self, d_vec, d_j, d_out, learned=True):
+        fn = lib().wtg_lsel_select_batch if learned else lib().wtg_lsel_plain_batch
+        _check(fn(self.ctx.h, self.h, d_vec.ptr, d_j.ptr, d_j.size, d_out.ptr))
+        return d_out
+
+    def select(self, vec, j, learned=True):
+        ctx = self.ctx
+        dv = ctx.to_device(np.ascontiguousarray(vec, np.uint32))
+        dj = ctx.to_device(np.ascontiguousarray(j, np.uint64))
+        do = ctx.empty(dj.size, np.uint64)
+        try:
+            self.query(dv, dj, do, learned)
+            return do.get()
+        finally:
+            dv.free(); dj.free(); do.free()
+
+    def free(self):
+        if self.h is not None:
+            _check(lib().wtg_lsel_destroy(self.ctx.h, self.h))
+            self.h = None
+
+
+def lsel_loss_grad(ctx, y, spacing, stride, alpha, beta, pos):
+    """(loss, grad) of one spline model by the GPU training kernels."""
+    dy = ctx.to_device(np.ascontiguousarray(y, np.float32))
+    dp = ctx.to_device(np.ascontiguousarray(pos, np.uint64))
+    dg = ctx.empty(len(y), np.float32)
+    loss = C.c_double(0)
+    try:
+        _check(lib().wtg_lsel_loss_grad(ctx.h, dy.ptr, len(y), spacing, stride, alpha, beta, dp.ptr,
+                                        len(pos), C.byref(loss), dg.ptr))
+        return loss.value, dg.get()
+    finally:
+        dy.free(); dp.free(); dg.free()
 
 
 class WtHuff:

@@ -3,6 +3,7 @@
 // generation is parallel and reproducible.
 #include <stdint.h>
 #include <string.h>
+#include <cstring>
 #include <math.h>
 #include <thread>
 #include <vector>
@@ -92,5 +93,61 @@ extern "C" int wtg_gen_zipf_bytes(uint64_t seed, uint64_t n, double s, uint8_t* 
     if (r > 255) r = 255;
     out[i] = perm[r];
   });
+  return WTG_OK;
+}
+
+extern "C" int wtg_gen_select_vectors(uint64_t seed, uint32_t nvec, uint64_t nbits, uint64_t* words,
+                                      double* density) {
+  // nvec bitvectors of nbits bits, vector b at words + b*ceil(nbits/64); density
+  // d_b log-uniform in [0.01, 0.5].  Even b: Bernoulli(d_b) bits.  Odd b:
+  // alternating runs, 0-runs Geometric(mean 64(1-d_b)/d_b), 1-runs Geometric(mean 64),
+  // both >= 1, starting with a 0-run.
+  if (!words || nbits == 0) return WTG_ERR_ARG;
+  const uint64_t nw = (nbits + 63) / 64;
+  std::vector<double> d(nvec);
+  for (uint32_t b = 0; b < nvec; b++)
+    d[b] = exp(log(0.01) + (log(0.5) - log(0.01)) * (ctr_unit(seed ^ 0xD5D5D5D5ull, b) - 1e-300));
+  if (density)
+    for (uint32_t b = 0; b < nvec; b++) density[b] = d[b];
+  // even vectors: word-parallel Bernoulli
+  for (uint32_t b = 0; b < nvec; b += 2) {
+    const uint64_t sb = mix64(seed + 0x1000 + b);
+    const double db = d[b];
+    uint64_t* w = words + (uint64_t)b * nw;
+    par_for(nw, [&](uint64_t k) {
+      uint64_t x = 0;
+      for (int j = 0; j < 64; j++) {
+        const uint64_t i = k * 64 + j;
+        if (i < nbits && ctr_unit(sb, i) <= db) x |= 1ull << j;
+      }
+      w[k] = x;
+    });
+  }
+  // odd vectors: sequential runs, one thread per vector
+  std::vector<uint32_t> odd;
+  for (uint32_t b = 1; b < nvec; b += 2) odd.push_back(b);
+  std::vector<std::thread> th;
+  for (uint32_t b : odd) {
+    th.emplace_back([=]() {
+      uint64_t* w = words + (uint64_t)b * nw;
+      std::memset(w, 0, nw * 8);
+      const uint64_t sb = mix64(seed + 0x2000 + b);
+      const double l1 = log1p(-1.0 / 64), l0 = log1p(-1.0 / (64 * (1 - d[b]) / d[b]));
+      uint64_t pos = 0, k = 0;
+      bool one = false;
+      while (pos < nbits) {
+        const double lq = one ? l1 : l0;
+        const uint64_t len = 1 + (uint64_t)floor(log(ctr_unit(sb, k++)) / lq);
+        if (one) {
+          const uint64_t e = std::min(nbits, pos + len);
+          for (uint64_t i = pos; i < e; i++) w[i >> 6] |= 1ull << (i & 63);
+        }
+        pos += len;
+        one = !one;
+      }
+    });
+    if (th.size() >= 16) { for (auto& x : th) x.join(); th.clear(); }
+  }
+  for (auto& x : th) x.join();
   return WTG_OK;
 }
