@@ -172,6 +172,160 @@ def bench_lsel(pkg, args):
     return 0 if same else 1
 
 
+def bench_batch(pkg, args):
+    """Batched level-wise trees: 4096 byte sequences, n_i = floor(2^U(10,22)),
+    alphabets {4, 20, 256}; one hipGraph replay per step, against per-sequence
+    plain launches and the reference levelWT on a bounded sample."""
+    import ctypes as C
+    nseq = args.batch_seqs
+    log("generating %d sequences" % nseq)
+    n, sigma, sym = pkg.gen_batch(SEED + 7, nseq, 10.0, float(args.batch_log2max))
+    total = int(n.sum())
+    off = np.concatenate([[0], np.cumsum(n)[:-1]]).astype(np.uint64)
+    ctx = pkg.Context(0)
+    d = ctx.to_device(sym)
+    B = pkg.Batch(ctx, d, n, sigma, off)
+    L = np.array([pkg.levels_of_sigma(int(x)) for x in sigma])
+    log("batch: %d symbols" % total)
+    B.run()
+    ctx.set_blocking(False)
+    for _ in range(args.warmup):
+        B.run()
+    ctx.sync()
+    ctx.timer_start()
+    for _ in range(args.steps):
+        B.run()
+    ms = ctx.timer_stop() / args.steps
+    ctx.sync()
+    # per-sequence plain launches into one output array
+    wo = np.concatenate([[0], np.cumsum([pkg.lib().wtg_levels_words(int(k), int(l)) + 1
+                                         for k, l in zip(n, L)])]).astype(np.uint64)
+    dw = ctx.empty(int(wo[-1]), np.uint64)
+    lib = pkg.lib()
+
+    def per_seq():
+        for i in range(nseq):
+            if n[i]:
+                pkg._check(lib.wtg_levels_build(ctx.h, C.c_void_p(d.ptr.value + int(off[i])), 1, int(n[i]),
+                                                int(L[i]), C.c_void_p(dw.ptr.value + 8 * int(wo[i])), None, 0))
+    per_seq()
+    ctx.sync()
+    t0 = time.perf_counter()
+    ctx.timer_start()
+    per_seq()
+    ms_seq = ctx.timer_stop()
+    ctx.sync()
+    wall_seq = time.perf_counter() - t0
+    ctx.set_blocking(True)
+    # parity: the batch's sequences equal the per-sequence builds
+    allw = dw.get()
+    ok = True
+    rng = np.random.default_rng(1)
+    for i in list(rng.choice(nseq, 24, replace=False)) + [int(np.argmax(n))]:
+        e = allw[int(wo[i]):int(wo[i]) + pkg.lib().wtg_levels_words(int(n[i]), int(L[i]))]
+        ok &= bool(np.array_equal(B.extract(int(i)), e))
+    del allw
+    B_a = float((n.astype(np.float64) * (2 * L + L / 8.0)).sum())
+    out = {"metric": "batched WT build Gsym/s", "value": round(total / (ms / 1e3) / 1e9, 4), "unit": "Gsym/s",
+           "n_gpus": 1, "steps": args.steps, "warmup": args.warmup, "ms_per_step": round(ms, 4),
+           "higher_is_better": True, "dtype": "u8", "data": "synthetic",
+           "config": {"workload": "batched levelWT, %d sequences, n_i=floor(2^U(10,%d)), alphabets {4,20,256}"
+                                  % (nseq, args.batch_log2max), "symbols": total},
+           "build_roofline": {"B_a_bytes": B_a, "frac": round(B_a / (ms / 1e3) / 1e9 / HBM_PEAK_GBS, 4)},
+           "per_sequence_launches": {"ms": round(ms_seq, 3), "wall_ms": round(wall_seq * 1e3, 3),
+                                     "Gsym_s": round(total / (ms_seq / 1e3) / 1e9, 4)},
+           "speedup_vs_per_sequence": round(ms_seq / ms, 2), "parity_vs_per_sequence": ok}
+    if not args.no_cpu:
+        from oracle import oracle
+        ref = oracle.RefWT("levelWT", "u8")
+        acc_n, acc_t = 0, 0.0
+        for i in range(nseq):
+            if acc_n >= (1 << 26):
+                break
+            if n[i] == 0:
+                continue
+            s_i = sym[int(off[i]):int(off[i]) + int(n[i])]
+            if ref.ok:
+                acc_t += float(ref.time(s_i, 1 << int(L[i]), rounds=1).min())
+            else:
+                t0 = time.perf_counter()
+                oracle.levels_serial(s_i, int(L[i]))
+                acc_t += time.perf_counter() - t0
+            acc_n += int(n[i])
+        out["cpu_baseline"] = {"value": round(acc_n / acc_t / 1e9, 5), "unit": "Gsym/s",
+                               "cores": int(ref.workers()) if ref.ok else 1,
+                               "kind": "reference" if ref.ok else "port",
+                               "sample": "the first %d sequences' symbols (%d), one WT() each" % (i, acc_n)}
+    B.free(); d.free(); dw.free(); ctx.close()
+    print(json.dumps(out), flush=True)
+    return 0 if ok else 1
+
+
+def bench_fm(pkg, args):
+    """FM-index over synthetic DNA: host SA + BWT, wt_huff build on the GPU (timed
+    per build, input resident), then batched backward-search counts of
+    length-20 patterns (half exact substrings, half with 1-2 substitutions)."""
+    n = 1 << args.fm_log2n
+    log("generating %d DNA bytes" % n)
+    t = pkg.gen_dna(SEED + 3, n)
+    t0 = time.perf_counter()
+    sa = pkg.suffix_array(t)
+    sa_s = time.perf_counter() - t0
+    bwt = pkg.bwt_from_sa(t, sa)
+    pats = pkg.gen_patterns(SEED + 4, t, 20, args.fm_patterns)
+    ctx = pkg.Context(0)
+    d = ctx.to_device(bwt)
+    wt = ctx.huff_build(d)
+    wt.free()
+    ctx.kernel_timing_reset()
+    ctx.kernel_timing(True)
+    ctx.timer_start()
+    for _ in range(args.steps):
+        wt = ctx.huff_build(d)
+        wt.free()
+    build_ms = ctx.timer_stop() / args.steps
+    kt = {k: ctx.kernel_time(k) for k in ("huff_hist", "huff_count", "huff_scan", "huff_emit", "rank_select")}
+    ctx.kernel_timing(False)
+    wt = ctx.huff_build(d)
+    dp = ctx.to_device(pats.reshape(-1))
+    do = ctx.empty(pats.shape[0], np.uint64)
+    lib = pkg.lib()
+    pkg._check(lib.wtg_fm_count_batch(ctx.h, wt.h, dp.ptr, 20, pats.shape[0], do.ptr))
+    ctx.timer_start()
+    for _ in range(args.steps):
+        pkg._check(lib.wtg_fm_count_batch(ctx.h, wt.h, dp.ptr, 20, pats.shape[0], do.ptr))
+    q_ms = ctx.timer_stop() / args.steps
+    got = do.get()
+    from oracle import oracle
+    ok = bool(np.array_equal(got, oracle.sa_count(t, sa, pats)))
+    bits = int(wt.shape.bv_size)
+    B_c = n * 1.0 + 2.0 * bits + bits / 8.0   # n*w + sum_l a_l*2w + n*H/8 (w = 1, sum_l a_l = bits)
+    emit_ms = kt["huff_emit"][0] / max(kt["huff_emit"][1], 1)
+    out = {"metric": "wt_huff build over a BWT Gsym/s; FM count Mpat/s", "value": round(n / (build_ms / 1e3) / 1e9, 4),
+           "unit": "Gsym/s", "n_gpus": 1, "steps": args.steps, "higher_is_better": True, "dtype": "u8",
+           "data": "synthetic", "config": {"workload": "FM-index over 2^%d-byte order-5 Markov DNA, %d length-20 patterns"
+                                                      % (args.fm_log2n, pats.shape[0])},
+           "build_ms": round(build_ms, 4), "bits": bits, "mean_code_len": round(bits / n, 4),
+           "build_roofline": {"B_c_bytes": B_c, "frac": round(B_c / (build_ms / 1e3) / 1e9 / HBM_PEAK_GBS, 4)},
+           "kernels_ms": {k: round(v[0] / max(v[1], 1), 4) for k, v in kt.items()},
+           "emit_GBps": round((n + bits / 8.0) / (emit_ms / 1e3) / 1e9, 1) if emit_ms else None,
+           "count_ms": round(q_ms, 4), "count_Mpat_s": round(pats.shape[0] / (q_ms / 1e3) / 1e6, 2),
+           "counts_match_sa_binary_search": ok, "host_sa_s": round(sa_s, 2)}
+    if not args.no_cpu:
+        # CPU: the oracle's serial restatement of the Huffman tree bits (wt_huff_par.hpp:92-117,
+        # wt_pc_par.hpp:193-257) on a prefix of the BWT
+        m = min(n, 1 << 25)
+        t0 = time.perf_counter()
+        oracle.Huff(bwt[:m])
+        dt = time.perf_counter() - t0
+        out["cpu_baseline"] = {"value": round(m / dt / 1e9, 5), "unit": "Gsym/s", "cores": 1, "kind": "port",
+                               "sample": "serial Huffman-shape + tree bits + rank9 + select restatement on the "
+                                         "first %d BWT bytes" % m}
+    wt.free(); d.free(); dp.free(); do.free(); ctx.close()
+    print(json.dumps(out), flush=True)
+    return 0 if ok else 1
+
+
 def main():
     ap = argparse.ArgumentParser()
     ap.add_argument("--gpus", type=int, default=1)
@@ -186,9 +340,13 @@ def main():
                          "the reference point for HBM byte counters)")
     ap.add_argument("--queries", type=int, default=10_000_000,
                     help="rank and select queries on the built tree (0: skip)")
-    ap.add_argument("--workload", default="wt_int", choices=["wt_int", "lsel"],
+    ap.add_argument("--workload", default="wt_int", choices=["wt_int", "lsel", "batch", "fm"],
                     help="wt_int: the tree build (the headline line); lsel: learned vs plain select")
     ap.add_argument("--lsel-log2bits", type=int, default=26)
+    ap.add_argument("--batch-seqs", type=int, default=4096)
+    ap.add_argument("--fm-log2n", type=int, default=27)
+    ap.add_argument("--fm-patterns", type=int, default=1_000_000)
+    ap.add_argument("--batch-log2max", type=int, default=22)
     ap.add_argument("--lsel-queries", type=int, default=100_000_000)
     args = ap.parse_args()
     if args.gpus != 1:
@@ -198,6 +356,10 @@ def main():
     pkg = load_pkg()
     if args.workload == "lsel":
         return bench_lsel(pkg, args)
+    if args.workload == "batch":
+        return bench_batch(pkg, args)
+    if args.workload == "fm":
+        return bench_fm(pkg, args)
     n, L, w = 1 << args.log2n, args.levels, 4
     log("generating n=%d uint32 symbols in [0,2^%d)" % (n, L))
     s = pkg.gen_uniform(SEED, n, 1 << L, np.uint32)

@@ -226,6 +226,27 @@ int wtg_bwt_from_sa(const uint8_t* text, const int64_t* sa, uint64_t n, uint8_t*
 int wtg_fm_count_batch(wtg_ctx* ctx, const wtg_huff* bwt_wt, const uint8_t* d_pats,
                        uint32_t m, uint64_t npat, uint64_t* d_out);
 
+/* ------------------------------------------------ batched level-wise trees */
+/* Many WT(s_i, n_i, a_i) builds (wavelet/levelWT/WT.h:43, levels
+ * L_i = max(1, log2Up(a_i)), a_i <= 256, byte symbols) in one go.  Sequence i is
+ * d_sym[h_off[i], h_off[i] + n_i).  Sequences are grouped by L_i; a group
+ * holds at most 2^16 sequences and 2^32 - 1 symbols.  Output (device, owned by
+ * the batch): level l of sequence i is the bit range
+ * [first_bit + l*level_stride, ... + n_i) of the batch words (wtg_batch_seq). */
+typedef struct wtg_batch wtg_batch;
+int wtg_batch_create(wtg_ctx* ctx, const uint8_t* d_sym, uint32_t nseq, const uint64_t* h_off,
+                     const uint64_t* h_n, const uint32_t* h_sigma, wtg_batch** out);
+int wtg_batch_destroy(wtg_ctx* ctx, wtg_batch* b);
+/* 1 (default): the first run records every kernel of the batch as one hipGraph,
+ * later runs replay it with a single launch; 0: plain stream launches */
+int wtg_batch_set_graph(wtg_batch* b, int use_graph);
+int wtg_batch_run(wtg_ctx* ctx, wtg_batch* b);
+int wtg_batch_words(const wtg_batch* b, const uint64_t** d_words, uint64_t* nwords);
+int wtg_batch_seq(const wtg_batch* b, uint32_t seq, uint32_t* levels, uint64_t* first_bit,
+                  uint64_t* level_stride);
+/* sequence `seq` in the WT() layout (level l at bit l*n_i): wtg_levels_words(n_i, L_i) words */
+int wtg_batch_extract(wtg_ctx* ctx, const wtg_batch* b, uint32_t seq, uint64_t* d_out);
+
 /* ------------------------------------------------ learned select (added form) */
 /* No counterpart in the reference: select over a batch of bitvectors where a
  * per-vector piecewise-linear model, trained on the GPU on the answers of the
@@ -275,6 +296,13 @@ uint64_t wtg_splitmix64(uint64_t* state);
 int wtg_gen_uniform(uint64_t seed, uint64_t n, uint64_t range, int sym_bytes, void* out);
 int wtg_gen_geometric(uint64_t seed, uint64_t n, double p, uint64_t cap, int sym_bytes, void* out);
 int wtg_gen_zipf_bytes(uint64_t seed, uint64_t n, double s, uint8_t* out);
+/* Batched workload: nseq sequences, n_i = floor(2^U(log2_min, log2_max)),
+ * alphabet a_i uniform over alphabets[0..nalpha); symbols (host, caller-sized
+ * sum(n_i) bytes, sequence i at offset sum_{k<i} n_k) uniform in [0, a_i).
+ * With sym == NULL only n and sigma are written. */
+int wtg_gen_batch(uint64_t seed, uint32_t nseq, double log2_min, double log2_max,
+                  const uint32_t* alphabets, uint32_t nalpha, uint64_t* n, uint32_t* sigma,
+                  uint8_t* sym);
 /* nvec bitvectors of nbits bits (vector b at words + b*ceil(nbits/64)), density
  * d_b log-uniform in [0.01, 0.5] (density may be NULL): even b Bernoulli(d_b);
  * odd b alternating runs, 0-runs Geometric(mean 64(1-d_b)/d_b), 1-runs

@@ -118,6 +118,14 @@ SIGNATURES = {
     "wtg_lsel_loss_grad": (i32, [vp, vp, u32, u32, u32, C.c_double, C.c_double, vp, u64,
                                  C.POINTER(C.c_double), vp]),
     "wtg_gen_select_vectors": (i32, [u64, u32, u64, vp, vp]),
+    "wtg_gen_batch": (i32, [u64, u32, C.c_double, C.c_double, vp, u32, vp, vp, vp]),
+    "wtg_batch_create": (i32, [vp, vp, u32, vp, vp, vp, C.POINTER(vp)]),
+    "wtg_batch_destroy": (i32, [vp, vp]),
+    "wtg_batch_set_graph": (i32, [vp, i32]),
+    "wtg_batch_run": (i32, [vp, vp]),
+    "wtg_batch_words": (i32, [vp, C.POINTER(vp), U64P]),
+    "wtg_batch_seq": (i32, [vp, u32, C.POINTER(u32), U64P, U64P]),
+    "wtg_batch_extract": (i32, [vp, vp, u32, vp]),
     "wtg_splitmix64": (u64, [U64P]),
     "wtg_gen_uniform": (i32, [u64, u64, u64, i32, vp]),
     "wtg_gen_geometric": (i32, [u64, u64, C.c_double, u64, i32, vp]),
@@ -190,6 +198,20 @@ def gen_select_vectors(seed, nvec, nbits):
     d = np.empty(nvec, np.float64)
     _check(lib().wtg_gen_select_vectors(seed, nvec, nbits, _p(w), _p(d)))
     return w, d
+
+
+def gen_batch(seed, nseq, log2_min=10.0, log2_max=22.0, alphabets=(4, 20, 256), symbols=True):
+    """(n[nseq], sigma[nseq], packed symbols or None) of the batched workload."""
+    n = np.empty(nseq, np.uint64)
+    sg = np.empty(nseq, np.uint32)
+    al = np.ascontiguousarray(alphabets, np.uint32)
+    _check(lib().wtg_gen_batch(seed, nseq, log2_min, log2_max, _p(al), al.size, _p(n), _p(sg), None))
+    sym = None
+    if symbols:
+        sym = np.empty(int(n.sum()), np.uint8)
+        _check(lib().wtg_gen_batch(seed, nseq, log2_min, log2_max, _p(al), al.size, _p(n), _p(sg),
+                                   _p(sym)))
+    return n, sg, sym
 
 
 def gen_dna(seed, n):
@@ -486,6 +508,56 @@ def lsel_loss_grad(ctx, y, spacing, stride, alpha, beta, pos):
         return loss.value, dg.get()
     finally:
         dy.free(); dp.free(); dg.free()
+
+
+class Batch:
+    """Batched level-wise trees of byte sequences packed in one device array."""
+
+    def __init__(self, ctx, d_sym, n, sigma, off=None):
+        self.ctx = ctx
+        n = np.ascontiguousarray(n, np.uint64)
+        sigma = np.ascontiguousarray(sigma, np.uint32)
+        if off is None:
+            off = np.concatenate([[0], np.cumsum(n)[:-1]]).astype(np.uint64)
+        off = np.ascontiguousarray(off, np.uint64)
+        self.n, self.sigma, self.off = n, sigma, off
+        h = C.c_void_p()
+        _check(lib().wtg_batch_create(ctx.h, d_sym.ptr, n.size, _p(off), _p(n), _p(sigma), C.byref(h)))
+        self.h = h
+
+    def set_graph(self, on):
+        _check(lib().wtg_batch_set_graph(self.h, 1 if on else 0))
+
+    def run(self):
+        _check(lib().wtg_batch_run(self.ctx.h, self.h))
+
+    def seq(self, i):
+        L, fb, st = C.c_uint32(), C.c_uint64(), C.c_uint64()
+        _check(lib().wtg_batch_seq(self.h, i, C.byref(L), C.byref(fb), C.byref(st)))
+        return L.value, fb.value, st.value
+
+    def words(self):
+        p, nw = C.c_void_p(), C.c_uint64()
+        _check(lib().wtg_batch_words(self.h, C.byref(p), C.byref(nw)))
+        a = np.empty(nw.value, np.uint64)
+        _check(lib().wtg_memcpy_d2h(self.ctx.h, _p(a), p, a.nbytes))
+        return a
+
+    def extract(self, i):
+        """Sequence i's tree bits in the WT() layout."""
+        L = self.seq(i)[0]
+        nw = lib().wtg_levels_words(int(self.n[i]), L)
+        d = self.ctx.empty(max(nw, 1), np.uint64)
+        try:
+            _check(lib().wtg_batch_extract(self.ctx.h, self.h, i, d.ptr))
+            return d.get()[:nw]
+        finally:
+            d.free()
+
+    def free(self):
+        if self.h is not None:
+            _check(lib().wtg_batch_destroy(self.ctx.h, self.h))
+            self.h = None
 
 
 class WtHuff:

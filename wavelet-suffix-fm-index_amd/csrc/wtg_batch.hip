@@ -1,0 +1,317 @@
+// wtg_batch.hip -- batched level-wise wavelet trees of many uneven byte
+// sequences (the per-sequence call WT(s_i, n_i, a_i) of wavelet/levelWT/WT.h:43
+// for thousands of i at once).
+//
+// Sequences are grouped by their level count L = max(1, log2Up(a_i)).  Within a
+// group the sequences are concatenated and each symbol becomes the key
+// (group-local sequence id << L) | symbol (16 or 32 bits).  The keys are then
+// already ordered by their top bits, so the level pipeline starts at level
+// `sbits` (the sequence-id bits) with the sequence lengths as that level's node
+// counts: levels sbits .. sbits+L-1 of the keys are exactly levels 0 .. L-1 of
+// every sequence, side by side.  The group's output is level-major: level l of
+// sequence i is the bit range [l*N_g + P_i, l*N_g + P_i + n_i) of the group's
+// words (N_g = group total, P_i = offset of i in the group).
+//
+// The first wtg_batch_run records the whole batch (key expansion + every
+// level of every group) as one hipGraph; later runs replay it with one launch.
+#include "wtg_internal.hpp"
+#include <algorithm>
+#include <cstring>
+#include <vector>
+
+namespace {
+
+template <typename K>
+__global__ __launch_bounds__(256) void k_batch_keys(const uint8_t* __restrict__ sym,
+                                                    const uint64_t* __restrict__ seqoff,
+                                                    const uint64_t* __restrict__ pref, uint32_t m, uint64_t n,
+                                                    uint32_t L, K* __restrict__ keys, uint32_t* __restrict__ err) {
+  __shared__ uint32_t s_first;
+  const uint64_t b0 = (uint64_t)blockIdx.x * 4096;
+  if (threadIdx.x == 0) {  // last sequence with pref <= b0
+    uint32_t lo = 0, hi = m;
+    while (hi - lo > 1) {
+      const uint32_t mid = (lo + hi) / 2;
+      if (pref[mid] <= b0) lo = mid; else hi = mid;
+    }
+    s_first = lo;
+  }
+  __syncthreads();
+  const uint64_t p0 = b0 + (uint64_t)threadIdx.x * 16;
+  uint32_t s = s_first;
+  bool bad = false;
+  for (int k = 0; k < 16; k++) {
+    const uint64_t p = p0 + k;
+    if (p >= n) break;
+    while (s + 1 < m && pref[s + 1] <= p) s++;
+    const uint32_t x = sym[seqoff[s] + (p - pref[s])];
+    bad |= (x >> L) != 0;
+    keys[p] = (K)(((uint64_t)s << L) | x);
+  }
+  if (bad) atomicOr(err, 2u);  // reported as an input symbol >= 2^levels
+}
+
+// copies `cnt` <= 64 bits starting at bit `b` of w (bits past the end are 0)
+__device__ __forceinline__ uint64_t bits_at(const uint64_t* __restrict__ w, uint64_t b, uint32_t cnt) {
+  const uint64_t i = b >> 6;
+  const uint32_t sh = (uint32_t)(b & 63);
+  uint64_t v = w[i] >> sh;
+  if (sh && sh + cnt > 64) v |= w[i + 1] << (64 - sh);
+  return cnt == 64 ? v : v & ((1ull << cnt) - 1);
+}
+
+// one sequence's levels in the WT() layout (level l at l*n_i)
+__global__ void k_batch_extract(const uint64_t* __restrict__ src, uint64_t first, uint64_t stride, uint64_t ni,
+                                uint32_t L, uint64_t* __restrict__ out) {
+  const uint64_t total = ni * L;
+  const uint64_t k = (uint64_t)blockIdx.x * blockDim.x + threadIdx.x;
+  if (k * 64 >= total) return;
+  uint64_t val = 0;
+  uint64_t b = k * 64;
+  const uint64_t e = (b + 64 < total) ? b + 64 : total;
+  while (b < e) {
+    const uint64_t l = b / ni, i = b - l * ni;
+    const uint64_t rem = ni - i < e - b ? ni - i : e - b;
+    const uint32_t cnt = (uint32_t)rem;
+    val |= bits_at(src, first + l * stride + i, cnt) << (b - k * 64);
+    b += cnt;
+  }
+  out[k] = val;
+}
+
+struct Group {
+  uint32_t L = 0, sbits = 0;
+  int key_bytes = 4;
+  uint64_t n = 0;
+  std::vector<uint32_t> seqs;   // batch indices in group order
+  uint64_t* d_seqoff = nullptr;
+  uint64_t* d_pref = nullptr;
+  uint32_t* d_cnt = nullptr;
+  void* d_keys = nullptr;
+  uint64_t word_base = 0;
+};
+}  // namespace
+
+struct wtg_batch {
+  const uint8_t* d_sym = nullptr;
+  uint32_t nseq = 0;
+  std::vector<uint64_t> n, off;
+  std::vector<uint32_t> L, grp, idx;
+  std::vector<uint64_t> pref;   // offset of each sequence inside its group
+  std::vector<Group> groups;
+  uint64_t* words = nullptr;
+  uint64_t nwords = 0;
+  size_t ws_need = 0;
+  bool graph = true;
+  hipGraph_t g = nullptr;
+  hipGraphExec_t exec = nullptr;
+  void* graph_ws = nullptr;     // workspace the recorded graph addresses
+};
+
+static void batch_release(wtg_ctx* ctx, wtg_batch* b) {
+  if (!b) return;
+  if (b->exec) hipGraphExecDestroy(b->exec);
+  if (b->g) hipGraphDestroy(b->g);
+  for (auto& G : b->groups) {
+    wtg_free(ctx, G.d_seqoff);
+    wtg_free(ctx, G.d_pref);
+    wtg_free(ctx, G.d_cnt);
+    wtg_free(ctx, G.d_keys);
+  }
+  wtg_free(ctx, b->words);
+  delete b;
+}
+
+extern "C" int wtg_batch_destroy(wtg_ctx* ctx, wtg_batch* b) {
+  if (!b) return WTG_OK;
+  WTG_REQUIRE(ctx, "null ctx");
+  batch_release(ctx, b);
+  return WTG_OK;
+}
+
+static int log2up_u64(uint64_t x) { return wtg_log2up_host(x); }
+
+extern "C" int wtg_batch_create(wtg_ctx* ctx, const uint8_t* d_sym, uint32_t nseq, const uint64_t* h_off,
+                                const uint64_t* h_n, const uint32_t* h_sigma, wtg_batch** out) {
+  WTG_REQUIRE(ctx && out && nseq >= 1 && h_off && h_n && h_sigma, "wtg_batch_create: null argument");
+  WTG_TRY(wtg_set_device(ctx));
+  *out = nullptr;
+  wtg_batch* b = new wtg_batch();
+  b->d_sym = d_sym;
+  b->nseq = nseq;
+  b->n.assign(h_n, h_n + nseq);
+  b->off.assign(h_off, h_off + nseq);
+  b->L.resize(nseq);
+  b->grp.resize(nseq);
+  b->idx.resize(nseq);
+  b->pref.resize(nseq);
+  int rc = WTG_OK;
+  uint64_t total = 0;
+  for (uint32_t i = 0; i < nseq; i++) {
+    if (h_sigma[i] < 1 || h_sigma[i] > 256) {
+      wtg_set_error("wtg_batch_create: alphabet size must be in [1, 256]");
+      batch_release(ctx, b);
+      return WTG_ERR_ARG;
+    }
+    b->L[i] = (uint32_t)std::max(1, log2up_u64(h_sigma[i]));
+    total += h_n[i];
+  }
+  if (total && !d_sym) {
+    wtg_set_error("wtg_batch_create: null symbols");
+    batch_release(ctx, b);
+    return WTG_ERR_ARG;
+  }
+  // groups by L, sequences in batch order inside a group
+  for (uint32_t L = 1; L <= 8; L++) {
+    Group G;
+    G.L = L;
+    for (uint32_t i = 0; i < nseq; i++)
+      if (b->L[i] == L) G.seqs.push_back(i);
+    if (G.seqs.empty()) continue;
+    const uint32_t m = (uint32_t)G.seqs.size();
+    G.sbits = (uint32_t)log2up_u64(m);
+    G.key_bytes = G.sbits + L <= 16 ? 2 : 4;
+    std::vector<uint64_t> so(m), pr(m + 1, 0);
+    std::vector<uint32_t> cnt(1ull << G.sbits, 0);
+    for (uint32_t t = 0; t < m; t++) {
+      const uint32_t i = G.seqs[t];
+      so[t] = b->off[i];
+      pr[t + 1] = pr[t] + b->n[i];
+      cnt[t] = (uint32_t)b->n[i];
+      b->grp[i] = (uint32_t)b->groups.size();
+      b->idx[i] = t;
+      b->pref[i] = pr[t];
+    }
+    G.n = pr[m];
+    if (G.n >= (1ull << 32) || G.sbits + L > 24) {
+      wtg_set_error("wtg_batch_create: a group exceeds 2^32 symbols or 2^16 sequences");
+      b->groups.push_back(G);
+      batch_release(ctx, b);
+      return WTG_ERR_UNSUPPORTED;
+    }
+    G.word_base = b->nwords;
+    b->nwords += wtg_levels_words(G.n, L) + 1;
+    if ((rc = wtg_malloc(ctx, (void**)&G.d_seqoff, m * 8)) != WTG_OK ||
+        (rc = wtg_malloc(ctx, (void**)&G.d_pref, (m + 1) * 8)) != WTG_OK ||
+        (rc = wtg_malloc(ctx, (void**)&G.d_cnt, cnt.size() * 4)) != WTG_OK ||
+        (rc = wtg_malloc(ctx, &G.d_keys, std::max<uint64_t>(G.n, 1) * G.key_bytes)) != WTG_OK) {
+      b->groups.push_back(G);
+      batch_release(ctx, b);
+      return rc;
+    }
+    if ((rc = wtg_memcpy_h2d(ctx, G.d_seqoff, so.data(), m * 8)) != WTG_OK ||
+        (rc = wtg_memcpy_h2d(ctx, G.d_pref, pr.data(), (m + 1) * 8)) != WTG_OK ||
+        (rc = wtg_memcpy_h2d(ctx, G.d_cnt, cnt.data(), cnt.size() * 4)) != WTG_OK) {
+      b->groups.push_back(G);
+      batch_release(ctx, b);
+      return rc;
+    }
+    b->ws_need = std::max(b->ws_need, wtg_levels_ws_bytes(G.n, G.sbits + L, G.key_bytes));
+    b->groups.push_back(G);
+  }
+  if ((rc = wtg_malloc(ctx, (void**)&b->words, std::max<uint64_t>(b->nwords, 1) * 8)) != WTG_OK) {
+    batch_release(ctx, b);
+    return rc;
+  }
+  if ((rc = wtg_fill_u64(ctx, b->words, std::max<uint64_t>(b->nwords, 1), 0)) != WTG_OK ||
+      (rc = wtg_ws_reserve(ctx, b->ws_need)) != WTG_OK || (rc = wtg_finish(ctx)) != WTG_OK) {
+    batch_release(ctx, b);
+    return rc;
+  }
+  *out = b;
+  return WTG_OK;
+}
+
+// every kernel of the batch, in stream order (no allocation, no host sync)
+static int batch_enqueue(wtg_ctx* ctx, wtg_batch* b) {
+  for (auto& G : b->groups) {
+    if (G.n == 0) continue;
+    const unsigned grid = (unsigned)wtg_ceil_div(G.n, 4096);
+    const uint32_t m = (uint32_t)G.seqs.size();
+    if (G.key_bytes == 2)
+      WTG_LAUNCH(ctx, "batch_keys", k_batch_keys<uint16_t>, dim3(grid), dim3(256), 0, b->d_sym, G.d_seqoff,
+                 G.d_pref, m, G.n, G.L, (uint16_t*)G.d_keys, ctx->d_flag);
+    else
+      WTG_LAUNCH(ctx, "batch_keys", k_batch_keys<uint32_t>, dim3(grid), dim3(256), 0, b->d_sym, G.d_seqoff,
+                 G.d_pref, m, G.n, G.L, (uint32_t*)G.d_keys, ctx->d_flag);
+    WTG_LAUNCH_CHECK();
+    WTG_TRY(wtg_levels_build_from(ctx, G.d_keys, G.key_bytes, G.n, G.sbits + G.L, G.sbits,
+                                  G.sbits ? G.d_cnt : nullptr, b->words + G.word_base));
+  }
+  return WTG_OK;
+}
+
+extern "C" int wtg_batch_set_graph(wtg_batch* b, int use_graph) {
+  WTG_REQUIRE(b, "null batch");
+  b->graph = use_graph != 0;
+  return WTG_OK;
+}
+
+extern "C" int wtg_batch_run(wtg_ctx* ctx, wtg_batch* b) {
+  WTG_REQUIRE(ctx && b, "wtg_batch_run: null argument");
+  WTG_TRY(wtg_set_device(ctx));
+  WTG_TRY(wtg_ws_reserve(ctx, b->ws_need));
+  if (!b->graph) {
+    WTG_TRY(batch_enqueue(ctx, b));
+    return wtg_finish(ctx);
+  }
+  if (b->exec && b->graph_ws != ctx->ws) {  // the workspace moved: record again
+    hipGraphExecDestroy(b->exec);
+    hipGraphDestroy(b->g);
+    b->exec = nullptr;
+    b->g = nullptr;
+  }
+  if (!b->exec) {
+    const bool kt = ctx->ktime;
+    ctx->ktime = false;  // no timing events inside the recorded graph
+    WTG_CHECK_HIP(hipStreamBeginCapture(ctx->stream, hipStreamCaptureModeThreadLocal));
+    int rc = batch_enqueue(ctx, b);
+    hipGraph_t g = nullptr;
+    const hipError_t e = hipStreamEndCapture(ctx->stream, &g);
+    ctx->ktime = kt;
+    if (rc != WTG_OK) {
+      if (g) hipGraphDestroy(g);
+      return rc;
+    }
+    WTG_CHECK_HIP(e);
+    b->g = g;
+    WTG_CHECK_HIP(hipGraphInstantiate(&b->exec, b->g, nullptr, nullptr, 0));
+    b->graph_ws = ctx->ws;
+  }
+  {
+    wtg_kscope ks(ctx, "batch_graph");
+    WTG_CHECK_HIP(hipGraphLaunch(b->exec, ctx->stream));
+  }
+  return wtg_finish(ctx);
+}
+
+extern "C" int wtg_batch_words(const wtg_batch* b, const uint64_t** d_words, uint64_t* nwords) {
+  WTG_REQUIRE(b, "null batch");
+  if (d_words) *d_words = b->words;
+  if (nwords) *nwords = b->nwords;
+  return WTG_OK;
+}
+
+extern "C" int wtg_batch_seq(const wtg_batch* b, uint32_t seq, uint32_t* levels, uint64_t* first_bit,
+                             uint64_t* level_stride) {
+  WTG_REQUIRE(b && seq < b->nseq, "wtg_batch_seq: bad sequence");
+  const Group& G = b->groups[b->grp[seq]];
+  if (levels) *levels = b->L[seq];
+  if (first_bit) *first_bit = G.word_base * 64 + b->pref[seq];
+  if (level_stride) *level_stride = G.n;
+  return WTG_OK;
+}
+
+extern "C" int wtg_batch_extract(wtg_ctx* ctx, const wtg_batch* b, uint32_t seq, uint64_t* d_out) {
+  WTG_REQUIRE(ctx && b && seq < b->nseq && d_out, "wtg_batch_extract: bad argument");
+  WTG_TRY(wtg_set_device(ctx));
+  const Group& G = b->groups[b->grp[seq]];
+  const uint64_t ni = b->n[seq];
+  const uint32_t L = b->L[seq];
+  const uint64_t nw = wtg_levels_words(ni, L);
+  if (!nw) return WTG_OK;
+  WTG_LAUNCH(ctx, "batch_extract", k_batch_extract, dim3((unsigned)wtg_ceil_div(nw, 256)), dim3(256), 0,
+             b->words + G.word_base, b->pref[seq], G.n, ni, L, d_out);
+  WTG_LAUNCH_CHECK();
+  return wtg_finish(ctx);
+}

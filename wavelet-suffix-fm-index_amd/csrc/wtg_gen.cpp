@@ -151,3 +151,28 @@ extern "C" int wtg_gen_select_vectors(uint64_t seed, uint32_t nvec, uint64_t nbi
   for (auto& x : th) x.join();
   return WTG_OK;
 }
+
+extern "C" int wtg_gen_batch(uint64_t seed, uint32_t nseq, double log2_min, double log2_max,
+                             const uint32_t* alphabets, uint32_t nalpha, uint64_t* n, uint32_t* sigma,
+                             uint8_t* sym) {
+  // n_i = floor(2^U(log2_min, log2_max)), a_i uniform over alphabets, symbols
+  // uniform in [0, a_i) (multiply-shift of a counter-based draw)
+  if (!n || !sigma || !alphabets || nalpha == 0 || !(log2_max >= log2_min) || log2_min < 0) return WTG_ERR_ARG;
+  std::vector<uint64_t> off(nseq + 1, 0);
+  for (uint32_t i = 0; i < nseq; i++) {
+    const double u = ctr_unit(seed ^ 0xB47C4ull, 2 * (uint64_t)i) - 1e-300;
+    n[i] = (uint64_t)floor(exp2(log2_min + (log2_max - log2_min) * u));
+    sigma[i] = alphabets[ctr_rand(seed ^ 0xB47C4ull, 2 * (uint64_t)i + 1) % nalpha];
+    if (sigma[i] == 0 || sigma[i] > 256) return WTG_ERR_ARG;
+    off[i + 1] = off[i] + n[i];
+  }
+  if (!sym) return WTG_OK;
+  par_for(nseq, [&](uint64_t i) {
+    const uint64_t si = mix64(seed + 0x3000 + i);
+    for (uint64_t k = 0; k < n[i]; k++) {
+      const unsigned __int128 p = (unsigned __int128)ctr_rand(si, k) * sigma[i];
+      sym[off[i] + k] = (uint8_t)(uint64_t)(p >> 64);
+    }
+  });
+  return WTG_OK;
+}

@@ -598,9 +598,52 @@ int launch_level(wtg_ctx* ctx, const LevelArgs& la, uint64_t tiles, int mode, bo
   return WTG_OK;
 }
 
+// ones of the level-`start` bit per level-`start` node (the c1 a split would
+// have counted); wave-aggregated when a wave's 16*64 keys share one node
+template <typename T>
+__global__ __launch_bounds__(256) void k_prefix_ones(const T* __restrict__ keys, uint64_t n, uint32_t bitpos,
+                                                     uint32_t node_shift, uint32_t* __restrict__ c1) {
+  const uint64_t base = ((uint64_t)blockIdx.x * 256 + threadIdx.x) * 16;
+  uint64_t first = WTG_NONE, last = 0;
+  uint32_t ones = 0;
+  uint64_t cur = WTG_NONE;
+  bool mixed = false;
+  for (int k = 0; k < 16; k++) {
+    if (base + k >= n) break;
+    const uint64_t x = (uint64_t)keys[base + k];
+    const uint64_t node = x >> node_shift;
+    if (cur != WTG_NONE && node != cur) mixed = true;
+    if (first == WTG_NONE) first = node;
+    last = node;
+    cur = node;
+    ones += (uint32_t)((x >> bitpos) & 1u);
+  }
+  const uint64_t f0 = __shfl((unsigned long long)first, 0);
+  const bool same = !mixed && (first == WTG_NONE || (first == f0 && last == f0));
+  if (__all(same)) {
+    const uint32_t t = wave_sum(ones);
+    if (lane_id() == 0 && t && f0 != WTG_NONE) atomicAdd(&c1[f0], t);
+    return;
+  }
+  for (int k = 0; k < 16; k++) {
+    if (base + k >= n) break;
+    const uint64_t x = (uint64_t)keys[base + k];
+    if ((x >> bitpos) & 1u) atomicAdd(&c1[x >> node_shift], 1u);
+  }
+}
+
+__global__ void k_copy_u32(const uint32_t* __restrict__ src, uint32_t* __restrict__ dst, uint64_t n) {
+  const uint64_t i = (uint64_t)blockIdx.x * blockDim.x + threadIdx.x;
+  if (i < n) dst[i] = src[i];
+}
+
+// Level pipeline over levels [first, L) of keys < 2^L.  first > 0: the keys
+// are already grouped by their top `first` bits (node of level `first`), whose
+// counts are in d_cnt_start[2^first]; bits of level k go to bit (k-first)*n.
 template <typename T>
 int levels_build_t(wtg_ctx* ctx, const T* d_sym, uint64_t n, uint32_t L, uint64_t* d_words,
-                   wtg_node* d_nodes, uint64_t node_slots, unsigned long long* d_distinct) {
+                   wtg_node* d_nodes, uint64_t node_slots, unsigned long long* d_distinct,
+                   uint32_t first = 0, const uint32_t* d_cnt_start = nullptr) {
   const int w = sizeof(T);
   LevelPlan P = plan_levels(n, L, w);
   WTG_TRY(wtg_ws_reserve(ctx, P.total));
@@ -615,8 +658,8 @@ int levels_build_t(wtg_ctx* ctx, const T* d_sym, uint64_t n, uint32_t L, uint64_
   uint32_t* zc = (uint32_t*)(ws + P.off_zc);
   uint32_t* zpre = (uint32_t*)(ws + P.off_zpre);
   void* buf[2] = {ws + P.off_buf0, ws + P.off_buf1};
-  const uint64_t nwords = wtg_levels_words(n, L);
-  const bool want_nodes = d_nodes && node_slots;
+  const uint64_t nwords = wtg_levels_words(n, L - first);
+  const bool want_nodes = d_nodes && node_slots && first == 0;
 
   // clear counts, scan states and counters (one contiguous range)
   WTG_TRY(wtg_fill_bytes_zero(ctx, ws + P.off_cnt, P.off_start - P.off_cnt));
@@ -625,33 +668,33 @@ int levels_build_t(wtg_ctx* ctx, const T* d_sym, uint64_t n, uint32_t L, uint64_
 
   const bool vec = ((uintptr_t)d_sym & 15) == 0;
   const void* src = d_sym;
-  for (uint32_t k = 0; k < L; k++) {
+  for (uint32_t k = first; k < L; k++) {
     const bool split = k + 1 < L;
     uint32_t* zck = zc + (size_t)k * (P.tiles + 1);
     LevelArgs la{};
     la.src = src;
     la.dst = buf[k & 1];
     la.words = d_words;
-    la.bit_base = (uint64_t)k * n;
+    la.bit_base = (uint64_t)(k - first) * n;
     la.n = (uint32_t)n;
     la.bitpos = L - 1 - k;
     la.node_shift = L - k;
     la.nnodes = 1u << k;
-    la.check_shift = (k == 0 && L < 64) ? L : 64;
+    la.check_shift = (k == first && L < 64) ? L : 64;
     la.tab = tab + heap_off(k);
     la.c1 = c1 + P.c1off[k + 1];
     la.reps = P.reps[k + 1];
     la.zc = zck;
     la.zc_next = split ? zc + (size_t)(k + 1) * (P.tiles + 1) : nullptr;
-    la.next_base = (uint64_t)(k + 1) * n;
+    la.next_base = (uint64_t)(k + 1 - first) * n;
     la.zpre = zpre;
     la.err = ctx->d_flag;
-    const bool kvec = k == 0 ? vec : true;
+    const bool kvec = k == first ? vec : true;
     if (n > 0) {
-      // level 0: bits pass (its bit words + zeros per tile); every later level's
-      // bits and tile zeros were produced by the previous split
-      if (k == 0) WTG_TRY(launch_level<T>(ctx, la, P.tiles, 0, kvec, false));
-      if (split || k == 0) {
+      // first level: bits pass (its bit words + zeros per tile); every later
+      // level's bits and tile zeros were produced by the previous split
+      if (k == first) WTG_TRY(launch_level<T>(ctx, la, P.tiles, 0, kvec, false));
+      if (split || k == first) {
         WTG_LAUNCH(ctx, "scan_tiles", k_scan_u32<uint32_t>, dim3((unsigned)P.scan_tiles), dim3(256), 0, zck, zpre,
                    P.tiles, scst + (size_t)k * (P.scan_tiles + 1), ctr + k, ctx->d_flag);
         WTG_LAUNCH_CHECK();
@@ -660,15 +703,24 @@ int levels_build_t(wtg_ctx* ctx, const T* d_sym, uint64_t n, uint32_t L, uint64_
         // CNT_0 = n, CNT_1 = (zeros, ones) of the top bit
         WTG_LAUNCH(ctx, "cnt_init", k_cnt_from_zeros, dim3(1), dim3(64), 0, cnt, zpre + P.tiles, n);
         WTG_LAUNCH_CHECK();
+      } else if (k == first) {
+        // CNT_start from the caller, and the level's ones per node
+        const uint64_t nn = 1ull << first;
+        WTG_LAUNCH(ctx, "cnt_start", k_copy_u32, dim3((unsigned)wtg_ceil_div(nn, 256)), dim3(256), 0,
+                   d_cnt_start, cnt + heap_off(first), nn);
+        WTG_LAUNCH_CHECK();
+        WTG_LAUNCH(ctx, "prefix_ones", k_prefix_ones<T>, dim3((unsigned)wtg_ceil_div(n, 4096)), dim3(256), 0,
+                   (const T*)src, n, L - 1 - k, L - k, c1 + P.c1off[k]);
+        WTG_LAUNCH_CHECK();
       }
     }
-    if (k == 0 && d_distinct && L == 1) {
+    if (k == 0 && first == 0 && d_distinct && L == 1) {
       WTG_LAUNCH(ctx, "distinct", k_count_nonzero, dim3(1), dim3(256), 0, cnt + heap_off(1), (uint64_t)2,
                  d_distinct);
       WTG_LAUNCH_CHECK();
     }
     // node table of level k (node starts always: cheap, needed by WTnodes)
-    if (k >= 1 && n > 0 && P.reps[k] > 1) {
+    if (k >= 1 && k > first && n > 0 && P.reps[k] > 1) {
       WTG_LAUNCH(ctx, "c1_reduce", k_c1_reduce, dim3((unsigned)(1u << k)), dim3(256), 0, c1 + P.c1off[k],
                  P.reps[k], (uint64_t)(1ull << k), c1 + P.c1off[k]);
       WTG_LAUNCH_CHECK();
@@ -741,6 +793,23 @@ int wtg_levels_build_async(wtg_ctx* ctx, const void* d_sym, int sym_bytes, uint6
     case 4: return levels_build_t<uint32_t>(ctx, (const uint32_t*)d_sym, n, levels, d_words, d_nodes, node_slots, d_distinct);
     default: return levels_build_t<uint64_t>(ctx, (const uint64_t*)d_sym, n, levels, d_words, d_nodes, node_slots, d_distinct);
   }
+}
+
+int wtg_levels_build_from(wtg_ctx* ctx, const void* d_keys, int key_bytes, uint64_t n, uint32_t levels,
+                          uint32_t start, const uint32_t* d_cnt_start, uint64_t* d_words) {
+  WTG_TRY(levels_check(ctx, key_bytes, n, levels));
+  WTG_REQUIRE(start < levels && (start == 0 || d_cnt_start), "levels_build_from: bad start level");
+  WTG_REQUIRE(n == 0 || (d_keys && d_words), "null device buffer");
+  switch (key_bytes) {
+    case 1: return levels_build_t<uint8_t>(ctx, (const uint8_t*)d_keys, n, levels, d_words, nullptr, 0, nullptr, start, d_cnt_start);
+    case 2: return levels_build_t<uint16_t>(ctx, (const uint16_t*)d_keys, n, levels, d_words, nullptr, 0, nullptr, start, d_cnt_start);
+    case 4: return levels_build_t<uint32_t>(ctx, (const uint32_t*)d_keys, n, levels, d_words, nullptr, 0, nullptr, start, d_cnt_start);
+    default: return levels_build_t<uint64_t>(ctx, (const uint64_t*)d_keys, n, levels, d_words, nullptr, 0, nullptr, start, d_cnt_start);
+  }
+}
+
+size_t wtg_levels_ws_bytes(uint64_t n, uint32_t levels, int key_bytes) {
+  return plan_levels(n, levels, key_bytes).total;
 }
 
 extern "C" int wtg_levels_build(wtg_ctx* ctx, const void* d_sym, int sym_bytes, uint64_t n,
