@@ -399,8 +399,9 @@ __global__ __launch_bounds__(256) void k_level(LevelArgs a) {
     }
     if (tid == 0) s_nb[NW] = 0;
     __syncthreads();
-    // ---- lane-consecutive stores of the staged tile
-#pragma unroll 4
+    // ---- lane-consecutive stores of the staged tile (unrolled by 16: the
+    // rounds' LDS reads and stores in flight together)
+#pragma unroll 16
     for (int r = 0; r < ROUNDS; r++) {
       const uint32_t q = (uint32_t)(r * 256 + tid);
       if (q < valid) {
