@@ -326,6 +326,74 @@ def bench_fm(pkg, args):
     return 0 if ok else 1
 
 
+def bench_bytes(pkg, args):
+    """Byte wavelet tree (levelWT): n = 2^24 uint8 Zipf(1.1) symbols over a seeded
+    byte permutation, 8 levels + rank directory per step; 10^6 uniform rank(c, i)
+    queries; bit-exact against the compiled reference serialWT."""
+    n, L = 1 << 24, 8
+    s = pkg.gen_zipf_bytes(SEED + 1, n, 1.1)
+    ctx = pkg.Context(0)
+    d = ctx.to_device(s)
+    nw = pkg.lib().wtg_levels_words(n, L)
+    dw = ctx.empty(nw, np.uint64)
+    D = ctx.empty(pkg.lib().wtg_rank9_words(n * L), np.uint64)
+    lib = pkg.lib()
+
+    def step():
+        ctx.levels_build(d, n, L, dw)
+        pkg._check(lib.wtg_rank9_build(ctx.h, dw.ptr, n * L, D.ptr))
+    ctx.set_blocking(False)
+    for _ in range(args.warmup):
+        step()
+    ctx.sync()
+    ctx.timer_start()
+    for _ in range(args.steps):
+        step()
+    ms = ctx.timer_stop() / args.steps
+    ctx.sync()
+    ctx.set_blocking(True)
+    words = dw.get()
+    from oracle import oracle
+    ref = oracle.RefWT("serialWT", "u8")
+    exp = ref.build(s, 256)[0] if ref.ok else oracle.levels_serial(s, L)[0]
+    ok = bool(np.array_equal(words, exp))
+    wt = ctx.wtint_build(d, n, L)
+    rng = np.random.default_rng(SEED + 2)
+    qi = rng.integers(0, n + 1, 1_000_000).astype(np.uint64)
+    qc = rng.integers(0, 256, 1_000_000).astype(np.uint64)
+    di, dc, do = ctx.to_device(qi), ctx.to_device(qc), ctx.empty(qi.size, np.uint64)
+    pkg._check(lib.wtg_wtint_rank_batch(ctx.h, wt.h, di.ptr, dc.ptr, qi.size, do.ptr))
+    ctx.timer_start()
+    for _ in range(args.steps):
+        pkg._check(lib.wtg_wtint_rank_batch(ctx.h, wt.h, di.ptr, dc.ptr, qi.size, do.ptr))
+    q_ms = ctx.timer_stop() / args.steps
+    got = do.get()
+    for k in range(0, 1000):
+        c, i = int(qc[k]), int(qi[k])
+        ok &= int(got[k]) == int((s[:i] == c).sum())
+    B_a = n * (2 * L * 1 + L / 8.0)
+    out = {"metric": "byte WT build Gsym/s", "value": round(n / (ms / 1e3) / 1e9, 4), "unit": "Gsym/s",
+           "n_gpus": 1, "steps": args.steps, "higher_is_better": True, "dtype": "u8", "data": "synthetic",
+           "config": {"workload": "levelWT bytes, n=2^24 Zipf(1.1) over a byte permutation, 8 levels + rank9"},
+           "ms_per_step": round(ms, 4),
+           "build_roofline": {"B_a_bytes": B_a, "frac": round(B_a / (ms / 1e3) / 1e9 / HBM_PEAK_GBS, 4)},
+           "rank_queries_Mq_s": round(qi.size / (q_ms / 1e3) / 1e6, 1),
+           "bit_exact_vs_reference_serialWT": ok}
+    if not args.no_cpu:
+        lw = oracle.RefWT("levelWT", "u8")
+        if lw.ok:
+            t = float(lw.time(s, 256, rounds=3).min())
+            out["cpu_baseline"] = {"value": round(n / t / 1e9, 5), "unit": "Gsym/s", "cores": int(lw.workers()),
+                                   "kind": "reference", "sample": "reference levelWT (OpenMP, byte build) on the "
+                                                                  "whole input, best of 3 (tree only)"}
+    for a in (d, dw, D, di, dc, do):
+        a.free()
+    wt.free()
+    ctx.close()
+    print(json.dumps(out), flush=True)
+    return 0 if ok else 1
+
+
 def main():
     ap = argparse.ArgumentParser()
     ap.add_argument("--gpus", type=int, default=1)
@@ -340,7 +408,7 @@ def main():
                          "the reference point for HBM byte counters)")
     ap.add_argument("--queries", type=int, default=10_000_000,
                     help="rank and select queries on the built tree (0: skip)")
-    ap.add_argument("--workload", default="wt_int", choices=["wt_int", "lsel", "batch", "fm"],
+    ap.add_argument("--workload", default="wt_int", choices=["wt_int", "bytes", "lsel", "batch", "fm"],
                     help="wt_int: the tree build (the headline line); lsel: learned vs plain select")
     ap.add_argument("--lsel-log2bits", type=int, default=26)
     ap.add_argument("--batch-seqs", type=int, default=4096)
@@ -356,6 +424,8 @@ def main():
     pkg = load_pkg()
     if args.workload == "lsel":
         return bench_lsel(pkg, args)
+    if args.workload == "bytes":
+        return bench_bytes(pkg, args)
     if args.workload == "batch":
         return bench_batch(pkg, args)
     if args.workload == "fm":

@@ -229,8 +229,8 @@ int wtg_fm_count_batch(wtg_ctx* ctx, const wtg_huff* bwt_wt, const uint8_t* d_pa
 /* ------------------------------------------------ batched level-wise trees */
 /* Many WT(s_i, n_i, a_i) builds (wavelet/levelWT/WT.h:43, levels
  * L_i = max(1, log2Up(a_i)), a_i <= 256, byte symbols) in one go.  Sequence i is
- * d_sym[h_off[i], h_off[i] + n_i).  Sequences are grouped by L_i; a group
- * holds at most 2^16 sequences and 2^32 - 1 symbols.  Output (device, owned by
+ * d_sym[h_off[i], h_off[i] + n_i).  Sequences are grouped by L_i, at most
+ * 2^(16-L_i) per group; a group holds at most 2^32 - 1 symbols.  Output (device, owned by
  * the batch): level l of sequence i is the bit range
  * [first_bit + l*level_stride, ... + n_i) of the batch words (wtg_batch_seq). */
 typedef struct wtg_batch wtg_batch;

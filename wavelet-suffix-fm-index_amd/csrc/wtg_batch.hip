@@ -2,9 +2,9 @@
 // sequences (the per-sequence call WT(s_i, n_i, a_i) of wavelet/levelWT/WT.h:43
 // for thousands of i at once).
 //
-// Sequences are grouped by their level count L = max(1, log2Up(a_i)).  Within a
-// group the sequences are concatenated and each symbol becomes the key
-// (group-local sequence id << L) | symbol (16 or 32 bits).  The keys are then
+// Sequences are grouped by their level count L = max(1, log2Up(a_i)), at most
+// 2^(16-L) per group.  Within a group the sequences are concatenated and each
+// symbol becomes the 16-bit key (group-local sequence id << L) | symbol.  The keys are then
 // already ordered by their top bits, so the level pipeline starts at level
 // `sbits` (the sequence-id bits) with the sequence lengths as that level's node
 // counts: levels sbits .. sbits+L-1 of the keys are exactly levels 0 .. L-1 of
@@ -21,34 +21,82 @@
 
 namespace {
 
+// keys of 4096 consecutive positions of the group per block, lane-consecutive
+// (coalesced byte loads and key stores); the block's sequence boundaries are
+// cached in LDS.  Also counts, per sequence, the ones of its top symbol bit
+// (level 0 of every sequence = the first level the pipeline builds).
+constexpr int BK_SEQ_CACHE = 64;
 template <typename K>
 __global__ __launch_bounds__(256) void k_batch_keys(const uint8_t* __restrict__ sym,
                                                     const uint64_t* __restrict__ seqoff,
                                                     const uint64_t* __restrict__ pref, uint32_t m, uint64_t n,
-                                                    uint32_t L, K* __restrict__ keys, uint32_t* __restrict__ err) {
-  __shared__ uint32_t s_first;
+                                                    uint32_t L, K* __restrict__ keys, uint32_t* __restrict__ ones,
+                                                    uint32_t* __restrict__ err) {
+  __shared__ uint32_t s_first, s_cnt;
+  __shared__ uint64_t s_pref[BK_SEQ_CACHE + 1], s_off[BK_SEQ_CACHE];
+  __shared__ uint32_t s_ones[BK_SEQ_CACHE];
   const uint64_t b0 = (uint64_t)blockIdx.x * 4096;
-  if (threadIdx.x == 0) {  // last sequence with pref <= b0
+  const uint64_t b1 = b0 + 4096 < n ? b0 + 4096 : n;
+  if (threadIdx.x == 0) {  // sequences covering [b0, b1)
     uint32_t lo = 0, hi = m;
     while (hi - lo > 1) {
       const uint32_t mid = (lo + hi) / 2;
       if (pref[mid] <= b0) lo = mid; else hi = mid;
     }
+    uint32_t l2 = lo, h2 = m;
+    while (h2 - l2 > 1) {
+      const uint32_t mid = (l2 + h2) / 2;
+      if (pref[mid] < b1) l2 = mid; else h2 = mid;
+    }
     s_first = lo;
+    s_cnt = l2 - lo + 1;
   }
   __syncthreads();
-  const uint64_t p0 = b0 + (uint64_t)threadIdx.x * 16;
-  uint32_t s = s_first;
+  const uint32_t f = s_first, cnt = s_cnt;
+  const bool cached = cnt <= BK_SEQ_CACHE;
+  if (cached) {
+    for (uint32_t k = threadIdx.x; k <= cnt; k += blockDim.x) s_pref[k] = pref[f + k];
+    for (uint32_t k = threadIdx.x; k < cnt; k += blockDim.x) { s_off[k] = seqoff[f + k]; s_ones[k] = 0; }
+  }
+  __syncthreads();
   bool bad = false;
+  uint32_t s = 0;  // sequence of the current element, relative to f
+  uint32_t run = 0, run_s = 0xFFFFFFFFu;
   for (int k = 0; k < 16; k++) {
-    const uint64_t p = p0 + k;
-    if (p >= n) break;
-    while (s + 1 < m && pref[s + 1] <= p) s++;
-    const uint32_t x = sym[seqoff[s] + (p - pref[s])];
+    const uint64_t p = b0 + (uint64_t)k * 256 + threadIdx.x;
+    if (p >= b1) break;
+    uint64_t ps, po;
+    if (cached) {
+      while (s + 1 < cnt && s_pref[s + 1] <= p) s++;
+      ps = s_pref[s];
+      po = s_off[s];
+    } else {
+      while (f + s + 1 < m && pref[f + s + 1] <= p) s++;
+      ps = pref[f + s];
+      po = seqoff[f + s];
+    }
+    const uint32_t x = sym[po + (p - ps)];
     bad |= (x >> L) != 0;
-    keys[p] = (K)(((uint64_t)s << L) | x);
+    keys[p] = (K)(((uint64_t)(f + s) << L) | x);
+    const uint32_t top = (x >> (L - 1)) & 1u;
+    if (run_s != s) {
+      if (run) {
+        if (cached) atomicAdd(&s_ones[run_s], run); else atomicAdd(&ones[f + run_s], run);
+      }
+      run_s = s;
+      run = 0;
+    }
+    run += top;
+  }
+  if (run) {
+    if (cached) atomicAdd(&s_ones[run_s], run); else atomicAdd(&ones[f + run_s], run);
   }
   if (bad) atomicOr(err, 2u);  // reported as an input symbol >= 2^levels
+  if (cached) {
+    __syncthreads();
+    for (uint32_t k = threadIdx.x; k < cnt; k += blockDim.x)
+      if (s_ones[k]) atomicAdd(&ones[f + k], s_ones[k]);
+  }
 }
 
 // copies `cnt` <= 64 bits starting at bit `b` of w (bits past the end are 0)
@@ -87,6 +135,7 @@ struct Group {
   uint64_t* d_seqoff = nullptr;
   uint64_t* d_pref = nullptr;
   uint32_t* d_cnt = nullptr;
+  uint32_t* d_ones = nullptr;   // ones of each sequence's top symbol bit (2^sbits)
   void* d_keys = nullptr;
   uint64_t word_base = 0;
 };
@@ -116,6 +165,7 @@ static void batch_release(wtg_ctx* ctx, wtg_batch* b) {
     wtg_free(ctx, G.d_seqoff);
     wtg_free(ctx, G.d_pref);
     wtg_free(ctx, G.d_cnt);
+    wtg_free(ctx, G.d_ones);
     wtg_free(ctx, G.d_keys);
   }
   wtg_free(ctx, b->words);
@@ -161,13 +211,23 @@ extern "C" int wtg_batch_create(wtg_ctx* ctx, const uint8_t* d_sym, uint32_t nse
     batch_release(ctx, b);
     return WTG_ERR_ARG;
   }
-  // groups by L, sequences in batch order inside a group
+  // groups by L, sequences in batch order inside a group, at most 2^(16-L)
+  // sequences per group so that every key fits 16 bits
+  std::vector<Group> plan;
   for (uint32_t L = 1; L <= 8; L++) {
-    Group G;
-    G.L = L;
+    std::vector<uint32_t> all;
     for (uint32_t i = 0; i < nseq; i++)
-      if (b->L[i] == L) G.seqs.push_back(i);
-    if (G.seqs.empty()) continue;
+      if (b->L[i] == L) all.push_back(i);
+    const size_t cap = (size_t)1 << (16 - L);
+    for (size_t a0 = 0; a0 < all.size(); a0 += cap) {
+      Group G;
+      G.L = L;
+      G.seqs.assign(all.begin() + a0, all.begin() + std::min(all.size(), a0 + cap));
+      plan.push_back(G);
+    }
+  }
+  for (Group& G : plan) {
+    const uint32_t L = G.L;
     const uint32_t m = (uint32_t)G.seqs.size();
     G.sbits = (uint32_t)log2up_u64(m);
     G.key_bytes = G.sbits + L <= 16 ? 2 : 4;
@@ -184,7 +244,7 @@ extern "C" int wtg_batch_create(wtg_ctx* ctx, const uint8_t* d_sym, uint32_t nse
     }
     G.n = pr[m];
     if (G.n >= (1ull << 32) || G.sbits + L > 24) {
-      wtg_set_error("wtg_batch_create: a group exceeds 2^32 symbols or 2^16 sequences");
+      wtg_set_error("wtg_batch_create: a group of sequences exceeds 2^32 - 1 symbols");
       b->groups.push_back(G);
       batch_release(ctx, b);
       return WTG_ERR_UNSUPPORTED;
@@ -194,6 +254,7 @@ extern "C" int wtg_batch_create(wtg_ctx* ctx, const uint8_t* d_sym, uint32_t nse
     if ((rc = wtg_malloc(ctx, (void**)&G.d_seqoff, m * 8)) != WTG_OK ||
         (rc = wtg_malloc(ctx, (void**)&G.d_pref, (m + 1) * 8)) != WTG_OK ||
         (rc = wtg_malloc(ctx, (void**)&G.d_cnt, cnt.size() * 4)) != WTG_OK ||
+        (rc = wtg_malloc(ctx, (void**)&G.d_ones, cnt.size() * 4)) != WTG_OK ||
         (rc = wtg_malloc(ctx, &G.d_keys, std::max<uint64_t>(G.n, 1) * G.key_bytes)) != WTG_OK) {
       b->groups.push_back(G);
       batch_release(ctx, b);
@@ -228,15 +289,17 @@ static int batch_enqueue(wtg_ctx* ctx, wtg_batch* b) {
     if (G.n == 0) continue;
     const unsigned grid = (unsigned)wtg_ceil_div(G.n, 4096);
     const uint32_t m = (uint32_t)G.seqs.size();
+    WTG_TRY(wtg_fill_bytes_zero(ctx, G.d_ones, (1ull << G.sbits) * 4));
     if (G.key_bytes == 2)
       WTG_LAUNCH(ctx, "batch_keys", k_batch_keys<uint16_t>, dim3(grid), dim3(256), 0, b->d_sym, G.d_seqoff,
-                 G.d_pref, m, G.n, G.L, (uint16_t*)G.d_keys, ctx->d_flag);
+                 G.d_pref, m, G.n, G.L, (uint16_t*)G.d_keys, G.d_ones, ctx->d_flag);
     else
       WTG_LAUNCH(ctx, "batch_keys", k_batch_keys<uint32_t>, dim3(grid), dim3(256), 0, b->d_sym, G.d_seqoff,
-                 G.d_pref, m, G.n, G.L, (uint32_t*)G.d_keys, ctx->d_flag);
+                 G.d_pref, m, G.n, G.L, (uint32_t*)G.d_keys, G.d_ones, ctx->d_flag);
     WTG_LAUNCH_CHECK();
     WTG_TRY(wtg_levels_build_from(ctx, G.d_keys, G.key_bytes, G.n, G.sbits + G.L, G.sbits,
-                                  G.sbits ? G.d_cnt : nullptr, b->words + G.word_base));
+                                  G.sbits ? G.d_cnt : nullptr, b->words + G.word_base,
+                                  G.sbits ? G.d_ones : nullptr));
   }
   return WTG_OK;
 }

@@ -75,39 +75,35 @@ __device__ __forceinline__ void wave_sync_lds() {
   __builtin_amdgcn_fence(__ATOMIC_ACQUIRE, "wavefront");
 }
 
-// byte histogram: per chunk of 64 bytes, lanes with equal bytes are counted
-// together (8 rounds of ballots), the rest by LDS atomics
+// byte histogram: LDS atomics into 4 replicas per wave (lane & 3), so lanes
+// holding the same byte mostly hit different words; replicas summed at the end
+constexpr int HF_HIST_PIECES = 8;  // pieces of 1 KiB per wave
 __global__ __launch_bounds__(HF_THREADS) void k_huff_hist(const uint8_t* __restrict__ text, uint64_t n,
                                                           unsigned long long* __restrict__ hist, bool aligned) {
-  __shared__ uint32_t s_h[256];
-  for (int k = threadIdx.x; k < 256; k += HF_THREADS) s_h[k] = 0;
+  __shared__ uint32_t s_h[HF_THREADS / 64][4][256];
+  uint32_t* flat = &s_h[0][0][0];
+  for (int k = threadIdx.x; k < (HF_THREADS / 64) * 4 * 256; k += HF_THREADS) flat[k] = 0;
   __syncthreads();
-  const int lane = lane_id();
-  const uint64_t waves = (uint64_t)gridDim.x * (HF_THREADS / 64);
-  const uint64_t pieces = (n + HF_PIECE - 1) / HF_PIECE;
-  for (uint64_t pc = (uint64_t)blockIdx.x * (HF_THREADS / 64) + (threadIdx.x >> 6); pc < pieces; pc += waves) {
-    const uint64_t base = pc * HF_PIECE + (uint64_t)lane * 16;
+  const int lane = lane_id(), w = threadIdx.x >> 6;
+  uint32_t* h = s_h[w][lane & 3];
+  const uint64_t base0 = ((uint64_t)blockIdx.x * (HF_THREADS / 64) + w) * HF_HIST_PIECES * HF_PIECE;
+#pragma unroll
+  for (int pc = 0; pc < HF_HIST_PIECES; pc++) {
+    const uint64_t base = base0 + (uint64_t)pc * HF_PIECE + (uint64_t)lane * 16;
+    if (base >= n) break;
     uint8_t b[16];
     const uint4 v = load16(text, n, base, aligned);
     memcpy(b, &v, 16);
 #pragma unroll
-    for (int k = 0; k < 16; k++) {
-      const bool valid = base + k < n;
-      const uint32_t c = b[k];
-      uint64_t rem = wave_ballot(valid);
-      for (int it = 0; it < 8 && rem; it++) {
-        const int ld = __builtin_ctzll(rem);
-        const uint32_t s = __builtin_amdgcn_readlane(c, ld);
-        const uint64_t m = wave_ballot(valid && c == s);
-        rem &= ~m;
-        if (lane == ld) atomicAdd(&s_h[s], popc64(m));
-      }
-      if ((rem >> lane) & 1ull) atomicAdd(&s_h[c], 1u);
-    }
+    for (int k = 0; k < 16; k++)
+      if (base + k < n) atomicAdd(&h[b[k]], 1u);
   }
   __syncthreads();
-  for (int k = threadIdx.x; k < 256; k += HF_THREADS)
-    if (s_h[k]) atomicAdd(&hist[k], (unsigned long long)s_h[k]);
+  for (int c = threadIdx.x; c < 256; c += HF_THREADS) {
+    uint32_t t = 0;
+    for (int r = 0; r < (HF_THREADS / 64) * 4; r++) t += flat[r * 256 + c];
+    if (t) atomicAdd(&hist[c], (unsigned long long)t);
+  }
 }
 
 // per-node values held across the wave: lane j keeps node j + 64q in a[q]
@@ -240,6 +236,12 @@ __global__ __launch_bounds__(HF_THREADS) void k_huff_emit(SegArgs a) {
     wave_sync_lds();
     // local run starts: exclusive scan of this segment's node counts
     NodeRegs lstart{{0, 0, 0, 0}}, run{{0, 0, 0, 0}};
+    uint64_t gpos[4];  // each run's global bit position, loaded up front (lane j: node j+64q)
+#pragma unroll
+    for (int q = 0; q < 4; q++) {
+      const uint32_t v = (uint32_t)(64 * q + lane);
+      gpos[q] = v < a.n_inner ? a.pos[(uint64_t)v * a.nseg + s] : 0ull;
+    }
     uint32_t carry = 0;
 #pragma unroll
     for (int q = 0; q < 4; q++) {
@@ -288,7 +290,10 @@ __global__ __launch_bounds__(HF_THREADS) void k_huff_emit(SegArgs a) {
     for (uint32_t v = 0; v < a.n_inner; v++) {
       const uint32_t len = run.get(v);
       if (!len) continue;
-      const uint64_t g0 = a.pos[(uint64_t)v * a.nseg + s];
+      const uint32_t qv = v >> 6;
+      const uint64_t gq = qv == 0 ? gpos[0] : qv == 1 ? gpos[1] : qv == 2 ? gpos[2] : gpos[3];
+      const uint64_t g0 = ((uint64_t)(uint32_t)__builtin_amdgcn_readlane((int)(uint32_t)(gq >> 32), (int)(v & 63)) << 32) |
+                          (uint32_t)__builtin_amdgcn_readlane((int)(uint32_t)gq, (int)(v & 63));
       const int64_t ls = (int64_t)lstart.get(v);
       const uint64_t w0 = g0 >> 6, w1 = (g0 + len - 1) >> 6;
       for (uint64_t kw = w0 + lane; kw <= w1; kw += 64) {
@@ -614,7 +619,7 @@ extern "C" int wtg_huff_build(wtg_ctx* ctx, const uint8_t* d_text, uint64_t n, w
   if ((rc = wtg_malloc(ctx, (void**)&wt->d_tab, sizeof(HuffTab))) != WTG_OK) goto fail;
   if ((rc = wtg_fill_u64(ctx, wt->d_hist, 256, 0)) != WTG_OK) goto fail;
   {
-    const unsigned grid = (unsigned)std::min<uint64_t>(wtg_ceil_div(n, 4 * HF_PIECE), 2048);
+    const unsigned grid = (unsigned)wtg_ceil_div(n, (uint64_t)(HF_THREADS / 64) * HF_HIST_PIECES * HF_PIECE);
     WTG_LAUNCH(ctx, "huff_hist", k_huff_hist, dim3(grid), dim3(HF_THREADS), 0, d_text, n,
                (unsigned long long*)wt->d_hist, ((uintptr_t)d_text & 15) == 0);
     WTG_LAUNCH_CHECK();

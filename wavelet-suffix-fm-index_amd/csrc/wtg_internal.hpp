@@ -75,8 +75,10 @@ int wtg_levels_build_async(wtg_ctx* ctx, const void* d_sym, int sym_bytes, uint6
                            uint64_t node_slots, unsigned long long* d_distinct);
 // level pipeline over levels [start, levels) of keys already grouped by their
 // top `start` bits (counts in d_cnt_start[2^start]); level k at bit (k-start)*n
+// (d_ones_start: the first level's ones per node when the caller has them)
 int wtg_levels_build_from(wtg_ctx* ctx, const void* d_keys, int key_bytes, uint64_t n, uint32_t levels,
-                          uint32_t start, const uint32_t* d_cnt_start, uint64_t* d_words);
+                          uint32_t start, const uint32_t* d_cnt_start, uint64_t* d_words,
+                          const uint32_t* d_ones_start = nullptr);
 size_t wtg_levels_ws_bytes(uint64_t n, uint32_t levels, int key_bytes);
 int wtg_rank_select_run(wtg_ctx* ctx, const uint64_t* d_words, uint64_t nbits, uint64_t* d_dir,
                         bool build, wtg_select* sel1, wtg_select* sel0);

@@ -644,7 +644,8 @@ __global__ void k_copy_u32(const uint32_t* __restrict__ src, uint32_t* __restric
 template <typename T>
 int levels_build_t(wtg_ctx* ctx, const T* d_sym, uint64_t n, uint32_t L, uint64_t* d_words,
                    wtg_node* d_nodes, uint64_t node_slots, unsigned long long* d_distinct,
-                   uint32_t first = 0, const uint32_t* d_cnt_start = nullptr) {
+                   uint32_t first = 0, const uint32_t* d_cnt_start = nullptr,
+                   const uint32_t* d_ones_start = nullptr) {
   const int w = sizeof(T);
   LevelPlan P = plan_levels(n, L, w);
   WTG_TRY(wtg_ws_reserve(ctx, P.total));
@@ -710,8 +711,13 @@ int levels_build_t(wtg_ctx* ctx, const T* d_sym, uint64_t n, uint32_t L, uint64_
         WTG_LAUNCH(ctx, "cnt_start", k_copy_u32, dim3((unsigned)wtg_ceil_div(nn, 256)), dim3(256), 0,
                    d_cnt_start, cnt + heap_off(first), nn);
         WTG_LAUNCH_CHECK();
-        WTG_LAUNCH(ctx, "prefix_ones", k_prefix_ones<T>, dim3((unsigned)wtg_ceil_div(n, 4096)), dim3(256), 0,
-                   (const T*)src, n, L - 1 - k, L - k, c1 + P.c1off[k]);
+        if (d_ones_start) {
+          WTG_LAUNCH(ctx, "ones_start", k_copy_u32, dim3((unsigned)wtg_ceil_div(nn, 256)), dim3(256), 0,
+                     d_ones_start, c1 + P.c1off[k], nn);
+        } else {
+          WTG_LAUNCH(ctx, "prefix_ones", k_prefix_ones<T>, dim3((unsigned)wtg_ceil_div(n, 4096)), dim3(256), 0,
+                     (const T*)src, n, L - 1 - k, L - k, c1 + P.c1off[k]);
+        }
         WTG_LAUNCH_CHECK();
       }
     }
@@ -797,15 +803,16 @@ int wtg_levels_build_async(wtg_ctx* ctx, const void* d_sym, int sym_bytes, uint6
 }
 
 int wtg_levels_build_from(wtg_ctx* ctx, const void* d_keys, int key_bytes, uint64_t n, uint32_t levels,
-                          uint32_t start, const uint32_t* d_cnt_start, uint64_t* d_words) {
+                          uint32_t start, const uint32_t* d_cnt_start, uint64_t* d_words,
+                          const uint32_t* d_ones_start) {
   WTG_TRY(levels_check(ctx, key_bytes, n, levels));
   WTG_REQUIRE(start < levels && (start == 0 || d_cnt_start), "levels_build_from: bad start level");
   WTG_REQUIRE(n == 0 || (d_keys && d_words), "null device buffer");
   switch (key_bytes) {
-    case 1: return levels_build_t<uint8_t>(ctx, (const uint8_t*)d_keys, n, levels, d_words, nullptr, 0, nullptr, start, d_cnt_start);
-    case 2: return levels_build_t<uint16_t>(ctx, (const uint16_t*)d_keys, n, levels, d_words, nullptr, 0, nullptr, start, d_cnt_start);
-    case 4: return levels_build_t<uint32_t>(ctx, (const uint32_t*)d_keys, n, levels, d_words, nullptr, 0, nullptr, start, d_cnt_start);
-    default: return levels_build_t<uint64_t>(ctx, (const uint64_t*)d_keys, n, levels, d_words, nullptr, 0, nullptr, start, d_cnt_start);
+    case 1: return levels_build_t<uint8_t>(ctx, (const uint8_t*)d_keys, n, levels, d_words, nullptr, 0, nullptr, start, d_cnt_start, d_ones_start);
+    case 2: return levels_build_t<uint16_t>(ctx, (const uint16_t*)d_keys, n, levels, d_words, nullptr, 0, nullptr, start, d_cnt_start, d_ones_start);
+    case 4: return levels_build_t<uint32_t>(ctx, (const uint32_t*)d_keys, n, levels, d_words, nullptr, 0, nullptr, start, d_cnt_start, d_ones_start);
+    default: return levels_build_t<uint64_t>(ctx, (const uint64_t*)d_keys, n, levels, d_words, nullptr, 0, nullptr, start, d_cnt_start, d_ones_start);
   }
 }
 
