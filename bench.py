@@ -406,6 +406,8 @@ def main():
     ap.add_argument("--calibrate-copy", action="store_true",
                     help="first copy 1 GiB device-to-device twice (a plain copy of known size, "
                          "the reference point for HBM byte counters)")
+    ap.add_argument("--no-geometric", dest="geometric", action="store_false",
+                    help="skip the geometric(1e-4) second input")
     ap.add_argument("--queries", type=int, default=10_000_000,
                     help="rank and select queries on the built tree (0: skip)")
     ap.add_argument("--workload", default="wt_int", choices=["wt_int", "bytes", "lsel", "batch", "fm"],
@@ -477,6 +479,30 @@ def main():
         if int(lv.sum()) != int(((s >> (L - 1 - l)) & 1).sum()):
             ok = False
     del bits, words
+    # the second input of the same workload: geometric(p = 1e-4) values capped at
+    # 2^20 - 1, the same 20 levels (wt_int max_level = 20)
+    geo = None
+    if args.geometric:
+        g = pkg.gen_geometric(SEED + 9, n, 1e-4, (1 << L) - 1, np.uint32)
+        dg = ctx.to_device(g)
+        ctx.levels_build(dg, n, L, dw)
+        ctx.set_blocking(False)
+        ctx.timer_start()
+        for _ in range(3):
+            ctx.levels_build(dg, n, L, dw)
+        gms = ctx.timer_stop() / 3
+        ctx.sync()
+        ctx.set_blocking(True)
+        gw = dw.get()
+        gok = True
+        gb = np.unpackbits(gw.view(np.uint8), bitorder="little")
+        for l in range(L):
+            if int(gb[l * n:(l + 1) * n].sum()) != int(((g >> (L - 1 - l)) & 1).sum()):
+                gok = False
+        del gb, gw
+        dg.free()
+        geo = {"Gsym_s": round(n / (gms / 1e3) / 1e9, 4), "ms": round(gms, 4), "level_popcounts_match": gok}
+
     d.free()
     dw.free()
     ctx.close()
@@ -506,6 +532,7 @@ def main():
                            "hist_ms_per_step": round(hist_ms / args.steps, 4)},
         "parity": {"level_popcounts_match": ok},
         "rank_select": extras,
+        "geometric_input": geo,
         "wall_s": round(wall, 3),
     }
     if not args.no_cpu:

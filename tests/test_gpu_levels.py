@@ -114,3 +114,13 @@ def test_levels_large_bit_exact(ctx, wsfm, orc):
     bits = np.unpackbits(w.view(np.uint8), bitorder="little")[:n * L].reshape(L, n)
     for l in range(L):
         assert bits[l].sum() == int(((s >> (L - 1 - l)) & 1).sum())
+
+
+def test_levels_geometric_twenty_levels(wsfm, orc, ctx):
+    # the second wt_int input kind: geometric(1e-4) values capped at 2^20 - 1, 20 levels
+    n, L = 1 << 21, 20
+    s = wsfm.gen_geometric(21, n, 1e-4, (1 << L) - 1, np.uint32)
+    print("geometric", n, L, flush=True)
+    words, _ = ctx.levels(s, L)
+    exp = orc.levels_closed(s, L)
+    assert np.array_equal(words, exp)

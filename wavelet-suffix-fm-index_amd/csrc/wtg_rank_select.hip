@@ -9,13 +9,16 @@
 // superblock k of 4096 arguments, a = pos(4096k), e = pos(min(4096k+4095,m-1));
 // e-a > logn^4 -> "long" (all 4096 positions) else "mini" (pos(4096k+64t)-a).
 //
-// One thread per directory entry (512 bits).  A block of 256 entries is one
-// look-back tile.  The same thread emits, for each argument rank j in its
-// entry with j%64==0, the low 32 bits of pos(j) straight into slot j/64 of the
-// final mini array (mini offsets are < logn^4 < 2^32, so subtracting the low
-// 32 bits of the superblock start later gives the exact offset), and the
-// superblock starts/ends.  A second kernel classifies long/mini superblocks
-// and rebases the mini offsets; a third fills the rare long superblocks.
+// Two passes over the words: ones per 512-word tile (k_rank_tile_sums) and,
+// after their exclusive scan, one lane per word (k_rank_words): lanes 8i..8i+7
+// of a wave round form directory entry i (the 9-bit fields are the lanes'
+// prefix differences, OR-ed across the 8 lanes), and each lane emits the
+// sample of the argument rank j%64==0 its word holds (at most one per bit
+// value): the low 32 bits of pos(j) straight into slot j/64 of the final mini
+// array (mini offsets are < logn^4 < 2^32, so subtracting the low 32 bits of
+// the superblock start later gives the exact offset), and the superblock
+// starts/ends.  A second kernel classifies long/mini superblocks and rebases
+// the mini offsets; a third fills the rare long superblocks.
 #include "wtg_internal.hpp"
 #include <cstring>
 #include <vector>
@@ -38,137 +41,127 @@ struct RankSelArgs {
   uint64_t* states;
   uint32_t* counter;
   uint32_t* err;
+  const uint64_t* tpre;  // BUILD: ones before each tile (exclusive scan of k_rank_tile_sums)
   unsigned long long* ones_total;  // out: total ones
   SelOut sel[2];         // [0] = select0, [1] = select1
   int want[2];
 };
 
-__device__ __forceinline__ void emit_samples(const uint64_t* wk, const uint32_t* cum, uint64_t pre,
-                                             uint32_t cnt, uint64_t wbase, bool inv,
-                                             const uint64_t* vmask, const SelOut& o) {
-  // argument ranks j in [pre, pre+cnt) of this entry
-  if (cnt == 0) return;
-  const uint64_t end = pre + cnt;
-  for (uint64_t j = (pre + 63) & ~63ull; j < end; j += 64) {
-    uint32_t r = (uint32_t)(j - pre);
-    int k = 0;
+// ones per tile of 512 words (64 directory entries), one wave per tile,
+// lane-consecutive loads
+constexpr int RS_TILE_WORDS = 512;
+__global__ __launch_bounds__(256) void k_rank_tile_sums(const uint64_t* __restrict__ words, uint64_t W,
+                                                        uint64_t ntiles, uint32_t* __restrict__ tsum) {
+  const uint64_t t = (uint64_t)blockIdx.x * 4 + (threadIdx.x >> 6);
+  if (t >= ntiles) return;
+  const uint64_t w0 = t * RS_TILE_WORDS + lane_id();
+  uint32_t c = 0;
 #pragma unroll
-    for (int q = 1; q < 8; q++) k += (cum[q] <= r) ? 1 : 0;
-    uint64_t x = inv ? (~wk[k] & vmask[k]) : wk[k];
-    uint64_t pos = (wbase + k) * 64 + select_in_word(x, r - cum[k]);
+  for (int k = 0; k < RS_TILE_WORDS / 64; k++) {
+    const uint64_t wi = w0 + (uint64_t)k * 64;
+    c += wi < W ? popc64(words[wi]) : 0u;
+  }
+  c = wave_sum(c);
+  if (lane_id() == 0) tsum[t] = c;
+}
+
+// Directory entries and select samples, one lane per word, one wave per tile
+// of 512 words.  Lanes 8i..8i+7 of a round hold directory entry i of the round;
+// a word holds at most one argument rank that is a multiple of 64 (per bit
+// value), so each lane emits at most one sample per select structure.
+__device__ __forceinline__ void word_sample(uint64_t x, uint64_t r, uint32_t c, uint64_t w, const SelOut& o) {
+  if (c == 0) return;
+  const uint64_t j = (r + 63) & ~63ull;
+  if (j < r + c) {
+    const uint64_t pos = (w << 6) + select_in_word(x, (uint32_t)(j - r));
     o.mini[j >> 6] = (uint32_t)pos;
     if ((j & 4095) == 0) o.sbstart[j >> 12] = pos;
   }
-  // superblock end: rank j with j%4096 == 4095
-  uint64_t je = ((pre >> 12) << 12) + 4095;
-  if (je < pre) je += 4096;
-  if (je < end) {
-    uint32_t r = (uint32_t)(je - pre);
-    int k = 0;
-#pragma unroll
-    for (int q = 1; q < 8; q++) k += (cum[q] <= r) ? 1 : 0;
-    uint64_t x = inv ? (~wk[k] & vmask[k]) : wk[k];
-    o.sbend[je >> 12] = (wbase + k) * 64 + select_in_word(x, r - cum[k]);
-  }
+  uint64_t je = ((r >> 12) << 12) + 4095;
+  if (je < r) je += 4096;
+  if (je < r + c) o.sbend[je >> 12] = (w << 6) + select_in_word(x, (uint32_t)(je - r));
 }
 
-template <bool BUILD>
-__global__ __launch_bounds__(256) void k_rank_select(RankSelArgs a) {
-  __shared__ uint32_t s_wave[4];
-  __shared__ uint32_t s_tile;
-  __shared__ uint64_t s_pre;
-  __shared__ unsigned long long s_last[2];
-  uint32_t tile;
-  if (BUILD) {
-    if (threadIdx.x == 0) s_tile = atomicAdd(a.counter, 1u);
-    if (threadIdx.x < 2) s_last[threadIdx.x] = 0;
-    __syncthreads();
-    tile = s_tile;
-  } else {
-    if (threadIdx.x < 2) s_last[threadIdx.x] = 0;
-    __syncthreads();
-    tile = blockIdx.x;
-  }
-  const uint64_t b = (uint64_t)tile * 256 + threadIdx.x;
-  const bool act = b <= a.S;
-  uint64_t wk[8], vm[8];
-  uint32_t c1[9];
-  c1[0] = 0;
-  const uint64_t wb = b * 8;
+// OR of x over the 8 lanes of the caller's group (lanes 8i..8i+7), valid in
+// the group's first lane: two quad permutations, then a row rotation by 12
+__device__ __forceinline__ uint32_t dpp_or8(uint32_t x) {
+  x |= (uint32_t)__builtin_amdgcn_update_dpp(0, (int)x, 0xB1, 0xF, 0xF, false);   // quad_perm(1,0,3,2)
+  x |= (uint32_t)__builtin_amdgcn_update_dpp(0, (int)x, 0x4E, 0xF, 0xF, false);   // quad_perm(2,3,0,1)
+  x |= (uint32_t)__builtin_amdgcn_update_dpp(0, (int)x, 0x12C, 0xF, 0xF, false);  // row_ror:12 (lane i <- lane i+4 mod 16)
+  return x;
+}
+
+__global__ __launch_bounds__(256) void k_rank_words(RankSelArgs a, uint64_t ntiles, bool write_dir) {
+  const uint64_t t = (uint64_t)blockIdx.x * 4 + (threadIdx.x >> 6);
+  if (t >= ntiles) return;
+  const int lane = lane_id();
+  const uint32_t j = (uint32_t)lane & 7u;
+  uint64_t carry = a.tpre[t];
+  uint64_t last1 = 0, last0 = 0;
+  for (int r = 0; r < RS_TILE_WORDS / 64; r++) {
+    const uint64_t w = t * RS_TILE_WORDS + (uint64_t)r * 64 + lane;
+    const uint64_t x = w < a.W ? a.words[w] : 0ull;
+    const uint64_t lo = w * 64;
+    const uint64_t vm = lo >= a.nbits ? 0ull : (a.nbits - lo >= 64 ? ~0ull : ((1ull << (a.nbits - lo)) - 1));
+    const uint32_t c = popc64(x);
+    // wave prefix, prefix inside the lane's 8-word entry and wave total from
+    // the 7 bit planes of c (ballots and popcounts, no cross-lane shuffles)
+    const uint64_t lt = (1ull << lane) - 1;
+    const uint64_t gm = lt & ~((1ull << (lane & ~7)) - 1);
+    uint32_t ex = 0, gex = 0, tot = 0;
 #pragma unroll
-  for (int k = 0; k < 8; k++) {
-    const uint64_t wi = wb + k;
-    wk[k] = (act && wi < a.W) ? a.words[wi] : 0ull;
-    // valid-bit mask of the word (positions < nbits)
-    uint64_t lo = wi * 64;
-    vm[k] = (lo >= a.nbits) ? 0ull : ((a.nbits - lo >= 64) ? ~0ull : ((1ull << (a.nbits - lo)) - 1));
-    c1[k + 1] = c1[k] + popc64(wk[k]);
+    for (int bp = 0; bp < 7; bp++) {
+      const uint64_t B = wave_ballot((c >> bp) & 1u);
+      ex += popc64(B & lt) << bp;
+      gex += popc64(B & gm) << bp;
+      tot += popc64(B) << bp;
+    }
+    const uint64_t pre = carry + ex;  // ones before word w
+    carry += tot;
+    const uint64_t b = w >> 3;         // directory entry
+    if (write_dir) {
+      const uint64_t m = a.W > 8 * b ? (a.W - 8 * b < 8 ? a.W - 8 * b : 8) : 0;  // valid words of the entry
+      const uint64_t f = (j >= 1 && j <= m && j <= 7) ? (uint64_t)gex << (63 - 9 * j) : 0ull;
+      const uint32_t flo = dpp_or8((uint32_t)f), fhi = dpp_or8((uint32_t)(f >> 32));
+      if (j == 0 && b <= a.S) {
+        a.D[2 * b] = pre;
+        a.D[2 * b + 1] = ((uint64_t)fhi << 32) | flo;
+      }
+    }
+    if (w < a.W) {
+      if (a.want[1]) {
+        word_sample(x, pre, c, w, a.sel[1]);
+        if (c) last1 = lo + (63 - __clzll(x)) + 1;
+      }
+      if (a.want[0]) {
+        const uint64_t z = ~x & vm;
+        const uint64_t zpre = (lo < a.nbits ? lo : a.nbits) - pre;
+        word_sample(z, zpre, popc64(z), w, a.sel[0]);
+        if (z) last0 = lo + (63 - __clzll(z)) + 1;
+      }
+      if (w + 1 == a.W) *a.ones_total = pre + c;
+    }
   }
-  const uint32_t tot1 = act ? c1[8] : 0;
-  uint64_t pre1;
-  if (BUILD) {
-    uint32_t btot;
-    uint32_t ex = 0;
-    {
-      const int w = threadIdx.x >> 6, l = lane_id();
-      uint32_t incl = wave_incl_scan(tot1);
-      if (l == 63) s_wave[w] = incl;
-      __syncthreads();
-      uint32_t off = 0, tt = 0;
+  // last argument position (+1): only the tile holding the last one (the last
+  // zero) writes it, a plain store; the scanned tile counts identify that tile
+  const uint64_t T = a.tpre[ntiles], tp0 = a.tpre[t], tp1 = a.tpre[t + 1];
+  const uint64_t v0 = t * RS_TILE_WORDS * 64 < a.nbits ? t * RS_TILE_WORDS * 64 : a.nbits;
+  const uint64_t v1 = (t + 1) * RS_TILE_WORDS * 64 < a.nbits ? (t + 1) * RS_TILE_WORDS * 64 : a.nbits;
+  const bool has_last1 = tp0 < T && tp1 == T;
+  const bool has_last0 = v0 - tp0 < a.nbits - T && v1 - tp1 == a.nbits - T;
+  if (has_last1 || has_last0) {
 #pragma unroll
-      for (int q = 0; q < 4; q++) { uint32_t t = s_wave[q]; if (q < w) off += t; tt += t; }
-      btot = tt;
-      ex = incl - tot1 + off;
+    for (int d = 1; d < 64; d <<= 1) {
+      const uint64_t o1 = (uint64_t)__shfl_xor((unsigned long long)last1, d);
+      const uint64_t o0 = (uint64_t)__shfl_xor((unsigned long long)last0, d);
+      last1 = o1 > last1 ? o1 : last1;
+      last0 = o0 > last0 ? o0 : last0;
     }
-    if (threadIdx.x < 64) {
-      uint64_t p = lb_exclusive(a.states, tile, btot, a.err);
-      if (threadIdx.x == 0) s_pre = p;
-    }
-    __syncthreads();
-    pre1 = s_pre + ex;
-    if (act) {
-      // fields: m = valid words of this entry (0 for the padding entry)
-      const uint64_t m = (a.W > wb) ? ((a.W - wb < 8) ? a.W - wb : 8) : 0;
-      uint64_t f = 0;
-      uint64_t i;
-      for (i = 1; i < m; i++) f |= (uint64_t)c1[i] << (63 - 9 * i);
-      if (m == 0) i = 1;
-      if ((i & 7) != 0) f |= (uint64_t)c1[i] << (63 - 9 * (i & 7));
-      a.D[2 * b] = pre1;
-      a.D[2 * b + 1] = f;
-      if (b == a.S) *a.ones_total = pre1 + tot1;
-    }
-  } else {
-    pre1 = act ? a.D[2 * b] : 0;
-  }
-  if (act && a.want[1]) {
-    emit_samples(wk, c1, pre1, tot1, wb, false, vm, a.sel[1]);
-    if (tot1) {
-      int k = 7;
-      while (wk[k] == 0) k--;
-      unsigned long long lp = (wb + k) * 64 + (63 - __clzll(wk[k])) + 1;
-      atomicMax(&s_last[1], lp);
+    if (lane == 0) {
+      if (a.want[1] && has_last1) *a.sel[1].last = last1;
+      if (a.want[0] && has_last0) *a.sel[0].last = last0;
     }
   }
-  if (act && a.want[0]) {
-    uint32_t c0[9];
-    c0[0] = 0;
-#pragma unroll
-    for (int k = 0; k < 8; k++) c0[k + 1] = c0[k] + popc64(~wk[k] & vm[k]);
-    const uint64_t vbits_before = (b * 512 < a.nbits) ? b * 512 : a.nbits;
-    const uint64_t pre0 = vbits_before - pre1;
-    emit_samples(wk, c0, pre0, c0[8], wb, true, vm, a.sel[0]);
-    if (c0[8]) {
-      int k = 7;
-      while ((~wk[k] & vm[k]) == 0) k--;
-      uint64_t z = ~wk[k] & vm[k];
-      unsigned long long lp = (wb + k) * 64 + (63 - __clzll(z)) + 1;
-      atomicMax(&s_last[0], lp);
-    }
-  }
-  __syncthreads();
-  if (threadIdx.x < 2 && a.want[threadIdx.x] && s_last[threadIdx.x])
-    atomicMax(a.sel[threadIdx.x].last, s_last[threadIdx.x]);
 }
 
 // ----------------------------------------------------- select assembly
@@ -359,6 +352,12 @@ int wtg_rank_select_run(wtg_ctx* ctx, const uint64_t* d_words, uint64_t nbits, u
   // workspace: states | counter | ones_total | last[2] | sbend[2] | finish scratch
   size_t o = 0;
   const size_t o_states = o; o = wtg_align256(o + (tiles + 1) * 8);
+  // word tiles: every entry 0..S has its first word 8b inside them
+  const uint64_t wtiles = wtg_ceil_div(8 * (S + 1), RS_TILE_WORDS);
+  const uint64_t stiles = wtg_ceil_div(wtiles, 4096);
+  const size_t o_tsum = o; o = wtg_align256(o + (wtiles + 1) * 4);
+  const size_t o_tpre = o; o = wtg_align256(o + (wtiles + 1) * 8);
+  const size_t o_scst = o; o = wtg_align256(o + (stiles + 1) * 8);
   const size_t o_ctr = o; o = wtg_align256(o + 256);
   const size_t o_end0 = o; o = wtg_align256(o + sbmax * 8);
   const size_t o_end1 = o; o = wtg_align256(o + sbmax * 8);
@@ -399,10 +398,19 @@ int wtg_rank_select_run(wtg_ctx* ctx, const uint64_t* d_words, uint64_t nbits, u
       }
     return WTG_OK;
   }
-  if (build) {
-    WTG_LAUNCH(ctx, "rank_select", k_rank_select<true>, dim3((unsigned)tiles), dim3(256), 0, ra);
-  } else {
-    WTG_LAUNCH(ctx, "rank_select", k_rank_select<false>, dim3((unsigned)tiles), dim3(256), 0, ra);
+  {
+    // ones per 512-word tile, their exclusive scan, then one pass writing the
+    // directory entries and the select samples (a wave per tile)
+    ra.tpre = (const uint64_t*)(ws + o_tpre);
+    const unsigned g4 = (unsigned)wtg_ceil_div(wtiles, 4);
+    WTG_LAUNCH(ctx, "rank_sums", k_rank_tile_sums, dim3(g4), dim3(256), 0, d_words, W, wtiles,
+               (uint32_t*)(ws + o_tsum));
+    WTG_LAUNCH_CHECK();
+    WTG_LAUNCH(ctx, "rank_scan", k_scan_u32<uint64_t>, dim3((unsigned)stiles), dim3(256), 0,
+               (const uint32_t*)(ws + o_tsum), (uint64_t*)(ws + o_tpre), wtiles, (uint64_t*)(ws + o_scst), ctr + 1,
+               ctx->d_flag);
+    WTG_LAUNCH_CHECK();
+    WTG_LAUNCH(ctx, "rank_select", k_rank_words, dim3(g4), dim3(256), 0, ra, wtiles, build);
   }
   WTG_LAUNCH_CHECK();
   uint64_t m1 = 0;
