@@ -151,19 +151,25 @@ def bench_lsel(pkg, args):
         ms = ctx.timer_stop() / args.steps
         res[name] = {"Mq_s": round(nq / (ms / 1e3) / 1e6, 1), "ms": round(ms, 3)}
     same = bool(np.array_equal(out_p.get(), out_l.get()))
-    nl = sum(1 for i in infos if i["spacing"])
-    model_bytes = sum(i["nknots"] * 4 for i in infos if i["spacing"])
+    nl = sum(1 for i in infos if i["learned"])
+    model_bytes = sum(i["nknots"] * 8 + ((i["ones"] >> i["lut_log"]) + 1) * 4 for i in infos if i["learned"])
+    plain_bytes = sum(i["nbits"] // 8 for i in infos)
     out = {"metric": "select Mq/s learned vs plain (bit-exact)", "value": res["learned"]["Mq_s"],
            "unit": "Mq/s", "n_gpus": 1, "steps": args.steps, "higher_is_better": True,
            "dtype": "u64", "data": "synthetic",
            "config": {"workload": "learned select, 64 bitvectors x 2^%d bits, %d uniform queries"
                                   % (args.lsel_log2bits, nq)},
            "plain_Mq_s": res["plain"]["Mq_s"], "speedup": round(res["learned"]["Mq_s"] / res["plain"]["Mq_s"], 3),
-           "identical_answers": same, "learned_vectors": nl, "model_bytes": model_bytes,
+           "identical_answers": same, "learned_vectors": nl,
+           "trained_vectors": sum(1 for i in infos if i["learned"] and i["trained"]),
+           "model_bytes": model_bytes, "model_bytes_per_bitvector_byte": round(model_bytes / plain_bytes, 4),
            "build_and_train_s": round(build_s, 3),
-           "eps_max_learned": max([i["eps"] for i in infos if i["spacing"]] or [0]),
-           "per_vector": [{"d": round(float(dens[b]), 4), "spacing": infos[b]["spacing"],
-                           "eps": infos[b]["eps"], "eps_init": infos[b]["eps_init"]} for b in range(nvec)]}
+           "eps_max_learned": max([i["eps"] for i in infos if i["learned"]] or [0]),
+           "per_vector": [{"d": round(float(dens[b]), 4), "learned": infos[b]["learned"],
+                           "trained": infos[b]["trained"], "nknots": infos[b]["nknots"],
+                           "eps": infos[b]["eps"], "eps_init": infos[b]["eps_init"],
+                           "loss_init": round(infos[b]["loss_init"], 1),
+                           "loss_final": round(infos[b]["loss_final"], 1)} for b in range(nvec)]}
     ls.free()
     for a in dv + [dvq, djq, out_p, out_l]:
         a.free()

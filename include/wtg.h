@@ -249,46 +249,65 @@ int wtg_batch_extract(wtg_ctx* ctx, const wtg_batch* b, uint32_t seq, uint64_t* 
 
 /* ------------------------------------------------ learned select (added form) */
 /* No counterpart in the reference: select over a batch of bitvectors where a
- * per-vector piecewise-linear model, trained on the GPU on the answers of the
- * plain sampled select (select_support_mcl_par.hpp:564-611), predicts select(j)
- * and rank9 (rank_support_v_par.hpp:131-142) plus a bounded word scan recovers
- * the exact position.  Answers always equal the plain method's. */
+ * per-vector piecewise-linear model predicts select(j) and the rank9 directory
+ * (rank_support_v_par.hpp:131-142) recovers the exact position.  The model's
+ * knots are placed by a greedy error cone over the vector's ones and trained on
+ * the GPU (forward/backward kernels, Adam) on the answers of the plain sampled
+ * select (select_support_mcl_par.hpp:564-611).  Answers always equal the plain
+ * method's. */
 typedef struct {
   uint32_t train_stride;  /* training samples select(T), select(2T), ...; power of two (16) */
-  uint32_t spacing_max;   /* largest knot spacing tried, in arguments; power of two (4096) */
-  uint32_t spacing_min;   /* smallest, >= 2*train_stride (64) */
-  uint32_t epochs;        /* Adam steps per spacing (200) */
+  uint32_t max_gap;       /* longest knot interval in arguments; power of two, 2..2^24 (4096) */
+  uint32_t tol;           /* knot placement tolerance in bits, 1 <= tol < eps_max - 1 (256) */
+  uint32_t epochs;        /* Adam steps (200) */
   float lr;               /* Adam step in bit positions (4) */
-  uint32_t eps_max;       /* accepted error window in bits (512) */
+  uint32_t eps_max;       /* accepted error window in bits, <= 512 (512) */
 } wtg_lsel_params;
 typedef struct {
   uint64_t nbits, ones;
-  uint32_t spacing;       /* knot spacing kept; 0 = the vector answers with plain select */
+  uint32_t learned;       /* 1: answered through the model; 0: by the plain sampled select */
   uint32_t nknots;
-  uint64_t eps;           /* max |prediction - select(j)| over every j (exhaustive) */
-  uint64_t eps_init;      /* the same for the untrained knots at that spacing */
-  double loss_init, loss_final;  /* mean squared residual on the training samples */
+  uint64_t eps;           /* max |prediction - select(j)| over every j (exhaustive), model in use */
+  uint64_t eps_init;      /* the same for the placed, untrained knots */
+  double loss_init, loss_final;  /* mean squared residual on the training samples, placed / in use */
+  uint32_t trained;       /* 1: the Adam-trained knots are in use; 0: the placed ones (training
+                             raised eps above eps_max) */
+  uint32_t lut_log;       /* knot lookup table: one entry per 2^lut_log arguments */
 } wtg_lsel_vec_info;
+/* A vector's model on the device: d_knots = nknots pairs {uint32 j; float y}
+ * (j strictly increasing from 1 to ones), pred(j) = alpha*j + beta + linear
+ * interpolation of y; d_lut[i] = the knot interval holding argument i << lut_log. */
+typedef struct {
+  const void* d_knots;
+  uint32_t nknots, lut_log;
+  const uint32_t* d_lut;
+  uint64_t lut_size;
+  double alpha, beta;
+  uint32_t learned, _pad;
+} wtg_lsel_model_view;
 typedef struct wtg_lsel wtg_lsel;
 /* d_words[b]: device words of vector b (nbits[b] bits, zero padding word after
- * them), 1 <= nvec <= 256; params NULL = defaults.  Builds rank9 + select1 per vector, trains. */
+ * them), 1 <= nvec <= 256; params NULL = defaults.  Builds rank9 + select1 per
+ * vector, places and trains the models, measures eps exhaustively. */
 int wtg_lsel_build(wtg_ctx* ctx, uint32_t nvec, const uint64_t* const* d_words,
                    const uint64_t* nbits, const wtg_lsel_params* params, wtg_lsel** out);
 int wtg_lsel_destroy(wtg_ctx* ctx, wtg_lsel* ls);
 int wtg_lsel_info(const wtg_lsel* ls, uint32_t vec, wtg_lsel_vec_info* out);
+int wtg_lsel_model(const wtg_lsel* ls, uint32_t vec, wtg_lsel_model_view* out);
 /* out[q] = select1 of vector d_vec[q] at rank d_j[q] (1-based); WTG_NONE when
- * d_j[q] is 0 or above the vector's ones */
+ * d_j[q] is 0 or above the vector's ones, or d_vec[q] >= nvec */
 int wtg_lsel_select_batch(wtg_ctx* ctx, const wtg_lsel* ls, const uint32_t* d_vec,
                           const uint64_t* d_j, uint64_t nq, uint64_t* d_out);
 /* the same queries answered by the plain sampled select only */
 int wtg_lsel_plain_batch(wtg_ctx* ctx, const wtg_lsel* ls, const uint32_t* d_vec,
                          const uint64_t* d_j, uint64_t nq, uint64_t* d_out);
 /* Training loss (mean squared residual) and its gradient for one model, by the
- * training kernels: knots d_y[nknots] every `spacing` arguments, samples
- * d_pos[t] = select((t+1)*stride), baseline alpha*j + beta. */
-int wtg_lsel_loss_grad(wtg_ctx* ctx, const float* d_y, uint32_t nknots, uint32_t spacing,
-                       uint32_t stride, double alpha, double beta, const uint64_t* d_pos,
-                       uint64_t ns, double* loss, float* d_grad);
+ * training kernels: d_knots = nknots pairs {uint32 j; float y} (j increasing),
+ * samples d_pos[t] = select((t+1)*stride) (those outside [j_0, j_last] unseen),
+ * baseline alpha*j + beta. */
+int wtg_lsel_loss_grad(wtg_ctx* ctx, const void* d_knots, uint32_t nknots, uint32_t stride,
+                       double alpha, double beta, const uint64_t* d_pos, uint64_t ns,
+                       double* loss, float* d_grad);
 
 /* ------------------------------------------------ synthetic workloads (host) */
 /* Seeded generators of the benchmark inputs (splitmix64 streams). */

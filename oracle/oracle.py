@@ -326,27 +326,87 @@ class RefWT:
 
 
 # ----------------------------------------------------------- learned select
-def spline_predict(y, spacing, alpha, beta, j):
-    """Piecewise-linear select model (float64): alpha*j + beta + knot interpolation."""
-    j = np.asarray(j, np.float64)
-    k = np.floor(j / spacing).astype(np.int64)
-    f = (j - k * spacing) / spacing
+def spline_interval(J, j):
+    """Knot interval k of each argument j: J[k] <= j < J[k+1], clamped to [0, K-2]."""
+    J = np.asarray(J, np.float64)
+    return np.clip(np.searchsorted(J, np.asarray(j, np.float64), side="right") - 1, 0, J.size - 2)
+
+
+def spline_predict(J, y, alpha, beta, j):
+    """Continuous piecewise-linear select model (float64): alpha*j + beta plus the
+    linear interpolation of residual knots y at strictly increasing arguments J."""
+    J = np.asarray(J, np.float64)
     y = np.asarray(y, np.float64)
+    j = np.asarray(j, np.float64)
+    k = spline_interval(J, j)
+    f = (j - J[k]) / (J[k + 1] - J[k])
     return alpha * j + beta + (1 - f) * y[k] + f * y[k + 1]
 
 
-def spline_loss_grad(y, spacing, stride, alpha, beta, pos):
+def spline_loss_grad(J, y, stride, alpha, beta, pos):
     """Mean squared residual of the model on samples pos[t] = select((t+1)*stride)
-    and its gradient with respect to the knots (float64, closed form)."""
+    (those outside [J[0], J[-1]] unseen, the mean still over all of them) and its
+    gradient with respect to the knots (float64, closed form)."""
+    J = np.asarray(J, np.float64)
     y = np.asarray(y, np.float64)
     pos = np.asarray(pos, np.float64)
     ns = pos.size
     j = (np.arange(ns, dtype=np.float64) + 1) * stride
-    k = np.floor(j / spacing).astype(np.int64)
-    f = (j - k * spacing) / spacing
+    keep = (j >= J[0]) & (j <= J[-1])
+    j, pos = j[keep], pos[keep]
+    k = spline_interval(J, j)
+    f = (j - J[k]) / (J[k + 1] - J[k])
     e = alpha * j + beta + (1 - f) * y[k] + f * y[k + 1] - pos
     loss = float((e * e).sum() / ns)
     g = np.zeros_like(y)
     np.add.at(g, k, 2.0 / ns * e * (1 - f))
     np.add.at(g, k + 1, 2.0 / ns * e * f)
     return loss, g
+
+
+def place_knots(ones_pos, tol, gap):
+    """Greedy error-cone knot placement of the learned select (pure Python).
+
+    ones_pos[i] = position of the (i+1)-th one.  Arguments are cut into chunks
+    of `gap`; in each chunk a knot sits on the first one, and from the last knot
+    (x0, y0) the cone of slopes s with |y0 + s*(j - x0) - pos(j)| <= tol for every
+    one seen shrinks until it is empty; the knot then goes to the previous
+    argument, on the cone's middle slope.  The chunk's last argument closes it.
+    Returns (J, value): knot arguments and float64 knot positions (every one is
+    within tol of the interpolation).  Float operations in the order of the GPU
+    placement kernel, unfused."""
+    m = len(ones_pos)
+    J, V = [], []
+    for c in range((m + gap - 1) // gap):
+        j0 = c * gap + 1
+        j1 = min((c + 1) * gap, m)
+        x0 = float(j0)
+        y0 = float(ones_pos[j0 - 1])
+        lo, hi = -1e300, 1e300
+        jp = j0
+        J.append(j0)
+        V.append(y0)
+        for jj in range(j0 + 1, j1 + 1):
+            P = float(ones_pos[jj - 1])
+            dx = float(jj) - x0
+            nlo = max(lo, (P - tol - y0) / dx)
+            nhi = min(hi, (P + tol - y0) / dx)
+            if nlo > nhi:
+                ye = y0 + 0.5 * (lo + hi) * (float(jp) - x0)
+                J.append(jp)
+                V.append(ye)
+                x0, y0 = float(jp), ye
+                dx = float(jj) - x0
+                nlo = (P - tol - y0) / dx
+                nhi = (P + tol - y0) / dx
+            lo, hi = nlo, nhi
+            jp = jj
+        if float(jp) > x0:
+            J.append(jp)
+            V.append(y0 + 0.5 * (lo + hi) * (float(jp) - x0))
+    return np.array(J, np.uint64), np.array(V, np.float64)
+
+
+def model_line(alpha, beta, j):
+    """alpha*j + beta, unfused (as the GPU model computes it)."""
+    return alpha * np.asarray(j, np.float64) + beta

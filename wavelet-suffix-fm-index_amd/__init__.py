@@ -47,14 +47,24 @@ class HuffShape(C.Structure):
 
 
 class LselParams(C.Structure):
-    _fields_ = [("train_stride", u32), ("spacing_max", u32), ("spacing_min", u32),
+    _fields_ = [("train_stride", u32), ("max_gap", u32), ("tol", u32),
                 ("epochs", u32), ("lr", C.c_float), ("eps_max", u32)]
 
 
 class LselInfo(C.Structure):
-    _fields_ = [("nbits", u64), ("ones", u64), ("spacing", u32), ("nknots", u32),
+    _fields_ = [("nbits", u64), ("ones", u64), ("learned", u32), ("nknots", u32),
                 ("eps", u64), ("eps_init", u64), ("loss_init", C.c_double),
-                ("loss_final", C.c_double)]
+                ("loss_final", C.c_double), ("trained", u32), ("lut_log", u32)]
+
+
+class LselModelView(C.Structure):
+    _fields_ = [("d_knots", C.c_void_p), ("nknots", u32), ("lut_log", u32),
+                ("d_lut", C.c_void_p), ("lut_size", u64), ("alpha", C.c_double),
+                ("beta", C.c_double), ("learned", u32), ("_pad", u32)]
+
+
+# a knot of the learned-select model: {uint32 j; float y}
+KNOT_DTYPE = np.dtype([("j", "<u4"), ("y", "<f4")])
 
 
 # name -> (restype, argtypes); every symbol here is declared in include/wtg.h
@@ -113,9 +123,10 @@ SIGNATURES = {
     "wtg_lsel_build": (i32, [vp, u32, C.POINTER(vp), U64P, C.POINTER(LselParams), C.POINTER(vp)]),
     "wtg_lsel_destroy": (i32, [vp, vp]),
     "wtg_lsel_info": (i32, [vp, u32, C.POINTER(LselInfo)]),
+    "wtg_lsel_model": (i32, [vp, u32, C.POINTER(LselModelView)]),
     "wtg_lsel_select_batch": (i32, [vp, vp, vp, vp, u64, vp]),
     "wtg_lsel_plain_batch": (i32, [vp, vp, vp, vp, u64, vp]),
-    "wtg_lsel_loss_grad": (i32, [vp, vp, u32, u32, u32, C.c_double, C.c_double, vp, u64,
+    "wtg_lsel_loss_grad": (i32, [vp, vp, u32, u32, C.c_double, C.c_double, vp, u64,
                                  C.POINTER(C.c_double), vp]),
     "wtg_gen_select_vectors": (i32, [u64, u32, u64, vp, vp]),
     "wtg_gen_batch": (i32, [u64, u32, C.c_double, C.c_double, vp, u32, vp, vp, vp]),
@@ -474,6 +485,17 @@ class LearnedSelect:
         _check(lib().wtg_lsel_info(self.h, b, C.byref(i)))
         return {k: getattr(i, k) for k, _ in LselInfo._fields_}
 
+    def model(self, b):
+        """(knots as a KNOT_DTYPE array, lookup table, lut_log, alpha, beta) of vector b."""
+        m = LselModelView()
+        _check(lib().wtg_lsel_model(self.h, b, C.byref(m)))
+        knots = np.zeros(m.nknots, KNOT_DTYPE)
+        lut = np.zeros(m.lut_size, np.uint32)
+        if m.nknots:
+            _check(lib().wtg_memcpy_d2h(self.ctx.h, _p(knots), C.c_void_p(m.d_knots), knots.nbytes))
+            _check(lib().wtg_memcpy_d2h(self.ctx.h, _p(lut), C.c_void_p(m.d_lut), lut.nbytes))
+        return knots, lut, m.lut_log, m.alpha, m.beta
+
     def query(self, d_vec, d_j, d_out, learned=True):
         fn = lib().wtg_lsel_select_batch if learned else lib().wtg_lsel_plain_batch
         _check(fn(self.ctx.h, self.h, d_vec.ptr, d_j.ptr, d_j.size, d_out.ptr))
@@ -496,18 +518,22 @@ class LearnedSelect:
             self.h = None
 
 
-def lsel_loss_grad(ctx, y, spacing, stride, alpha, beta, pos):
-    """(loss, grad) of one spline model by the GPU training kernels."""
-    dy = ctx.to_device(np.ascontiguousarray(y, np.float32))
+def lsel_loss_grad(ctx, J, y, stride, alpha, beta, pos):
+    """(loss, grad) of one spline model (knots at arguments J, values y) by the
+    GPU training kernels."""
+    knots = np.zeros(len(y), KNOT_DTYPE)
+    knots["j"] = J
+    knots["y"] = y
+    dk = ctx.to_device(knots)
     dp = ctx.to_device(np.ascontiguousarray(pos, np.uint64))
     dg = ctx.empty(len(y), np.float32)
     loss = C.c_double(0)
     try:
-        _check(lib().wtg_lsel_loss_grad(ctx.h, dy.ptr, len(y), spacing, stride, alpha, beta, dp.ptr,
+        _check(lib().wtg_lsel_loss_grad(ctx.h, dk.ptr, len(y), stride, alpha, beta, dp.ptr,
                                         len(pos), C.byref(loss), dg.ptr))
         return loss.value, dg.get()
     finally:
-        dy.free(); dp.free(); dg.free()
+        dk.free(); dp.free(); dg.free()
 
 
 class Batch:

@@ -1,22 +1,34 @@
 // wtg_lsel.hip -- learned select over a batch of bitvectors.
 //
 // Per bitvector: rank9 directory + select_support_mcl<1> samples (the plain
-// method), then a piecewise-linear model of j -> select(j):
-//   pred(j) = alpha*j + beta + (1-f)*y[k] + f*y[k+1],  k = j / S, f = (j mod S) / S
+// method), then a continuous piecewise-linear model of j -> select(j) with
+// knots at chosen arguments 1 = J_0 < J_1 < ... < J_{K-1} = ones:
+//   pred(j) = alpha*j + beta + (1-f)*y[k] + f*y[k+1],   J_k <= j < J_{k+1},
+//   f = (j - J_k) / (J_{k+1} - J_k)
 // alpha, beta: the line through the first and last training sample (fp64);
-// y: fp32 residual knots every S arguments, trained on the GPU (forward +
-// backward kernel, Adam) to minimise the mean squared error on the training
-// samples select(T), select(2T), ... that the plain method answers.
-// The error window eps = max_j |round(pred(j)) - select(j)| is then measured
-// over EVERY one of the vector (exhaustive kernel).  Knot spacings are tried
-// from the largest down; the first with eps <= eps_max is kept, otherwise the
-// vector answers with the plain sampled select.
+// y: fp32 residual knots.
 //
-// Query: p = clamp(round(pred(j))); r = rank9(p); the answer is the (j-r)-th
-// one at/after p, or the (r-j+1)-th one before p -- at most eps/64 + 1 words
-// away.  The result is exact whatever the model predicts; eps only bounds the
-// scan.  The plain sampled select stays available (wtg_lsel_plain_batch) as
-// the judge.
+// Build:
+//   1. placement (k_lsel_place): the arguments are cut into chunks of
+//      `max_gap`, one thread per chunk.  A greedy cone walks the chunk's ones
+//      in order from the last knot and sets a knot where no line from it stays
+//      within `tol` bits of every one seen, so |pred(j) - select(j)| <= tol for
+//      every j.  A run of ones is one segment; the gap between two runs costs
+//      two knots.  Counted in one pass, written in a second.
+//   2. training: forward + backward kernel (thread per knot interval) and Adam
+//      on every knot, minimising the mean squared error on the training
+//      samples select(T), select(2T), ... that the plain method answers.
+//   3. the error window eps = max_j |round(pred(j)) - select(j)| is measured
+//      over EVERY one (exhaustive kernel).  If training raised it above
+//      eps_max, the placed knots are restored (their eps <= tol + 1).
+//
+// Query: a lookup table indexed by j >> lut_log gives the knot interval (a
+// short forward probe finishes it); the prediction p names the 512-bit rank9
+// block b; eps <= 512 puts select(j) in blocks b-1..b+1, whose directory
+// entries give the block and, by their 9-bit word counts, the word; one word
+// read and an in-word select finish.  The answer is exact whatever the model
+// predicts: outside that window the plain select answers.  The plain sampled
+// select stays available (wtg_lsel_plain_batch) as the judge.
 #include "wtg_internal.hpp"
 #include <algorithm>
 #include <cmath>
@@ -36,57 +48,92 @@ __device__ __forceinline__ uint64_t wave_max_u64(uint64_t v) {
   return v;
 }
 
+struct Knot {
+  uint32_t j;   // argument (1-based rank)
+  float y;      // residual against alpha*j + beta
+};
+
 struct LselDesc {
   const uint64_t* words;
   const uint64_t* dir;
   SelQ s1;
   uint64_t nbits, ones;
-  const float* y;        // nknots residual knots (learned vectors)
+  uint64_t nblk;         // rank9 entries 0..nblk
+  const Knot* knots;
+  const uint32_t* lut;   // (ones >> lut_log) + 1 entries: knot interval of argument i << lut_log
   double alpha, beta;
-  uint32_t slog;         // log2 of the knot spacing
-  uint32_t nknots;
+  uint32_t lut_log, nknots;
   int32_t learned;
   int32_t _pad;
 };
 
-// the model's prediction as the position it starts from
+// the model line, unfused (the placement and the oracle compute it the same way)
+__device__ __forceinline__ double lsel_line(double alpha, double beta, double j) {
+#pragma clang fp contract(off)
+  return alpha * j + beta;
+}
+
+// the model's prediction for argument j (1 <= j <= ones), clamped to the vector
 __device__ __forceinline__ uint64_t lsel_predict(const LselDesc& d, uint64_t j) {
-  const uint64_t k = j >> d.slog;
-  const float f = (float)(j & ((1ull << d.slog) - 1)) / (float)(1ull << d.slog);
-  const float s = (1.0f - f) * d.y[k] + f * d.y[k + 1];
-  const double p = fma(d.alpha, (double)j, d.beta) + (double)s;
+  uint32_t k = d.lut[j >> d.lut_log];
+  Knot a = d.knots[k], b = d.knots[k + 1];
+  while (j >= b.j && k + 2 < d.nknots) {
+    ++k;
+    a = b;
+    b = d.knots[k + 1];
+  }
+  const float f = (float)(j - a.j) / (float)(b.j - a.j);
+  const float s = (1.0f - f) * a.y + f * b.y;
+  const double p = lsel_line(d.alpha, d.beta, (double)j) + (double)s;
   const double hi = (double)(d.nbits - 1);
   return p <= 0.0 ? 0ull : p >= hi ? d.nbits - 1 : (uint64_t)llrint(p);
 }
 
-// exact select1(j) from a start position p: rank9 at p, then a word scan
-// forward or backward to the j-th one
+// exact select1(j) from a predicted position p: the rank9 entries of blocks
+// b-1..b+2 (b = p/512) bracket the argument's block, its 9-bit fields the word
 __device__ __forceinline__ uint64_t lsel_correct(const LselDesc& d, uint64_t j, uint64_t p) {
-  const uint64_t r = rank9_dev(d.dir, d.words, p);
-  const uint64_t W = (d.nbits + 63) >> 6;
-  uint64_t w = p >> 6;
-  const uint32_t o = (uint32_t)(p & 63);
-  if (r < j) {
-    uint64_t k = j - r;
-    uint64_t x = d.words[w] & (~0ull << o);
-    for (;;) {
-      const uint32_t c = popc64(x);
-      if (c >= k) return (w << 6) + select_in_word(x, (uint32_t)(k - 1));
-      k -= c;
-      if (++w >= W) return WTG_NONE;
-      x = d.words[w];
+  const uint64_t b = p >> 9;
+  const uint64_t b0 = b ? b - 1 : 0;
+  uint64_t st[4], fl[3];
+  if (b0 + 3 <= d.nblk) {
+    const uint4* q = reinterpret_cast<const uint4*>(d.dir + 2 * b0);  // 16-byte entries
+    uint4 e[3];
+#pragma unroll
+    for (int i = 0; i < 3; i++) e[i] = q[i];
+    const uint64_t s3 = d.dir[2 * (b0 + 3)];
+#pragma unroll
+    for (int i = 0; i < 3; i++) {
+      st[i] = ((uint64_t)e[i].y << 32) | e[i].x;
+      fl[i] = ((uint64_t)e[i].w << 32) | e[i].z;
     }
+    st[3] = s3;
   } else {
-    uint64_t k = r - j + 1;
-    uint64_t x = o ? d.words[w] & ((1ull << o) - 1) : 0ull;
-    for (;;) {
-      const uint32_t c = popc64(x);
-      if (c >= k) return (w << 6) + select_in_word(x, (uint32_t)(c - k));
-      k -= c;
-      if (w == 0) return WTG_NONE;
-      x = d.words[--w];
+#pragma unroll
+    for (int i = 0; i < 4; i++) {
+      const uint64_t B = b0 + i;
+      st[i] = B <= d.nblk ? d.dir[2 * B] : d.ones;
+      if (i < 3) fl[i] = B <= d.nblk ? d.dir[2 * B + 1] : 0ull;
     }
   }
+  if (st[0] >= j || st[3] < j) return select_plain(d.s1, j);  // outside the window
+  const uint32_t i = (uint32_t)(st[1] < j) + (uint32_t)(st[2] < j);
+  const uint64_t B = b0 + i;
+  const uint64_t s = i == 0 ? st[0] : i == 1 ? st[1] : st[2];
+  const uint64_t F = i == 0 ? fl[0] : i == 1 ? fl[1] : fl[2];
+  const uint32_t r = (uint32_t)(j - s);  // 1-based rank inside the block (<= 512)
+  const uint64_t W = (d.nbits + 63) >> 6;
+  const uint32_t mv = W - 8 * B < 8 ? (uint32_t)(W - 8 * B) : 8u;  // words of the block
+  uint32_t k = 0, ck = 0;
+#pragma unroll
+  for (uint32_t m = 1; m < 8; m++) {
+    const uint32_t c = (uint32_t)(F >> (63 - 9 * m)) & 0x1FFu;
+    if (m < mv && c < r) {
+      k = m;
+      ck = c;
+    }
+  }
+  const uint64_t w = 8 * B + k;
+  return (w << 6) + select_in_word(d.words[w], r - ck - 1);
 }
 
 template <bool LEARNED>
@@ -113,127 +160,199 @@ __global__ __launch_bounds__(256) void k_lsel_query(const LselDesc* __restrict__
   }
 }
 
+// ------------------------------------------------------------- placement
+struct PlaceVec {
+  SelQ s1;
+  const uint64_t* words;
+  uint64_t ones;
+  uint64_t cbase;        // first flat chunk index of this vector
+  double alpha, beta;
+};
+
+__device__ __forceinline__ uint32_t find_base(const uint64_t* base, uint32_t nv, uint64_t flat) {
+  uint32_t lo = 0, hi = nv;  // last vector with base <= flat
+  while (hi - lo > 1) {
+    const uint32_t mid = (lo + hi) / 2;
+    if (base[mid] <= flat) lo = mid; else hi = mid;
+  }
+  return lo;
+}
+
+// Greedy knot placement over one chunk of arguments [j0, j1] (thread per
+// chunk).  Count pass (off == nullptr): cnt[flat] = knots of the chunk.
+// Emit pass: the knots go to knots[off[flat] ..], at most off[flat+1]-off[flat]
+// of them (a different count sets *err).
+__global__ __launch_bounds__(256) void k_lsel_place(const PlaceVec* __restrict__ pv, const uint64_t* __restrict__ cbase,
+                                                    uint32_t nv, uint64_t nflat, uint32_t gap_log, double tol,
+                                                    uint32_t* __restrict__ cnt, const uint32_t* __restrict__ off,
+                                                    Knot* __restrict__ knots, uint32_t* __restrict__ err) {
+#pragma clang fp contract(off)
+  const uint64_t flat = (uint64_t)blockIdx.x * blockDim.x + threadIdx.x;
+  if (flat >= nflat) return;
+  const PlaceVec& v = pv[find_base(cbase, nv, flat)];
+  const uint64_t c = flat - v.cbase;
+  const uint64_t j0 = (c << gap_log) + 1;
+  const uint64_t j1 = ((c + 1) << gap_log) < v.ones ? ((c + 1) << gap_log) : v.ones;
+  const uint32_t lim = off ? off[flat + 1] - off[flat] : 0u;
+  Knot* out = off ? knots + off[flat] : nullptr;
+  uint32_t n = 0;
+  auto put = [&](uint64_t jj, double val) {
+    if (out && n < lim) out[n] = Knot{(uint32_t)jj, (float)(val - lsel_line(v.alpha, v.beta, (double)jj))};
+    n++;
+  };
+  if (j0 <= j1) {
+    const uint64_t pos = select_plain(v.s1, j0);
+    uint64_t w = pos >> 6;
+    uint64_t x = v.words[w] & (~0ull << (pos & 63));
+    x &= x - 1;  // the one at pos itself
+    double x0 = (double)j0, y0 = (double)pos, lo = -1e300, hi = 1e300;
+    uint64_t jp = j0;
+    put(j0, y0);
+    for (uint64_t jj = j0 + 1; jj <= j1; jj++) {
+      while (!x) x = v.words[++w];  // the jj-th one exists: jj <= ones
+      const double P = (double)((w << 6) + (uint64_t)__builtin_ctzll(x));
+      x &= x - 1;
+      double dx = (double)jj - x0;
+      double nlo = (P - tol - y0) / dx, nhi = (P + tol - y0) / dx;
+      nlo = nlo > lo ? nlo : lo;
+      nhi = nhi < hi ? nhi : hi;
+      if (nlo > nhi) {
+        // no line from (x0, y0) fits: knot at the previous argument
+        const double ye = y0 + 0.5 * (lo + hi) * ((double)jp - x0);
+        put(jp, ye);
+        x0 = (double)jp;
+        y0 = ye;
+        dx = (double)jj - x0;
+        nlo = (P - tol - y0) / dx;
+        nhi = (P + tol - y0) / dx;
+      }
+      lo = nlo;
+      hi = nhi;
+      jp = jj;
+    }
+    if ((double)jp > x0) put(jp, y0 + 0.5 * (lo + hi) * ((double)jp - x0));
+  }
+  if (!off) cnt[flat] = n;
+  else if (n != lim) *err = 1u;
+}
+
+// lookup table: thread per knot interval k, entries i with i << lut_log in
+// the interval's arguments [J_k, J_{k+1}) (the first from 0, the last to ones)
+__global__ __launch_bounds__(256) void k_lsel_lut(const Knot* __restrict__ K, uint32_t nk, uint64_t ones,
+                                                  uint32_t lut_log, uint32_t* __restrict__ lut) {
+  const uint32_t k = blockIdx.x * blockDim.x + threadIdx.x;
+  if (k + 1 >= nk) return;
+  const uint64_t lo = k == 0 ? 0ull : K[k].j;
+  const uint64_t hi = k + 2 == nk ? ones + 1 : K[k + 1].j;
+  const uint64_t nlut = (ones >> lut_log) + 1;
+  for (uint64_t i = (lo + (1ull << lut_log) - 1) >> lut_log; (i << lut_log) < hi && i < nlut; i++)
+    lut[i] = k;
+}
+
 // ------------------------------------------------------------- training
-// One spline model's view for the training kernels (several models are
-// trained together; `intervals` indexes a flat space of knot intervals).
+// One model's view for the training kernels.  Several models train together:
+// their knots are consecutive in one array; thread g handles the interval
+// starting at global knot g (none for a model's last knot).
 struct TrainVec {
   const uint64_t* pos;   // ns training samples: pos[t] = select(T*(t+1))
   uint64_t ns;
-  float* y;              // nknots parameters
+  uint64_t kfirst;       // global index of the model's first knot
+  uint32_t nknots, tlog;
+  double alpha, beta;
+};
+
+struct TrainBufs {
+  Knot* K;
   float* m1;             // Adam moments
   float* m2;
   float* gl;             // per interval: sum e*(1-f), sum e*f, sum e^2
   float* gr;
   float* le;
-  uint64_t ibase;        // first flat interval index of this model
-  double alpha, beta;
-  uint32_t slog, tlog, nknots;
-  uint32_t _pad;
+  const uint64_t* kfirst;  // [nv] model first knots (for the vector search)
 };
 
-__device__ __forceinline__ uint32_t find_vec(const TrainVec* tv, uint32_t nv, uint64_t flat) {
-  uint32_t lo = 0, hi = nv;  // last model with ibase <= flat
-  while (hi - lo > 1) {
-    const uint32_t mid = (lo + hi) / 2;
-    if (tv[mid].ibase <= flat) lo = mid; else hi = mid;
+// forward + backward: samples j = T*u in [J_k, J_{k+1}) (the last interval
+// includes J_{K-1}); samples outside [J_0, J_{K-1}] are not seen
+__global__ __launch_bounds__(256) void k_lsel_fwd_bwd(const TrainVec* __restrict__ tv, uint32_t nv, TrainBufs tb,
+                                                      uint64_t nk_total) {
+  const uint64_t g = (uint64_t)blockIdx.x * blockDim.x + threadIdx.x;
+  if (g >= nk_total) return;
+  const TrainVec& v = tv[find_base(tb.kfirst, nv, g)];
+  const uint64_t k = g - v.kfirst;
+  if (k + 1 >= v.nknots) {
+    tb.gl[g] = 0.f;
+    tb.gr[g] = 0.f;
+    tb.le[g] = 0.f;
+    return;
   }
-  return lo;
-}
-
-// forward + backward: thread per knot interval k; samples j in [kS, (k+1)S)
-__global__ __launch_bounds__(256) void k_lsel_fwd_bwd(const TrainVec* __restrict__ tv, uint32_t nv,
-                                                      uint64_t nint) {
-  const uint64_t flat = (uint64_t)blockIdx.x * blockDim.x + threadIdx.x;
-  if (flat >= nint) return;
-  const uint32_t b = find_vec(tv, nv, flat);
-  const TrainVec& v = tv[b];
-  const uint64_t k = flat - v.ibase;
-  if (k + 1 >= v.nknots) return;
-  const uint64_t S = 1ull << v.slog, T = 1ull << v.tlog;
-  // samples t with T*(t+1) in [kS, (k+1)S) and 1 <= t+1 <= ns
-  uint64_t t1 = (k * S) >> v.tlog;
+  const Knot a = tb.K[g], b = tb.K[g + 1];
+  const uint64_t T = 1ull << v.tlog;
+  uint64_t t1 = ((uint64_t)a.j + T - 1) >> v.tlog;
   if (t1 < 1) t1 = 1;
-  uint64_t t2 = (((k + 1) * S) >> v.tlog) - 1;
+  uint64_t t2 = (k + 2 == v.nknots ? (uint64_t)b.j : (uint64_t)b.j - 1) >> v.tlog;
   if (t2 > v.ns) t2 = v.ns;
-  const float y0 = v.y[k], y1 = v.y[k + 1];
-  float a = 0.f, c = 0.f, e2 = 0.f;
+  const float span = (float)(b.j - a.j);
+  float sa = 0.f, sc = 0.f, e2 = 0.f;
   for (uint64_t u = t1; u <= t2; u++) {
     const uint64_t j = u * T;
-    const float f = (float)(j - k * S) / (float)S;
-    const float target = (float)((double)v.pos[u - 1] - fma(v.alpha, (double)j, v.beta));
-    const float e = (1.0f - f) * y0 + f * y1 - target;
-    a += e * (1.0f - f);
-    c += e * f;
+    const float f = (float)(j - a.j) / span;
+    const float target = (float)((double)v.pos[u - 1] - lsel_line(v.alpha, v.beta, (double)j));
+    const float e = (1.0f - f) * a.y + f * b.y - target;
+    sa += e * (1.0f - f);
+    sc += e * f;
     e2 += e * e;
   }
-  v.gl[k] = a;
-  v.gr[k] = c;
-  v.le[k] = e2;
+  tb.gl[g] = sa;
+  tb.gr[g] = sc;
+  tb.le[g] = e2;
 }
 
-// Adam step on every knot: grad = 2/ns * (gl[k] + gr[k-1])
-__global__ __launch_bounds__(256) void k_lsel_adam(const TrainVec* __restrict__ tv, uint32_t nv,
-                                                   uint64_t nknots_total, const uint64_t* __restrict__ kbase,
-                                                   float lr, float b1, float b2, float bc1, float bc2) {
-  const uint64_t flat = (uint64_t)blockIdx.x * blockDim.x + threadIdx.x;
-  if (flat >= nknots_total) return;
-  uint32_t lo = 0, hi = nv;
-  while (hi - lo > 1) {
-    const uint32_t mid = (lo + hi) / 2;
-    if (kbase[mid] <= flat) lo = mid; else hi = mid;
+// d loss / d y_k = 2/ns * (gl[k] + gr[k-1])
+__device__ __forceinline__ float knot_grad(const TrainVec& v, const TrainBufs& tb, uint64_t g) {
+  const uint64_t k = g - v.kfirst;
+  float s = 0.f;
+  if (k + 1 < v.nknots) s += tb.gl[g];
+  if (k >= 1) s += tb.gr[g - 1];
+  return s * (2.0f / (float)(v.ns ? v.ns : 1));
+}
+
+__global__ __launch_bounds__(256) void k_lsel_adam(const TrainVec* __restrict__ tv, uint32_t nv, TrainBufs tb,
+                                                   uint64_t nk_total, float lr, float b1, float b2, float bc1,
+                                                   float bc2) {
+  const uint64_t g = (uint64_t)blockIdx.x * blockDim.x + threadIdx.x;
+  if (g >= nk_total) return;
+  const float gr = knot_grad(tv[find_base(tb.kfirst, nv, g)], tb, g);
+  const float m = b1 * tb.m1[g] + (1.f - b1) * gr;
+  const float s = b2 * tb.m2[g] + (1.f - b2) * gr * gr;
+  tb.m1[g] = m;
+  tb.m2[g] = s;
+  tb.K[g].y -= lr * (m / bc1) / (sqrtf(s / bc2) + 1e-6f);
+}
+
+// the gradient alone (for checks), one model
+__global__ void k_lsel_grad(const TrainVec* __restrict__ tv, TrainBufs tb, float* __restrict__ grad) {
+  const uint64_t g = (uint64_t)blockIdx.x * blockDim.x + threadIdx.x;
+  if (g >= tv[0].nknots) return;
+  grad[g] = knot_grad(tv[0], tb, g);
+}
+
+// sum of squared residuals of model b (training loss numerator)
+__global__ __launch_bounds__(256) void k_lsel_loss(const TrainVec* __restrict__ tv, uint32_t b, TrainBufs tb,
+                                                   double* __restrict__ out) {
+  const TrainVec& v = tv[b];
+  double acc = 0;
+  for (uint64_t k = threadIdx.x; k + 1 < v.nknots; k += blockDim.x) acc += (double)tb.le[v.kfirst + k];
+  __shared__ double s[256];
+  s[threadIdx.x] = acc;
+  __syncthreads();
+  for (int h = 128; h > 0; h >>= 1) {
+    if ((int)threadIdx.x < h) s[threadIdx.x] += s[threadIdx.x + h];
+    __syncthreads();
   }
-  const TrainVec& v = tv[lo];
-  const uint64_t k = flat - kbase[lo];
-  if (k >= v.nknots) return;
-  float g = 0.f;
-  if (k + 1 < v.nknots) g += v.gl[k];
-  if (k >= 1) g += v.gr[k - 1];
-  g *= 2.0f / (float)(v.ns ? v.ns : 1);
-  const float m = b1 * v.m1[k] + (1.f - b1) * g;
-  const float s = b2 * v.m2[k] + (1.f - b2) * g * g;
-  v.m1[k] = m;
-  v.m2[k] = s;
-  v.y[k] -= lr * (m / bc1) / (sqrtf(s / bc2) + 1e-6f);
+  if (threadIdx.x == 0) out[b] = s[0];
 }
 
-// the gradient alone (for checks): grad[k] = 2/ns * (gl[k] + gr[k-1])
-__global__ void k_lsel_grad(const TrainVec* __restrict__ tv, float* __restrict__ grad) {
-  const TrainVec& v = tv[0];
-  const uint64_t k = (uint64_t)blockIdx.x * blockDim.x + threadIdx.x;
-  if (k >= v.nknots) return;
-  float g = 0.f;
-  if (k + 1 < v.nknots) g += v.gl[k];
-  if (k >= 1) g += v.gr[k - 1];
-  grad[k] = g * 2.0f / (float)(v.ns ? v.ns : 1);
-}
-
-// initial knots: the residual at the sample nearest to j = kS
-__global__ void k_lsel_init(const TrainVec* __restrict__ tv, uint32_t nv, uint64_t nknots_total,
-                            const uint64_t* __restrict__ kbase) {
-  const uint64_t flat = (uint64_t)blockIdx.x * blockDim.x + threadIdx.x;
-  if (flat >= nknots_total) return;
-  uint32_t lo = 0, hi = nv;
-  while (hi - lo > 1) {
-    const uint32_t mid = (lo + hi) / 2;
-    if (kbase[mid] <= flat) lo = mid; else hi = mid;
-  }
-  const TrainVec& v = tv[lo];
-  const uint64_t k = flat - kbase[lo];
-  if (k >= v.nknots) return;
-  float y = 0.f;
-  if (v.ns) {
-    uint64_t t = (k << v.slog) >> v.tlog;  // sample index + 1 at j = kS
-    if (t < 1) t = 1;
-    if (t > v.ns) t = v.ns;
-    const uint64_t j = t << v.tlog;
-    y = (float)((double)v.pos[t - 1] - fma(v.alpha, (double)j, v.beta));
-  }
-  v.y[k] = y;
-  v.m1[k] = 0.f;
-  v.m2[k] = 0.f;
-}
-
-// exhaustive error window: thread per word, every one of the word
+// exhaustive error window of vector b: thread per word, every one of the word
 __global__ __launch_bounds__(256) void k_lsel_eps(const LselDesc* __restrict__ descs, uint32_t b,
                                                   unsigned long long* __restrict__ eps) {
   const LselDesc d = descs[b];
@@ -256,22 +375,6 @@ __global__ __launch_bounds__(256) void k_lsel_eps(const LselDesc* __restrict__ d
   }
   mx = wave_max_u64(mx);
   if (lane_id() == 0 && mx) atomicMax(&eps[b], (unsigned long long)mx);
-}
-
-// sum of squared residuals of one model (training loss numerator)
-__global__ __launch_bounds__(256) void k_lsel_loss(const TrainVec* __restrict__ tv, uint32_t b,
-                                                   double* __restrict__ out) {
-  const TrainVec& v = tv[b];
-  double acc = 0;
-  for (uint64_t k = threadIdx.x; k + 1 < v.nknots; k += blockDim.x) acc += (double)v.le[k];
-  __shared__ double s[256];
-  s[threadIdx.x] = acc;
-  __syncthreads();
-  for (int h = 128; h > 0; h >>= 1) {
-    if ((int)threadIdx.x < h) s[threadIdx.x] += s[threadIdx.x + h];
-    __syncthreads();
-  }
-  if (threadIdx.x == 0) out[b] = s[0];
 }
 
 // training samples: pos[t] = select1(T*(t+1)) by the plain sampled select
@@ -323,16 +426,186 @@ static int alloc_block(wtg_ctx* ctx, wtg_lsel* ls, size_t bytes, void** p) {
   return WTG_OK;
 }
 
+static int sync_copy(wtg_ctx* ctx, void* dst, const void* src, size_t bytes, hipMemcpyKind kind) {
+  WTG_CHECK_HIP(hipMemcpyAsync(dst, src, bytes, kind, ctx->stream));
+  WTG_CHECK_HIP(hipStreamSynchronize(ctx->stream));
+  return WTG_OK;
+}
+
+// knot placement, lookup tables, training and the eps check for the vectors
+// in `act` (all memory goes to ls->blocks, released by the caller on failure)
+static int lsel_models(wtg_ctx* ctx, wtg_lsel* ls, const wtg_lsel_params& P, const uint64_t* nbits,
+                       const std::vector<uint64_t*>& pos, const std::vector<uint64_t>& ns,
+                       const std::vector<uint32_t>& act) {
+  const uint32_t nvec = ls->nvec;
+  const uint32_t tlog = (uint32_t)__builtin_ctz(P.train_stride);
+  const uint32_t gap_log = (uint32_t)__builtin_ctz(P.max_gap);
+  const uint32_t na = (uint32_t)act.size();
+  // 3. knot placement: count, scan (host), emit
+  std::vector<PlaceVec> pv(na);
+  std::vector<uint64_t> cbase(na + 1, 0);
+  for (uint32_t a = 0; a < na; a++) {
+    const LselDesc& d = ls->desc[act[a]];
+    pv[a] = PlaceVec{d.s1, d.words, d.ones, cbase[a], d.alpha, d.beta};
+    cbase[a + 1] = cbase[a] + wtg_ceil_div(d.ones, P.max_gap);
+  }
+  const uint64_t nflat = cbase[na];
+  PlaceVec* d_pv = nullptr;
+  uint64_t* d_cbase = nullptr;
+  uint32_t *d_cnt = nullptr, *d_err = nullptr;
+  WTG_TRY(alloc_block(ctx, ls, na * sizeof(PlaceVec), (void**)&d_pv));
+  WTG_TRY(alloc_block(ctx, ls, (na + 1) * 8, (void**)&d_cbase));
+  WTG_TRY(alloc_block(ctx, ls, (nflat + 1) * 4, (void**)&d_cnt));
+  WTG_TRY(alloc_block(ctx, ls, 256, (void**)&d_err));
+  WTG_CHECK_HIP(hipMemcpyAsync(d_pv, pv.data(), na * sizeof(PlaceVec), hipMemcpyHostToDevice, ctx->stream));
+  WTG_CHECK_HIP(hipMemcpyAsync(d_cbase, cbase.data(), (na + 1) * 8, hipMemcpyHostToDevice, ctx->stream));
+  WTG_TRY(wtg_fill_u64(ctx, d_err, 1, 0));
+  const unsigned gp = (unsigned)wtg_ceil_div(nflat, 256);
+  WTG_LAUNCH(ctx, "lsel_place", k_lsel_place, dim3(gp), dim3(256), 0, d_pv, d_cbase, na, nflat, gap_log,
+             (double)P.tol, d_cnt, (const uint32_t*)nullptr, (Knot*)nullptr, d_err);
+  WTG_LAUNCH_CHECK();
+  std::vector<uint32_t> off(nflat + 1, 0);
+  WTG_TRY(sync_copy(ctx, off.data(), d_cnt, nflat * 4, hipMemcpyDeviceToHost));
+  uint64_t tot = 0;
+  for (uint64_t c = 0; c < nflat; c++) {
+    const uint32_t k = off[c];
+    off[c] = (uint32_t)tot;
+    tot += k;
+  }
+  WTG_REQUIRE(tot < (1ull << 32), "wtg_lsel_build: too many knots");
+  off[nflat] = (uint32_t)tot;
+  const uint64_t nk_total = tot;
+  Knot *d_K = nullptr, *d_K0 = nullptr;
+  float *m1 = nullptr, *m2 = nullptr, *gl = nullptr, *gr = nullptr, *le = nullptr;
+  uint64_t* d_kfirst = nullptr;
+  WTG_TRY(alloc_block(ctx, ls, nk_total * sizeof(Knot) + 256, (void**)&d_K));
+  WTG_TRY(alloc_block(ctx, ls, nk_total * sizeof(Knot) + 256, (void**)&d_K0));
+  WTG_TRY(alloc_block(ctx, ls, nk_total * 20 + 256, (void**)&m1));
+  m2 = m1 + nk_total;
+  gl = m2 + nk_total;
+  gr = gl + nk_total;
+  le = gr + nk_total;
+  WTG_TRY(alloc_block(ctx, ls, (na + 1) * 8, (void**)&d_kfirst));
+  WTG_CHECK_HIP(hipMemcpyAsync(d_cnt, off.data(), (nflat + 1) * 4, hipMemcpyHostToDevice, ctx->stream));
+  WTG_LAUNCH(ctx, "lsel_place", k_lsel_place, dim3(gp), dim3(256), 0, d_pv, d_cbase, na, nflat, gap_log,
+             (double)P.tol, (uint32_t*)nullptr, (const uint32_t*)d_cnt, d_K, d_err);
+  WTG_LAUNCH_CHECK();
+  uint32_t perr = 0;
+  WTG_TRY(sync_copy(ctx, &perr, d_err, 4, hipMemcpyDeviceToHost));
+  WTG_REQUIRE(perr == 0, "wtg_lsel_build: knot placement passes disagree");
+  WTG_CHECK_HIP(hipMemcpyAsync(d_K0, d_K, nk_total * sizeof(Knot), hipMemcpyDeviceToDevice, ctx->stream));
+  WTG_TRY(wtg_fill_u64(ctx, m1, nk_total * 5 / 2 + 1, 0));
+  // 4. per vector: knots, lookup table
+  std::vector<TrainVec> tv(na);
+  std::vector<uint64_t> kfirst(na + 1, 0);
+  for (uint32_t a = 0; a < na; a++) {
+    const uint32_t b = act[a];
+    LselDesc& d = ls->desc[b];
+    const uint64_t k0 = off[cbase[a]], k1 = off[cbase[a + 1]];
+    kfirst[a] = k0;
+    tv[a] = TrainVec{pos[b], ns[b], k0, (uint32_t)(k1 - k0), tlog, d.alpha, d.beta};
+    d.knots = d_K + k0;
+    d.nknots = (uint32_t)(k1 - k0);
+    uint32_t lg = 0;
+    const uint64_t avg = d.ones / (d.nknots > 1 ? d.nknots - 1 : 1);
+    while (lg < 20 && (2ull << lg) <= avg) lg++;
+    d.lut_log = lg;
+    uint32_t* lut = nullptr;
+    const uint64_t nlut = (d.ones >> lg) + 1;
+    WTG_TRY(alloc_block(ctx, ls, (nlut + 2) * 4, (void**)&lut));
+    WTG_TRY(wtg_fill_u64(ctx, lut, (nlut + 1) / 2, 0));  // every entry is written below; 0 keeps reads in range
+    d.lut = lut;
+    WTG_LAUNCH(ctx, "lsel_lut", k_lsel_lut, dim3((unsigned)wtg_ceil_div(d.nknots, 256)), dim3(256), 0, d.knots,
+               d.nknots, d.ones, lg, lut);
+    WTG_LAUNCH_CHECK();
+    d.learned = 1;
+    ls->info[b].nknots = d.nknots;
+    ls->info[b].lut_log = lg;
+  }
+  kfirst[na] = nk_total;
+  TrainVec* d_tv = nullptr;
+  unsigned long long* d_eps = nullptr;
+  double* d_loss = nullptr;
+  WTG_TRY(alloc_block(ctx, ls, na * sizeof(TrainVec), (void**)&d_tv));
+  WTG_TRY(alloc_block(ctx, ls, nvec * 8, (void**)&d_eps));
+  WTG_TRY(alloc_block(ctx, ls, na * 8, (void**)&d_loss));
+  WTG_CHECK_HIP(hipMemcpyAsync(d_tv, tv.data(), na * sizeof(TrainVec), hipMemcpyHostToDevice, ctx->stream));
+  WTG_CHECK_HIP(hipMemcpyAsync(d_kfirst, kfirst.data(), (na + 1) * 8, hipMemcpyHostToDevice, ctx->stream));
+  WTG_TRY(lsel_upload_desc(ctx, ls));
+  TrainBufs tb{d_K, m1, m2, gl, gr, le, d_kfirst};
+  const unsigned gk = (unsigned)wtg_ceil_div(nk_total, 256);
+  // eps (exhaustive) and training loss of the knots now in d_K
+  auto measure = [&](std::vector<uint64_t>& eps, std::vector<double>& loss) -> int {
+    WTG_TRY(wtg_fill_u64(ctx, d_eps, nvec, 0));
+    WTG_LAUNCH(ctx, "lsel_fwd_bwd", k_lsel_fwd_bwd, dim3(gk), dim3(256), 0, d_tv, na, tb, nk_total);
+    WTG_LAUNCH_CHECK();
+    for (uint32_t a = 0; a < na; a++) {
+      const uint32_t b = act[a];
+      const unsigned ge = (unsigned)std::min<uint64_t>(wtg_ceil_div((nbits[b] + 63) / 64, 256), 2048);
+      WTG_LAUNCH(ctx, "lsel_eps", k_lsel_eps, dim3(ge), dim3(256), 0, ls->d_desc, b, d_eps);
+      WTG_LAUNCH_CHECK();
+      WTG_LAUNCH(ctx, "lsel_loss", k_lsel_loss, dim3(1), dim3(256), 0, d_tv, a, tb, d_loss);
+      WTG_LAUNCH_CHECK();
+    }
+    std::vector<unsigned long long> e(nvec);
+    std::vector<double> l(na);
+    WTG_CHECK_HIP(hipMemcpyAsync(e.data(), d_eps, nvec * 8, hipMemcpyDeviceToHost, ctx->stream));
+    WTG_CHECK_HIP(hipMemcpyAsync(l.data(), d_loss, na * 8, hipMemcpyDeviceToHost, ctx->stream));
+    WTG_CHECK_HIP(hipStreamSynchronize(ctx->stream));
+    for (uint32_t a = 0; a < na; a++) {
+      eps[a] = e[act[a]];
+      loss[a] = ns[act[a]] ? l[a] / (double)ns[act[a]] : 0.0;
+    }
+    return WTG_OK;
+  };
+  std::vector<uint64_t> eps0(na), eps1(na);
+  std::vector<double> loss0(na), loss1(na);
+  WTG_TRY(measure(eps0, loss0));
+  // 5. Adam on every knot of every model
+  const float b1 = 0.9f, b2 = 0.999f;
+  float p1 = 1.f, p2 = 1.f;
+  for (uint32_t e = 0; e < P.epochs; e++) {
+    WTG_LAUNCH(ctx, "lsel_fwd_bwd", k_lsel_fwd_bwd, dim3(gk), dim3(256), 0, d_tv, na, tb, nk_total);
+    WTG_LAUNCH_CHECK();
+    p1 *= b1;
+    p2 *= b2;
+    WTG_LAUNCH(ctx, "lsel_adam", k_lsel_adam, dim3(gk), dim3(256), 0, d_tv, na, tb, nk_total, P.lr, b1, b2,
+               1.f - p1, 1.f - p2);
+    WTG_LAUNCH_CHECK();
+  }
+  WTG_TRY(measure(eps1, loss1));
+  // 6. keep the trained knots where eps <= eps_max, else restore the placed ones
+  for (uint32_t a = 0; a < na; a++) {
+    const uint32_t b = act[a];
+    wtg_lsel_vec_info& I = ls->info[b];
+    I.eps_init = eps0[a];
+    I.loss_init = loss0[a];
+    I.loss_final = loss1[a];
+    I.trained = eps1[a] <= P.eps_max ? 1u : 0u;
+    I.eps = I.trained ? eps1[a] : eps0[a];
+    if (!I.trained) {
+      WTG_CHECK_HIP(hipMemcpyAsync(d_K + tv[a].kfirst, d_K0 + tv[a].kfirst, tv[a].nknots * sizeof(Knot),
+                                   hipMemcpyDeviceToDevice, ctx->stream));
+    }
+    ls->desc[b].learned = I.eps <= P.eps_max ? 1 : 0;
+    I.learned = (uint32_t)ls->desc[b].learned;
+  }
+  WTG_TRY(lsel_upload_desc(ctx, ls));
+  return WTG_OK;
+}
+
 extern "C" int wtg_lsel_build(wtg_ctx* ctx, uint32_t nvec, const uint64_t* const* d_words,
                               const uint64_t* nbits, const wtg_lsel_params* prm, wtg_lsel** out) {
   WTG_REQUIRE(ctx && out && d_words && nbits && nvec >= 1 && nvec <= 256, "wtg_lsel_build: bad argument (1 <= nvec <= 256)");
-  wtg_lsel_params P{16, 4096, 64, 200, 4.0f, 512};
+  wtg_lsel_params P{16, 4096, 256, 200, 4.0f, 512};
   if (prm) P = *prm;
   auto pow2 = [](uint32_t x) { return x && !(x & (x - 1)); };
-  WTG_REQUIRE(pow2(P.train_stride) && pow2(P.spacing_max) && pow2(P.spacing_min) &&
-                  P.spacing_min >= 2 * P.train_stride && P.spacing_max >= P.spacing_min,
-              "wtg_lsel_build: spacings and stride must be powers of two, spacing_min >= 2*stride");
-  for (uint32_t b = 0; b < nvec; b++) WTG_REQUIRE(d_words[b] && nbits[b] >= 1, "wtg_lsel_build: empty vector");
+  WTG_REQUIRE(pow2(P.train_stride) && pow2(P.max_gap) && P.max_gap >= 2 && P.max_gap <= (1u << 24),
+              "wtg_lsel_build: train_stride and max_gap must be powers of two, 2 <= max_gap <= 2^24");
+  WTG_REQUIRE(P.eps_max >= 2 && P.eps_max <= 512 && P.tol >= 1 && P.tol + 1 < P.eps_max,
+              "wtg_lsel_build: need 2 <= eps_max <= 512 and 1 <= tol < eps_max - 1");
+  for (uint32_t b = 0; b < nvec; b++)
+    WTG_REQUIRE(d_words[b] && nbits[b] >= 1 && nbits[b] < (1ull << 40), "wtg_lsel_build: empty vector");
   WTG_TRY(wtg_set_device(ctx));
   *out = nullptr;
   wtg_lsel* ls = new wtg_lsel();
@@ -345,10 +618,7 @@ extern "C" int wtg_lsel_build(wtg_ctx* ctx, uint32_t nvec, const uint64_t* const
   const uint32_t tlog = (uint32_t)__builtin_ctz(P.train_stride);
   std::vector<uint64_t*> pos(nvec, nullptr);
   std::vector<uint64_t> ns(nvec, 0);
-  unsigned long long* d_eps = nullptr;
-  double* d_loss = nullptr;
-  TrainVec* d_tv = nullptr;
-  uint64_t* d_kbase = nullptr;
+  std::vector<uint32_t> act;  // vectors that get a model
   // 1. the plain method: rank9 + select1 per vector
   for (uint32_t b = 0; b < nvec && rc == WTG_OK; b++) {
     std::memset(&ls->sel[b], 0, sizeof(wtg_select));
@@ -363,6 +633,7 @@ extern "C" int wtg_lsel_build(wtg_ctx* ctx, uint32_t nvec, const uint64_t* const
     d.s1 = SelQ{s.superblock, s.mini, s.long_index, s.long_pool, d_words[b], (nbits[b] + 63) / 64 + 1, 1};
     d.nbits = nbits[b];
     d.ones = s.arg_cnt;
+    d.nblk = ((nbits[b] + 63) / 64) >> 3;
     ls->info[b].nbits = nbits[b];
     ls->info[b].ones = s.arg_cnt;
   }
@@ -372,7 +643,7 @@ extern "C" int wtg_lsel_build(wtg_ctx* ctx, uint32_t nvec, const uint64_t* const
   // 2. training samples and the baseline line
   for (uint32_t b = 0; b < nvec; b++) {
     ns[b] = ls->desc[b].ones >> tlog;
-    if (ns[b] < 2) continue;
+    if (ns[b] < 2 || ls->desc[b].ones >= (1ull << 32)) continue;  // too few ones (or u32 knots): plain select
     if ((rc = alloc_block(ctx, ls, ns[b] * 8, (void**)&pos[b])) != WTG_OK) goto fail;
     const unsigned g = (unsigned)std::min<uint64_t>(wtg_ceil_div(ns[b], 256), 4096);
     WTG_LAUNCH(ctx, "lsel_samples", k_lsel_samples, dim3(g), dim3(256), 0, ls->d_desc, b, tlog, ns[b], pos[b]);
@@ -388,125 +659,9 @@ extern "C" int wtg_lsel_build(wtg_ctx* ctx, uint32_t nvec, const uint64_t* const
     const double j0 = (double)P.train_stride, j1 = (double)(ns[b] << tlog);
     ls->desc[b].alpha = ((double)last - (double)first) / (j1 - j0);
     ls->desc[b].beta = (double)first - ls->desc[b].alpha * j0;
+    act.push_back(b);
   }
-  if ((rc = alloc_block(ctx, ls, nvec * 8, (void**)&d_eps)) != WTG_OK) goto fail;
-  if ((rc = alloc_block(ctx, ls, nvec * 8, (void**)&d_loss)) != WTG_OK) goto fail;
-  if ((rc = alloc_block(ctx, ls, nvec * sizeof(TrainVec), (void**)&d_tv)) != WTG_OK) goto fail;
-  if ((rc = alloc_block(ctx, ls, (nvec + 1) * 8, (void**)&d_kbase)) != WTG_OK) goto fail;
-  {
-    // 3. spacing search: train every still-open vector at spacing S, measure eps
-    std::vector<char> done(nvec, 0);
-    for (uint32_t b = 0; b < nvec; b++) done[b] = ns[b] < 2;  // too few ones: plain select
-    for (uint32_t S = P.spacing_max; S >= P.spacing_min; S >>= 1) {
-      std::vector<uint32_t> act;
-      for (uint32_t b = 0; b < nvec; b++)
-        if (!done[b]) act.push_back(b);
-      if (act.empty()) break;
-      const uint32_t slog = (uint32_t)__builtin_ctz(S);
-      std::vector<TrainVec> tv(act.size());
-      std::vector<uint64_t> kbase(act.size() + 1, 0);
-      uint64_t nint = 0, nk = 0;
-      for (size_t a = 0; a < act.size(); a++) {
-        const uint32_t b = act[a];
-        TrainVec& v = tv[a];
-        std::memset(&v, 0, sizeof(v));
-        v.nknots = (uint32_t)((ls->desc[b].ones >> slog) + 2);
-        v.ibase = nint;
-        kbase[a] = nk;
-        nint += v.nknots - 1;
-        nk += v.nknots;
-      }
-      kbase[act.size()] = nk;
-      // one block per round: y | m1 | m2 | gl | gr | le
-      float* blk = nullptr;
-      if ((rc = alloc_block(ctx, ls, (size_t)nk * 6 * 4 + 256, (void**)&blk)) != WTG_OK) goto fail;
-      for (size_t a = 0; a < act.size(); a++) {
-        const uint32_t b = act[a];
-        TrainVec& v = tv[a];
-        float* base = blk + kbase[a] * 6;
-        v.y = base;
-        v.m1 = base + v.nknots;
-        v.m2 = base + 2 * (uint64_t)v.nknots;
-        v.gl = base + 3 * (uint64_t)v.nknots;
-        v.gr = base + 4 * (uint64_t)v.nknots;
-        v.le = base + 5 * (uint64_t)v.nknots;
-        v.pos = pos[b];
-        v.ns = ns[b];
-        v.alpha = ls->desc[b].alpha;
-        v.beta = ls->desc[b].beta;
-        v.slog = slog;
-        v.tlog = tlog;
-      }
-      const uint32_t na = (uint32_t)act.size();
-      WTG_CHECK_HIP(hipMemcpyAsync(d_tv, tv.data(), na * sizeof(TrainVec), hipMemcpyHostToDevice, ctx->stream));
-      WTG_CHECK_HIP(hipMemcpyAsync(d_kbase, kbase.data(), (na + 1) * 8, hipMemcpyHostToDevice, ctx->stream));
-      const unsigned gk = (unsigned)wtg_ceil_div(nk, 256), gi = (unsigned)wtg_ceil_div(nint, 256);
-      WTG_LAUNCH(ctx, "lsel_init", k_lsel_init, dim3(gk), dim3(256), 0, d_tv, na, nk, d_kbase);
-      WTG_LAUNCH_CHECK();
-      // eps and loss of the initial (untrained) knots
-      auto measure = [&](std::vector<uint64_t>& eps, std::vector<double>& loss) -> int {
-        for (size_t a = 0; a < act.size(); a++) {
-          LselDesc& d = ls->desc[act[a]];
-          d.y = tv[a].y;
-          d.slog = slog;
-          d.nknots = tv[a].nknots;
-        }
-        WTG_TRY(lsel_upload_desc(ctx, ls));
-        WTG_TRY(wtg_fill_u64(ctx, d_eps, nvec, 0));
-        WTG_LAUNCH(ctx, "lsel_fwd_bwd", k_lsel_fwd_bwd, dim3(gi), dim3(256), 0, d_tv, na, nint);
-        WTG_LAUNCH_CHECK();
-        for (size_t a = 0; a < act.size(); a++) {
-          const uint32_t b = act[a];
-          const unsigned ge = (unsigned)std::min<uint64_t>(wtg_ceil_div((nbits[b] + 63) / 64, 256), 2048);
-          WTG_LAUNCH(ctx, "lsel_eps", k_lsel_eps, dim3(ge), dim3(256), 0, ls->d_desc, b, d_eps);
-          WTG_LAUNCH_CHECK();
-          WTG_LAUNCH(ctx, "lsel_loss", k_lsel_loss, dim3(1), dim3(256), 0, d_tv, (uint32_t)a, d_loss);
-          WTG_LAUNCH_CHECK();
-          WTG_CHECK_HIP(hipMemcpyAsync(&loss[a], d_loss + a, 8, hipMemcpyDeviceToHost, ctx->stream));
-        }
-        std::vector<unsigned long long> e(nvec);
-        WTG_CHECK_HIP(hipMemcpyAsync(e.data(), d_eps, nvec * 8, hipMemcpyDeviceToHost, ctx->stream));
-        WTG_CHECK_HIP(hipStreamSynchronize(ctx->stream));
-        for (size_t a = 0; a < act.size(); a++) eps[a] = e[act[a]];
-        return WTG_OK;
-      };
-      std::vector<uint64_t> eps0(na), eps1(na);
-      std::vector<double> loss0(na), loss1(na);
-      if ((rc = measure(eps0, loss0)) != WTG_OK) goto fail;
-      const float b1 = 0.9f, b2 = 0.999f;
-      float p1 = 1.f, p2 = 1.f;
-      for (uint32_t e = 0; e < P.epochs; e++) {
-        WTG_LAUNCH(ctx, "lsel_fwd_bwd", k_lsel_fwd_bwd, dim3(gi), dim3(256), 0, d_tv, na, nint);
-        WTG_LAUNCH_CHECK();
-        p1 *= b1;
-        p2 *= b2;
-        WTG_LAUNCH(ctx, "lsel_adam", k_lsel_adam, dim3(gk), dim3(256), 0, d_tv, na, nk, d_kbase, P.lr, b1, b2,
-                   1.f - p1, 1.f - p2);
-        WTG_LAUNCH_CHECK();
-      }
-      if ((rc = measure(eps1, loss1)) != WTG_OK) goto fail;
-      for (size_t a = 0; a < act.size(); a++) {
-        const uint32_t b = act[a];
-        wtg_lsel_vec_info& I = ls->info[b];
-        const bool last_try = (S >> 1) < P.spacing_min;
-        if (eps1[a] <= P.eps_max || last_try) {
-          I.eps = eps1[a];
-          I.eps_init = eps0[a];
-          I.loss_init = ns[b] ? loss0[a] / (double)ns[b] : 0;
-          I.loss_final = ns[b] ? loss1[a] / (double)ns[b] : 0;
-          I.nknots = tv[a].nknots;
-          I.spacing = eps1[a] <= P.eps_max ? S : 0;
-          ls->desc[b].learned = eps1[a] <= P.eps_max ? 1 : 0;
-          done[b] = 1;
-        } else {
-          ls->desc[b].learned = 0;
-        }
-      }
-    }
-    for (uint32_t b = 0; b < nvec; b++)
-      if (!ls->desc[b].learned) ls->info[b].spacing = 0;
-    if ((rc = lsel_upload_desc(ctx, ls)) != WTG_OK) goto fail;
-  }
+  if (!act.empty() && (rc = lsel_models(ctx, ls, P, nbits, pos, ns, act)) != WTG_OK) goto fail;
   if ((rc = wtg_finish(ctx)) != WTG_OK) goto fail;
   *out = ls;
   return WTG_OK;
@@ -518,6 +673,23 @@ fail:
 extern "C" int wtg_lsel_info(const wtg_lsel* ls, uint32_t vec, wtg_lsel_vec_info* out) {
   WTG_REQUIRE(ls && out && vec < ls->nvec, "wtg_lsel_info: bad argument");
   *out = ls->info[vec];
+  return WTG_OK;
+}
+
+extern "C" int wtg_lsel_model(const wtg_lsel* ls, uint32_t vec, wtg_lsel_model_view* out) {
+  WTG_REQUIRE(ls && out && vec < ls->nvec, "wtg_lsel_model: bad argument");
+  const LselDesc& d = ls->desc[vec];
+  std::memset(out, 0, sizeof(*out));
+  out->learned = (uint32_t)d.learned;
+  if (d.knots) {
+    out->d_knots = d.knots;
+    out->nknots = d.nknots;
+    out->d_lut = d.lut;
+    out->lut_log = d.lut_log;
+    out->lut_size = (d.ones >> d.lut_log) + 1;
+  }
+  out->alpha = d.alpha;
+  out->beta = d.beta;
   return WTG_OK;
 }
 
@@ -547,43 +719,32 @@ extern "C" int wtg_lsel_plain_batch(wtg_ctx* ctx, const wtg_lsel* ls, const uint
   return lsel_query(ctx, ls, d_vec, d_j, nq, d_out, false);
 }
 
-extern "C" int wtg_lsel_loss_grad(wtg_ctx* ctx, const float* d_y, uint32_t nknots, uint32_t spacing,
-                                  uint32_t stride, double alpha, double beta, const uint64_t* d_pos, uint64_t ns,
-                                  double* loss, float* d_grad) {
+extern "C" int wtg_lsel_loss_grad(wtg_ctx* ctx, const void* d_knots, uint32_t nknots, uint32_t stride,
+                                  double alpha, double beta, const uint64_t* d_pos, uint64_t ns, double* loss,
+                                  float* d_grad) {
   auto pow2 = [](uint32_t x) { return x && !(x & (x - 1)); };
-  WTG_REQUIRE(ctx && d_y && d_pos && d_grad && loss && nknots >= 2 && pow2(spacing) && pow2(stride) &&
-                  spacing >= stride,
+  WTG_REQUIRE(ctx && d_knots && d_pos && d_grad && loss && nknots >= 2 && pow2(stride),
               "wtg_lsel_loss_grad: bad argument");
-  WTG_REQUIRE(((ns * (uint64_t)stride) >> __builtin_ctz(spacing)) + 2 <= nknots,
-              "wtg_lsel_loss_grad: knots do not cover the samples");
   WTG_TRY(wtg_set_device(ctx));
-  // scratch: TrainVec | gl | gr | le | loss
-  const size_t o_tv = 0, o_gl = wtg_align256(sizeof(TrainVec));
+  // scratch: TrainVec | kfirst | gl | gr | le | loss
+  const size_t o_tv = 0, o_kf = wtg_align256(sizeof(TrainVec)), o_gl = o_kf + 256;
   const size_t o_gr = wtg_align256(o_gl + (size_t)nknots * 4), o_le = wtg_align256(o_gr + (size_t)nknots * 4);
   const size_t o_l = wtg_align256(o_le + (size_t)nknots * 4), total = o_l + 256;
   WTG_TRY(wtg_ws_reserve(ctx, total));
   char* ws = (char*)ctx->ws;
-  TrainVec v{};
-  v.pos = d_pos;
-  v.ns = ns;
-  v.y = const_cast<float*>(d_y);
-  v.gl = (float*)(ws + o_gl);
-  v.gr = (float*)(ws + o_gr);
-  v.le = (float*)(ws + o_le);
-  v.ibase = 0;
-  v.alpha = alpha;
-  v.beta = beta;
-  v.slog = (uint32_t)__builtin_ctz(spacing);
-  v.tlog = (uint32_t)__builtin_ctz(stride);
-  v.nknots = nknots;
+  const TrainVec v{d_pos, ns, 0, nknots, (uint32_t)__builtin_ctz(stride), alpha, beta};
+  const uint64_t kf = 0;
   WTG_CHECK_HIP(hipMemcpyAsync(ws + o_tv, &v, sizeof(v), hipMemcpyHostToDevice, ctx->stream));
+  WTG_CHECK_HIP(hipMemcpyAsync(ws + o_kf, &kf, 8, hipMemcpyHostToDevice, ctx->stream));
   const TrainVec* d_tv = (const TrainVec*)(ws + o_tv);
-  WTG_LAUNCH(ctx, "lsel_fwd_bwd", k_lsel_fwd_bwd, dim3((unsigned)wtg_ceil_div(nknots - 1, 256)), dim3(256), 0,
-             d_tv, 1u, (uint64_t)(nknots - 1));
+  TrainBufs tb{(Knot*)const_cast<void*>(d_knots), nullptr, nullptr, (float*)(ws + o_gl), (float*)(ws + o_gr),
+               (float*)(ws + o_le), (const uint64_t*)(ws + o_kf)};
+  const unsigned g = (unsigned)wtg_ceil_div(nknots, 256);
+  WTG_LAUNCH(ctx, "lsel_fwd_bwd", k_lsel_fwd_bwd, dim3(g), dim3(256), 0, d_tv, 1u, tb, (uint64_t)nknots);
   WTG_LAUNCH_CHECK();
-  WTG_LAUNCH(ctx, "lsel_grad", k_lsel_grad, dim3((unsigned)wtg_ceil_div(nknots, 256)), dim3(256), 0, d_tv, d_grad);
+  WTG_LAUNCH(ctx, "lsel_grad", k_lsel_grad, dim3(g), dim3(256), 0, d_tv, tb, d_grad);
   WTG_LAUNCH_CHECK();
-  WTG_LAUNCH(ctx, "lsel_loss", k_lsel_loss, dim3(1), dim3(256), 0, d_tv, 0u, (double*)(ws + o_l));
+  WTG_LAUNCH(ctx, "lsel_loss", k_lsel_loss, dim3(1), dim3(256), 0, d_tv, 0u, tb, (double*)(ws + o_l));
   WTG_LAUNCH_CHECK();
   double l = 0;
   WTG_CHECK_HIP(hipMemcpyAsync(&l, ws + o_l, 8, hipMemcpyDeviceToHost, ctx->stream));
