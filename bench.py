@@ -132,7 +132,8 @@ def bench_lsel(pkg, args):
     del w
     log("build + train")
     t0 = time.perf_counter()
-    ls = pkg.LearnedSelect(ctx, dv, [nb] * nvec)
+    prm = pkg.LselParams(16, args.lsel_gap, args.lsel_tol, args.lsel_epochs, 4.0, 512)
+    ls = pkg.LearnedSelect(ctx, dv, [nb] * nvec, prm)
     build_s = time.perf_counter() - t0
     infos = [ls.info(b) for b in range(nvec)]
     rng = np.random.default_rng(SEED + 6)
@@ -158,7 +159,8 @@ def bench_lsel(pkg, args):
            "unit": "Mq/s", "n_gpus": 1, "steps": args.steps, "higher_is_better": True,
            "dtype": "u64", "data": "synthetic",
            "config": {"workload": "learned select, 64 bitvectors x 2^%d bits, %d uniform queries"
-                                  % (args.lsel_log2bits, nq)},
+                                  % (args.lsel_log2bits, nq), "tol": args.lsel_tol, "max_gap": args.lsel_gap,
+                      "epochs": args.lsel_epochs},
            "plain_Mq_s": res["plain"]["Mq_s"], "speedup": round(res["learned"]["Mq_s"] / res["plain"]["Mq_s"], 3),
            "identical_answers": same, "learned_vectors": nl,
            "trained_vectors": sum(1 for i in infos if i["learned"] and i["trained"]),
@@ -424,6 +426,9 @@ def main():
     ap.add_argument("--fm-patterns", type=int, default=1_000_000)
     ap.add_argument("--batch-log2max", type=int, default=22)
     ap.add_argument("--lsel-queries", type=int, default=100_000_000)
+    ap.add_argument("--lsel-tol", type=int, default=480, help="learned select: knot placement tolerance (bits)")
+    ap.add_argument("--lsel-gap", type=int, default=4096, help="learned select: longest knot interval (ones)")
+    ap.add_argument("--lsel-epochs", type=int, default=200, help="learned select: Adam steps")
     args = ap.parse_args()
     if args.gpus != 1:
         print("bench.py measures one GPU only: the level build of this path runs on a single "

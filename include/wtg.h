@@ -258,7 +258,7 @@ int wtg_batch_extract(wtg_ctx* ctx, const wtg_batch* b, uint32_t seq, uint64_t* 
 typedef struct {
   uint32_t train_stride;  /* training samples select(T), select(2T), ...; power of two (16) */
   uint32_t max_gap;       /* longest knot interval in arguments; power of two, 2..2^24 (4096) */
-  uint32_t tol;           /* knot placement tolerance in bits, 1 <= tol < eps_max - 1 (256) */
+  uint32_t tol;           /* knot placement tolerance in bits, 1 <= tol < eps_max - 1 (480) */
   uint32_t epochs;        /* Adam steps (200) */
   float lr;               /* Adam step in bit positions (4) */
   uint32_t eps_max;       /* accepted error window in bits, <= 512 (512) */

@@ -237,9 +237,107 @@ struct LevelArgs {
   uint64_t next_base;    // split pass: (l+1)*n, first bit of level l+1
   const uint32_t* zpre;  // split pass: exclusive prefix of zc (in)
   uint32_t* err;
+  uint32_t tile0;        // tile of block 0
 };
 
 #define LV_STEPS 4
+
+// Group phase of a split tile: per group of the next level (contiguous slots
+// [sG, eG) -> contiguous destinations [dG, dG+len)), one wave per group, one
+// output word per lane: level l+1's bit words (interior words stored, the two
+// edge words OR-ed), the group's next-bit ones (level l+1's table) and its
+// zeros per destination tile of level l+1.
+template <typename T, bool COUNT, int TILE>
+__device__ __forceinline__ void split_groups(const LevelArgs& a, uint32_t tile, uint32_t valid, uint32_t zt,
+                                             uint64_t zprev, uint64_t oprev, bool cached, uint64_t pfirst,
+                                             const T* s_stage, const uint64_t* s_nb, const uint64_t* s_head,
+                                             const uint2* s_tab) {
+  constexpr int NW = TILE / 64;
+  const int w = threadIdx.x >> 6, lane = lane_id();
+  uint32_t* c1 = COUNT ? a.c1 + (uint64_t)(tile & (a.reps - 1)) * (2ull * a.nnodes) : nullptr;
+  constexpr int HPL = (NW + 63) / 64;  // head words per lane
+  uint64_t hwl[HPL];
+  uint32_t hc = 0;
+#pragma unroll
+  for (int j = 0; j < HPL; j++) {
+    const int idx = lane * HPL + j;
+    hwl[j] = (idx < NW) ? s_head[idx] : 0ull;
+    hc += popc64(hwl[j]);
+  }
+  const uint32_t hincl = wave_incl_scan(hc);
+  const uint32_t hex = hincl - hc;
+  const uint32_t ng = __shfl(hincl, 63);
+  auto group_start = [&](uint32_t gi) -> uint32_t {
+    const uint64_t own = wave_ballot(hex <= gi && gi < hincl);
+    const int ol = (int)__builtin_ctzll(own);
+    uint32_t slot = 0;
+    if (lane == ol) {
+      uint32_t r = gi - hex;
+#pragma unroll
+      for (int j = 0; j < HPL; j++) {
+        const uint32_t c = popc64(hwl[j]);
+        if (r < c) { slot = (uint32_t)(lane * HPL + j) * 64 + select_in_word(hwl[j], r); break; }
+        r -= c;
+      }
+    }
+    return __shfl(slot, ol);
+  };
+  for (uint32_t gi = (uint32_t)w; gi < ng; gi += 4) {
+    const uint32_t sG = group_start(gi);
+    const uint32_t eG = (gi + 1 < ng) ? group_start(gi + 1) : (uint32_t)valid;
+    const uint32_t len = eG - sG;
+    const uint64_t x0 = (uint64_t)s_stage[sG];
+    const uint64_t pnode = x0 >> a.node_shift;
+    const uint64_t g = x0 >> a.bitpos;
+    uint2 ab;
+    if (cached && pnode - pfirst < TAB_CACHE) ab = s_tab[pnode - pfirst];
+    else if (pnode < a.nnodes) ab = a.tab[pnode];
+    else { if (lane == 0) atomicOr(a.err, FLAG_BADSYM); continue; }
+    const uint64_t dG = (sG < zt) ? zprev + sG + ab.x : oprev + (sG - zt) + ab.y;
+    if (dG + len > a.n || len == 0) { if (lane == 0) atomicOr(a.err, FLAG_BADSYM); continue; }
+    const uint64_t b0 = (dG / TILE + 1) * TILE;  // first destination-tile boundary
+    const int64_t rel0 = (dG + len > b0) ? (int64_t)(b0 - dG) : (int64_t)len;
+    const uint64_t GB0 = a.next_base + dG;
+    const uint64_t k0 = GB0 >> 6, k1 = (GB0 + len - 1) >> 6;
+    uint32_t o_lo = 0, o_hi = 0;
+    for (uint64_t k = k0 + lane; k <= k1; k += 64) {
+      const int64_t off = (int64_t)(k * 64) - (int64_t)GB0;  // group bit at word bit 0
+      const int64_t P = (int64_t)sG + off;                   // its slot
+      uint64_t val;
+      if (P >= 0) {
+        const uint32_t w0 = (uint32_t)(P >> 6), sh = (uint32_t)(P & 63);
+        val = s_nb[w0] >> sh;
+        if (sh) val |= s_nb[w0 + 1] << (64 - sh);
+      } else {
+        val = s_nb[0] << (uint32_t)(-P);
+      }
+      const int64_t lo = off < 0 ? -off : 0;
+      const int64_t hi = ((int64_t)len - off < 64) ? (int64_t)len - off : 64;
+      const uint64_t msk = (hi >= 64 ? ~0ull : ((1ull << hi) - 1)) & (~0ull << lo);
+      val &= msk;
+      if (lo == 0 && hi == 64) a.words[k] = val;
+      else if (val) atomicOr((unsigned long long*)&a.words[k], (unsigned long long)val);
+      const int64_t cut = rel0 - off;  // bits below `cut` belong to the first destination tile
+      const uint64_t mlo = cut <= 0 ? 0ull : (cut >= 64 ? ~0ull : ((1ull << cut) - 1));
+      o_lo += popc64(val & mlo);
+      o_hi += popc64(val & ~mlo);
+    }
+    o_lo = wave_sum(o_lo);
+    o_hi = wave_sum(o_hi);
+    if (lane == 0) {
+      const uint32_t ones = o_lo + o_hi;
+      if (COUNT && ones && g < 2ull * a.nnodes) atomicAdd(&c1[g], ones);
+      const uint64_t D0 = dG / TILE;
+      const uint32_t z_lo = (uint32_t)rel0 - o_lo;
+      if (z_lo) atomicAdd(&a.zc_next[D0], z_lo);
+      if ((int64_t)len > rel0) {
+        const uint32_t z_hi = (uint32_t)(len - rel0) - o_hi;
+        if (z_hi) atomicAdd(&a.zc_next[D0 + 1], z_hi);
+      }
+    }
+  }
+}
+
 
 // MODE 0 ("bits"): the level's bit words + the tile's zero count.
 // MODE 1 ("split"): the stable scatter of the tile into the next level, with
@@ -263,7 +361,7 @@ __global__ __launch_bounds__(256) void k_level(LevelArgs a) {
   __shared__ uint2 s_tab[SCATTER ? TAB_CACHE : 1];
 
   const int tid = threadIdx.x, w = tid >> 6, lane = lane_id();
-  const uint32_t tile = blockIdx.x;
+  const uint32_t tile = a.tile0 + blockIdx.x;
   const uint64_t tile_start = (uint64_t)tile * TILE;
   const uint64_t n = a.n;
   const uint64_t valid = (n - tile_start < (uint64_t)TILE) ? n - tile_start : (uint64_t)TILE;
@@ -417,93 +515,197 @@ __global__ __launch_bounds__(256) void k_level(LevelArgs a) {
         else atomicOr(a.err, FLAG_BADSYM);
       }
     }
-    // ---- per group (contiguous slots [sG, eG) -> contiguous dests [dG, dG+len)):
-    // one wave per group, one output word per lane: level l+1's bit words
-    // (interior words stored, the two edge words OR-ed), the group's next-bit
-    // ones (level l+1's table) and its zeros per destination tile of level l+1
-    uint32_t* c1 = COUNT ? a.c1 + (uint64_t)(tile & (a.reps - 1)) * (2ull * a.nnodes) : nullptr;
-    constexpr int HPL = (NW + 63) / 64;  // head words per lane
-    uint64_t hwl[HPL];
-    uint32_t hc = 0;
+    split_groups<T, COUNT, TILE>(a, tile, (uint32_t)valid, zt, zprev, oprev, cached, pfirst, s_stage, s_nb,
+                                 s_head, s_tab);
+  }
+}
+
+// Raw buffer store of one key at element index i of a buffer described as
+// `nbytes` long: the hardware drops stores at or past the end (no branch).
+__device__ __forceinline__ __amdgpu_buffer_rsrc_t key_rsrc(void* base, uint32_t nbytes) {
+  return __builtin_amdgcn_make_buffer_rsrc(base, (short)0, (int)nbytes, 0x00020000);
+}
+template <typename T>
+__device__ __forceinline__ void key_store(__amdgpu_buffer_rsrc_t r, uint32_t i, T v) {
+  constexpr int NT = 2;  // non-temporal
+  if constexpr (sizeof(T) == 1) __builtin_amdgcn_raw_buffer_store_b8((uint8_t)v, r, (int)i, 0, NT);
+  else if constexpr (sizeof(T) == 2) __builtin_amdgcn_raw_buffer_store_b16((uint16_t)v, r, (int)(i * 2), 0, NT);
+  else if constexpr (sizeof(T) == 4) __builtin_amdgcn_raw_buffer_store_b32((uint32_t)v, r, (int)(i * 4), 0, NT);
+  else {
+    typedef uint32_t u32x2 __attribute__((ext_vector_type(2)));
+    const u32x2 d = {(uint32_t)(uint64_t)v, (uint32_t)((uint64_t)v >> 32)};
+    __builtin_amdgcn_raw_buffer_store_b64(d, r, (int)(i * 8), 0, NT);
+  }
+}
+
+// Split pass over FULL tiles (every tile but a partial last one), in 32-bit
+// index arithmetic (n < 2^32) and without per-element validity tests: the
+// same stable scatter, next-level bit words and counts as k_level<T, 1>.
+// Needs node_shift < 8*sizeof(T) (the host sends level 0 of 32-level uint32
+// keys to k_level).  Chunk zero prefixes by a 16-lane DPP scan; slot-order
+// heads and next-level bits from two LDS reads per element; stores from the
+// node table cached in LDS (a tile's nodes are consecutive).
+template <typename T, bool VEC, bool COUNT>
+__global__ __launch_bounds__(256) void k_split(LevelArgs a) {
+  constexpr int V = VecOf<T>::V;
+  constexpr int TILE = 256 * LV_STEPS * V;
+  constexpr int NW = TILE / 64;
+  constexpr int ROUNDS = TILE / 256;
+  constexpr int NPL = (V == 16) ? 5 : (V == 8) ? 4 : (V == 4) ? 3 : 2;
+  constexpr int NCH = LV_STEPS * 4;  // (step, wave) chunks, chunk s*4+w
+  static_assert(NCH <= 16, "chunk scan covers one DPP row");
+  __shared__ uint32_t s_wz[NCH];
+  __shared__ uint64_t s_nb[NW + 1];
+  __shared__ uint64_t s_head[NW];
+  __shared__ T s_stage[TILE];
+  __shared__ uint2 s_tab[TAB_CACHE];
+  __shared__ T s_ends[2];
+
+  const int tid = threadIdx.x, w = tid >> 6, lane = lane_id();
+  const uint32_t tile = blockIdx.x;
+  const uint32_t ts = tile * (uint32_t)TILE;
+  const T* src = (const T*)a.src + ts;
+  const uint32_t zprev = a.zpre[tile];
+
+  T x[LV_STEPS][V];
 #pragma unroll
-    for (int j = 0; j < HPL; j++) {
-      const int idx = lane * HPL + j;
-      hwl[j] = (idx < NW) ? s_head[idx] : 0ull;
-      hc += popc64(hwl[j]);
+  for (int s = 0; s < LV_STEPS; s++) {
+    const uint32_t e0 = (uint32_t)(s * 256 + tid) * V;
+    if (VEC) {
+      const v4u32 q = __builtin_nontemporal_load((const v4u32*)(src + e0));
+      const T* qp = (const T*)&q;
+#pragma unroll
+      for (int v = 0; v < V; v++) x[s][v] = qp[v];
+    } else {
+#pragma unroll
+      for (int v = 0; v < V; v++) x[s][v] = src[e0 + v];
     }
-    const uint32_t hincl = wave_incl_scan(hc);
-    const uint32_t hex = hincl - hc;
-    const uint32_t ng = __shfl(hincl, 63);
-    auto group_start = [&](uint32_t gi) -> uint32_t {
-      const uint64_t own = wave_ballot(hex <= gi && gi < hincl);
-      const int ol = (int)__builtin_ctzll(own);
-      uint32_t slot = 0;
-      if (lane == ol) {
-        uint32_t r = gi - hex;
+  }
+  // this level's bits, the lane's zero rank inside its chunk, chunk zeros
+  uint32_t mk[LV_STEPS], pre[LV_STEPS];
 #pragma unroll
-        for (int j = 0; j < HPL; j++) {
-          const uint32_t c = popc64(hwl[j]);
-          if (r < c) { slot = (uint32_t)(lane * HPL + j) * 64 + select_in_word(hwl[j], r); break; }
-          r -= c;
-        }
-      }
-      return __shfl(slot, ol);
-    };
-    for (uint32_t gi = (uint32_t)w; gi < ng; gi += 4) {
-      const uint32_t sG = group_start(gi);
-      const uint32_t eG = (gi + 1 < ng) ? group_start(gi + 1) : (uint32_t)valid;
-      const uint32_t len = eG - sG;
-      const uint64_t x0 = (uint64_t)s_stage[sG];
-      const uint64_t pnode = x0 >> a.node_shift;
-      const uint64_t g = x0 >> a.bitpos;
-      uint2 ab;
-      if (cached && pnode - pfirst < TAB_CACHE) ab = s_tab[pnode - pfirst];
-      else if (pnode < a.nnodes) ab = a.tab[pnode];
-      else { if (lane == 0) atomicOr(a.err, FLAG_BADSYM); continue; }
-      const uint64_t dG = (sG < zt) ? zprev + sG + ab.x : oprev + (sG - zt) + ab.y;
-      if (dG + len > n || len == 0) { if (lane == 0) atomicOr(a.err, FLAG_BADSYM); continue; }
-      const uint64_t b0 = (dG / TILE + 1) * TILE;  // first destination-tile boundary
-      const int64_t rel0 = (dG + len > b0) ? (int64_t)(b0 - dG) : (int64_t)len;
-      const uint64_t GB0 = a.next_base + dG;
-      const uint64_t k0 = GB0 >> 6, k1 = (GB0 + len - 1) >> 6;
-      uint32_t o_lo = 0, o_hi = 0;
-      for (uint64_t k = k0 + lane; k <= k1; k += 64) {
-        const int64_t off = (int64_t)(k * 64) - (int64_t)GB0;  // group bit at word bit 0
-        const int64_t P = (int64_t)sG + off;                   // its slot
-        uint64_t val;
-        if (P >= 0) {
-          const uint32_t w0 = (uint32_t)(P >> 6), sh = (uint32_t)(P & 63);
-          val = s_nb[w0] >> sh;
-          if (sh) val |= s_nb[w0 + 1] << (64 - sh);
-        } else {
-          val = s_nb[0] << (uint32_t)(-P);
-        }
-        const int64_t lo = off < 0 ? -off : 0;
-        const int64_t hi = ((int64_t)len - off < 64) ? (int64_t)len - off : 64;
-        const uint64_t msk = (hi >= 64 ? ~0ull : ((1ull << hi) - 1)) & (~0ull << lo);
-        val &= msk;
-        if (lo == 0 && hi == 64) a.words[k] = val;
-        else if (val) atomicOr((unsigned long long*)&a.words[k], (unsigned long long)val);
-        const int64_t cut = rel0 - off;  // bits below `cut` belong to the first destination tile
-        const uint64_t mlo = cut <= 0 ? 0ull : (cut >= 64 ? ~0ull : ((1ull << cut) - 1));
-        o_lo += popc64(val & mlo);
-        o_hi += popc64(val & ~mlo);
-      }
-      o_lo = wave_sum(o_lo);
-      o_hi = wave_sum(o_hi);
-      if (lane == 0) {
-        const uint32_t ones = o_lo + o_hi;
-        if (COUNT && ones && g < 2ull * a.nnodes) atomicAdd(&c1[g], ones);
-        const uint64_t D0 = dG / TILE;
-        const uint32_t z_lo = (uint32_t)rel0 - o_lo;
-        if (z_lo) atomicAdd(&a.zc_next[D0], z_lo);
-        if ((int64_t)len > rel0) {
-          const uint32_t z_hi = (uint32_t)(len - rel0) - o_hi;
-          if (z_hi) atomicAdd(&a.zc_next[D0 + 1], z_hi);
-        }
+  for (int s = 0; s < LV_STEPS; s++) {
+    uint32_t m = 0;
+#pragma unroll
+    for (int v = 0; v < V; v++) m |= (uint32_t)((x[s][v] >> a.bitpos) & 1u) << v;
+    mk[s] = m;
+    uint32_t tot;
+    pre[s] = plane_scan<NPL>((uint32_t)V - (uint32_t)__popc(m), tot);
+    if (lane == 0) s_wz[s * 4 + w] = tot;
+  }
+  if (tid == 0) s_ends[0] = x[0][0];
+  if (tid == 255) s_ends[1] = x[LV_STEPS - 1][V - 1];
+  __syncthreads();
+
+  // chunk prefixes: lane c < NCH holds chunk c's zeros; inclusive scan in row 0
+  const uint32_t cz = lane < NCH ? s_wz[lane] : 0u;
+  uint32_t ci = cz;
+  ci += (uint32_t)__builtin_amdgcn_update_dpp(0, (int)ci, 0x111, 0xF, 0xF, false);  // row_shr:1
+  ci += (uint32_t)__builtin_amdgcn_update_dpp(0, (int)ci, 0x112, 0xF, 0xF, false);  // row_shr:2
+  ci += (uint32_t)__builtin_amdgcn_update_dpp(0, (int)ci, 0x114, 0xF, 0xF, false);  // row_shr:4
+  ci += (uint32_t)__builtin_amdgcn_update_dpp(0, (int)ci, 0x118, 0xF, 0xF, false);  // row_shr:8
+  const uint32_t cx = ci - cz;
+  const uint32_t zt = (uint32_t)__builtin_amdgcn_readlane((int)ci, NCH - 1);
+  const int wu = __builtin_amdgcn_readfirstlane(w);
+  uint32_t woff[LV_STEPS];
+#pragma unroll
+  for (int s = 0; s < LV_STEPS; s++) woff[s] = (uint32_t)__builtin_amdgcn_readlane((int)cx, s * 4 + wu);
+  if (tid == 0) a.zc[tile] = zt;  // unused by the split; keeps zc like k_level's
+
+  // node-table entries of the tile's node range into LDS
+  const uint64_t pfirst = (uint64_t)s_ends[0] >> a.node_shift;
+  const uint64_t plast = (uint64_t)s_ends[1] >> a.node_shift;
+  const bool cached = plast >= pfirst && plast - pfirst < TAB_CACHE && plast < a.nnodes;
+  if (cached && (uint64_t)tid <= plast - pfirst) s_tab[tid] = a.tab[pfirst + tid];
+
+  // stable 0/1 partition of the tile into LDS
+#pragma unroll
+  for (int s = 0; s < LV_STEPS; s++) {
+    const uint32_t m = mk[s];
+    const uint32_t zb = woff[s] + pre[s];                      // slot of the lane's first zero
+    const uint32_t ob = zt + (uint32_t)(s * 256 + tid) * V - zb;  // slot of its first one
+#pragma unroll
+    for (int v = 0; v < V; v++) {
+      const uint32_t ov = (uint32_t)__popc(m & ((1u << v) - 1u));  // ones before v
+      const uint32_t slot = ((m >> v) & 1u) ? ob + ov : zb + (uint32_t)v - ov;
+      s_stage[slot] = x[s][v];
+    }
+  }
+  __syncthreads();
+
+  // slot-order bit arrays: next level's bit (s_nb) and group starts (s_head,
+  // group = (l+1)-prefix)
+  const uint32_t nb = a.bitpos - 1;
+  const T hlim = (T)((T)1 << a.bitpos);
+#pragma unroll 4
+  for (int r = 0; r < ROUNDS; r++) {
+    const uint32_t q = (uint32_t)(r * 256 + tid);
+    const T xv = s_stage[q];
+    const T pv = s_stage[q ? q - 1 : 0];
+    const bool head = q == 0 || (T)(xv ^ pv) >= hlim;
+    const uint64_t H = wave_ballot(head);
+    const uint64_t C = wave_ballot((xv >> nb) & 1u);
+    if (lane == 0) {
+      s_nb[q >> 6] = C;
+      s_head[q >> 6] = H;
+    }
+  }
+  if (tid == 0) s_nb[NW] = 0;
+  __syncthreads();
+
+  // lane-consecutive stores of the staged tile
+  const uint32_t oprev = ts - zprev;
+  T* dst = (T*)a.dst;
+  bool bad = false;
+  if (cached && (uint64_t)a.n * sizeof(T) < (1ull << 31)) {
+    // destinations past n (only from invalid keys, which the bits pass flags)
+    // are dropped by the buffer range check
+    const __amdgpu_buffer_rsrc_t rs = key_rsrc(dst, a.n * (uint32_t)sizeof(T));
+    const uint32_t pf = (uint32_t)pfirst;
+    const uint32_t b0 = zprev + (uint32_t)tid, b1 = oprev - zt + (uint32_t)tid;
+#pragma unroll 16
+    for (int r = 0; r < ROUNDS; r++) {
+      const uint32_t q = (uint32_t)(r * 256 + tid);
+      const T xv = s_stage[q];
+      uint32_t pi = (uint32_t)(xv >> a.node_shift) - pf;
+      pi = pi < TAB_CACHE - 1 ? pi : TAB_CACHE - 1;
+      const uint2 ab = s_tab[pi];
+      const uint32_t dest = q < zt ? b0 + (uint32_t)(r * 256) + ab.x : b1 + (uint32_t)(r * 256) + ab.y;
+      key_store<T>(rs, dest, xv);
+    }
+  } else if (cached) {
+    const uint32_t pf = (uint32_t)pfirst;
+    const uint32_t b0 = zprev + (uint32_t)tid, b1 = oprev - zt + (uint32_t)tid;
+#pragma unroll 16
+    for (int r = 0; r < ROUNDS; r++) {
+      const uint32_t q = (uint32_t)(r * 256 + tid);
+      const T xv = s_stage[q];
+      uint32_t pi = (uint32_t)(xv >> a.node_shift) - pf;
+      pi = pi < TAB_CACHE - 1 ? pi : TAB_CACHE - 1;
+      const uint2 ab = s_tab[pi];
+      const uint32_t dest = q < zt ? b0 + (uint32_t)(r * 256) + ab.x : b1 + (uint32_t)(r * 256) + ab.y;
+      if (dest < a.n) __builtin_nontemporal_store(xv, dst + dest);
+      else bad = true;
+    }
+  } else {
+#pragma unroll 4
+    for (int r = 0; r < ROUNDS; r++) {
+      const uint32_t q = (uint32_t)(r * 256 + tid);
+      const T xv = s_stage[q];
+      const uint64_t pnode = (uint64_t)xv >> a.node_shift;
+      if (pnode < a.nnodes) {
+        const uint2 ab = a.tab[pnode];
+        const uint32_t dest = q < zt ? zprev + q + ab.x : oprev + (q - zt) + ab.y;
+        if (dest < a.n) __builtin_nontemporal_store(xv, dst + dest);
+        else bad = true;
+      } else {
+        bad = true;
       }
     }
   }
+  if (bad) atomicOr(a.err, FLAG_BADSYM);
+  split_groups<T, COUNT, TILE>(a, tile, (uint32_t)TILE, zt, zprev, oprev, cached, pfirst, s_stage, s_nb, s_head,
+                               s_tab);
 }
 
 // sums the replicas of a level's next-bit counters: out[g] = sum_r c1[r*ng + g]
@@ -586,15 +788,34 @@ void pick_level(bool count, void (*&k)(LevelArgs)) {
   else k = k_level<T, MODE, VEC, false>;
 }
 
+template <typename T, bool VEC>
+void pick_split(bool count, void (*&k)(LevelArgs)) {
+  if (count) k = k_split<T, VEC, true>;
+  else k = k_split<T, VEC, false>;
+}
+
 template <typename T>
 int launch_level(wtg_ctx* ctx, const LevelArgs& la, uint64_t tiles, int mode, bool vec, bool count) {
+  constexpr uint64_t TILE = 256ull * LV_STEPS * VecOf<T>::V;
   void (*k)(LevelArgs) = nullptr;
   if (mode == 1) {
+    // full tiles: k_split; a partial last tile (and a node shift as wide as
+    // the key): k_level
+    const uint64_t full = la.node_shift < 8 * sizeof(T) ? (uint64_t)la.n / TILE : 0;
+    if (full) {
+      if (vec) pick_split<T, true>(count, k); else pick_split<T, false>(count, k);
+      WTG_LAUNCH(ctx, "level_split", k, dim3((unsigned)full), dim3(256), 0, la);
+      WTG_LAUNCH_CHECK();
+    }
+    if (full == tiles) return WTG_OK;
     if (vec) pick_level<T, 1, true>(count, k); else pick_level<T, 1, false>(count, k);
+    LevelArgs lt = la;
+    lt.tile0 = (uint32_t)full;
+    WTG_LAUNCH(ctx, "level_split_tail", k, dim3((unsigned)(tiles - full)), dim3(256), 0, lt);
   } else {
     if (vec) pick_level<T, 0, true>(false, k); else pick_level<T, 0, false>(false, k);
+    WTG_LAUNCH(ctx, "level_bits", k, dim3((unsigned)tiles), dim3(256), 0, la);
   }
-  WTG_LAUNCH(ctx, mode == 1 ? "level_split" : "level_bits", k, dim3((unsigned)tiles), dim3(256), 0, la);
   WTG_LAUNCH_CHECK();
   return WTG_OK;
 }

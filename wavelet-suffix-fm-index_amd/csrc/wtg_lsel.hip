@@ -597,7 +597,7 @@ static int lsel_models(wtg_ctx* ctx, wtg_lsel* ls, const wtg_lsel_params& P, con
 extern "C" int wtg_lsel_build(wtg_ctx* ctx, uint32_t nvec, const uint64_t* const* d_words,
                               const uint64_t* nbits, const wtg_lsel_params* prm, wtg_lsel** out) {
   WTG_REQUIRE(ctx && out && d_words && nbits && nvec >= 1 && nvec <= 256, "wtg_lsel_build: bad argument (1 <= nvec <= 256)");
-  wtg_lsel_params P{16, 4096, 256, 200, 4.0f, 512};
+  wtg_lsel_params P{16, 4096, 480, 200, 4.0f, 512};
   if (prm) P = *prm;
   auto pow2 = [](uint32_t x) { return x && !(x & (x - 1)); };
   WTG_REQUIRE(pow2(P.train_stride) && pow2(P.max_gap) && P.max_gap >= 2 && P.max_gap <= (1u << 24),
