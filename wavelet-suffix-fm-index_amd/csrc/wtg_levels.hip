@@ -242,103 +242,142 @@ struct LevelArgs {
 
 #define LV_STEPS 4
 
+// wave-uniform copies of a value (readfirstlane) and of one lane's value
+template <typename U>
+__device__ __forceinline__ U rfl(U v) {
+  if constexpr (sizeof(U) == 8) {
+    const uint32_t lo = (uint32_t)__builtin_amdgcn_readfirstlane((int)(uint32_t)(uint64_t)v);
+    const uint32_t hi = (uint32_t)__builtin_amdgcn_readfirstlane((int)(uint32_t)((uint64_t)v >> 32));
+    return (U)(((uint64_t)hi << 32) | lo);
+  } else {
+    return (U)(uint32_t)__builtin_amdgcn_readfirstlane((int)(uint32_t)v);
+  }
+}
+__device__ __forceinline__ uint32_t rl32(uint32_t v, int l) { return (uint32_t)__builtin_amdgcn_readlane((int)v, l); }
+__device__ __forceinline__ uint64_t rl64(uint64_t v, int l) {
+  return ((uint64_t)rl32((uint32_t)(v >> 32), l) << 32) | rl32((uint32_t)v, l);
+}
 // Group phase of a split tile: per group of the next level (contiguous slots
 // [sG, eG) -> contiguous destinations [dG, dG+len)), one wave per group, one
 // output word per lane: level l+1's bit words (interior words stored, the two
 // edge words OR-ed), the group's next-bit ones (level l+1's table) and its
-// zeros per destination tile of level l+1.
+// zeros per destination tile of level l+1.  Group starts and the ones counts
+// come from per-lane prefixes of the slot-order head / next-bit words read
+// back as wave-uniform values, so the per-group bookkeeping runs on the
+// scalar unit.
 template <typename T, bool COUNT, int TILE>
 __device__ __forceinline__ void split_groups(const LevelArgs& a, uint32_t tile, uint32_t valid, uint32_t zt,
                                              uint64_t zprev, uint64_t oprev, bool cached, uint64_t pfirst,
                                              const T* s_stage, const uint64_t* s_nb, const uint64_t* s_head,
                                              const uint2* s_tab) {
   constexpr int NW = TILE / 64;
+  constexpr int HPL = (NW + 63) / 64;  // head / next-bit words per lane
   const int w = threadIdx.x >> 6, lane = lane_id();
   uint32_t* c1 = COUNT ? a.c1 + (uint64_t)(tile & (a.reps - 1)) * (2ull * a.nnodes) : nullptr;
-  constexpr int HPL = (NW + 63) / 64;  // head words per lane
-  uint64_t hwl[HPL];
-  uint32_t hc = 0;
+  uint64_t hwl[HPL], nwl[HPL];
+  uint32_t hc = 0, oc = 0;
 #pragma unroll
   for (int j = 0; j < HPL; j++) {
     const int idx = lane * HPL + j;
     hwl[j] = (idx < NW) ? s_head[idx] : 0ull;
+    nwl[j] = (idx < NW) ? s_nb[idx] : 0ull;
     hc += popc64(hwl[j]);
+    oc += popc64(nwl[j]);
   }
-  const uint32_t hincl = wave_incl_scan(hc);
-  const uint32_t hex = hincl - hc;
-  const uint32_t ng = __shfl(hincl, 63);
+  const uint32_t hincl = wave_incl_scan(hc), hex = hincl - hc;
+  const uint32_t oincl = wave_incl_scan(oc), oex = oincl - oc;
+  const uint32_t ng = rl32(hincl, 63);
+  const uint32_t otot = rl32(oincl, 63);
+  // next-level ones among slots [0, x) (x uniform, <= valid)
+  auto ones_before = [&](uint32_t x) -> uint32_t {
+    const uint32_t wi = x >> 6;
+    if (wi >= (uint32_t)NW) return otot;
+    const int ln = (int)(wi / HPL);
+    const uint32_t jj = wi % HPL;
+    uint32_t r = rl32(oex, ln);
+#pragma unroll
+    for (int j = 0; j < HPL; j++) {
+      const uint64_t wv = rl64(nwl[j], ln);
+      if ((uint32_t)j < jj) r += popc64(wv);
+      else if ((uint32_t)j == jj) r += popc64(wv & ((1ull << (x & 63)) - 1));
+    }
+    return r;
+  };
+  // slot of the gi-th group start (gi uniform, < ng)
   auto group_start = [&](uint32_t gi) -> uint32_t {
     const uint64_t own = wave_ballot(hex <= gi && gi < hincl);
     const int ol = (int)__builtin_ctzll(own);
+    uint32_t r = gi - rl32(hex, ol);
     uint32_t slot = 0;
-    if (lane == ol) {
-      uint32_t r = gi - hex;
 #pragma unroll
-      for (int j = 0; j < HPL; j++) {
-        const uint32_t c = popc64(hwl[j]);
-        if (r < c) { slot = (uint32_t)(lane * HPL + j) * 64 + select_in_word(hwl[j], r); break; }
-        r -= c;
-      }
+    for (int j = 0; j < HPL; j++) {
+      const uint64_t hv = rl64(hwl[j], ol);
+      const uint32_t c = popc64(hv);
+      if (r < c) { slot = (uint32_t)(ol * HPL + j) * 64 + select_in_word(hv, r); break; }
+      r -= c;
     }
-    return __shfl(slot, ol);
+    return slot;
   };
   for (uint32_t gi = (uint32_t)w; gi < ng; gi += 4) {
     const uint32_t sG = group_start(gi);
-    const uint32_t eG = (gi + 1 < ng) ? group_start(gi + 1) : (uint32_t)valid;
+    const uint32_t eG = (gi + 1 < ng) ? group_start(gi + 1) : valid;
     const uint32_t len = eG - sG;
-    const uint64_t x0 = (uint64_t)s_stage[sG];
+    const uint64_t x0 = (uint64_t)rfl<T>(s_stage[sG]);
     const uint64_t pnode = x0 >> a.node_shift;
     const uint64_t g = x0 >> a.bitpos;
     uint2 ab;
-    if (cached && pnode - pfirst < TAB_CACHE) ab = s_tab[pnode - pfirst];
-    else if (pnode < a.nnodes) ab = a.tab[pnode];
-    else { if (lane == 0) atomicOr(a.err, FLAG_BADSYM); continue; }
+    if (cached && pnode - pfirst < TAB_CACHE) {
+      const uint2 t = s_tab[pnode - pfirst];
+      ab = make_uint2(rfl<uint32_t>(t.x), rfl<uint32_t>(t.y));
+    } else if (pnode < a.nnodes) {
+      ab = a.tab[pnode];
+    } else {
+      if (lane == 0) atomicOr(a.err, FLAG_BADSYM);
+      continue;
+    }
     const uint64_t dG = (sG < zt) ? zprev + sG + ab.x : oprev + (sG - zt) + ab.y;
     if (dG + len > a.n || len == 0) { if (lane == 0) atomicOr(a.err, FLAG_BADSYM); continue; }
     const uint64_t b0 = (dG / TILE + 1) * TILE;  // first destination-tile boundary
-    const int64_t rel0 = (dG + len > b0) ? (int64_t)(b0 - dG) : (int64_t)len;
+    const uint32_t rel0 = (dG + len > b0) ? (uint32_t)(b0 - dG) : len;
     const uint64_t GB0 = a.next_base + dG;
     const uint64_t k0 = GB0 >> 6, k1 = (GB0 + len - 1) >> 6;
-    uint32_t o_lo = 0, o_hi = 0;
-    for (uint64_t k = k0 + lane; k <= k1; k += 64) {
-      const int64_t off = (int64_t)(k * 64) - (int64_t)GB0;  // group bit at word bit 0
-      const int64_t P = (int64_t)sG + off;                   // its slot
+    const int32_t off0 = (int32_t)(k0 * 64 - GB0);  // in (-64, 0]
+    const uint32_t nwo = (uint32_t)(k1 - k0 + 1);
+    for (uint32_t t = (uint32_t)lane; t < nwo; t += 64) {
+      const int32_t off = off0 + (int32_t)(t * 64);  // group bit at word bit 0
+      const int32_t P = (int32_t)sG + off;           // its slot
       uint64_t val;
       if (P >= 0) {
-        const uint32_t w0 = (uint32_t)(P >> 6), sh = (uint32_t)(P & 63);
+        const uint32_t w0 = (uint32_t)P >> 6, sh = (uint32_t)P & 63;
         val = s_nb[w0] >> sh;
         if (sh) val |= s_nb[w0 + 1] << (64 - sh);
       } else {
         val = s_nb[0] << (uint32_t)(-P);
       }
-      const int64_t lo = off < 0 ? -off : 0;
-      const int64_t hi = ((int64_t)len - off < 64) ? (int64_t)len - off : 64;
+      const int32_t lo = off < 0 ? -off : 0;
+      const int32_t hi = ((int32_t)len - off < 64) ? (int32_t)len - off : 64;
       const uint64_t msk = (hi >= 64 ? ~0ull : ((1ull << hi) - 1)) & (~0ull << lo);
       val &= msk;
+      const uint64_t k = k0 + t;
       if (lo == 0 && hi == 64) a.words[k] = val;
       else if (val) atomicOr((unsigned long long*)&a.words[k], (unsigned long long)val);
-      const int64_t cut = rel0 - off;  // bits below `cut` belong to the first destination tile
-      const uint64_t mlo = cut <= 0 ? 0ull : (cut >= 64 ? ~0ull : ((1ull << cut) - 1));
-      o_lo += popc64(val & mlo);
-      o_hi += popc64(val & ~mlo);
     }
-    o_lo = wave_sum(o_lo);
-    o_hi = wave_sum(o_hi);
     if (lane == 0) {
-      const uint32_t ones = o_lo + o_hi;
+      const uint32_t os = ones_before(sG);
+      const uint32_t ones = ones_before(eG) - os;
+      const uint32_t o_lo = ones_before(sG + rel0) - os;
+      const uint32_t o_hi = ones - o_lo;
       if (COUNT && ones && g < 2ull * a.nnodes) atomicAdd(&c1[g], ones);
       const uint64_t D0 = dG / TILE;
-      const uint32_t z_lo = (uint32_t)rel0 - o_lo;
+      const uint32_t z_lo = rel0 - o_lo;
       if (z_lo) atomicAdd(&a.zc_next[D0], z_lo);
-      if ((int64_t)len > rel0) {
-        const uint32_t z_hi = (uint32_t)(len - rel0) - o_hi;
+      if (len > rel0) {
+        const uint32_t z_hi = (len - rel0) - o_hi;
         if (z_hi) atomicAdd(&a.zc_next[D0 + 1], z_hi);
       }
     }
   }
 }
-
-
 // MODE 0 ("bits"): the level's bit words + the tile's zero count.
 // MODE 1 ("split"): the stable scatter of the tile into the next level, with
 //                   the tile's zero prefix taken from the scanned counts.
