@@ -463,6 +463,7 @@ def main():
         ctx.levels_build(d, n, L, dw)
     ctx.sync()
     ctx.kernel_timing_reset()
+    ctx.kernel_timing_only("level_")  # events around the level kernels only
     ctx.kernel_timing(True)
     ctx.sync()
     log("timing %d steps" % args.steps)
@@ -535,7 +536,7 @@ def main():
         "roofline": {"bound": "hbm", "achieved": round(achieved, 1), "peak": HBM_PEAK_GBS,
                      "unit": "GB/s", "frac": round(achieved / HBM_PEAK_GBS, 4),
                      "traffic": measured_traffic(args.log2n, L),
-                     "kernel": "k_level<uint32,scatter> (level_split)",
+                     "kernel": "k_split<uint32> (level_split)",
                      "bytes_per_launch": split_bytes, "avg_launch_ms": round(split_avg_s * 1e3, 4),
                      "launches": split_n},
         "build_roofline": {"B_a_bytes": B_a, "frac": round(B_a / (step_ms / 1e3) / 1e9 / HBM_PEAK_GBS, 4),

@@ -67,6 +67,9 @@ int wtg_timer_stop(wtg_ctx* ctx, float* ms);
 int wtg_kernel_timing(wtg_ctx* ctx, int enable);
 int wtg_kernel_time(wtg_ctx* ctx, const char* name, double* total_ms, uint64_t* launches);
 int wtg_kernel_timing_reset(wtg_ctx* ctx);
+/* Restrict per-kernel timing to launches whose name starts with name_prefix
+ * (NULL or "" = every launch), so the other launches carry no events. */
+int wtg_kernel_timing_only(wtg_ctx* ctx, const char* name_prefix);
 
 /* ------------------------------------------------ (a) level-wise wavelet tree */
 typedef struct {

@@ -88,6 +88,7 @@ SIGNATURES = {
     "wtg_kernel_timing": (i32, [vp, i32]),
     "wtg_kernel_time": (i32, [vp, C.c_char_p, C.POINTER(C.c_double), U64P]),
     "wtg_kernel_timing_reset": (i32, [vp]),
+    "wtg_kernel_timing_only": (i32, [vp, C.c_char_p]),
     "wtg_levels_words": (u64, [u64, u32]),
     "wtg_levels_build": (i32, [vp, vp, i32, u64, u32, vp, vp, u64]),
     "wtg_remap": (i32, [vp, vp, i32, u64, U64P]),
@@ -378,6 +379,10 @@ class Context:
 
     def kernel_timing_reset(self):
         _check(lib().wtg_kernel_timing_reset(self.h))
+
+    def kernel_timing_only(self, prefix):
+        """time only launches whose name starts with prefix (None: all)"""
+        _check(lib().wtg_kernel_timing_only(self.h, prefix.encode() if prefix else None))
 
     # ---- (a) levels
     def levels_build(self, d_sym, n, L, d_words, d_nodes=None, slots=0):

@@ -213,14 +213,14 @@ static hipEvent_t kpool_get(wtg_ctx* ctx) {
   return e;
 }
 wtg_kscope::wtg_kscope(wtg_ctx* c, const char* n) : ctx(c), name(n) {
-  if (ctx->ktime) {
+  if (ctx->ktime && (ctx->kfilter.empty() || std::strncmp(n, ctx->kfilter.c_str(), ctx->kfilter.size()) == 0)) {
     a = kpool_get(ctx);
     b = kpool_get(ctx);
     hipEventRecord(a, ctx->stream);
   }
 }
 wtg_kscope::~wtg_kscope() {
-  if (ctx->ktime && a && b) {
+  if (a && b) {
     hipEventRecord(b, ctx->stream);
     ctx->kpending.push_back({std::string(name), a, b});
   }
@@ -245,6 +245,12 @@ extern "C" int wtg_kernel_timing(wtg_ctx* ctx, int enable) {
   WTG_TRY(wtg_set_device(ctx));
   WTG_TRY(kflush(ctx));
   ctx->ktime = enable != 0;
+  return WTG_OK;
+}
+
+extern "C" int wtg_kernel_timing_only(wtg_ctx* ctx, const char* name_prefix) {
+  WTG_REQUIRE(ctx, "null ctx");
+  ctx->kfilter = name_prefix ? name_prefix : "";
   return WTG_OK;
 }
 extern "C" int wtg_kernel_time(wtg_ctx* ctx, const char* name, double* total_ms, uint64_t* launches) {

@@ -42,6 +42,7 @@ struct wtg_ctx {
   hipEvent_t t0 = nullptr, t1 = nullptr;
   // per-kernel timing
   bool ktime = false;
+  std::string kfilter;   // time only launches whose name starts with this (empty: all)
   std::vector<wtg_kpending> kpending;
   std::vector<hipEvent_t> kpool;
   std::map<std::string, wtg_kstat> kstats;

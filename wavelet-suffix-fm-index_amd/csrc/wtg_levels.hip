@@ -42,12 +42,13 @@ typedef uint32_t v4u32 __attribute__((ext_vector_type(4)));
 
 static __host__ __device__ __forceinline__ uint64_t heap_off(uint32_t j) { return (1ull << j) - 1; }
 // Counter replicas for the next-bit counts of the 2^j groups of level j: a
-// group receives about 2*tiles/2^j tile contributions; replicas keep that
-// near 64 per counter (power of two, at most 1024).
+// tile adds to replica (tile mod reps).  A skewed input can send nearly every
+// tile's contribution to one group, so the replica count follows the tile
+// count (at most 1024, and at most 2^17 counters per level), not the mean
+// group size.
 static uint32_t c1_reps(uint32_t j, uint64_t tiles) {
-  uint64_t per = (2 * tiles) >> j;
   uint32_t r = 1;
-  while (r < 1024 && (uint64_t)r * 64 < per) r <<= 1;
+  while (r < 1024 && r < tiles && ((uint64_t)r << (j + 1)) <= (1ull << 17)) r <<= 1;
   return r;
 }
 
@@ -238,6 +239,7 @@ struct LevelArgs {
   const uint32_t* zpre;  // split pass: exclusive prefix of zc (in)
   uint32_t* err;
   uint32_t tile0;        // tile of block 0
+  uint32_t store;        // split pass: 0 = the next level's keys are not needed (last split)
 };
 
 #define LV_STEPS 4
@@ -541,7 +543,7 @@ __global__ __launch_bounds__(256) void k_level(LevelArgs a) {
 #pragma unroll 16
     for (int r = 0; r < ROUNDS; r++) {
       const uint32_t q = (uint32_t)(r * 256 + tid);
-      if (q < valid) {
+      if (q < valid && a.store) {
         const T xv = s_stage[q];
         const uint64_t pnode = (uint64_t)xv >> a.node_shift;
         uint2 ab;
@@ -696,7 +698,9 @@ __global__ __launch_bounds__(256) void k_split(LevelArgs a) {
   const uint32_t oprev = ts - zprev;
   T* dst = (T*)a.dst;
   bool bad = false;
-  if (cached && (uint64_t)a.n * sizeof(T) < (1ull << 31)) {
+  if (!a.store) {
+    // last split: only the next level's bits and counts are needed
+  } else if (cached && (uint64_t)a.n * sizeof(T) < (1ull << 31)) {
     // destinations past n (only from invalid keys, which the bits pass flags)
     // are dropped by the buffer range check
     const __amdgpu_buffer_rsrc_t rs = key_rsrc(dst, a.n * (uint32_t)sizeof(T));
@@ -747,17 +751,18 @@ __global__ __launch_bounds__(256) void k_split(LevelArgs a) {
                                s_tab);
 }
 
-// sums the replicas of a level's next-bit counters: out[g] = sum_r c1[r*ng + g]
-__global__ __launch_bounds__(256) void k_c1_reduce(const uint32_t* __restrict__ c1, uint32_t reps,
-                                                   uint64_t ngroups, uint32_t* __restrict__ out) {
-  __shared__ uint32_t s_wave[4];
-  const uint64_t g = blockIdx.x;
+// adds replicas 1.. of a level's next-bit counters into replica 0:
+// c1[g] += sum_{r>=1} c1[r*ng + g]; thread per group (coalesced rows), block
+// row y covers replicas [1 + y*C1_RCH, 1 + (y+1)*C1_RCH)
+#define C1_RCH 32
+__global__ __launch_bounds__(256) void k_c1_reduce(uint32_t* __restrict__ c1, uint32_t reps, uint64_t ngroups) {
+  const uint64_t g = (uint64_t)blockIdx.x * 256 + threadIdx.x;
+  if (g >= ngroups) return;
+  const uint32_t r0 = 1 + blockIdx.y * C1_RCH;
+  const uint32_t r1 = r0 + C1_RCH < reps ? r0 + C1_RCH : reps;
   uint32_t acc = 0;
-  for (uint32_t r = threadIdx.x; r < reps; r += 256) acc += c1[(uint64_t)r * ngroups + g];
-  acc = wave_sum(acc);
-  if (lane_id() == 0) s_wave[threadIdx.x >> 6] = acc;
-  __syncthreads();
-  if (threadIdx.x == 0) out[g] = s_wave[0] + s_wave[1] + s_wave[2] + s_wave[3];
+  for (uint32_t r = r0; r < r1; r++) acc += c1[(uint64_t)r * ngroups + g];
+  if (acc) atomicAdd(&c1[g], acc);
 }
 
 __global__ void k_cnt_from_zeros(uint32_t* __restrict__ cnt, const uint32_t* zeros, uint64_t n) {
@@ -843,7 +848,8 @@ int launch_level(wtg_ctx* ctx, const LevelArgs& la, uint64_t tiles, int mode, bo
     const uint64_t full = la.node_shift < 8 * sizeof(T) ? (uint64_t)la.n / TILE : 0;
     if (full) {
       if (vec) pick_split<T, true>(count, k); else pick_split<T, false>(count, k);
-      WTG_LAUNCH(ctx, "level_split", k, dim3((unsigned)full), dim3(256), 0, la);
+      // the last split (no key stores) is timed under its own name
+      WTG_LAUNCH(ctx, la.store ? "level_split" : "level_split_last", k, dim3((unsigned)full), dim3(256), 0, la);
       WTG_LAUNCH_CHECK();
     }
     if (full == tiles) return WTG_OK;
@@ -936,6 +942,7 @@ int levels_build_t(wtg_ctx* ctx, const T* d_sym, uint64_t n, uint32_t L, uint64_
     LevelArgs la{};
     la.src = src;
     la.dst = buf[k & 1];
+    la.store = k + 2 < L ? 1u : 0u;  // the last split's keys are never read
     la.words = d_words;
     la.bit_base = (uint64_t)(k - first) * n;
     la.n = (uint32_t)n;
@@ -988,8 +995,9 @@ int levels_build_t(wtg_ctx* ctx, const T* d_sym, uint64_t n, uint32_t L, uint64_
     }
     // node table of level k (node starts always: cheap, needed by WTnodes)
     if (k >= 1 && k > first && n > 0 && P.reps[k] > 1) {
-      WTG_LAUNCH(ctx, "c1_reduce", k_c1_reduce, dim3((unsigned)(1u << k)), dim3(256), 0, c1 + P.c1off[k],
-                 P.reps[k], (uint64_t)(1ull << k), c1 + P.c1off[k]);
+      WTG_LAUNCH(ctx, "c1_reduce", k_c1_reduce,
+                 dim3((unsigned)wtg_ceil_div(1ull << k, 256), (unsigned)wtg_ceil_div(P.reps[k] - 1, C1_RCH)), dim3(256),
+                 0, c1 + P.c1off[k], P.reps[k], (uint64_t)(1ull << k));
       WTG_LAUNCH_CHECK();
     }
     {
