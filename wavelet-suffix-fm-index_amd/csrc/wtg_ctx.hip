@@ -1,6 +1,7 @@
 // wtg_ctx.hip -- context, memory, timing and fill kernels of the C ABI.
 #include "wtg_internal.hpp"
 #include <cstring>
+#include <cstdlib>
 
 static thread_local std::string g_err;
 void wtg_set_error(const std::string& msg) { g_err = msg; }
@@ -43,6 +44,18 @@ extern "C" int wtg_ctx_create(int device, wtg_ctx** out) {
   WTG_CHECK_HIP(hipMemset(c->d_flag, 0, 256));
   WTG_CHECK_HIP(hipEventCreate(&c->t0));
   WTG_CHECK_HIP(hipEventCreate(&c->t1));
+  // library allocations come stream-ordered from the device's default pool,
+  // which keeps freed memory for reuse (WTG_NO_POOL=1: plain hipMalloc)
+  if (!std::getenv("WTG_NO_POOL")) {
+    int pools = 0;
+    hipMemPool_t pool = nullptr;
+    uint64_t keep = UINT64_MAX;
+    if (hipDeviceGetAttribute(&pools, hipDeviceAttributeMemoryPoolsSupported, device) == hipSuccess && pools &&
+        hipDeviceGetDefaultMemPool(&pool, device) == hipSuccess &&
+        hipMemPoolSetAttribute(pool, hipMemPoolAttrReleaseThreshold, &keep) == hipSuccess)
+      c->pooled = true;
+    hipGetLastError();
+  }
   WTG_CHECK_HIP(hipDeviceSynchronize());
   *out = c;
   return WTG_OK;
@@ -113,7 +126,7 @@ int wtg_ws_reserve(wtg_ctx* ctx, size_t bytes) {
 extern "C" int wtg_malloc(wtg_ctx* ctx, void** dptr, uint64_t bytes) {
   WTG_REQUIRE(ctx && dptr, "wtg_malloc: null argument");
   WTG_TRY(wtg_set_device(ctx));
-  hipError_t e = hipMalloc(dptr, bytes ? bytes : 1);
+  hipError_t e = ctx->pooled ? hipMallocAsync(dptr, bytes ? bytes : 1, ctx->stream) : hipMalloc(dptr, bytes ? bytes : 1);
   if (e != hipSuccess) {
     hipGetLastError();
     *dptr = nullptr;
@@ -126,8 +139,12 @@ extern "C" int wtg_free(wtg_ctx* ctx, void* dptr) {
   WTG_REQUIRE(ctx, "null ctx");
   if (!dptr) return WTG_OK;
   WTG_TRY(wtg_set_device(ctx));
-  WTG_CHECK_HIP(hipStreamSynchronize(ctx->stream));
-  WTG_CHECK_HIP(hipFree(dptr));
+  if (ctx->pooled) {
+    WTG_CHECK_HIP(hipFreeAsync(dptr, ctx->stream));  // after the stream's earlier work
+  } else {
+    WTG_CHECK_HIP(hipStreamSynchronize(ctx->stream));
+    WTG_CHECK_HIP(hipFree(dptr));
+  }
   return WTG_OK;
 }
 extern "C" int wtg_memcpy_h2d(wtg_ctx* ctx, void* dst, const void* src, uint64_t bytes) {

@@ -41,6 +41,7 @@ struct wtg_ctx {
   // stream timer
   hipEvent_t t0 = nullptr, t1 = nullptr;
   // per-kernel timing
+  bool pooled = false;   // wtg_malloc / wtg_free use the stream-ordered pool
   bool ktime = false;
   std::string kfilter;   // time only launches whose name starts with this (empty: all)
   std::vector<wtg_kpending> kpending;

@@ -10,11 +10,11 @@
 // e-a > logn^4 -> "long" (all 4096 positions) else "mini" (pos(4096k+64t)-a).
 //
 // Two passes over the words: ones per 512-word tile (k_rank_tile_sums) and,
-// after their exclusive scan, one lane per word (k_rank_words): lanes 8i..8i+7
-// of a wave round form directory entry i (the 9-bit fields are the lanes'
-// prefix differences, OR-ed across the 8 lanes), and each lane emits the
-// sample of the argument rank j%64==0 its word holds (at most one per bit
-// value): the low 32 bits of pos(j) straight into slot j/64 of the final mini
+// after their exclusive scan, one lane per directory entry (k_rank_entries):
+// the lane's 8 word counts give the entry's 9-bit fields, a DPP scan of the
+// entry totals its prefix, and for each word the lane emits the sample of the
+// argument rank j%64==0 the word holds (at most one per bit value): the low
+// 32 bits of pos(j) straight into slot j/64 of the final mini
 // array (mini offsets are < logn^4 < 2^32, so subtracting the low 32 bits of
 // the superblock start later gives the exact offset), and the superblock
 // starts/ends.  A second kernel classifies long/mini superblocks and rebases
@@ -82,64 +82,77 @@ __device__ __forceinline__ void word_sample(uint64_t x, uint64_t r, uint32_t c, 
   if (je < r + c) o.sbend[je >> 12] = (w << 6) + select_in_word(x, (uint32_t)(je - r));
 }
 
-// OR of x over the 8 lanes of the caller's group (lanes 8i..8i+7), valid in
-// the group's first lane: two quad permutations, then a row rotation by 12
-__device__ __forceinline__ uint32_t dpp_or8(uint32_t x) {
-  x |= (uint32_t)__builtin_amdgcn_update_dpp(0, (int)x, 0xB1, 0xF, 0xF, false);   // quad_perm(1,0,3,2)
-  x |= (uint32_t)__builtin_amdgcn_update_dpp(0, (int)x, 0x4E, 0xF, 0xF, false);   // quad_perm(2,3,0,1)
-  x |= (uint32_t)__builtin_amdgcn_update_dpp(0, (int)x, 0x12C, 0xF, 0xF, false);  // row_ror:12 (lane i <- lane i+4 mod 16)
-  return x;
+// inclusive scan over the 64 lanes: DPP shifts inside each row of 16, then the
+// row totals read back as scalars
+__device__ __forceinline__ uint32_t wave_incl_scan_dpp(uint32_t v) {
+  v += (uint32_t)__builtin_amdgcn_update_dpp(0, (int)v, 0x111, 0xF, 0xF, false);  // row_shr:1
+  v += (uint32_t)__builtin_amdgcn_update_dpp(0, (int)v, 0x112, 0xF, 0xF, false);  // row_shr:2
+  v += (uint32_t)__builtin_amdgcn_update_dpp(0, (int)v, 0x114, 0xF, 0xF, false);  // row_shr:4
+  v += (uint32_t)__builtin_amdgcn_update_dpp(0, (int)v, 0x118, 0xF, 0xF, false);  // row_shr:8
+  const uint32_t r0 = (uint32_t)__builtin_amdgcn_readlane((int)v, 15);
+  const uint32_t r1 = (uint32_t)__builtin_amdgcn_readlane((int)v, 31);
+  const uint32_t r2 = (uint32_t)__builtin_amdgcn_readlane((int)v, 47);
+  const uint32_t row = (uint32_t)lane_id() >> 4;
+  return v + (row == 0 ? 0u : row == 1 ? r0 : row == 2 ? r0 + r1 : r0 + r1 + r2);
 }
 
-__global__ __launch_bounds__(256) void k_rank_words(RankSelArgs a, uint64_t ntiles, bool write_dir) {
+// Directory entries and select samples: one lane per directory entry (8
+// words), one wave per tile of 512 words.  The lane's word counts give the
+// entry's 9-bit fields directly; one DPP scan of the entry totals gives each
+// entry's prefix inside the tile.
+__global__ __launch_bounds__(256) void k_rank_entries(RankSelArgs a, uint64_t ntiles, bool write_dir, bool aligned) {
   const uint64_t t = (uint64_t)blockIdx.x * 4 + (threadIdx.x >> 6);
   if (t >= ntiles) return;
   const int lane = lane_id();
-  const uint32_t j = (uint32_t)lane & 7u;
-  uint64_t carry = a.tpre[t];
-  uint64_t last1 = 0, last0 = 0;
-  for (int r = 0; r < RS_TILE_WORDS / 64; r++) {
-    const uint64_t w = t * RS_TILE_WORDS + (uint64_t)r * 64 + lane;
-    const uint64_t x = w < a.W ? a.words[w] : 0ull;
-    const uint64_t lo = w * 64;
-    const uint64_t vm = lo >= a.nbits ? 0ull : (a.nbits - lo >= 64 ? ~0ull : ((1ull << (a.nbits - lo)) - 1));
-    const uint32_t c = popc64(x);
-    // wave prefix, prefix inside the lane's 8-word entry and wave total from
-    // the 7 bit planes of c (ballots and popcounts, no cross-lane shuffles)
-    const uint64_t lt = (1ull << lane) - 1;
-    const uint64_t gm = lt & ~((1ull << (lane & ~7)) - 1);
-    uint32_t ex = 0, gex = 0, tot = 0;
+  const uint64_t e = t * 64 + (uint64_t)lane;  // directory entry
+  const uint64_t w0 = e * 8;
+  uint64_t x[8];
+  if (aligned && w0 + 8 <= a.W) {
+    const uint4* q = reinterpret_cast<const uint4*>(a.words + w0);
 #pragma unroll
-    for (int bp = 0; bp < 7; bp++) {
-      const uint64_t B = wave_ballot((c >> bp) & 1u);
-      ex += popc64(B & lt) << bp;
-      gex += popc64(B & gm) << bp;
-      tot += popc64(B) << bp;
+    for (int k = 0; k < 4; k++) {
+      const uint4 v = q[k];
+      x[2 * k] = ((uint64_t)v.y << 32) | v.x;
+      x[2 * k + 1] = ((uint64_t)v.w << 32) | v.z;
     }
-    const uint64_t pre = carry + ex;  // ones before word w
-    carry += tot;
-    const uint64_t b = w >> 3;         // directory entry
-    if (write_dir) {
-      const uint64_t m = a.W > 8 * b ? (a.W - 8 * b < 8 ? a.W - 8 * b : 8) : 0;  // valid words of the entry
-      const uint64_t f = (j >= 1 && j <= m && j <= 7) ? (uint64_t)gex << (63 - 9 * j) : 0ull;
-      const uint32_t flo = dpp_or8((uint32_t)f), fhi = dpp_or8((uint32_t)(f >> 32));
-      if (j == 0 && b <= a.S) {
-        a.D[2 * b] = pre;
-        a.D[2 * b + 1] = ((uint64_t)fhi << 32) | flo;
-      }
-    }
+  } else {
+#pragma unroll
+    for (int k = 0; k < 8; k++) x[k] = w0 + k < a.W ? a.words[w0 + k] : 0ull;
+  }
+  uint32_t f[9];  // f[j] = ones in words [w0, w0 + j)
+  f[0] = 0;
+#pragma unroll
+  for (int k = 0; k < 8; k++) f[k + 1] = f[k] + popc64(x[k]);
+  const uint32_t incl = wave_incl_scan_dpp(f[8]);
+  const uint64_t pre = a.tpre[t] + (incl - f[8]);  // ones before word w0
+  if (write_dir && e <= a.S) {
+    const uint64_t m = a.W > w0 ? (a.W - w0 < 8 ? a.W - w0 : 8) : 0;  // words of the entry
+    uint64_t F = 0;
+#pragma unroll
+    for (int j = 1; j < 8; j++)
+      if ((uint64_t)j <= m) F |= (uint64_t)f[j] << (63 - 9 * j);
+    a.D[2 * e] = pre;
+    a.D[2 * e + 1] = F;
+  }
+  uint64_t last1 = 0, last0 = 0;
+#pragma unroll
+  for (int k = 0; k < 8; k++) {
+    const uint64_t w = w0 + k;
     if (w < a.W) {
+      const uint32_t c = f[k + 1] - f[k];
+      const uint64_t r = pre + f[k];
+      const uint64_t lo = w * 64;
       if (a.want[1]) {
-        word_sample(x, pre, c, w, a.sel[1]);
-        if (c) last1 = lo + (63 - __clzll(x)) + 1;
+        word_sample(x[k], r, c, w, a.sel[1]);
+        if (c) last1 = lo + (63 - __clzll(x[k])) + 1;
       }
       if (a.want[0]) {
-        const uint64_t z = ~x & vm;
-        const uint64_t zpre = (lo < a.nbits ? lo : a.nbits) - pre;
-        word_sample(z, zpre, popc64(z), w, a.sel[0]);
+        const uint64_t vm = a.nbits - lo >= 64 ? ~0ull : ((1ull << (a.nbits - lo)) - 1);
+        const uint64_t z = ~x[k] & vm;
+        word_sample(z, lo - r, popc64(z), w, a.sel[0]);
         if (z) last0 = lo + (63 - __clzll(z)) + 1;
       }
-      if (w + 1 == a.W) *a.ones_total = pre + c;
+      if (w + 1 == a.W) *a.ones_total = r + c;
     }
   }
   // last argument position (+1): only the tile holding the last one (the last
@@ -410,7 +423,8 @@ int wtg_rank_select_run(wtg_ctx* ctx, const uint64_t* d_words, uint64_t nbits, u
                (const uint32_t*)(ws + o_tsum), (uint64_t*)(ws + o_tpre), wtiles, (uint64_t*)(ws + o_scst), ctr + 1,
                ctx->d_flag);
     WTG_LAUNCH_CHECK();
-    WTG_LAUNCH(ctx, "rank_select", k_rank_words, dim3(g4), dim3(256), 0, ra, wtiles, build);
+    WTG_LAUNCH(ctx, "rank_select", k_rank_entries, dim3(g4), dim3(256), 0, ra, wtiles, build,
+               ((uintptr_t)d_words & 15) == 0);
   }
   WTG_LAUNCH_CHECK();
   uint64_t m1 = 0;
