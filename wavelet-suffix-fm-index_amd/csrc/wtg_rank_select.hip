@@ -20,6 +20,7 @@
 // starts/ends.  A second kernel classifies long/mini superblocks and rebases
 // the mini offsets; a third fills the rare long superblocks.
 #include "wtg_internal.hpp"
+#include <algorithm>
 #include <cstring>
 #include <vector>
 
@@ -51,18 +52,24 @@ struct RankSelArgs {
 // lane-consecutive loads
 constexpr int RS_TILE_WORDS = 512;
 __global__ __launch_bounds__(256) void k_rank_tile_sums(const uint64_t* __restrict__ words, uint64_t W,
-                                                        uint64_t ntiles, uint32_t* __restrict__ tsum) {
-  const uint64_t t = (uint64_t)blockIdx.x * 4 + (threadIdx.x >> 6);
-  if (t >= ntiles) return;
-  const uint64_t w0 = t * RS_TILE_WORDS + lane_id();
-  uint32_t c = 0;
+                                                        uint64_t ntiles, uint32_t* __restrict__ tsum, bool aligned) {
+  const uint64_t waves = (uint64_t)gridDim.x * 4;
+  for (uint64_t t = (uint64_t)blockIdx.x * 4 + (threadIdx.x >> 6); t < ntiles; t += waves) {
+    const uint64_t q0 = t * (RS_TILE_WORDS / 2) + lane_id();  // 16-byte pairs of words
+    uint32_t c = 0;
 #pragma unroll
-  for (int k = 0; k < RS_TILE_WORDS / 64; k++) {
-    const uint64_t wi = w0 + (uint64_t)k * 64;
-    c += wi < W ? popc64(words[wi]) : 0u;
+    for (int k = 0; k < RS_TILE_WORDS / 128; k++) {
+      const uint64_t wi = 2 * (q0 + (uint64_t)k * 64);
+      if (aligned && wi + 2 <= W) {
+        const uint4 v = reinterpret_cast<const uint4*>(words)[wi >> 1];
+        c += popc64(((uint64_t)v.y << 32) | v.x) + popc64(((uint64_t)v.w << 32) | v.z);
+      } else {
+        c += (wi < W ? popc64(words[wi]) : 0u) + (wi + 1 < W ? popc64(words[wi + 1]) : 0u);
+      }
+    }
+    c = wave_sum(c);
+    if (lane_id() == 0) tsum[t] = c;
   }
-  c = wave_sum(c);
-  if (lane_id() == 0) tsum[t] = c;
 }
 
 // Directory entries and select samples, one lane per word, one wave per tile
@@ -193,18 +200,26 @@ struct AsmArgs {
 
 __device__ __forceinline__ uint32_t hi_dev(uint64_t x) { return x ? 63 - __clzll(x) : 0; }
 
-__global__ void k_select_assemble(AsmArgs a) {
-  const uint64_t idx = (uint64_t)blockIdx.x * blockDim.x + threadIdx.x;
-  if (idx >= a.sb * 64) return;
-  const uint64_t k = idx >> 6, t = idx & 63;
+// thread per 4 mini slots (one 16-byte load and store): rebase the samples of
+// mini superblocks to their start, clear those of long superblocks; the first
+// thread of a superblock writes its widths and long flag
+__global__ __launch_bounds__(256) void k_select_assemble(AsmArgs a) {
+  const uint64_t idx4 = (uint64_t)blockIdx.x * blockDim.x + threadIdx.x;
+  if (idx4 >= a.sb * 16) return;
+  const uint64_t k = idx4 >> 4;  // superblock (64 slots = 16 threads)
   const uint64_t st = a.sbstart[k];
   const uint64_t en = (k + 1 < a.sb) ? a.sbend[k] : (uint64_t)(*a.last) - 1;
   const bool lng = (en - st) > (uint64_t)a.logn4;
   const uint64_t nsamp = (a.m + 63) / 64;
-  uint32_t v = 0;
-  if (!lng && idx < nsamp) v = a.mini[idx] - (uint32_t)st;
-  a.mini[idx] = v;
-  if (t == 0) {
+  const uint64_t i0 = idx4 * 4;
+  uint4 v = reinterpret_cast<const uint4*>(a.mini)[idx4];
+  const uint32_t s32 = (uint32_t)st;
+  v.x = (!lng && i0 + 0 < nsamp) ? v.x - s32 : 0u;
+  v.y = (!lng && i0 + 1 < nsamp) ? v.y - s32 : 0u;
+  v.z = (!lng && i0 + 2 < nsamp) ? v.z - s32 : 0u;
+  v.w = (!lng && i0 + 3 < nsamp) ? v.w - s32 : 0u;
+  reinterpret_cast<uint4*>(a.mini)[idx4] = v;
+  if ((idx4 & 15) == 0) {
     a.mini_width[k] = lng ? 0 : (uint8_t)(hi_dev(en - st) + 1);
     a.long_width[k] = lng ? (uint8_t)(hi_dev(en) + 1) : 0;
     a.long_flag[k] = lng ? 1u : 0u;
@@ -330,7 +345,7 @@ int sel_finish(wtg_ctx* ctx, const uint64_t* d_words, uint64_t nbits, wtg_select
   aa.mini = s->mini; aa.mini_width = s->mini_width; aa.long_width = s->long_width;
   aa.long_flag = flag; aa.sbstart = s->superblock; aa.sbend = sbend; aa.last = last;
   aa.m = m; aa.sb = sb; aa.logn4 = s->logn4;
-  WTG_LAUNCH(ctx, "select_assemble", k_select_assemble, dim3((unsigned)wtg_ceil_div(sb * 64, 256)),
+  WTG_LAUNCH(ctx, "select_assemble", k_select_assemble, dim3((unsigned)wtg_ceil_div(sb * 16, 256)),
              dim3(256), 0, aa);
   WTG_LAUNCH_CHECK();
   WTG_LAUNCH(ctx, "select_scan", k_scan_u32<uint32_t>, dim3((unsigned)wtg_ceil_div(sb, 4096)), dim3(256), 0,
@@ -417,7 +432,7 @@ int wtg_rank_select_run(wtg_ctx* ctx, const uint64_t* d_words, uint64_t nbits, u
     ra.tpre = (const uint64_t*)(ws + o_tpre);
     const unsigned g4 = (unsigned)wtg_ceil_div(wtiles, 4);
     WTG_LAUNCH(ctx, "rank_sums", k_rank_tile_sums, dim3(g4), dim3(256), 0, d_words, W, wtiles,
-               (uint32_t*)(ws + o_tsum));
+               (uint32_t*)(ws + o_tsum), ((uintptr_t)d_words & 15) == 0);
     WTG_LAUNCH_CHECK();
     WTG_LAUNCH(ctx, "rank_scan", k_scan_u32<uint64_t>, dim3((unsigned)stiles), dim3(256), 0,
                (const uint32_t*)(ws + o_tsum), (uint64_t*)(ws + o_tpre), wtiles, (uint64_t*)(ws + o_scst), ctr + 1,
