@@ -21,13 +21,17 @@
 
 namespace {
 
-// keys of 4096 consecutive positions of the group per block, lane-consecutive
-// (coalesced byte loads and key stores); the block's sequence boundaries are
-// cached in LDS.  Also counts, per sequence, the ones of its top symbol bit
-// (level 0 of every sequence = the first level the pipeline builds).
+// Keys of one group: key(p) = (sequence << L) | symbol for the group's
+// positions p (sequences concatenated in group order).  A thread builds 16
+// consecutive keys: inside one sequence (the common case) from five aligned
+// dword loads of the source bytes (funnel-shifted), stored as 16-byte vectors;
+// across a sequence boundary or at the block's end one key at a time.  The
+// block's sequence boundaries are cached in LDS.  Also counts, per sequence,
+// the ones of its top symbol bit (level 0 of every sequence = the first level
+// the pipeline builds).
 constexpr int BK_SEQ_CACHE = 64;
 template <typename K>
-__global__ __launch_bounds__(256) void k_batch_keys(const uint8_t* __restrict__ sym,
+__global__ __launch_bounds__(256) void k_batch_keys(const uint8_t* __restrict__ sym, uint64_t sym_end, bool fast,
                                                     const uint64_t* __restrict__ seqoff,
                                                     const uint64_t* __restrict__ pref, uint32_t m, uint64_t n,
                                                     uint32_t L, K* __restrict__ keys, uint32_t* __restrict__ ones,
@@ -55,41 +59,91 @@ __global__ __launch_bounds__(256) void k_batch_keys(const uint8_t* __restrict__ 
   const uint32_t f = s_first, cnt = s_cnt;
   const bool cached = cnt <= BK_SEQ_CACHE;
   if (cached) {
-    for (uint32_t k = threadIdx.x; k <= cnt; k += blockDim.x) s_pref[k] = pref[f + k];
+    for (uint32_t k = threadIdx.x; k <= cnt; k += blockDim.x) s_pref[k] = k + f < m ? pref[f + k] : n;
     for (uint32_t k = threadIdx.x; k < cnt; k += blockDim.x) { s_off[k] = seqoff[f + k]; s_ones[k] = 0; }
   }
   __syncthreads();
+  auto seq_start = [&](uint32_t s) -> uint64_t { return cached ? s_pref[s] : pref[f + s]; };
+  auto seq_end = [&](uint32_t s) -> uint64_t { return cached ? s_pref[s + 1] : (f + s + 1 < m ? pref[f + s + 1] : n); };
+  auto seq_off = [&](uint32_t s) -> uint64_t { return cached ? s_off[s] : seqoff[f + s]; };
+  auto add_ones = [&](uint32_t s, uint32_t c) {
+    if (!c) return;
+    if (cached) atomicAdd(&s_ones[s], c); else atomicAdd(&ones[f + s], c);
+  };
   bool bad = false;
-  uint32_t s = 0;  // sequence of the current element, relative to f
-  uint32_t run = 0, run_s = 0xFFFFFFFFu;
-  for (int k = 0; k < 16; k++) {
-    const uint64_t p = b0 + (uint64_t)k * 256 + threadIdx.x;
-    if (p >= b1) break;
-    uint64_t ps, po;
-    if (cached) {
-      while (s + 1 < cnt && s_pref[s + 1] <= p) s++;
-      ps = s_pref[s];
-      po = s_off[s];
-    } else {
-      while (f + s + 1 < m && pref[f + s + 1] <= p) s++;
-      ps = pref[f + s];
-      po = seqoff[f + s];
+  const uint64_t p0 = b0 + (uint64_t)threadIdx.x * 16;
+  if (p0 < b1) {
+    // sequence of p0 (relative to f): last s with start <= p0
+    uint32_t s = 0, hi = cnt;
+    while (hi - s > 1) {
+      const uint32_t mid = (s + hi) / 2;
+      if (seq_start(mid) <= p0) s = mid; else hi = mid;
     }
-    const uint32_t x = sym[po + (p - ps)];
-    bad |= (x >> L) != 0;
-    keys[p] = (K)(((uint64_t)(f + s) << L) | x);
-    const uint32_t top = (x >> (L - 1)) & 1u;
-    if (run_s != s) {
-      if (run) {
-        if (cached) atomicAdd(&s_ones[run_s], run); else atomicAdd(&ones[f + run_s], run);
+    const uint64_t ps = seq_start(s), pe = seq_end(s);
+    const uint64_t src = seq_off(s) + (p0 - ps);
+    const uint32_t S = (f + s) << L;
+    const uintptr_t ab = ((uintptr_t)(sym + src)) & ~(uintptr_t)3;
+    if (fast && p0 + 16 <= pe && p0 + 16 <= b1 && (uint64_t)(ab - (uintptr_t)sym) + 20 <= sym_end) {
+      const uint32_t* q = reinterpret_cast<const uint32_t*>(ab);
+      const uint32_t sh = (uint32_t)(((uintptr_t)(sym + src)) & 3);
+      uint32_t d[5];
+#pragma unroll
+      for (int k = 0; k < 5; k++) d[k] = q[k];
+      uint32_t wv[4];
+#pragma unroll
+      for (int k = 0; k < 4; k++) wv[k] = __builtin_amdgcn_alignbyte(d[k + 1], d[k], sh);
+      const uint32_t hib = (0xFFu << L) & 0xFFu;  // symbol bits at or above L
+      const uint32_t badm = hib * 0x01010101u;
+      uint32_t c = 0;
+#pragma unroll
+      for (int k = 0; k < 4; k++) {
+        bad |= (wv[k] & badm) != 0;
+        c += (uint32_t)__popc((wv[k] >> (L - 1)) & 0x01010101u);
       }
-      run_s = s;
-      run = 0;
+      if constexpr (sizeof(K) == 2) {
+        const uint32_t SS = S | (S << 16);
+        uint4 o[2];
+        uint32_t* ov = reinterpret_cast<uint32_t*>(o);
+#pragma unroll
+        for (int k = 0; k < 4; k++) {
+          ov[2 * k] = SS | (wv[k] & 0xFFu) | ((wv[k] & 0xFF00u) << 8);
+          ov[2 * k + 1] = SS | ((wv[k] >> 16) & 0xFFu) | ((wv[k] >> 8) & 0xFF0000u);
+        }
+        uint4* kp = reinterpret_cast<uint4*>(keys + p0);
+        kp[0] = o[0];
+        kp[1] = o[1];
+      } else {
+        uint4* kp = reinterpret_cast<uint4*>(keys + p0);
+#pragma unroll
+        for (int k = 0; k < 4; k++) {
+          uint4 o;
+          o.x = S | (wv[k] & 0xFFu);
+          o.y = S | ((wv[k] >> 8) & 0xFFu);
+          o.z = S | ((wv[k] >> 16) & 0xFFu);
+          o.w = S | (wv[k] >> 24);
+          kp[k] = o;
+        }
+      }
+      add_ones(s, c);
+    } else {
+      uint32_t run = 0;
+      uint64_t pend = pe;
+      for (int k = 0; k < 16; k++) {
+        const uint64_t p = p0 + k;
+        if (p >= b1) break;
+        while (p >= pend) {  // next sequence (those of length 0 are skipped)
+          add_ones(s, run);
+          run = 0;
+          s++;
+          pend = seq_end(s);
+        }
+        const uint32_t x = sym[seq_off(s) + (p - seq_start(s))];
+        bad |= (x >> L) != 0;
+        keys[p] = (K)((((uint64_t)(f + s)) << L) | x);
+        run += (x >> (L - 1)) & 1u;
+      }
+      add_ones(s, run);
     }
-    run += top;
-  }
-  if (run) {
-    if (cached) atomicAdd(&s_ones[run_s], run); else atomicAdd(&ones[f + run_s], run);
   }
   if (bad) atomicOr(err, 2u);  // reported as an input symbol >= 2^levels
   if (cached) {
@@ -143,6 +197,8 @@ struct Group {
 
 struct wtg_batch {
   const uint8_t* d_sym = nullptr;
+  uint64_t sym_end = 0;         // max(off_i + n_i): bytes of d_sym the sequences span
+  bool sym_fast = false;        // d_sym is 4-byte aligned: aligned dword loads of the symbols
   uint32_t nseq = 0;
   std::vector<uint64_t> n, off;
   std::vector<uint32_t> L, grp, idx;
@@ -188,6 +244,7 @@ extern "C" int wtg_batch_create(wtg_ctx* ctx, const uint8_t* d_sym, uint32_t nse
   *out = nullptr;
   wtg_batch* b = new wtg_batch();
   b->d_sym = d_sym;
+  b->sym_fast = ((uintptr_t)d_sym & 3) == 0;
   b->nseq = nseq;
   b->n.assign(h_n, h_n + nseq);
   b->off.assign(h_off, h_off + nseq);
@@ -205,6 +262,7 @@ extern "C" int wtg_batch_create(wtg_ctx* ctx, const uint8_t* d_sym, uint32_t nse
     }
     b->L[i] = (uint32_t)std::max(1, log2up_u64(h_sigma[i]));
     total += h_n[i];
+    if (h_n[i]) b->sym_end = std::max(b->sym_end, h_off[i] + h_n[i]);
   }
   if (total && !d_sym) {
     wtg_set_error("wtg_batch_create: null symbols");
@@ -291,11 +349,11 @@ static int batch_enqueue(wtg_ctx* ctx, wtg_batch* b) {
     const uint32_t m = (uint32_t)G.seqs.size();
     WTG_TRY(wtg_fill_bytes_zero(ctx, G.d_ones, (1ull << G.sbits) * 4));
     if (G.key_bytes == 2)
-      WTG_LAUNCH(ctx, "batch_keys", k_batch_keys<uint16_t>, dim3(grid), dim3(256), 0, b->d_sym, G.d_seqoff,
-                 G.d_pref, m, G.n, G.L, (uint16_t*)G.d_keys, G.d_ones, ctx->d_flag);
+      WTG_LAUNCH(ctx, "batch_keys", k_batch_keys<uint16_t>, dim3(grid), dim3(256), 0, b->d_sym, b->sym_end,
+                 b->sym_fast, G.d_seqoff, G.d_pref, m, G.n, G.L, (uint16_t*)G.d_keys, G.d_ones, ctx->d_flag);
     else
-      WTG_LAUNCH(ctx, "batch_keys", k_batch_keys<uint32_t>, dim3(grid), dim3(256), 0, b->d_sym, G.d_seqoff,
-                 G.d_pref, m, G.n, G.L, (uint32_t*)G.d_keys, G.d_ones, ctx->d_flag);
+      WTG_LAUNCH(ctx, "batch_keys", k_batch_keys<uint32_t>, dim3(grid), dim3(256), 0, b->d_sym, b->sym_end,
+                 b->sym_fast, G.d_seqoff, G.d_pref, m, G.n, G.L, (uint32_t*)G.d_keys, G.d_ones, ctx->d_flag);
     WTG_LAUNCH_CHECK();
     WTG_TRY(wtg_levels_build_from(ctx, G.d_keys, G.key_bytes, G.n, G.sbits + G.L, G.sbits,
                                   G.sbits ? G.d_cnt : nullptr, b->words + G.word_base,
