@@ -15,9 +15,11 @@
 //   3. k_huff_emit   each wave walks its segment in 64-byte chunks, level by
 //                    level; per (chunk, level) the lanes are grouped by node with
 //                    ballots, each group's bits compacted into one word (lane
-//                    scatter with ds_permute + ballot) and appended to the node's
-//                    run in a per-wave LDS staging area; at the segment end every
-//                    run is written out as whole words (edge words OR-ed).
+//                    scatter with ds_permute + ballot) and OR-ed onto the node's
+//                    run in a per-wave LDS staging area (run starts and lengths
+//                    in per-wave LDS arrays, read as wave-uniform values); at the
+//                    segment end every run is written out as whole words (edge
+//                    words OR-ed).
 // Then rank9 + select1 + select0 over the bits, and the per-node rank bases.
 #include "wtg_internal.hpp"
 #include <cstring>
@@ -106,22 +108,6 @@ __global__ __launch_bounds__(HF_THREADS) void k_huff_hist(const uint8_t* __restr
   }
 }
 
-// per-node values held across the wave: lane j keeps node j + 64q in a[q]
-struct NodeRegs {
-  uint32_t a[4];
-  __device__ __forceinline__ uint32_t get(uint32_t v) const {
-    const uint32_t q = v >> 6;
-    const uint32_t x = q == 0 ? a[0] : q == 1 ? a[1] : q == 2 ? a[2] : a[3];
-    return (uint32_t)__builtin_amdgcn_readlane((int)x, (int)(v & 63));
-  }
-  __device__ __forceinline__ void add(uint32_t v, uint32_t d, int lane) {
-    if ((uint32_t)lane == (v & 63)) {
-      const uint32_t q = v >> 6;
-      if (q == 0) a[0] += d; else if (q == 1) a[1] += d; else if (q == 2) a[2] += d; else a[3] += d;
-    }
-  }
-};
-
 struct HuffLds {
   uint64_t code[256];
   uint16_t child[256][2];
@@ -185,12 +171,16 @@ __device__ __forceinline__ void load_piece(uint8_t* s_txt, const uint8_t* __rest
 // pass 2: cnt[v][s]
 __global__ __launch_bounds__(HF_THREADS) void k_huff_count(SegArgs a) {
   __shared__ HuffLds L;
+  __shared__ uint32_t s_cnt[HF_THREADS / 64][256];  // per wave: count of each inner node
   load_tab(L, a.tab);
   __syncthreads();
-  const int lane = lane_id();
+  const int lane = lane_id(), w = threadIdx.x >> 6;
+  uint32_t* cnt = s_cnt[w];
   const uint64_t waves = (uint64_t)gridDim.x * (HF_THREADS / 64);
-  for (uint64_t s = (uint64_t)blockIdx.x * (HF_THREADS / 64) + (threadIdx.x >> 6); s < a.nseg; s += waves) {
-    NodeRegs cnt{{0, 0, 0, 0}};
+  for (uint64_t s = (uint64_t)blockIdx.x * (HF_THREADS / 64) + w; s < a.nseg; s += waves) {
+#pragma unroll
+    for (int q = 0; q < 4; q++) cnt[64 * q + lane] = 0;
+    wave_sync_lds();
     for (uint32_t off = 0; off < a.S; off += HF_PIECE) {
       const uint64_t base = s * a.S + off + (uint64_t)lane * 16;
       if (base - (uint64_t)lane * 16 >= a.n) break;
@@ -202,15 +192,17 @@ __global__ __launch_bounds__(HF_THREADS) void k_huff_count(SegArgs a) {
         const uint32_t word = q == 0 ? wv.x : q == 1 ? wv.y : q == 2 ? wv.z : wv.w;
         const uint32_t c = (word >> (8 * (k & 3))) & 0xFFu;
         walk_chunk(L, c, base + k < a.n, [&](uint32_t v, uint64_t m, uint64_t, uint32_t) {
-          cnt.add(v, popc64(m), lane);
+          if (lane == 0) cnt[v] += popc64(m);
         });
       }
     }
+    wave_sync_lds();
 #pragma unroll
     for (int q = 0; q < 4; q++) {
       const uint32_t v = (uint32_t)(64 * q + lane);
-      if (v < a.n_inner) a.cnt[(uint64_t)v * a.nseg + s] = cnt.a[q];
+      if (v < a.n_inner) a.cnt[(uint64_t)v * a.nseg + s] = cnt[v];
     }
+    wave_sync_lds();
   }
 }
 
@@ -225,17 +217,20 @@ __device__ __forceinline__ uint64_t stage_bits(const uint64_t* stg, int64_t d) {
 // pass 3: tree bits
 __global__ __launch_bounds__(HF_THREADS) void k_huff_emit(SegArgs a) {
   __shared__ HuffLds L;
+  __shared__ uint32_t s_ls[HF_THREADS / 64][256];   // per wave: node run start in the staging area
+  __shared__ uint32_t s_run[HF_THREADS / 64][256];  // per wave: bits appended to the node's run
   extern __shared__ uint64_t s_dyn[];
   load_tab(L, a.tab);
   __syncthreads();
   const int lane = lane_id(), w = threadIdx.x >> 6;
   uint64_t* stg = s_dyn + (size_t)w * (a.stage_words + HF_PIECE / 8);
   uint8_t* s_txt = reinterpret_cast<uint8_t*>(stg + a.stage_words);
+  uint32_t* ls = s_ls[w];
+  uint32_t* rn = s_run[w];
   const uint64_t waves = (uint64_t)gridDim.x * (HF_THREADS / 64);
   for (uint64_t s = (uint64_t)blockIdx.x * (HF_THREADS / 64) + w; s < a.nseg; s += waves) {
     wave_sync_lds();
     // local run starts: exclusive scan of this segment's node counts
-    NodeRegs lstart{{0, 0, 0, 0}}, run{{0, 0, 0, 0}};
     uint64_t gpos[4];  // each run's global bit position, loaded up front (lane j: node j+64q)
 #pragma unroll
     for (int q = 0; q < 4; q++) {
@@ -248,7 +243,8 @@ __global__ __launch_bounds__(HF_THREADS) void k_huff_emit(SegArgs a) {
       const uint32_t v = (uint32_t)(64 * q + lane);
       const uint32_t c = v < a.n_inner ? a.cnt[(uint64_t)v * a.nseg + s] : 0u;
       const uint32_t inc = wave_incl_scan(c);
-      lstart.a[q] = carry + inc - c;
+      ls[v] = carry + inc - c;
+      rn[v] = 0;
       carry += (uint32_t)__shfl(inc, 63);
     }
     const uint32_t used = (carry + 63) / 64 + 2;
@@ -275,12 +271,14 @@ __global__ __launch_bounds__(HF_THREADS) void k_huff_emit(SegArgs a) {
             const int got = __builtin_amdgcn_ds_permute((int)(dst * 4), (int)bit);
             W = wave_ballot((uint32_t)lane < cm && got);
           }
-          const uint32_t at = lstart.get(v) + run.get(v);
-          run.add(v, cm, lane);
+          // the node's run so far (wave-uniform LDS reads); lane 0 appends
+          const uint32_t at = (uint32_t)__builtin_amdgcn_readfirstlane((int)(ls[v] + rn[v]));
           if (lane == 0) {
+            rn[v] += cm;
             const uint32_t wi = at >> 6, sh = at & 63;
-            stg[wi] |= W << sh;
-            if (sh && sh + cm > 64) stg[wi + 1] |= W >> (64 - sh);
+            atomicOr(reinterpret_cast<unsigned long long*>(&stg[wi]), (unsigned long long)(W << sh));
+            if (sh && sh + cm > 64)
+              atomicOr(reinterpret_cast<unsigned long long*>(&stg[wi + 1]), (unsigned long long)(W >> (64 - sh)));
           }
         });
       }
@@ -288,20 +286,20 @@ __global__ __launch_bounds__(HF_THREADS) void k_huff_emit(SegArgs a) {
     wave_sync_lds();
     // write every run of the segment as whole words
     for (uint32_t v = 0; v < a.n_inner; v++) {
-      const uint32_t len = run.get(v);
+      const uint32_t len = (uint32_t)__builtin_amdgcn_readfirstlane((int)rn[v]);
       if (!len) continue;
       const uint32_t qv = v >> 6;
       const uint64_t gq = qv == 0 ? gpos[0] : qv == 1 ? gpos[1] : qv == 2 ? gpos[2] : gpos[3];
       const uint64_t g0 = ((uint64_t)(uint32_t)__builtin_amdgcn_readlane((int)(uint32_t)(gq >> 32), (int)(v & 63)) << 32) |
                           (uint32_t)__builtin_amdgcn_readlane((int)(uint32_t)gq, (int)(v & 63));
-      const int64_t ls = (int64_t)lstart.get(v);
+      const int64_t lsv = (int64_t)(uint32_t)__builtin_amdgcn_readfirstlane((int)ls[v]);
       const uint64_t w0 = g0 >> 6, w1 = (g0 + len - 1) >> 6;
       for (uint64_t kw = w0 + lane; kw <= w1; kw += 64) {
         const uint64_t lo = kw * 64 > g0 ? kw * 64 : g0;
         const uint64_t hi = kw * 64 + 64 < g0 + len ? kw * 64 + 64 : g0 + len;
         const uint32_t nb = (uint32_t)(hi - lo), at = (uint32_t)(lo - kw * 64);
         const uint64_t mask = nb == 64 ? ~0ull : ((1ull << nb) - 1) << at;
-        const uint64_t val = stage_bits(stg, ls + (int64_t)(kw * 64) - (int64_t)g0) & mask;
+        const uint64_t val = stage_bits(stg, lsv + (int64_t)(kw * 64) - (int64_t)g0) & mask;
         if (nb == 64) a.words[kw] = val;
         else if (val) atomicOr(reinterpret_cast<unsigned long long*>(&a.words[kw]), (unsigned long long)val);
       }
