@@ -102,6 +102,24 @@ def test_bad_symbol_reports_error(ctx, wsfm):
     ctx.sync()
 
 
+@pytest.mark.parametrize("dt,L", [(np.uint32, 8), (np.uint8, 5), (np.uint16, 11)])
+def test_bad_symbol_in_full_tiles_reports_error(ctx, wsfm, dt, L):
+    # invalid keys inside full tiles (the full-tile split kernel): the build
+    # reports the error and every write stays inside the buffers; the context
+    # stays usable
+    rng = np.random.default_rng(L)
+    n = 100003
+    s = rng.integers(0, 1 << L, n).astype(dt)
+    s[[0, 50000, 77777]] = (1 << L) + 3
+    say("bad symbols in full tiles", np.dtype(dt).name, n, L)
+    with pytest.raises(wsfm.WtgError):
+        ctx.levels(s, L)
+    ctx.sync()
+    good = rng.integers(0, 1 << L, 5000).astype(dt)
+    w, _ = ctx.levels(good, L, with_nodes=False)
+    assert w.size == wsfm.lib().wtg_levels_words(good.size, L)
+
+
 def test_levels_large_bit_exact(ctx, wsfm, orc):
     """n = 2^24 uint32, L = 20 (the wt_int shape at 1/16 of its full size):
     bit-exact against the oracle, plus per-level one counts."""
