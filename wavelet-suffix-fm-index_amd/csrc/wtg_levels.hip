@@ -259,6 +259,84 @@ __device__ __forceinline__ uint32_t rl32(uint32_t v, int l) { return (uint32_t)_
 __device__ __forceinline__ uint64_t rl64(uint64_t v, int l) {
   return ((uint64_t)rl32((uint32_t)(v >> 32), l) << 32) | rl32((uint32_t)v, l);
 }
+// Groups of a split tile, one thread per group (tiles holding many small
+// groups): the same outputs as the wave-per-group path below.  s_hpre[i] /
+// s_npre[i] = group starts / next-level ones in slot-order words [0, i).
+#define SPLIT_GROUPS_WAVE 32
+template <typename T, bool COUNT, int TILE>
+__device__ __forceinline__ void split_groups_many(const LevelArgs& a, uint32_t tile, uint32_t valid, uint32_t zt,
+                                                  uint64_t zprev, uint64_t oprev, bool cached, uint64_t pfirst,
+                                                  const T* s_stage, const uint64_t* s_nb, const uint64_t* s_head,
+                                                  const uint2* s_tab, const uint32_t* s_hpre,
+                                                  const uint32_t* s_npre, uint32_t ng) {
+  constexpr int NW = TILE / 64;
+  uint32_t* c1 = COUNT ? a.c1 + (uint64_t)(tile & (a.reps - 1)) * (2ull * a.nnodes) : nullptr;
+  auto start_of = [&](uint32_t gi) -> uint32_t {  // slot of the gi-th group start
+    uint32_t lo = 0, hi = NW;                      // last word with s_hpre <= gi
+    while (hi - lo > 1) {
+      const uint32_t mid = (lo + hi) / 2;
+      if (s_hpre[mid] <= gi) lo = mid; else hi = mid;
+    }
+    return lo * 64 + select_in_word(s_head[lo], gi - s_hpre[lo]);
+  };
+  auto ones_before = [&](uint32_t x) -> uint32_t {  // next-level ones in slots [0, x)
+    const uint32_t wi = x >> 6;
+    if (wi >= (uint32_t)NW) return s_npre[NW];
+    return s_npre[wi] + popc64(s_nb[wi] & ((1ull << (x & 63)) - 1));
+  };
+  bool bad = false;
+  for (uint32_t gi = threadIdx.x; gi < ng; gi += blockDim.x) {
+    const uint32_t sG = start_of(gi);
+    const uint32_t eG = gi + 1 < ng ? start_of(gi + 1) : valid;
+    const uint32_t len = eG - sG;
+    const uint64_t x0 = (uint64_t)s_stage[sG];
+    const uint64_t pnode = x0 >> a.node_shift;
+    const uint64_t g = x0 >> a.bitpos;
+    uint2 ab;
+    if (cached && pnode - pfirst < TAB_CACHE) ab = s_tab[pnode - pfirst];
+    else if (pnode < a.nnodes) ab = a.tab[pnode];
+    else { bad = true; continue; }
+    const uint64_t dG = (sG < zt) ? zprev + sG + ab.x : oprev + (sG - zt) + ab.y;
+    if (dG + len > a.n || len == 0) { bad = true; continue; }
+    const uint64_t b0 = (dG / TILE + 1) * TILE;  // first destination-tile boundary
+    const uint32_t rel0 = (dG + len > b0) ? (uint32_t)(b0 - dG) : len;
+    const uint64_t GB0 = a.next_base + dG;
+    const uint64_t k0 = GB0 >> 6, k1 = (GB0 + len - 1) >> 6;
+    const int32_t off0 = (int32_t)(k0 * 64 - GB0);  // in (-64, 0]
+    for (uint64_t k = k0; k <= k1; k++) {
+      const int32_t off = off0 + (int32_t)((k - k0) * 64);
+      const int32_t P = (int32_t)sG + off;
+      uint64_t val;
+      if (P >= 0) {
+        const uint32_t w0 = (uint32_t)P >> 6, sh = (uint32_t)P & 63;
+        val = s_nb[w0] >> sh;
+        if (sh) val |= s_nb[w0 + 1] << (64 - sh);
+      } else {
+        val = s_nb[0] << (uint32_t)(-P);
+      }
+      const int32_t lo = off < 0 ? -off : 0;
+      const int32_t hi = ((int32_t)len - off < 64) ? (int32_t)len - off : 64;
+      const uint64_t msk = (hi >= 64 ? ~0ull : ((1ull << hi) - 1)) & (~0ull << lo);
+      val &= msk;
+      if (lo == 0 && hi == 64) a.words[k] = val;
+      else if (val) atomicOr((unsigned long long*)&a.words[k], (unsigned long long)val);
+    }
+    const uint32_t os = ones_before(sG);
+    const uint32_t ones = ones_before(eG) - os;
+    const uint32_t o_lo = ones_before(sG + rel0) - os;
+    const uint32_t o_hi = ones - o_lo;
+    if (COUNT && ones && g < 2ull * a.nnodes) atomicAdd(&c1[g], ones);
+    const uint64_t D0 = dG / TILE;
+    const uint32_t z_lo = rel0 - o_lo;
+    if (z_lo) atomicAdd(&a.zc_next[D0], z_lo);
+    if (len > rel0) {
+      const uint32_t z_hi = (len - rel0) - o_hi;
+      if (z_hi) atomicAdd(&a.zc_next[D0 + 1], z_hi);
+    }
+  }
+  if (bad) atomicOr(a.err, FLAG_BADSYM);
+}
+
 // Group phase of a split tile: per group of the next level (contiguous slots
 // [sG, eG) -> contiguous destinations [dG, dG+len)), one wave per group, one
 // output word per lane: level l+1's bit words (interior words stored, the two
@@ -290,6 +368,33 @@ __device__ __forceinline__ void split_groups(const LevelArgs& a, uint32_t tile, 
   const uint32_t oincl = wave_incl_scan(oc), oex = oincl - oc;
   const uint32_t ng = rl32(hincl, 63);
   const uint32_t otot = rl32(oincl, 63);
+  if (ng > SPLIT_GROUPS_WAVE) {
+    // many small groups (sparse or skewed inputs): one thread per group, with
+    // the head / next-bit word prefixes shared through LDS (ng is the same in
+    // every wave: all read the same slot-order arrays)
+    __shared__ uint32_t s_hpre[NW + 1], s_npre[NW + 1];
+    if (w == 0) {
+      uint32_t hh = hex, oo = oex;
+#pragma unroll
+      for (int j = 0; j < HPL; j++) {
+        const int idx = lane * HPL + j;
+        if (idx < NW) {
+          s_hpre[idx] = hh;
+          s_npre[idx] = oo;
+        }
+        hh += popc64(hwl[j]);
+        oo += popc64(nwl[j]);
+      }
+      if (lane == 0) {
+        s_hpre[NW] = ng;
+        s_npre[NW] = otot;
+      }
+    }
+    __syncthreads();
+    split_groups_many<T, COUNT, TILE>(a, tile, valid, zt, zprev, oprev, cached, pfirst, s_stage, s_nb, s_head, s_tab,
+                                      s_hpre, s_npre, ng);
+    return;
+  }
   // next-level ones among slots [0, x) (x uniform, <= valid)
   auto ones_before = [&](uint32_t x) -> uint32_t {
     const uint32_t wi = x >> 6;
