@@ -192,9 +192,13 @@ def bench_batch(pkg, args):
     off = np.concatenate([[0], np.cumsum(n)[:-1]]).astype(np.uint64)
     ctx = pkg.Context(0)
     d = ctx.to_device(sym)
-    B = pkg.Batch(ctx, d, n, sigma, off)
+    # the autotuner picks each alphabet width's group size on these sequences
+    t0 = time.perf_counter()
+    plan, tune_ms = pkg.Batch.tune(ctx, d, n, sigma, off, reps=2)
+    tune_s = time.perf_counter() - t0
+    B = pkg.Batch(ctx, d, n, sigma, off, plan=plan)
     L = np.array([pkg.levels_of_sigma(int(x)) for x in sigma])
-    log("batch: %d symbols" % total)
+    log("batch: %d symbols, plan %s" % (total, list(plan)))
     B.run()
     ctx.set_blocking(False)
     for _ in range(args.warmup):
@@ -242,7 +246,12 @@ def bench_batch(pkg, args):
            "build_roofline": {"B_a_bytes": B_a, "frac": round(B_a / (ms / 1e3) / 1e9 / HBM_PEAK_GBS, 4)},
            "per_sequence_launches": {"ms": round(ms_seq, 3), "wall_ms": round(wall_seq * 1e3, 3),
                                      "Gsym_s": round(total / (ms_seq / 1e3) / 1e9, 4)},
-           "speedup_vs_per_sequence": round(ms_seq / ms, 2), "parity_vs_per_sequence": ok}
+           "speedup_vs_per_sequence": round(ms_seq / ms, 2), "parity_vs_per_sequence": ok,
+           "autotune": {"cap_log_by_width": {int(w): int(plan[w]) for w in sorted(set(L.tolist()))},
+                        "candidates_ms": {int(w): [round(float(x), 3) for x in tune_ms[w]]
+                                          for w in sorted(set(L.tolist()))},
+                        "tune_s": round(tune_s, 3),
+                        "default_16bit_keys": {int(w): 16 - int(w) for w in sorted(set(L.tolist()))}}}
     if not args.no_cpu:
         from oracle import oracle
         ref = oracle.RefWT("levelWT", "u8")

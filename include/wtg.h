@@ -233,12 +233,29 @@ int wtg_fm_count_batch(wtg_ctx* ctx, const wtg_huff* bwt_wt, const uint8_t* d_pa
 /* Many WT(s_i, n_i, a_i) builds (wavelet/levelWT/WT.h:43, levels
  * L_i = max(1, log2Up(a_i)), a_i <= 256, byte symbols) in one go.  Sequence i is
  * d_sym[h_off[i], h_off[i] + n_i).  Sequences are grouped by L_i, at most
- * 2^(16-L_i) per group; a group holds at most 2^32 - 1 symbols.  Output (device, owned by
- * the batch): level l of sequence i is the bit range
- * [first_bit + l*level_stride, ... + n_i) of the batch words (wtg_batch_seq). */
+ * 2^cap_log[L_i] per group (default 2^(16-L_i): 16-bit keys); a group holds at most
+ * 2^32 - 1 symbols.  Output (device, owned by the batch): level l of sequence i is the
+ * bit range [first_bit + l*level_stride, ... + n_i) of the batch words (wtg_batch_seq). */
 typedef struct wtg_batch wtg_batch;
+/* Group sizes: cap_log[L] = log2 of the most sequences in one group of alphabet
+ * width L (1..8; cap_log + L <= 24; keys are 16-bit when the group's sequence bits
+ * + L <= 16, else 32-bit); 0xFF = the default 16 - L.  cap_log[0] is unused. */
+typedef struct {
+  uint8_t cap_log[9];
+} wtg_batch_plan;
 int wtg_batch_create(wtg_ctx* ctx, const uint8_t* d_sym, uint32_t nseq, const uint64_t* h_off,
                      const uint64_t* h_n, const uint32_t* h_sigma, wtg_batch** out);
+int wtg_batch_create_planned(wtg_ctx* ctx, const uint8_t* d_sym, uint32_t nseq, const uint64_t* h_off,
+                             const uint64_t* h_n, const uint32_t* h_sigma, const wtg_batch_plan* plan,
+                             wtg_batch** out);
+/* Autotuner: per alphabet width present, builds the batch's own sequences of that
+ * width with each candidate group size (2^(14-L), 2^(16-L): 16-bit keys; 2^(20-L),
+ * 2^(24-L): 32-bit keys), times the best of `reps` graph replays and returns the
+ * fastest plan; ms_out (nullable, 36 floats) gets the time of candidate c of width L
+ * at [L*4 + c] (-1: not tried). */
+int wtg_batch_tune(wtg_ctx* ctx, const uint8_t* d_sym, uint32_t nseq, const uint64_t* h_off,
+                   const uint64_t* h_n, const uint32_t* h_sigma, uint32_t reps, wtg_batch_plan* out,
+                   float* ms_out);
 int wtg_batch_destroy(wtg_ctx* ctx, wtg_batch* b);
 /* 1 (default): the first run records every kernel of the batch as one hipGraph,
  * later runs replay it with a single launch; 0: plain stream launches */

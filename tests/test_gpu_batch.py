@@ -87,3 +87,38 @@ def test_batch_permuted_offsets_and_bad_symbol(wsfm, orc, ctx):
         B.free()
         d.free()
         d2.free()
+
+
+def test_batch_plans_and_autotuner(wsfm, orc, ctx):
+    # the same sequences under other group sizes (32-bit keys in one group per
+    # width, smaller 16-bit-key groups) and under the autotuner's plan: every
+    # sequence's levels are unchanged
+    rng = np.random.default_rng(4)
+    nseq = 120
+    n = np.floor(2.0 ** rng.uniform(4, 13, nseq)).astype(np.uint64)
+    sigma = rng.choice([4, 20, 256, 3], nseq).astype(np.uint32)
+    sym = np.concatenate([rng.integers(0, s, int(k)).astype(np.uint8) for s, k in zip(sigma, n)])
+    off = np.concatenate([[0], np.cumsum(n)[:-1]]).astype(np.uint64)
+    d = ctx.to_device(sym)
+    ref = wsfm.Batch(ctx, d, n, sigma, off)
+    ref.run()
+    want = [ref.extract(i) for i in range(nseq)]
+    ref.free()
+    wide = np.full(9, 255, np.uint8)
+    small = np.full(9, 255, np.uint8)
+    for L in range(1, 9):
+        wide[L] = 24 - L
+        small[L] = max(0, 14 - L)
+    plan, ms = wsfm.Batch.tune(ctx, d, n, sigma, off, reps=1)
+    widths = sorted({orc.levels_of_sigma(int(s)) for s in sigma})
+    for L in widths:
+        assert plan[L] != 255 and (ms[L] >= 0).any()   # tried and chosen
+    for p in (wide, small, plan):
+        B = wsfm.Batch(ctx, d, n, sigma, off, plan=p)
+        try:
+            B.run()
+            for i in range(nseq):
+                assert np.array_equal(B.extract(i), want[i]), (list(p), i)
+        finally:
+            B.free()
+    d.free()

@@ -132,6 +132,8 @@ SIGNATURES = {
     "wtg_gen_select_vectors": (i32, [u64, u32, u64, vp, vp]),
     "wtg_gen_batch": (i32, [u64, u32, C.c_double, C.c_double, vp, u32, vp, vp, vp]),
     "wtg_batch_create": (i32, [vp, vp, u32, vp, vp, vp, C.POINTER(vp)]),
+    "wtg_batch_create_planned": (i32, [vp, vp, u32, vp, vp, vp, vp, C.POINTER(vp)]),
+    "wtg_batch_tune": (i32, [vp, vp, u32, vp, vp, vp, u32, vp, vp]),
     "wtg_batch_destroy": (i32, [vp, vp]),
     "wtg_batch_set_graph": (i32, [vp, i32]),
     "wtg_batch_run": (i32, [vp, vp]),
@@ -544,17 +546,37 @@ def lsel_loss_grad(ctx, J, y, stride, alpha, beta, pos):
 class Batch:
     """Batched level-wise trees of byte sequences packed in one device array."""
 
-    def __init__(self, ctx, d_sym, n, sigma, off=None):
+    def __init__(self, ctx, d_sym, n, sigma, off=None, plan=None):
+        """plan: None (default groups) or 9 cap_log values (see tune())."""
         self.ctx = ctx
+        n, sigma, off = Batch._arrays(n, sigma, off)
+        self.n, self.sigma, self.off = n, sigma, off
+        h = C.c_void_p()
+        if plan is None:
+            _check(lib().wtg_batch_create(ctx.h, d_sym.ptr, n.size, _p(off), _p(n), _p(sigma), C.byref(h)))
+        else:
+            p = np.ascontiguousarray(plan, np.uint8)
+            _check(lib().wtg_batch_create_planned(ctx.h, d_sym.ptr, n.size, _p(off), _p(n), _p(sigma), _p(p),
+                                                  C.byref(h)))
+        self.h = h
+
+    @staticmethod
+    def _arrays(n, sigma, off):
         n = np.ascontiguousarray(n, np.uint64)
         sigma = np.ascontiguousarray(sigma, np.uint32)
         if off is None:
             off = np.concatenate([[0], np.cumsum(n)[:-1]]).astype(np.uint64)
-        off = np.ascontiguousarray(off, np.uint64)
-        self.n, self.sigma, self.off = n, sigma, off
-        h = C.c_void_p()
-        _check(lib().wtg_batch_create(ctx.h, d_sym.ptr, n.size, _p(off), _p(n), _p(sigma), C.byref(h)))
-        self.h = h
+        return n, sigma, np.ascontiguousarray(off, np.uint64)
+
+    @staticmethod
+    def tune(ctx, d_sym, n, sigma, off=None, reps=2):
+        """The autotuner's plan (9 cap_log values, 255 = default) and its timings
+        (ms[L][c] of candidate c for alphabet width L, -1 = not tried)."""
+        n, sigma, off = Batch._arrays(n, sigma, off)
+        plan = np.zeros(9, np.uint8)
+        ms = np.zeros(36, np.float32)
+        _check(lib().wtg_batch_tune(ctx.h, d_sym.ptr, n.size, _p(off), _p(n), _p(sigma), reps, _p(plan), _p(ms)))
+        return plan, ms.reshape(9, 4)
 
     def set_graph(self, on):
         _check(lib().wtg_batch_set_graph(self.h, 1 if on else 0))

@@ -237,9 +237,18 @@ extern "C" int wtg_batch_destroy(wtg_ctx* ctx, wtg_batch* b) {
 
 static int log2up_u64(uint64_t x) { return wtg_log2up_host(x); }
 
-extern "C" int wtg_batch_create(wtg_ctx* ctx, const uint8_t* d_sym, uint32_t nseq, const uint64_t* h_off,
-                                const uint64_t* h_n, const uint32_t* h_sigma, wtg_batch** out) {
+// default group size: 16-bit keys (at most 2^(16-L) sequences per group)
+static uint32_t default_cap_log(uint32_t L) { return 16 - L; }
+
+extern "C" int wtg_batch_create_planned(wtg_ctx* ctx, const uint8_t* d_sym, uint32_t nseq, const uint64_t* h_off,
+                                        const uint64_t* h_n, const uint32_t* h_sigma, const wtg_batch_plan* bplan,
+                                        wtg_batch** out) {
   WTG_REQUIRE(ctx && out && nseq >= 1 && h_off && h_n && h_sigma, "wtg_batch_create: null argument");
+  uint32_t caplog[9];
+  for (uint32_t L = 1; L <= 8; L++) {
+    caplog[L] = bplan && bplan->cap_log[L] != 0xFF ? bplan->cap_log[L] : default_cap_log(L);
+    WTG_REQUIRE(caplog[L] + L <= 24, "wtg_batch_create: group size 2^cap_log needs cap_log + L <= 24");
+  }
   WTG_TRY(wtg_set_device(ctx));
   *out = nullptr;
   wtg_batch* b = new wtg_batch();
@@ -269,14 +278,14 @@ extern "C" int wtg_batch_create(wtg_ctx* ctx, const uint8_t* d_sym, uint32_t nse
     batch_release(ctx, b);
     return WTG_ERR_ARG;
   }
-  // groups by L, sequences in batch order inside a group, at most 2^(16-L)
-  // sequences per group so that every key fits 16 bits
+  // groups by L, sequences in batch order inside a group, at most 2^cap_log[L]
+  // sequences per group (default 2^(16-L): every key fits 16 bits)
   std::vector<Group> plan;
   for (uint32_t L = 1; L <= 8; L++) {
     std::vector<uint32_t> all;
     for (uint32_t i = 0; i < nseq; i++)
       if (b->L[i] == L) all.push_back(i);
-    const size_t cap = (size_t)1 << (16 - L);
+    const size_t cap = (size_t)1 << caplog[L];
     for (size_t a0 = 0; a0 < all.size(); a0 += cap) {
       Group G;
       G.L = L;
@@ -435,4 +444,63 @@ extern "C" int wtg_batch_extract(wtg_ctx* ctx, const wtg_batch* b, uint32_t seq,
              b->words + G.word_base, b->pref[seq], G.n, ni, L, d_out);
   WTG_LAUNCH_CHECK();
   return wtg_finish(ctx);
+}
+
+extern "C" int wtg_batch_create(wtg_ctx* ctx, const uint8_t* d_sym, uint32_t nseq, const uint64_t* h_off,
+                                const uint64_t* h_n, const uint32_t* h_sigma, wtg_batch** out) {
+  return wtg_batch_create_planned(ctx, d_sym, nseq, h_off, h_n, h_sigma, nullptr, out);
+}
+
+// Autotuner: for each alphabet width L present, time every candidate group
+// size on the batch's own sequences of that width (one recorded run, then the
+// best of `reps` graph replays) and keep the fastest.  Candidates: 2^(16-L-2)
+// and 2^(16-L) sequences (16-bit keys), 2^(20-L) and 2^(24-L) (32-bit keys;
+// one group holds every sequence of the width when they fit).
+extern "C" int wtg_batch_tune(wtg_ctx* ctx, const uint8_t* d_sym, uint32_t nseq, const uint64_t* h_off,
+                              const uint64_t* h_n, const uint32_t* h_sigma, uint32_t reps, wtg_batch_plan* out,
+                              float* ms_out) {
+  WTG_REQUIRE(ctx && out && nseq >= 1 && h_off && h_n && h_sigma && reps >= 1, "wtg_batch_tune: bad argument");
+  std::memset(out->cap_log, 0xFF, sizeof(out->cap_log));
+  if (ms_out)
+    for (int k = 0; k < 9 * 4; k++) ms_out[k] = -1.0f;
+  for (uint32_t L = 1; L <= 8; L++) {
+    std::vector<uint64_t> off, n;
+    std::vector<uint32_t> sig;
+    for (uint32_t i = 0; i < nseq; i++) {
+      WTG_REQUIRE(h_sigma[i] >= 1 && h_sigma[i] <= 256, "wtg_batch_tune: alphabet size must be in [1, 256]");
+      if ((uint32_t)std::max(1, log2up_u64(h_sigma[i])) != L) continue;
+      off.push_back(h_off[i]);
+      n.push_back(h_n[i]);
+      sig.push_back(h_sigma[i]);
+    }
+    if (off.empty()) continue;
+    const int cand[4] = {(int)(16 - L) - 2, (int)(16 - L), (int)(20 - L), (int)(24 - L)};
+    float best = 0.f;
+    for (int c = 0; c < 4; c++) {
+      if (cand[c] < 0) continue;
+      // a candidate no coarser than needed for the width's sequence count
+      const uint64_t m = off.size();
+      if (c > 0 && ((uint64_t)1 << cand[c - 1]) >= m && cand[c - 1] >= 0) continue;
+      wtg_batch_plan p;
+      std::memset(p.cap_log, 0xFF, sizeof(p.cap_log));
+      p.cap_log[L] = (uint8_t)cand[c];
+      wtg_batch* b = nullptr;
+      WTG_TRY(wtg_batch_create_planned(ctx, d_sym, (uint32_t)m, off.data(), n.data(), sig.data(), &p, &b));
+      int rc = wtg_batch_run(ctx, b);  // records the graph
+      float ms = 0.f, bestc = 1e30f;
+      for (uint32_t r = 0; r < reps && rc == WTG_OK; r++) {
+        if ((rc = wtg_timer_start(ctx)) == WTG_OK && (rc = wtg_batch_run(ctx, b)) == WTG_OK &&
+            (rc = wtg_timer_stop(ctx, &ms)) == WTG_OK)
+          bestc = std::min(bestc, ms);
+      }
+      wtg_batch_destroy(ctx, b);
+      if (rc != WTG_OK) return rc;
+      if (ms_out) ms_out[L * 4 + c] = bestc;
+      if (out->cap_log[L] == 0xFF || bestc < best) {
+        best = bestc;
+        out->cap_log[L] = (uint8_t)cand[c];
+      }
+    }
+  }
+  return WTG_OK;
 }
