@@ -238,8 +238,9 @@ int wtg_fm_count_batch(wtg_ctx* ctx, const wtg_huff* bwt_wt, const uint8_t* d_pa
  * bit range [first_bit + l*level_stride, ... + n_i) of the batch words (wtg_batch_seq). */
 typedef struct wtg_batch wtg_batch;
 /* Group sizes: cap_log[L] = log2 of the most sequences in one group of alphabet
- * width L (1..8; cap_log + L <= 24; keys are 16-bit when the group's sequence bits
- * + L <= 16, else 32-bit); 0xFF = the default 16 - L.  cap_log[0] is unused. */
+ * width L (1..8; cap_log + L <= 24; keys are 8-bit when the group's sequence bits
+ * + L <= 8, 16-bit when <= 16, else 32-bit); 0xFF = the default 16 - L.  cap_log[0]
+ * is unused. */
 typedef struct {
   uint8_t cap_log[9];
 } wtg_batch_plan;
@@ -249,10 +250,10 @@ int wtg_batch_create_planned(wtg_ctx* ctx, const uint8_t* d_sym, uint32_t nseq, 
                              const uint64_t* h_n, const uint32_t* h_sigma, const wtg_batch_plan* plan,
                              wtg_batch** out);
 /* Autotuner: per alphabet width present, builds the batch's own sequences of that
- * width with each candidate group size (2^(14-L), 2^(16-L): 16-bit keys; 2^(20-L),
- * 2^(24-L): 32-bit keys), times the best of `reps` graph replays and returns the
- * fastest plan; ms_out (nullable, 36 floats) gets the time of candidate c of width L
- * at [L*4 + c] (-1: not tried). */
+ * width with each candidate group size (2^(8-L): 8-bit keys; 2^(14-L), 2^(16-L):
+ * 16-bit keys; 2^(20-L), 2^(24-L): 32-bit keys; at most 64 groups), times the best of
+ * `reps` graph replays and returns the fastest plan; ms_out (nullable, 45 floats) gets
+ * the time of candidate c of width L at [L*5 + c] (-1: not tried). */
 int wtg_batch_tune(wtg_ctx* ctx, const uint8_t* d_sym, uint32_t nseq, const uint64_t* h_off,
                    const uint64_t* h_n, const uint32_t* h_sigma, uint32_t reps, wtg_batch_plan* out,
                    float* ms_out);

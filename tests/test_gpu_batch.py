@@ -106,14 +106,16 @@ def test_batch_plans_and_autotuner(wsfm, orc, ctx):
     ref.free()
     wide = np.full(9, 255, np.uint8)
     small = np.full(9, 255, np.uint8)
+    byte = np.full(9, 255, np.uint8)
     for L in range(1, 9):
         wide[L] = 24 - L
         small[L] = max(0, 14 - L)
+        byte[L] = max(2, 8 - L) if L <= 6 else 255   # 8-bit keys where at most 64 groups
     plan, ms = wsfm.Batch.tune(ctx, d, n, sigma, off, reps=1)
     widths = sorted({orc.levels_of_sigma(int(s)) for s in sigma})
     for L in widths:
         assert plan[L] != 255 and (ms[L] >= 0).any()   # tried and chosen
-    for p in (wide, small, plan):
+    for p in (wide, small, byte, plan):
         B = wsfm.Batch(ctx, d, n, sigma, off, plan=p)
         try:
             B.run()

@@ -574,9 +574,9 @@ class Batch:
         (ms[L][c] of candidate c for alphabet width L, -1 = not tried)."""
         n, sigma, off = Batch._arrays(n, sigma, off)
         plan = np.zeros(9, np.uint8)
-        ms = np.zeros(36, np.float32)
+        ms = np.zeros(45, np.float32)
         _check(lib().wtg_batch_tune(ctx.h, d_sym.ptr, n.size, _p(off), _p(n), _p(sigma), reps, _p(plan), _p(ms)))
-        return plan, ms.reshape(9, 4)
+        return plan, ms.reshape(9, 5)
 
     def set_graph(self, on):
         _check(lib().wtg_batch_set_graph(self.h, 1 if on else 0))

@@ -100,7 +100,14 @@ __global__ __launch_bounds__(256) void k_batch_keys(const uint8_t* __restrict__ 
         bad |= (wv[k] & badm) != 0;
         c += (uint32_t)__popc((wv[k] >> (L - 1)) & 0x01010101u);
       }
-      if constexpr (sizeof(K) == 2) {
+      if constexpr (sizeof(K) == 1) {
+        uint4 o;
+        o.x = wv[0] | S * 0x01010101u;
+        o.y = wv[1] | S * 0x01010101u;
+        o.z = wv[2] | S * 0x01010101u;
+        o.w = wv[3] | S * 0x01010101u;
+        *reinterpret_cast<uint4*>(keys + p0) = o;
+      } else if constexpr (sizeof(K) == 2) {
         const uint32_t SS = S | (S << 16);
         uint4 o[2];
         uint32_t* ov = reinterpret_cast<uint32_t*>(o);
@@ -297,7 +304,7 @@ extern "C" int wtg_batch_create_planned(wtg_ctx* ctx, const uint8_t* d_sym, uint
     const uint32_t L = G.L;
     const uint32_t m = (uint32_t)G.seqs.size();
     G.sbits = (uint32_t)log2up_u64(m);
-    G.key_bytes = G.sbits + L <= 16 ? 2 : 4;
+    G.key_bytes = G.sbits + L <= 8 ? 1 : G.sbits + L <= 16 ? 2 : 4;
     std::vector<uint64_t> so(m), pr(m + 1, 0);
     std::vector<uint32_t> cnt(1ull << G.sbits, 0);
     for (uint32_t t = 0; t < m; t++) {
@@ -357,7 +364,10 @@ static int batch_enqueue(wtg_ctx* ctx, wtg_batch* b) {
     const unsigned grid = (unsigned)wtg_ceil_div(G.n, 4096);
     const uint32_t m = (uint32_t)G.seqs.size();
     WTG_TRY(wtg_fill_bytes_zero(ctx, G.d_ones, (1ull << G.sbits) * 4));
-    if (G.key_bytes == 2)
+    if (G.key_bytes == 1)
+      WTG_LAUNCH(ctx, "batch_keys", k_batch_keys<uint8_t>, dim3(grid), dim3(256), 0, b->d_sym, b->sym_end,
+                 b->sym_fast, G.d_seqoff, G.d_pref, m, G.n, G.L, (uint8_t*)G.d_keys, G.d_ones, ctx->d_flag);
+    else if (G.key_bytes == 2)
       WTG_LAUNCH(ctx, "batch_keys", k_batch_keys<uint16_t>, dim3(grid), dim3(256), 0, b->d_sym, b->sym_end,
                  b->sym_fast, G.d_seqoff, G.d_pref, m, G.n, G.L, (uint16_t*)G.d_keys, G.d_ones, ctx->d_flag);
     else
@@ -453,16 +463,17 @@ extern "C" int wtg_batch_create(wtg_ctx* ctx, const uint8_t* d_sym, uint32_t nse
 
 // Autotuner: for each alphabet width L present, time every candidate group
 // size on the batch's own sequences of that width (one recorded run, then the
-// best of `reps` graph replays) and keep the fastest.  Candidates: 2^(16-L-2)
-// and 2^(16-L) sequences (16-bit keys), 2^(20-L) and 2^(24-L) (32-bit keys;
-// one group holds every sequence of the width when they fit).
+// best of `reps` graph replays) and keep the fastest.  Candidates: 2^(8-L)
+// sequences per group (8-bit keys), 2^(14-L) and 2^(16-L) (16-bit keys),
+// 2^(20-L) and 2^(24-L) (32-bit keys; one group holds every sequence of the
+// width when they fit).
 extern "C" int wtg_batch_tune(wtg_ctx* ctx, const uint8_t* d_sym, uint32_t nseq, const uint64_t* h_off,
                               const uint64_t* h_n, const uint32_t* h_sigma, uint32_t reps, wtg_batch_plan* out,
                               float* ms_out) {
   WTG_REQUIRE(ctx && out && nseq >= 1 && h_off && h_n && h_sigma && reps >= 1, "wtg_batch_tune: bad argument");
   std::memset(out->cap_log, 0xFF, sizeof(out->cap_log));
   if (ms_out)
-    for (int k = 0; k < 9 * 4; k++) ms_out[k] = -1.0f;
+    for (int k = 0; k < 9 * 5; k++) ms_out[k] = -1.0f;
   for (uint32_t L = 1; L <= 8; L++) {
     std::vector<uint64_t> off, n;
     std::vector<uint32_t> sig;
@@ -474,13 +485,14 @@ extern "C" int wtg_batch_tune(wtg_ctx* ctx, const uint8_t* d_sym, uint32_t nseq,
       sig.push_back(h_sigma[i]);
     }
     if (off.empty()) continue;
-    const int cand[4] = {(int)(16 - L) - 2, (int)(16 - L), (int)(20 - L), (int)(24 - L)};
+    const int cand[5] = {(int)(8 - L), (int)(16 - L) - 2, (int)(16 - L), (int)(20 - L), (int)(24 - L)};
     float best = 0.f;
-    for (int c = 0; c < 4; c++) {
+    for (int c = 0; c < 5; c++) {
       if (cand[c] < 0) continue;
       // a candidate no coarser than needed for the width's sequence count
       const uint64_t m = off.size();
       if (c > 0 && ((uint64_t)1 << cand[c - 1]) >= m && cand[c - 1] >= 0) continue;
+      if (wtg_ceil_div(m, (uint64_t)1 << cand[c]) > 64) continue;  // at most 64 groups per width
       wtg_batch_plan p;
       std::memset(p.cap_log, 0xFF, sizeof(p.cap_log));
       p.cap_log[L] = (uint8_t)cand[c];
@@ -495,7 +507,7 @@ extern "C" int wtg_batch_tune(wtg_ctx* ctx, const uint8_t* d_sym, uint32_t nseq,
       }
       wtg_batch_destroy(ctx, b);
       if (rc != WTG_OK) return rc;
-      if (ms_out) ms_out[L * 4 + c] = bestc;
+      if (ms_out) ms_out[L * 5 + c] = bestc;
       if (out->cap_log[L] == 0xFF || bestc < best) {
         best = bestc;
         out->cap_log[L] = (uint8_t)cand[c];
