@@ -31,6 +31,14 @@ def texts(wsfm):
     yield "two", (rng.random(3001) < 0.01).astype(np.uint8) + 5
     for n in (1, 2, 63, 64, 65, 1023, 1024, 1025, 4097):
         yield "tiny%d" % n, rng.integers(0, 4, n).astype(np.uint8) * 3 + 60
+    # alphabet sizes around the count kernel's per-lane counter widths
+    # (8, 16 and 32 inner nodes), skewed so the trees are unbalanced
+    for sig, n in ((9, 20011), (10, 3000), (17, 50007), (33, 65536 + 129), (34, 30001)):
+        p = 1.0 / np.arange(1, sig + 1) ** 0.7
+        yield "alpha%d" % sig, (rng.choice(sig, n, p=p / p.sum()) * 7 + 1).astype(np.uint8)
+
+
+NTEXTS = 20
 
 
 def check_shape(wt, oh):
@@ -60,9 +68,11 @@ def brute(t, rng, nq):
     return i, c, rank
 
 
-@pytest.mark.parametrize("case", range(15))
+@pytest.mark.parametrize("case", range(NTEXTS))
 def test_huff_build_and_queries(wsfm, orc, ctx, case):
-    name, t = list(texts(wsfm))[case]
+    cases = list(texts(wsfm))
+    assert len(cases) == NTEXTS
+    name, t = cases[case]
     print("huff case", name, t.size, flush=True)
     rng = np.random.default_rng(case)
     d = ctx.to_device(t)

@@ -7,7 +7,11 @@
 //   1. k_huff_hist   byte counts -> host builds the Huffman shape (a few us)
 //   2. k_huff_count  per (inner node v, segment s) count of text positions whose
 //                    code passes through v.  Segment = one wave's S consecutive
-//                    bytes.  Table laid out [v][s] with inner nodes in BFS order,
+//                    bytes.  Up to 32 inner nodes each lane adds a per-symbol
+//                    word of byte counters (one byte per node on the symbol's
+//                    path) for each of its bytes; larger trees take a segment
+//                    byte histogram and add it along each symbol's path.
+//                    Table laid out [v][s] with inner nodes in BFS order,
 //                    so ONE flat exclusive scan of it gives each (v, s) run's
 //                    global bit position: bv_pos[v] is the sum of the inner
 //                    nodes before v, the prefix within row v the bits of v before
@@ -77,33 +81,44 @@ __device__ __forceinline__ void wave_sync_lds() {
   __builtin_amdgcn_fence(__ATOMIC_ACQUIRE, "wavefront");
 }
 
-// byte histogram: LDS atomics into 4 replicas per wave (lane & 3), so lanes
-// holding the same byte mostly hit different words; replicas summed at the end
+// byte histogram: LDS atomics into HF_HIST_REPS replicas per wave (replica =
+// lane & (HF_HIST_REPS-1)), rows padded to 257 words so the replicas of one
+// byte sit in different banks (a low-entropy text such as DNA otherwise puts
+// every lane of an atomic on one of four banks).  All pieces are loaded before
+// counting.  Measured on 2^27 DNA bytes: 4 replicas 89 us, 8 or 16 replicas
+// 95 us, 4 or 16 unpadded 124 us.
 constexpr int HF_HIST_PIECES = 8;  // pieces of 1 KiB per wave
+constexpr int HF_HIST_REPS = 4;
+constexpr int HF_HIST_ROW = 257;
 __global__ __launch_bounds__(HF_THREADS) void k_huff_hist(const uint8_t* __restrict__ text, uint64_t n,
                                                           unsigned long long* __restrict__ hist, bool aligned) {
-  __shared__ uint32_t s_h[HF_THREADS / 64][4][256];
-  uint32_t* flat = &s_h[0][0][0];
-  for (int k = threadIdx.x; k < (HF_THREADS / 64) * 4 * 256; k += HF_THREADS) flat[k] = 0;
+  __shared__ uint32_t s_h[HF_THREADS / 64][HF_HIST_REPS * HF_HIST_ROW];
+  uint32_t* flat = &s_h[0][0];
+  for (int k = threadIdx.x; k < (HF_THREADS / 64) * HF_HIST_REPS * HF_HIST_ROW; k += HF_THREADS) flat[k] = 0;
   __syncthreads();
   const int lane = lane_id(), w = threadIdx.x >> 6;
-  uint32_t* h = s_h[w][lane & 3];
+  uint32_t* h = s_h[w] + (lane & (HF_HIST_REPS - 1)) * HF_HIST_ROW;
   const uint64_t base0 = ((uint64_t)blockIdx.x * (HF_THREADS / 64) + w) * HF_HIST_PIECES * HF_PIECE;
+  uint4 v[HF_HIST_PIECES];
 #pragma unroll
   for (int pc = 0; pc < HF_HIST_PIECES; pc++) {
     const uint64_t base = base0 + (uint64_t)pc * HF_PIECE + (uint64_t)lane * 16;
-    if (base >= n) break;
-    uint8_t b[16];
-    const uint4 v = load16(text, n, base, aligned);
-    memcpy(b, &v, 16);
+    v[pc] = base < n ? load16(text, n, base, aligned) : make_uint4(0, 0, 0, 0);
+  }
 #pragma unroll
-    for (int k = 0; k < 16; k++)
-      if (base + k < n) atomicAdd(&h[b[k]], 1u);
+  for (int pc = 0; pc < HF_HIST_PIECES; pc++) {
+    const uint64_t base = base0 + (uint64_t)pc * HF_PIECE + (uint64_t)lane * 16;
+    const uint32_t valid = base >= n ? 0u : (n - base >= 16 ? 16u : (uint32_t)(n - base));
+#pragma unroll
+    for (int k = 0; k < 16; k++) {
+      const uint32_t word = k < 4 ? v[pc].x : k < 8 ? v[pc].y : k < 12 ? v[pc].z : v[pc].w;
+      if ((uint32_t)k < valid) atomicAdd(&h[(word >> (8 * (k & 3))) & 0xFFu], 1u);
+    }
   }
   __syncthreads();
   for (int c = threadIdx.x; c < 256; c += HF_THREADS) {
     uint32_t t = 0;
-    for (int r = 0; r < (HF_THREADS / 64) * 4; r++) t += flat[r * 256 + c];
+    for (int r = 0; r < (HF_THREADS / 64) * HF_HIST_REPS; r++) t += flat[r * HF_HIST_ROW + c];
     if (t) atomicAdd(&hist[c], (unsigned long long)t);
   }
 }
@@ -168,16 +183,100 @@ __device__ __forceinline__ void load_piece(uint8_t* s_txt, const uint8_t* __rest
   *reinterpret_cast<uint4*>(s_txt + lane * 16) = load16(text, n, base + (uint64_t)lane * 16, aligned);
 }
 
-// pass 2: cnt[v][s]
-__global__ __launch_bounds__(HF_THREADS) void k_huff_count(SegArgs a) {
-  __shared__ HuffLds L;
-  __shared__ uint32_t s_cnt[HF_THREADS / 64][256];  // per wave: count of each inner node
-  load_tab(L, a.tab);
+// pass 2 for trees of at most 8*K inner nodes: per-lane byte counters.  s_exp[k][c]
+// holds a 1 in byte (v & 7) of word k = v >> 3 for every inner node v on the
+// path of symbol c, so adding s_exp[.][c] for each text byte counts, per lane,
+// the positions passing each node; a lane sees S/64 <= 64 bytes of a segment,
+// so a byte never overflows.  At the segment end the even and odd bytes are
+// split into 16-bit fields and summed over the wave.
+template <int K>
+__global__ __launch_bounds__(HF_THREADS) void k_huff_count_swar(SegArgs a) {
+  __shared__ uint64_t s_exp[K][256];
+  __shared__ uint32_t s_cnt[HF_THREADS / 64][8 * K];
+  for (int c = threadIdx.x; c < 256; c += HF_THREADS) {
+    uint64_t e[K];
+#pragma unroll
+    for (int k = 0; k < K; k++) e[k] = 0;
+    const uint32_t len = a.tab->len[c];
+    const uint64_t code = a.tab->code[c];
+    uint32_t node = 0;
+    for (uint32_t l = 0; l < len; l++) {
+#pragma unroll
+      for (int k = 0; k < K; k++)
+        if ((node >> 3) == (uint32_t)k) e[k] += 1ull << (8 * (node & 7));
+      node = a.tab->child[node][(code >> l) & 1];
+    }
+#pragma unroll
+    for (int k = 0; k < K; k++) s_exp[k][c] = e[k];
+  }
   __syncthreads();
   const int lane = lane_id(), w = threadIdx.x >> 6;
   uint32_t* cnt = s_cnt[w];
   const uint64_t waves = (uint64_t)gridDim.x * (HF_THREADS / 64);
+  constexpr uint64_t EVEN = 0x00FF00FF00FF00FFull;
   for (uint64_t s = (uint64_t)blockIdx.x * (HF_THREADS / 64) + w; s < a.nseg; s += waves) {
+    uint64_t acc[K];
+#pragma unroll
+    for (int k = 0; k < K; k++) acc[k] = 0;
+    for (uint32_t off = 0; off < a.S; off += HF_PIECE) {
+      const uint64_t base = s * a.S + off + (uint64_t)lane * 16;
+      if (base - (uint64_t)lane * 16 >= a.n) break;
+      const uint4 wv = load16(a.text, a.n, base, a.aligned);
+      const uint32_t valid = base >= a.n ? 0u : (a.n - base >= 16 ? 16u : (uint32_t)(a.n - base));
+#pragma unroll
+      for (int b = 0; b < 16; b++) {
+        const uint32_t word = b < 4 ? wv.x : b < 8 ? wv.y : b < 12 ? wv.z : wv.w;
+        const uint32_t c = (word >> (8 * (b & 3))) & 0xFFu;
+        if ((uint32_t)b < valid) {
+#pragma unroll
+          for (int k = 0; k < K; k++) acc[k] += s_exp[k][c];
+        }
+      }
+    }
+#pragma unroll
+    for (int k = 0; k < K; k++) {
+      const uint64_t ev = acc[k] & EVEN, od = (acc[k] >> 8) & EVEN;
+      // fields: ev lo = nodes 0,2; ev hi = 4,6; od lo = 1,3; od hi = 5,7 (of word k)
+      const uint32_t p0 = (uint32_t)__shfl(wave_incl_scan((uint32_t)ev), 63);
+      const uint32_t p1 = (uint32_t)__shfl(wave_incl_scan((uint32_t)(ev >> 32)), 63);
+      const uint32_t p2 = (uint32_t)__shfl(wave_incl_scan((uint32_t)od), 63);
+      const uint32_t p3 = (uint32_t)__shfl(wave_incl_scan((uint32_t)(od >> 32)), 63);
+      if (lane == 0) {
+        cnt[8 * k + 0] = p0 & 0xFFFFu; cnt[8 * k + 2] = p0 >> 16;
+        cnt[8 * k + 4] = p1 & 0xFFFFu; cnt[8 * k + 6] = p1 >> 16;
+        cnt[8 * k + 1] = p2 & 0xFFFFu; cnt[8 * k + 3] = p2 >> 16;
+        cnt[8 * k + 5] = p3 & 0xFFFFu; cnt[8 * k + 7] = p3 >> 16;
+      }
+    }
+    wave_sync_lds();
+    if ((uint32_t)lane < a.n_inner) a.cnt[(uint64_t)lane * a.nseg + s] = cnt[lane];
+    wave_sync_lds();
+  }
+}
+
+// pass 2 for larger trees: a byte histogram of the segment (LDS atomics into 4
+// padded replicas), then each present symbol adds its count to every inner node
+// on its path.
+__global__ __launch_bounds__(HF_THREADS) void k_huff_count_hist(SegArgs a) {
+  __shared__ uint32_t s_h[HF_THREADS / 64][4 * HF_HIST_ROW];
+  __shared__ uint32_t s_cnt[HF_THREADS / 64][256];
+  __shared__ uint64_t s_code[256];
+  __shared__ uint16_t s_child[256][2];
+  __shared__ uint8_t s_len[256];
+  for (int k = threadIdx.x; k < 256; k += HF_THREADS) {
+    s_code[k] = a.tab->code[k];
+    s_len[k] = a.tab->len[k];
+    s_child[k][0] = a.tab->child[k][0];
+    s_child[k][1] = a.tab->child[k][1];
+  }
+  __syncthreads();
+  const int lane = lane_id(), w = threadIdx.x >> 6;
+  uint32_t* hw = s_h[w];
+  uint32_t* h = hw + (lane & 3) * HF_HIST_ROW;
+  uint32_t* cnt = s_cnt[w];
+  const uint64_t waves = (uint64_t)gridDim.x * (HF_THREADS / 64);
+  for (uint64_t s = (uint64_t)blockIdx.x * (HF_THREADS / 64) + w; s < a.nseg; s += waves) {
+    for (int k = lane; k < 4 * HF_HIST_ROW; k += 64) hw[k] = 0;
 #pragma unroll
     for (int q = 0; q < 4; q++) cnt[64 * q + lane] = 0;
     wave_sync_lds();
@@ -185,15 +284,24 @@ __global__ __launch_bounds__(HF_THREADS) void k_huff_count(SegArgs a) {
       const uint64_t base = s * a.S + off + (uint64_t)lane * 16;
       if (base - (uint64_t)lane * 16 >= a.n) break;
       const uint4 wv = load16(a.text, a.n, base, a.aligned);
-      // counting is order-free: chunk k = byte k of every lane
-#pragma unroll 1
-      for (int k = 0; k < 16; k++) {
-        const uint32_t q = (uint32_t)k >> 2;
-        const uint32_t word = q == 0 ? wv.x : q == 1 ? wv.y : q == 2 ? wv.z : wv.w;
-        const uint32_t c = (word >> (8 * (k & 3))) & 0xFFu;
-        walk_chunk(L, c, base + k < a.n, [&](uint32_t v, uint64_t m, uint64_t, uint32_t) {
-          if (lane == 0) cnt[v] += popc64(m);
-        });
+      const uint32_t valid = base >= a.n ? 0u : (a.n - base >= 16 ? 16u : (uint32_t)(a.n - base));
+#pragma unroll
+      for (int b = 0; b < 16; b++) {
+        const uint32_t word = b < 4 ? wv.x : b < 8 ? wv.y : b < 12 ? wv.z : wv.w;
+        if ((uint32_t)b < valid) atomicAdd(&h[(word >> (8 * (b & 3))) & 0xFFu], 1u);
+      }
+    }
+    wave_sync_lds();
+#pragma unroll
+    for (int q = 0; q < 4; q++) {
+      const uint32_t c = (uint32_t)(64 * q + lane);
+      const uint32_t t = hw[c] + hw[HF_HIST_ROW + c] + hw[2 * HF_HIST_ROW + c] + hw[3 * HF_HIST_ROW + c];
+      const uint32_t len = t ? (uint32_t)s_len[c] : 0u;
+      const uint64_t code = s_code[c];
+      uint32_t node = 0;
+      for (uint32_t l = 0; l < len; l++) {
+        atomicAdd(&cnt[node], t);
+        node = s_child[node][(code >> l) & 1];
       }
     }
     wave_sync_lds();
@@ -674,7 +782,16 @@ extern "C" int wtg_huff_build(wtg_ctx* ctx, const uint8_t* d_text, uint64_t n, w
       a.words = wt->words;
       a.aligned = ((uintptr_t)d_text & 15) == 0;
       const unsigned grid = (unsigned)std::min<uint64_t>(wtg_ceil_div(nseg, 4), 8192);
-      WTG_LAUNCH(ctx, "huff_count", k_huff_count, dim3(grid), dim3(HF_THREADS), 0, a);
+      // S_bytes stays HF_PIECE for these tree sizes: 16 bytes per lane per segment
+      if (S.n_inner <= 8 && S_bytes <= 64 * 255) {
+        WTG_LAUNCH(ctx, "huff_count", k_huff_count_swar<1>, dim3(grid), dim3(HF_THREADS), 0, a);
+      } else if (S.n_inner <= 16 && S_bytes <= 64 * 255) {
+        WTG_LAUNCH(ctx, "huff_count", k_huff_count_swar<2>, dim3(grid), dim3(HF_THREADS), 0, a);
+      } else if (S.n_inner <= 32 && S_bytes <= 64 * 255) {
+        WTG_LAUNCH(ctx, "huff_count", k_huff_count_swar<4>, dim3(grid), dim3(HF_THREADS), 0, a);
+      } else {
+        WTG_LAUNCH(ctx, "huff_count", k_huff_count_hist, dim3(grid), dim3(HF_THREADS), 0, a);
+      }
       WTG_LAUNCH_CHECK();
       WTG_LAUNCH(ctx, "huff_scan", k_scan_u32<uint64_t>, dim3((unsigned)scan_tiles), dim3(256), 0,
                  (const uint32_t*)(ws + o_cnt), (uint64_t*)(ws + o_pos), entries, (uint64_t*)(ws + o_st),

@@ -226,21 +226,28 @@ __global__ __launch_bounds__(256) void k_select_assemble(AsmArgs a) {
   }
 }
 
+// also copies the long-superblock count ex[sb] to *nl_out
 __global__ void k_long_index(const uint32_t* __restrict__ flag, const uint32_t* __restrict__ ex,
-                             int64_t* __restrict__ li, uint64_t sb) {
+                             int64_t* __restrict__ li, uint64_t sb, uint32_t* __restrict__ nl_out) {
   const uint64_t k = (uint64_t)blockIdx.x * blockDim.x + threadIdx.x;
   if (k < sb) li[k] = flag[k] ? (int64_t)ex[k] : -1;
+  if (k == 0) *nl_out = ex[sb];
 }
 
-// one block per long superblock: enumerate the arguments in [a, e]
-__global__ __launch_bounds__(256) void k_long_fill(const uint64_t* __restrict__ words, uint64_t nbits,
-                                                   int bit, const uint64_t* __restrict__ sbstart,
-                                                   const uint64_t* __restrict__ sbend,
-                                                   const unsigned long long* last,
-                                                   const uint32_t* __restrict__ flag,
-                                                   const uint32_t* __restrict__ ex,
-                                                   uint64_t sb, uint64_t* __restrict__ pool) {
-  __shared__ uint32_t s_wave[4];
+// one block of LF_THREADS per long superblock: enumerate the arguments in
+// [a, e], LF_WORDS consecutive words per thread per round (all loads issued
+// before the block scan, so a round costs one memory latency)
+constexpr int LF_THREADS = 1024;
+constexpr int LF_WORDS = 4;
+__global__ __launch_bounds__(LF_THREADS) void k_long_fill(const uint64_t* __restrict__ words, uint64_t nbits,
+                                                          int bit, const uint64_t* __restrict__ sbstart,
+                                                          const uint64_t* __restrict__ sbend,
+                                                          const unsigned long long* last,
+                                                          const uint32_t* __restrict__ flag,
+                                                          const uint32_t* __restrict__ ex,
+                                                          uint64_t sb, uint64_t* __restrict__ pool) {
+  constexpr int NWV = LF_THREADS / 64;
+  __shared__ uint32_t s_wave[NWV];
   const uint64_t k = blockIdx.x;
   if (k >= sb || !flag[k]) return;
   const uint64_t st = sbstart[k];
@@ -248,30 +255,45 @@ __global__ __launch_bounds__(256) void k_long_fill(const uint64_t* __restrict__ 
   uint64_t* out = pool + (uint64_t)ex[k] * 4096;
   uint32_t run = 0;
   const uint64_t w0 = st >> 6, w1 = en >> 6;
-  for (uint64_t base = w0; base <= w1; base += 256) {
-    const uint64_t wi = base + threadIdx.x;
-    uint64_t x = 0;
-    if (wi <= w1) {
-      x = bit ? words[wi] : ~words[wi];
-      uint64_t lo = wi * 64;
-      if (lo < st) x &= ~0ull << (st - lo);
-      if (lo + 63 > en) x &= (en - lo == 63) ? ~0ull : ((1ull << (en - lo + 1)) - 1);
-      if (lo + 64 > nbits) x &= (nbits > lo) ? ((nbits - lo >= 64) ? ~0ull : ((1ull << (nbits - lo)) - 1)) : 0ull;
+  const uint64_t nwords = (nbits + 63) >> 6;
+  for (uint64_t base = w0; base <= w1; base += (uint64_t)LF_THREADS * LF_WORDS) {
+    const uint64_t wb = base + (uint64_t)threadIdx.x * LF_WORDS;
+    uint64_t x[LF_WORDS];
+#pragma unroll
+    for (int q = 0; q < LF_WORDS; q++) {
+      const uint64_t wi = wb + q;
+      x[q] = (wi <= w1 && wi < nwords) ? words[wi] : (bit ? 0ull : ~0ull);
     }
-    const uint32_t c = popc64(x);
+    uint32_t c = 0;
+#pragma unroll
+    for (int q = 0; q < LF_WORDS; q++) {
+      const uint64_t wi = wb + q;
+      uint64_t v = bit ? x[q] : ~x[q];
+      const uint64_t lo = wi * 64;
+      if (wi > w1) v = 0;
+      if (lo < st) v = (st - lo >= 64) ? 0ull : (v & (~0ull << (st - lo)));
+      if (lo + 63 > en) v &= (lo > en) ? 0ull : ((en - lo == 63) ? ~0ull : ((1ull << (en - lo + 1)) - 1));
+      if (lo + 64 > nbits) v &= (nbits > lo) ? ((nbits - lo >= 64) ? ~0ull : ((1ull << (nbits - lo)) - 1)) : 0ull;
+      x[q] = v;
+      c += popc64(v);
+    }
     const int w = threadIdx.x >> 6, l = lane_id();
-    uint32_t incl = wave_incl_scan(c);
+    const uint32_t incl = wave_incl_scan(c);
     if (l == 63) s_wave[w] = incl;
     __syncthreads();
     uint32_t off = 0, tt = 0;
 #pragma unroll
-    for (int q = 0; q < 4; q++) { uint32_t t = s_wave[q]; if (q < w) off += t; tt += t; }
+    for (int q = 0; q < NWV; q++) { const uint32_t t = s_wave[q]; if (q < w) off += t; tt += t; }
     __syncthreads();
     uint32_t r = run + incl - c + off;
-    while (x) {
-      if (r < 4096) out[r] = wi * 64 + (uint64_t)__builtin_ctzll(x);
-      r++;
-      x &= x - 1;
+#pragma unroll
+    for (int q = 0; q < LF_WORDS; q++) {
+      uint64_t v = x[q];
+      while (v) {
+        if (r < 4096) out[r] = (wb + q) * 64 + (uint64_t)__builtin_ctzll(v);
+        r++;
+        v &= v - 1;
+      }
     }
     run += tt;
   }
@@ -322,9 +344,13 @@ int sel_alloc_worst(wtg_ctx* ctx, uint64_t nbits, int bit, wtg_select* s) {
   return WTG_OK;
 }
 
-// finish one select structure once the arguments are counted
-int sel_finish(wtg_ctx* ctx, const uint64_t* d_words, uint64_t nbits, wtg_select* s, uint64_t m,
-               const uint64_t* sbend, const unsigned long long* last, char* scratch) {
+// finish one select structure once the arguments are counted, in two halves
+// so that both structures share one host read of their long-superblock
+// counts: sel_count stages the widths, the long flags and their scan, and
+// writes the number of long superblocks to *d_nl; sel_fill allocates and
+// fills the long pool once that number is on the host
+int sel_count(wtg_ctx* ctx, uint64_t nbits, wtg_select* s, uint64_t m, const uint64_t* sbend,
+              const unsigned long long* last, char* scratch, uint32_t* d_nl) {
   const uint64_t cap = (nbits + 63) / 64 * 64;
   s->arg_cnt = m;
   s->logn = (uint32_t)wtg_hi_host(cap) + 1;
@@ -335,7 +361,7 @@ int sel_finish(wtg_ctx* ctx, const uint64_t* d_words, uint64_t nbits, wtg_select
   WTG_TRY(wtg_malloc(ctx, (void**)&s->mini_width, sb ? sb : 1));
   WTG_TRY(wtg_malloc(ctx, (void**)&s->long_width, sb ? sb : 1));
   WTG_TRY(wtg_malloc(ctx, (void**)&s->long_index, (sb ? sb : 1) * 8));
-  if (sb == 0) return WTG_OK;
+  if (sb == 0) return wtg_fill_bytes_zero(ctx, d_nl, 4);
   uint32_t* flag = (uint32_t*)scratch;
   uint32_t* ex = (uint32_t*)(scratch + wtg_align256(sb * 4));
   uint64_t* st = (uint64_t*)(scratch + wtg_align256(sb * 4) + wtg_align256((sb + 1) * 4));
@@ -352,19 +378,23 @@ int sel_finish(wtg_ctx* ctx, const uint64_t* d_words, uint64_t nbits, wtg_select
              flag, ex, sb, st, ctr, ctx->d_flag);
   WTG_LAUNCH_CHECK();
   WTG_LAUNCH(ctx, "select_longidx", k_long_index, dim3((unsigned)wtg_ceil_div(sb, 256)), dim3(256), 0,
-             flag, ex, s->long_index, sb);
+             flag, ex, s->long_index, sb, d_nl);
   WTG_LAUNCH_CHECK();
-  uint32_t nl = 0;
-  WTG_CHECK_HIP(hipMemcpyAsync(&nl, ex + sb, 4, hipMemcpyDeviceToHost, ctx->stream));
-  WTG_CHECK_HIP(hipStreamSynchronize(ctx->stream));
+  return WTG_OK;
+}
+
+int sel_fill(wtg_ctx* ctx, const uint64_t* d_words, uint64_t nbits, wtg_select* s, uint32_t nl,
+             const uint64_t* sbend, const unsigned long long* last, const char* scratch) {
+  const uint64_t sb = s->sb;
   s->n_long = nl;
-  if (nl) {
-    WTG_TRY(wtg_malloc(ctx, (void**)&s->long_pool, (uint64_t)nl * 4096 * 8));
-    WTG_TRY(wtg_fill_u64(ctx, s->long_pool, (uint64_t)nl * 4096, 0));
-    WTG_LAUNCH(ctx, "select_longfill", k_long_fill, dim3((unsigned)sb), dim3(256), 0, d_words, nbits,
-               s->bit, s->superblock, sbend, last, flag, ex, sb, s->long_pool);
-    WTG_LAUNCH_CHECK();
-  }
+  if (sb == 0 || nl == 0) return WTG_OK;
+  const uint32_t* flag = (const uint32_t*)scratch;
+  const uint32_t* ex = (const uint32_t*)(scratch + wtg_align256(sb * 4));
+  WTG_TRY(wtg_malloc(ctx, (void**)&s->long_pool, (uint64_t)nl * 4096 * 8));
+  WTG_TRY(wtg_fill_u64(ctx, s->long_pool, (uint64_t)nl * 4096, 0));
+  WTG_LAUNCH(ctx, "select_longfill", k_long_fill, dim3((unsigned)sb), dim3(LF_THREADS), 0, d_words, nbits,
+             s->bit, s->superblock, sbend, last, flag, ex, sb, s->long_pool);
+  WTG_LAUNCH_CHECK();
   return WTG_OK;
 }
 
@@ -377,7 +407,7 @@ int wtg_rank_select_run(wtg_ctx* ctx, const uint64_t* d_words, uint64_t nbits, u
   const uint64_t entries = S + 1;
   const uint64_t tiles = wtg_ceil_div(entries, 256);
   const uint64_t sbmax = wtg_ceil_div(nbits, 4096) + 1;
-  // workspace: states | counter | ones_total | last[2] | sbend[2] | finish scratch
+  // workspace: states | counter | ones_total | last[2] | long counts[2] | sbend[2] | finish scratch[2]
   size_t o = 0;
   const size_t o_states = o; o = wtg_align256(o + (tiles + 1) * 8);
   // word tiles: every entry 0..S has its first word 8b inside them
@@ -389,7 +419,8 @@ int wtg_rank_select_run(wtg_ctx* ctx, const uint64_t* d_words, uint64_t nbits, u
   const size_t o_ctr = o; o = wtg_align256(o + 256);
   const size_t o_end0 = o; o = wtg_align256(o + sbmax * 8);
   const size_t o_end1 = o; o = wtg_align256(o + sbmax * 8);
-  const size_t o_fin = o; o = wtg_align256(o + 4 * wtg_align256((sbmax + 1) * 8) + 4096);
+  const size_t fin_bytes = wtg_align256(4 * wtg_align256((sbmax + 1) * 8) + 4096);
+  const size_t o_fin = o; o = wtg_align256(o + 2 * fin_bytes);
   WTG_TRY(wtg_ws_reserve(ctx, o));
   char* ws = (char*)ctx->ws;
   WTG_TRY(wtg_fill_bytes_zero(ctx, ws, o_end0));
@@ -461,10 +492,17 @@ int wtg_rank_select_run(wtg_ctx* ctx, const uint64_t* d_words, uint64_t nbits, u
     for (uint64_t k = 0; k < nt; k++) m1 += __builtin_popcountll(tailw[k]);
   }
   const uint64_t m[2] = {nbits - m1, m1};
-  for (int t = 0; t < 2; t++) {
-    if (!ra.want[t]) continue;
-    WTG_TRY(sel_finish(ctx, d_words, nbits, outs[t], m[t], ra.sel[t].sbend, last + t, ws + o_fin));
-  }
+  uint32_t* d_nl = (uint32_t*)(ws + o_ctr + 192);
+  for (int t = 0; t < 2; t++)
+    if (ra.want[t])
+      WTG_TRY(sel_count(ctx, nbits, outs[t], m[t], ra.sel[t].sbend, last + t, ws + o_fin + t * fin_bytes, d_nl + t));
+  if (!ra.want[0] && !ra.want[1]) return WTG_OK;
+  uint32_t nl[2] = {0, 0};
+  WTG_CHECK_HIP(hipMemcpyAsync(nl, d_nl, 8, hipMemcpyDeviceToHost, ctx->stream));
+  WTG_CHECK_HIP(hipStreamSynchronize(ctx->stream));
+  for (int t = 0; t < 2; t++)
+    if (ra.want[t])
+      WTG_TRY(sel_fill(ctx, d_words, nbits, outs[t], nl[t], ra.sel[t].sbend, last + t, ws + o_fin + t * fin_bytes));
   return WTG_OK;
 }
 
