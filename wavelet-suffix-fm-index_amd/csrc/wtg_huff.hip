@@ -322,6 +322,35 @@ __device__ __forceinline__ uint64_t stage_bits(const uint64_t* stg, int64_t d) {
   return sh ? (stg[i] >> sh) | (stg[i + 1] << (64 - sh)) : stg[i];
 }
 
+__device__ __forceinline__ uint64_t readlane64(uint64_t x, uint32_t l) {
+  return ((uint64_t)(uint32_t)__builtin_amdgcn_readlane((int)(uint32_t)(x >> 32), (int)l) << 32) |
+         (uint32_t)__builtin_amdgcn_readlane((int)(uint32_t)x, (int)l);
+}
+
+// writes every run of segment s from the wave's staging area as whole words
+// (edge words OR-ed); len(v), gstart(v), lstart(v) give node v's run length,
+// global bit and staging bit as wave-uniform values
+template <typename FL, typename FG, typename FS>
+__device__ __forceinline__ void write_runs(const SegArgs& a, uint64_t s, const uint64_t* stg, int lane, FL&& len_of,
+                                           FG&& gstart, FS&& lstart) {
+  for (uint32_t v = 0; v < a.n_inner; v++) {
+    const uint32_t len = len_of(v);
+    if (!len) continue;
+    const uint64_t g0 = gstart(v);
+    const int64_t lsv = (int64_t)lstart(v);
+    const uint64_t w0 = g0 >> 6, w1 = (g0 + len - 1) >> 6;
+    for (uint64_t kw = w0 + lane; kw <= w1; kw += 64) {
+      const uint64_t lo = kw * 64 > g0 ? kw * 64 : g0;
+      const uint64_t hi = kw * 64 + 64 < g0 + len ? kw * 64 + 64 : g0 + len;
+      const uint32_t nb = (uint32_t)(hi - lo), at = (uint32_t)(lo - kw * 64);
+      const uint64_t mask = nb == 64 ? ~0ull : ((1ull << nb) - 1) << at;
+      const uint64_t val = stage_bits(stg, lsv + (int64_t)(kw * 64) - (int64_t)g0) & mask;
+      if (nb == 64) a.words[kw] = val;
+      else if (val) atomicOr(reinterpret_cast<unsigned long long*>(&a.words[kw]), (unsigned long long)val);
+    }
+  }
+}
+
 // pass 3: tree bits
 __global__ __launch_bounds__(HF_THREADS) void k_huff_emit(SegArgs a) {
   __shared__ HuffLds L;
@@ -392,26 +421,139 @@ __global__ __launch_bounds__(HF_THREADS) void k_huff_emit(SegArgs a) {
       }
     }
     wave_sync_lds();
-    // write every run of the segment as whole words
-    for (uint32_t v = 0; v < a.n_inner; v++) {
-      const uint32_t len = (uint32_t)__builtin_amdgcn_readfirstlane((int)rn[v]);
-      if (!len) continue;
-      const uint32_t qv = v >> 6;
-      const uint64_t gq = qv == 0 ? gpos[0] : qv == 1 ? gpos[1] : qv == 2 ? gpos[2] : gpos[3];
-      const uint64_t g0 = ((uint64_t)(uint32_t)__builtin_amdgcn_readlane((int)(uint32_t)(gq >> 32), (int)(v & 63)) << 32) |
-                          (uint32_t)__builtin_amdgcn_readlane((int)(uint32_t)gq, (int)(v & 63));
-      const int64_t lsv = (int64_t)(uint32_t)__builtin_amdgcn_readfirstlane((int)ls[v]);
-      const uint64_t w0 = g0 >> 6, w1 = (g0 + len - 1) >> 6;
-      for (uint64_t kw = w0 + lane; kw <= w1; kw += 64) {
-        const uint64_t lo = kw * 64 > g0 ? kw * 64 : g0;
-        const uint64_t hi = kw * 64 + 64 < g0 + len ? kw * 64 + 64 : g0 + len;
-        const uint32_t nb = (uint32_t)(hi - lo), at = (uint32_t)(lo - kw * 64);
-        const uint64_t mask = nb == 64 ? ~0ull : ((1ull << nb) - 1) << at;
-        const uint64_t val = stage_bits(stg, lsv + (int64_t)(kw * 64) - (int64_t)g0) & mask;
-        if (nb == 64) a.words[kw] = val;
-        else if (val) atomicOr(reinterpret_cast<unsigned long long*>(&a.words[kw]), (unsigned long long)val);
+    write_runs(a, s, stg, lane, [&](uint32_t v) { return (uint32_t)__builtin_amdgcn_readfirstlane((int)rn[v]); },
+               [&](uint32_t v) {
+                 const uint32_t qv = v >> 6;
+                 const uint64_t gq = qv == 0 ? gpos[0] : qv == 1 ? gpos[1] : qv == 2 ? gpos[2] : gpos[3];
+                 return readlane64(gq, v & 63);
+               },
+               [&](uint32_t v) { return (uint32_t)__builtin_amdgcn_readfirstlane((int)ls[v]); });
+  }
+}
+
+// pass 3 for trees of at most 32 inner nodes and 1024-byte segments: lane i
+// takes bytes 16i..16i+15 of the segment and, node by node, gathers the branch
+// bits of its bytes that pass the node into a register (at most 16 bits); a wave
+// scan of the per-node bit counts (two nodes per scan, 16-bit fields) places
+// each lane's bits in the node's run of the LDS staging area (one or two LDS
+// ORs per node and lane).  The runs are then written out in one pass over all
+// the words they touch (lane k: the k-th word, its node found by a search over
+// the per-node word prefix).
+__global__ __launch_bounds__(HF_THREADS) void k_huff_emit_lane(SegArgs a) {
+  __shared__ uint64_t s_pb[256];  // per symbol: bit j = passes inner node j, bit 32+j = branch 1 there
+  extern __shared__ uint64_t s_dyn[];
+  for (int c = threadIdx.x; c < 256; c += HF_THREADS) {
+    uint64_t e = 0;
+    const uint32_t len = a.tab->len[c];
+    const uint64_t code = a.tab->code[c];
+    uint32_t node = 0;
+    for (uint32_t l = 0; l < len; l++) {
+      const uint32_t b = (uint32_t)(code >> l) & 1u;
+      e |= (1ull << node) | ((uint64_t)b << (32 + node));
+      node = a.tab->child[node][b];
+    }
+    s_pb[c] = e;
+  }
+  __syncthreads();
+  const int lane = lane_id(), w = threadIdx.x >> 6;
+  uint64_t* stg = s_dyn + (size_t)w * a.stage_words;
+  const uint64_t waves = (uint64_t)gridDim.x * (HF_THREADS / 64);
+  const uint32_t ni = a.n_inner;
+  // lane v < n_inner: node v's run in the segment (length, global start); lane
+  // i: text bytes 16i..16i+15.  The next segment's are loaded before the
+  // current one is processed.
+  struct SegIn {
+    uint32_t cv, valid;
+    uint64_t gp;
+    uint4 wv;
+  };
+  const bool has = (uint32_t)lane < ni;
+  auto load_seg = [&](uint64_t s) {
+    SegIn r;
+    r.cv = has ? a.cnt[(uint64_t)lane * a.nseg + s] : 0u;
+    r.gp = has ? a.pos[(uint64_t)lane * a.nseg + s] : 0ull;
+    const uint64_t base = s * HF_PIECE + (uint64_t)lane * 16;
+    r.valid = base >= a.n ? 0u : (a.n - base >= 16 ? 16u : (uint32_t)(a.n - base));
+    r.wv = r.valid ? load16(a.text, a.n, base, a.aligned) : make_uint4(0, 0, 0, 0);
+    return r;
+  };
+  uint64_t s = (uint64_t)blockIdx.x * (HF_THREADS / 64) + w;
+  SegIn cur{};
+  if (s < a.nseg) cur = load_seg(s);
+  for (; s < a.nseg; s += waves) {
+    SegIn nxt{};
+    if (s + waves < a.nseg) nxt = load_seg(s + waves);
+    const uint32_t cv = cur.cv;
+    const uint64_t gp = cur.gp;
+    const uint32_t inc = wave_incl_scan(cv);
+    const uint32_t lsv = inc - cv;
+    const uint32_t total = (uint32_t)__shfl((int)inc, 63);
+    // the 16 bytes' table entries (unconditional reads, masked: a conditional
+    // read becomes a branch per byte)
+    uint32_t pm[16], bm[16];
+#pragma unroll
+    for (int b = 0; b < 16; b++) {
+      const uint32_t word = b < 4 ? cur.wv.x : b < 8 ? cur.wv.y : b < 12 ? cur.wv.z : cur.wv.w;
+      const uint64_t e = s_pb[(word >> (8 * (b & 3))) & 0xFFu];
+      const uint32_t vm = (uint32_t)b < cur.valid ? ~0u : 0u;
+      pm[b] = (uint32_t)e & vm;
+      bm[b] = (uint32_t)(e >> 32) & vm;
+    }
+    const uint32_t used = (total + 63) / 64 + 2;
+    for (uint32_t k = lane; k < used; k += 64) stg[k] = 0;
+    wave_sync_lds();
+    for (uint32_t j = 0; j < ni; j += 2) {
+      uint32_t buf0 = 0, buf1 = 0, n0 = 0, n1 = 0;
+#pragma unroll
+      for (int b = 0; b < 16; b++) {
+        buf0 |= ((bm[b] >> j) & 1u) << n0;
+        n0 += (pm[b] >> j) & 1u;
+        buf1 |= ((bm[b] >> (j + 1)) & 1u) << n1;
+        n1 += (pm[b] >> (j + 1)) & 1u;
+      }
+      const uint32_t pk = n0 | (n1 << 16);
+      const uint32_t ex2 = wave_incl_scan(pk) - pk;
+#pragma unroll
+      for (int t = 0; t < 2; t++) {
+        const uint32_t cntj = t ? n1 : n0;
+        const uint32_t bj = t ? buf1 : buf0;
+        const uint32_t at = (uint32_t)__builtin_amdgcn_readlane((int)lsv, (int)(j + t)) + ((ex2 >> (16 * t)) & 0xFFFFu);
+        if (cntj) {
+          const uint32_t wi = at >> 6, sh = at & 63;
+          atomicOr(reinterpret_cast<unsigned long long*>(&stg[wi]), (unsigned long long)bj << sh);
+          if (sh + cntj > 64)
+            atomicOr(reinterpret_cast<unsigned long long*>(&stg[wi + 1]), (unsigned long long)bj >> (64 - sh));
+        }
       }
     }
+    wave_sync_lds();
+    // words touched by node v's run, their prefix over the nodes
+    const uint32_t wc = cv ? (uint32_t)(((gp + cv - 1) >> 6) - (gp >> 6) + 1) : 0u;
+    const uint32_t winc = wave_incl_scan(wc);
+    const uint32_t wst = winc - wc;
+    const uint32_t tw = (uint32_t)__shfl((int)winc, 63);
+    for (uint32_t k0 = 0; k0 < tw; k0 += 64) {  // uniform loop: the shuffles run on the full wave
+      const uint32_t k = k0 + lane;
+      uint32_t v = 0;
+      for (uint32_t q = 1; q < ni; q++)
+        if ((uint32_t)__builtin_amdgcn_readlane((int)wst, (int)q) <= k) v = q;
+      const uint32_t len = (uint32_t)__shfl((int)cv, (int)v);
+      const uint64_t g0 = ((uint64_t)(uint32_t)__shfl((int)(uint32_t)(gp >> 32), (int)v) << 32) |
+                          (uint32_t)__shfl((int)(uint32_t)gp, (int)v);
+      const int64_t lsx = (int64_t)(uint32_t)__shfl((int)lsv, (int)v);
+      const uint32_t wsv = (uint32_t)__shfl((int)wst, (int)v);
+      if (k >= tw) continue;
+      const uint64_t kw = (g0 >> 6) + (k - wsv);
+      const uint64_t lo = kw * 64 > g0 ? kw * 64 : g0;
+      const uint64_t hi = kw * 64 + 64 < g0 + len ? kw * 64 + 64 : g0 + len;
+      const uint32_t nb = (uint32_t)(hi - lo), at = (uint32_t)(lo - kw * 64);
+      const uint64_t mask = nb == 64 ? ~0ull : ((1ull << nb) - 1) << at;
+      const uint64_t val = stage_bits(stg, lsx + (int64_t)(kw * 64) - (int64_t)g0) & mask;
+      if (nb == 64) a.words[kw] = val;
+      else if (val) atomicOr(reinterpret_cast<unsigned long long*>(&a.words[kw]), (unsigned long long)val);
+    }
+    wave_sync_lds();
+    cur = nxt;
   }
 }
 
@@ -803,7 +945,12 @@ extern "C" int wtg_huff_build(wtg_ctx* ctx, const uint8_t* d_text, uint64_t n, w
         rc = WTG_ERR_UNSUPPORTED;
         goto fail;
       }
-      WTG_LAUNCH(ctx, "huff_emit", k_huff_emit, dim3(grid), dim3(HF_THREADS), lds, a);
+      if (S.n_inner <= 32 && S_bytes == HF_PIECE) {
+        WTG_LAUNCH(ctx, "huff_emit", k_huff_emit_lane, dim3(grid), dim3(HF_THREADS), (size_t)4 * a.stage_words * 8,
+                   a);
+      } else {
+        WTG_LAUNCH(ctx, "huff_emit", k_huff_emit, dim3(grid), dim3(HF_THREADS), lds, a);
+      }
       WTG_LAUNCH_CHECK();
     }
     if ((rc = wtg_rank_select_run(ctx, wt->words, S.bv_size, wt->dir, true, &wt->s1, &wt->s0)) != WTG_OK)
