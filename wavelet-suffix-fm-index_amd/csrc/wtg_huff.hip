@@ -16,12 +16,15 @@
 //                    global bit position: bv_pos[v] is the sum of the inner
 //                    nodes before v, the prefix within row v the bits of v before
 //                    segment s.
-//   3. k_huff_emit   each wave walks its segment in 64-byte chunks, level by
-//                    level; per (chunk, level) the lanes are grouped by node with
-//                    ballots, each group's bits compacted into one word (lane
-//                    scatter with ds_permute + ballot) and OR-ed onto the node's
-//                    run in a per-wave LDS staging area (run starts and lengths
-//                    in per-wave LDS arrays, read as wave-uniform values); at the
+//   3. k_huff_emit   up to 32 inner nodes (k_huff_emit_lane): each lane gathers
+//                    the branch bits of its 16 bytes node by node in registers
+//                    and a wave scan per node places them in the node's run of a
+//                    per-wave LDS staging area.  Larger trees (k_huff_emit): the
+//                    wave walks its segment in 64-byte chunks, level by level;
+//                    per (chunk, level) the lanes are sorted stably by node id
+//                    with one ballot per id bit, a ds_permute puts each node's
+//                    bits into one contiguous slice of a ballot word, and one
+//                    lane per node ORs its slice onto the node's run.  At the
 //                    segment end every run is written out as whole words (edge
 //                    words OR-ed).
 // Then rank9 + select1 + select0 over the bits, and the per-node rank bases.
@@ -44,6 +47,8 @@ struct HuffTab {
   uint64_t C[257];           // C[c] = #bytes < c
   uint16_t child[256][2];    // per inner node: inner index or HF_LEAF + symbol
   uint8_t len[256];          // code length, 0 = absent
+  uint8_t lvl_first[64];     // first inner node id at depth l (ids are BFS, so a level is a range)
+  uint8_t lvl_bits[64];      // bits for an id offset within level l plus one "none" value
   uint32_t n_inner, max_depth, sigma, single;  // single: the symbol when sigma == 1
 };
 }  // namespace
@@ -127,6 +132,7 @@ struct HuffLds {
   uint64_t code[256];
   uint16_t child[256][2];
   uint8_t len[256];
+  uint8_t lvl_first[64], lvl_bits[64];
 };
 
 __device__ __forceinline__ void load_tab(HuffLds& L, const HuffTab* __restrict__ T) {
@@ -135,32 +141,10 @@ __device__ __forceinline__ void load_tab(HuffLds& L, const HuffTab* __restrict__
     L.len[k] = T->len[k];
     L.child[k][0] = T->child[k][0];
     L.child[k][1] = T->child[k][1];
-  }
-}
-
-// Walks one chunk of 64 bytes (lane = text order) down the tree.  For every
-// (level, node) group calls f(v, m, bits) with m = the group's lanes and bits =
-// ballot of their branch bits.
-template <typename F>
-__device__ __forceinline__ void walk_chunk(const HuffLds& L, uint32_t c, bool valid, F&& f) {
-  const uint32_t len = valid ? L.len[c] : 0u;
-  const uint64_t code = L.code[c];
-  uint32_t node = 0;
-  for (uint32_t l = 0;; l++) {
-    const bool act = l < len;
-    const uint64_t am = wave_ballot(act);
-    if (!am) break;
-    const uint32_t bit = (uint32_t)(code >> l) & 1u;
-    const uint64_t bb = wave_ballot(act && bit);
-    uint64_t rem = am;
-    while (rem) {
-      const int ld = __builtin_ctzll(rem);
-      const uint32_t v = (uint32_t)__builtin_amdgcn_readlane((int)node, ld);
-      const uint64_t m = (rem == am && l == 0) ? am : wave_ballot(act && node == v);
-      rem &= ~m;
-      f(v, m, bb, bit);
+    if (k < 64) {
+      L.lvl_first[k] = T->lvl_first[k];
+      L.lvl_bits[k] = T->lvl_bits[k];
     }
-    if (act) node = L.child[node][bit];   // leaves drop out at l+1 == len
   }
 }
 
@@ -396,28 +380,43 @@ __global__ __launch_bounds__(HF_THREADS) void k_huff_emit(SegArgs a) {
       for (int k = 0; k < HF_PIECE / 64; k++) {
         const uint64_t p = pbase + (uint64_t)k * 64 + lane;
         const uint32_t c = s_txt[k * 64 + lane];
-        walk_chunk(L, c, p < a.n, [&](uint32_t v, uint64_t m, uint64_t bb, uint32_t bit) {
-          const uint32_t cm = popc64(m);
-          uint64_t W;
-          if (m == ~0ull) {
-            W = bb;
-          } else {
-            // lanes of m go to their rank, the others behind them: a permutation
-            const bool in = (m >> lane) & 1ull;
-            const uint32_t dst = in ? mbcnt64(m) : cm + mbcnt64(~m);
-            const int got = __builtin_amdgcn_ds_permute((int)(dst * 4), (int)bit);
-            W = wave_ballot((uint32_t)lane < cm && got);
+        const uint32_t len = p < a.n ? L.len[c] : 0u;
+        const uint64_t code = L.code[c];
+        uint32_t node = 0;
+        for (uint32_t l = 0;; l++) {
+          const bool act = l < len;
+          if (!wave_ballot(act)) break;
+          const uint32_t bit = act ? (uint32_t)(code >> l) & 1u : 0u;
+          // stable sort of the lanes by node id (finished lanes last): eq ends
+          // as the lanes with my node, lt counts the lanes with a smaller one.
+          // Ids are taken relative to the level's first node, in its bit width.
+          const uint32_t kbits = L.lvl_bits[l];
+          const uint32_t vid = act ? node - L.lvl_first[l] : (1u << kbits) - 1;
+          uint64_t eq = ~0ull;
+          uint32_t lt = 0;
+          for (int kb = (int)kbits - 1; kb >= 0; kb--) {
+            const uint32_t mb = (vid >> kb) & 1u;
+            const uint64_t B = wave_ballot(mb != 0);
+            lt += mb ? popc64(eq & ~B) : 0u;
+            eq &= mb ? B : ~B;
           }
-          // the node's run so far (wave-uniform LDS reads); lane 0 appends
-          const uint32_t at = (uint32_t)__builtin_amdgcn_readfirstlane((int)(ls[v] + rn[v]));
-          if (lane == 0) {
-            rn[v] += cm;
+          const uint32_t rank = mbcnt64(eq);
+          // lane lt + rank receives my bit: each node's bits become one
+          // contiguous slice of W, in text order
+          const int got = __builtin_amdgcn_ds_permute((int)((lt + rank) * 4), (int)bit);
+          const uint64_t W = wave_ballot(got != 0);
+          if (act && rank == 0) {  // one lane per node appends the node's slice
+            const uint32_t g = popc64(eq);
+            const uint32_t at = ls[node] + rn[node];
+            rn[node] += g;
+            const uint64_t slice = (W >> lt) & (g == 64 ? ~0ull : ((1ull << g) - 1));
             const uint32_t wi = at >> 6, sh = at & 63;
-            atomicOr(reinterpret_cast<unsigned long long*>(&stg[wi]), (unsigned long long)(W << sh));
-            if (sh && sh + cm > 64)
-              atomicOr(reinterpret_cast<unsigned long long*>(&stg[wi + 1]), (unsigned long long)(W >> (64 - sh)));
+            atomicOr(reinterpret_cast<unsigned long long*>(&stg[wi]), (unsigned long long)(slice << sh));
+            if (sh && sh + g > 64)
+              atomicOr(reinterpret_cast<unsigned long long*>(&stg[wi + 1]), (unsigned long long)(slice >> (64 - sh)));
           }
-        });
+          if (act) node = L.child[node][bit];  // leaves drop out at l+1 == len
+        }
       }
     }
     wave_sync_lds();
@@ -791,6 +790,30 @@ void build_shape(const uint64_t* cnt, uint64_t n, wtg_huff_shape& S, HuffTab& T)
       const int ch = S.child[k][b];
       T.child[inner_of[k]][b] = (uint16_t)(inner_of[ch] >= 0 ? (uint32_t)inner_of[ch]
                                                               : HF_LEAF + (uint32_t)S.sym[ch]);
+    }
+  }
+  // per-depth id ranges of the inner nodes (a range per depth when the ids are
+  // in BFS order; otherwise every level takes the full id width from 0)
+  {
+    std::vector<int> dep(ni, 0);
+    for (size_t k = 0; k < order.size(); k++) {
+      if (inner_of[k] < 0) continue;
+      int d = 0;
+      for (int x = (int)k; S.parent[x] >= 0; x = S.parent[x]) d++;
+      dep[inner_of[k]] = d;
+    }
+    bool bfs = true;
+    for (uint32_t v = 1; v < ni; v++) bfs = bfs && dep[v] >= dep[v - 1];
+    uint32_t full = 1;
+    while ((1u << full) - 1 < ni) full++;
+    for (int l = 0; l < 64; l++) {
+      uint32_t first = 0, cnt = 0;
+      for (uint32_t v = 0; v < ni; v++)
+        if (dep[v] == l) { if (!cnt) first = v; cnt++; }
+      uint32_t kb = 1;
+      while ((1u << kb) - 1 < cnt) kb++;
+      T.lvl_first[l] = bfs ? (uint8_t)first : 0;
+      T.lvl_bits[l] = bfs ? (uint8_t)kb : (uint8_t)full;
     }
   }
   uint32_t maxd = 0;
