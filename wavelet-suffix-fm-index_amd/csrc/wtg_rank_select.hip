@@ -226,57 +226,48 @@ __global__ __launch_bounds__(256) void k_select_assemble(AsmArgs a) {
   }
 }
 
-// also copies the long-superblock count ex[sb] to *nl_out
+// also lists the long superblocks (list[ex[k]] = k) and copies their count
+// ex[sb] to *nl_out
 __global__ void k_long_index(const uint32_t* __restrict__ flag, const uint32_t* __restrict__ ex,
-                             int64_t* __restrict__ li, uint64_t sb, uint32_t* __restrict__ nl_out) {
+                             int64_t* __restrict__ li, uint64_t sb, uint32_t* __restrict__ list,
+                             uint32_t* __restrict__ nl_out) {
   const uint64_t k = (uint64_t)blockIdx.x * blockDim.x + threadIdx.x;
-  if (k < sb) li[k] = flag[k] ? (int64_t)ex[k] : -1;
+  if (k < sb) {
+    li[k] = flag[k] ? (int64_t)ex[k] : -1;
+    if (flag[k]) list[ex[k]] = (uint32_t)k;
+  }
   if (k == 0) *nl_out = ex[sb];
 }
 
-// one block of LF_THREADS per long superblock: enumerate the arguments in
-// [a, e], LF_WORDS consecutive words per thread per round (all loads issued
-// before the block scan, so a round costs one memory latency)
+// one block of LF_THREADS per long superblock (list[blockIdx.x]): enumerate the
+// arguments in [a, e], one word per thread per round (a long superblock can
+// hold dense stretches, and each thread walks its word's set bits serially)
 constexpr int LF_THREADS = 1024;
-constexpr int LF_WORDS = 4;
 __global__ __launch_bounds__(LF_THREADS) void k_long_fill(const uint64_t* __restrict__ words, uint64_t nbits,
                                                           int bit, const uint64_t* __restrict__ sbstart,
                                                           const uint64_t* __restrict__ sbend,
                                                           const unsigned long long* last,
-                                                          const uint32_t* __restrict__ flag,
-                                                          const uint32_t* __restrict__ ex,
-                                                          uint64_t sb, uint64_t* __restrict__ pool) {
+                                                          const uint32_t* __restrict__ list, uint64_t sb,
+                                                          uint64_t* __restrict__ pool) {
   constexpr int NWV = LF_THREADS / 64;
   __shared__ uint32_t s_wave[NWV];
-  const uint64_t k = blockIdx.x;
-  if (k >= sb || !flag[k]) return;
+  const uint64_t k = list[blockIdx.x];
   const uint64_t st = sbstart[k];
   const uint64_t en = (k + 1 < sb) ? sbend[k] : (uint64_t)(*last) - 1;
-  uint64_t* out = pool + (uint64_t)ex[k] * 4096;
+  uint64_t* out = pool + (uint64_t)blockIdx.x * 4096;
   uint32_t run = 0;
   const uint64_t w0 = st >> 6, w1 = en >> 6;
-  const uint64_t nwords = (nbits + 63) >> 6;
-  for (uint64_t base = w0; base <= w1; base += (uint64_t)LF_THREADS * LF_WORDS) {
-    const uint64_t wb = base + (uint64_t)threadIdx.x * LF_WORDS;
-    uint64_t x[LF_WORDS];
-#pragma unroll
-    for (int q = 0; q < LF_WORDS; q++) {
-      const uint64_t wi = wb + q;
-      x[q] = (wi <= w1 && wi < nwords) ? words[wi] : (bit ? 0ull : ~0ull);
-    }
-    uint32_t c = 0;
-#pragma unroll
-    for (int q = 0; q < LF_WORDS; q++) {
-      const uint64_t wi = wb + q;
-      uint64_t v = bit ? x[q] : ~x[q];
+  for (uint64_t base = w0; base <= w1; base += LF_THREADS) {
+    const uint64_t wi = base + threadIdx.x;
+    uint64_t x = 0;
+    if (wi <= w1) {
+      x = bit ? words[wi] : ~words[wi];
       const uint64_t lo = wi * 64;
-      if (wi > w1) v = 0;
-      if (lo < st) v = (st - lo >= 64) ? 0ull : (v & (~0ull << (st - lo)));
-      if (lo + 63 > en) v &= (lo > en) ? 0ull : ((en - lo == 63) ? ~0ull : ((1ull << (en - lo + 1)) - 1));
-      if (lo + 64 > nbits) v &= (nbits > lo) ? ((nbits - lo >= 64) ? ~0ull : ((1ull << (nbits - lo)) - 1)) : 0ull;
-      x[q] = v;
-      c += popc64(v);
+      if (lo < st) x &= ~0ull << (st - lo);
+      if (lo + 63 > en) x &= (en - lo == 63) ? ~0ull : ((1ull << (en - lo + 1)) - 1);
+      if (lo + 64 > nbits) x &= (nbits > lo) ? ((nbits - lo >= 64) ? ~0ull : ((1ull << (nbits - lo)) - 1)) : 0ull;
     }
+    const uint32_t c = popc64(x);
     const int w = threadIdx.x >> 6, l = lane_id();
     const uint32_t incl = wave_incl_scan(c);
     if (l == 63) s_wave[w] = incl;
@@ -286,14 +277,10 @@ __global__ __launch_bounds__(LF_THREADS) void k_long_fill(const uint64_t* __rest
     for (int q = 0; q < NWV; q++) { const uint32_t t = s_wave[q]; if (q < w) off += t; tt += t; }
     __syncthreads();
     uint32_t r = run + incl - c + off;
-#pragma unroll
-    for (int q = 0; q < LF_WORDS; q++) {
-      uint64_t v = x[q];
-      while (v) {
-        if (r < 4096) out[r] = (wb + q) * 64 + (uint64_t)__builtin_ctzll(v);
-        r++;
-        v &= v - 1;
-      }
+    while (x) {
+      if (r < 4096) out[r] = wi * 64 + (uint64_t)__builtin_ctzll(x);
+      r++;
+      x &= x - 1;
     }
     run += tt;
   }
@@ -344,6 +331,25 @@ int sel_alloc_worst(wtg_ctx* ctx, uint64_t nbits, int bit, wtg_select* s) {
   return WTG_OK;
 }
 
+// scratch of one select structure's finish: long flags, their exclusive scan,
+// the scan's tile states and counter, and the list of long superblocks
+struct FinScratch {
+  uint32_t* flag;
+  uint32_t* ex;
+  uint64_t* st;
+  uint32_t* ctr;
+  uint32_t* list;
+};
+FinScratch fin_scratch(char* scratch, uint64_t sb) {
+  FinScratch f;
+  f.flag = (uint32_t*)scratch;
+  f.ex = (uint32_t*)(scratch + wtg_align256(sb * 4));
+  f.st = (uint64_t*)(scratch + wtg_align256(sb * 4) + wtg_align256((sb + 1) * 4));
+  f.ctr = (uint32_t*)((char*)f.st + wtg_align256((wtg_ceil_div(sb, 4096) + 1) * 8));
+  f.list = (uint32_t*)((char*)f.ctr + 256);
+  return f;
+}
+
 // finish one select structure once the arguments are counted, in two halves
 // so that both structures share one host read of their long-superblock
 // counts: sel_count stages the widths, the long flags and their scan, and
@@ -362,10 +368,11 @@ int sel_count(wtg_ctx* ctx, uint64_t nbits, wtg_select* s, uint64_t m, const uin
   WTG_TRY(wtg_malloc(ctx, (void**)&s->long_width, sb ? sb : 1));
   WTG_TRY(wtg_malloc(ctx, (void**)&s->long_index, (sb ? sb : 1) * 8));
   if (sb == 0) return wtg_fill_bytes_zero(ctx, d_nl, 4);
-  uint32_t* flag = (uint32_t*)scratch;
-  uint32_t* ex = (uint32_t*)(scratch + wtg_align256(sb * 4));
-  uint64_t* st = (uint64_t*)(scratch + wtg_align256(sb * 4) + wtg_align256((sb + 1) * 4));
-  uint32_t* ctr = (uint32_t*)((char*)st + wtg_align256((wtg_ceil_div(sb, 4096) + 1) * 8));
+  const FinScratch f = fin_scratch(scratch, sb);
+  uint32_t* flag = f.flag;
+  uint32_t* ex = f.ex;
+  uint64_t* st = f.st;
+  uint32_t* ctr = f.ctr;
   WTG_TRY(wtg_fill_bytes_zero(ctx, st, (char*)ctr + 256 - (char*)st));
   AsmArgs aa{};
   aa.mini = s->mini; aa.mini_width = s->mini_width; aa.long_width = s->long_width;
@@ -378,7 +385,7 @@ int sel_count(wtg_ctx* ctx, uint64_t nbits, wtg_select* s, uint64_t m, const uin
              flag, ex, sb, st, ctr, ctx->d_flag);
   WTG_LAUNCH_CHECK();
   WTG_LAUNCH(ctx, "select_longidx", k_long_index, dim3((unsigned)wtg_ceil_div(sb, 256)), dim3(256), 0,
-             flag, ex, s->long_index, sb, d_nl);
+             flag, ex, s->long_index, sb, f.list, d_nl);
   WTG_LAUNCH_CHECK();
   return WTG_OK;
 }
@@ -388,12 +395,11 @@ int sel_fill(wtg_ctx* ctx, const uint64_t* d_words, uint64_t nbits, wtg_select* 
   const uint64_t sb = s->sb;
   s->n_long = nl;
   if (sb == 0 || nl == 0) return WTG_OK;
-  const uint32_t* flag = (const uint32_t*)scratch;
-  const uint32_t* ex = (const uint32_t*)(scratch + wtg_align256(sb * 4));
+  const FinScratch f = fin_scratch(const_cast<char*>(scratch), sb);
   WTG_TRY(wtg_malloc(ctx, (void**)&s->long_pool, (uint64_t)nl * 4096 * 8));
   WTG_TRY(wtg_fill_u64(ctx, s->long_pool, (uint64_t)nl * 4096, 0));
-  WTG_LAUNCH(ctx, "select_longfill", k_long_fill, dim3((unsigned)sb), dim3(LF_THREADS), 0, d_words, nbits,
-             s->bit, s->superblock, sbend, last, flag, ex, sb, s->long_pool);
+  WTG_LAUNCH(ctx, "select_longfill", k_long_fill, dim3(nl), dim3(LF_THREADS), 0, d_words, nbits, s->bit,
+             s->superblock, sbend, last, f.list, sb, s->long_pool);
   WTG_LAUNCH_CHECK();
   return WTG_OK;
 }
