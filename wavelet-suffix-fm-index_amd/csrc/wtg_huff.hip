@@ -976,27 +976,39 @@ extern "C" int wtg_huff_build(wtg_ctx* ctx, const uint8_t* d_text, uint64_t n, w
       }
       WTG_LAUNCH_CHECK();
     }
-    if ((rc = wtg_rank_select_run(ctx, wt->words, S.bv_size, wt->dir, true, &wt->s1, &wt->s0)) != WTG_OK)
+    // the ones of the tree bits: each byte contributes the ones of its code
+    uint64_t ones = 0;
+    for (int c = 0; c < 256; c++) ones += hist[c] * (uint64_t)__builtin_popcountll(wt->tab.code[c]);
+    wtg_rs_pending pend;
+    if ((rc = wtg_rank_select_begin(ctx, wt->words, S.bv_size, wt->dir, true, &wt->s1, &wt->s0, ones, &pend)) !=
+        WTG_OK)
       goto fail;
+    // one host read for the long-superblock counts and the per-node rank bases
+    uint32_t nl[2] = {0, 0};
+    uint64_t rb[256];
     if (S.n_inner > 0) {
       WTG_LAUNCH(ctx, "huff_rbase", k_huff_rbase, dim3(1), dim3(256), 0, wt->d_tab, wt->dir, wt->words);
       WTG_LAUNCH_CHECK();
-      uint64_t rb[256];
-      if (hipMemcpyAsync(rb, wt->d_tab->rbase, sizeof(rb), hipMemcpyDeviceToHost, ctx->stream) != hipSuccess ||
-          hipStreamSynchronize(ctx->stream) != hipSuccess) {
-        wtg_set_error("wtg_huff_build: rank base copy failed");
-        rc = WTG_ERR_HIP;
-        goto fail;
-      }
-      std::memcpy(wt->tab.rbase, rb, sizeof(rb));
-      uint32_t ni = 0;
-      for (uint32_t v = 0; v < wt->shape.n_nodes; v++)
-        if (wt->shape.child[v][0] >= 0) wt->shape.bv_pos_rank[v] = rb[ni++];
     } else if (hipMemcpyAsync(wt->d_tab, &wt->tab, sizeof(HuffTab), hipMemcpyHostToDevice, ctx->stream) !=
                hipSuccess) {
       wtg_set_error("wtg_huff_build: table copy failed");
       rc = WTG_ERR_HIP;
       goto fail;
+    }
+    if ((pend.active && hipMemcpyAsync(nl, pend.d_nl, 8, hipMemcpyDeviceToHost, ctx->stream) != hipSuccess) ||
+        (S.n_inner > 0 &&
+         hipMemcpyAsync(rb, wt->d_tab->rbase, sizeof(rb), hipMemcpyDeviceToHost, ctx->stream) != hipSuccess) ||
+        hipStreamSynchronize(ctx->stream) != hipSuccess) {
+      wtg_set_error("wtg_huff_build: rank base copy failed");
+      rc = WTG_ERR_HIP;
+      goto fail;
+    }
+    if ((rc = wtg_rank_select_end(ctx, &pend, nl)) != WTG_OK) goto fail;
+    if (S.n_inner > 0) {
+      std::memcpy(wt->tab.rbase, rb, sizeof(rb));
+      uint32_t ni = 0;
+      for (uint32_t v = 0; v < wt->shape.n_nodes; v++)
+        if (wt->shape.child[v][0] >= 0) wt->shape.bv_pos_rank[v] = rb[ni++];
     }
   }
   if ((rc = wtg_finish(ctx)) != WTG_OK) goto fail;

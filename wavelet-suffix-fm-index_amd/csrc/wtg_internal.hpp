@@ -84,6 +84,25 @@ int wtg_levels_build_from(wtg_ctx* ctx, const void* d_keys, int key_bytes, uint6
 size_t wtg_levels_ws_bytes(uint64_t n, uint32_t levels, int key_bytes);
 int wtg_rank_select_run(wtg_ctx* ctx, const uint64_t* d_words, uint64_t nbits, uint64_t* d_dir,
                         bool build, wtg_select* sel1, wtg_select* sel0);
+// The same in two halves, for callers that fold the long-superblock counts'
+// host read into a read of their own: begin enqueues everything up to the two
+// counts (at pend->d_nl; the workspace must stay untouched until end), end
+// allocates and fills the long pools.  known_ones (build only): the number of
+// ones when the caller knows it, which saves a host read.
+constexpr uint64_t WTG_ONES_UNKNOWN = ~0ull;
+struct wtg_rs_pending {
+  bool active;
+  const uint64_t* d_words;
+  uint64_t nbits;
+  const uint32_t* d_nl;
+  wtg_select* outs[2];
+  const uint64_t* sbend[2];
+  const unsigned long long* last[2];
+  const char* fin[2];
+};
+int wtg_rank_select_begin(wtg_ctx* ctx, const uint64_t* d_words, uint64_t nbits, uint64_t* d_dir, bool build,
+                          wtg_select* sel1, wtg_select* sel0, uint64_t known_ones, wtg_rs_pending* pend);
+int wtg_rank_select_end(wtg_ctx* ctx, const wtg_rs_pending* pend, const uint32_t nl[2]);
 int wtg_max_value(wtg_ctx* ctx, const void* d_sym, int sym_bytes, uint64_t n, uint64_t* out);
 int wtg_fill_bytes_zero(wtg_ctx* ctx, void* d, uint64_t bytes);
 

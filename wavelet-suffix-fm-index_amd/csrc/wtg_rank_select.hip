@@ -406,8 +406,9 @@ int sel_fill(wtg_ctx* ctx, const uint64_t* d_words, uint64_t nbits, wtg_select* 
 
 }  // namespace
 
-int wtg_rank_select_run(wtg_ctx* ctx, const uint64_t* d_words, uint64_t nbits, uint64_t* d_dir,
-                        bool build, wtg_select* sel1, wtg_select* sel0) {
+int wtg_rank_select_begin(wtg_ctx* ctx, const uint64_t* d_words, uint64_t nbits, uint64_t* d_dir, bool build,
+                          wtg_select* sel1, wtg_select* sel0, uint64_t known_ones, wtg_rs_pending* pend) {
+  *pend = wtg_rs_pending{};
   const uint64_t cap = (nbits + 63) / 64 * 64;
   const uint64_t W = cap >> 6, S = cap >> 9;
   const uint64_t entries = S + 1;
@@ -480,7 +481,9 @@ int wtg_rank_select_run(wtg_ctx* ctx, const uint64_t* d_words, uint64_t nbits, u
   }
   WTG_LAUNCH_CHECK();
   uint64_t m1 = 0;
-  if (build) {
+  if (build && known_ones != WTG_ONES_UNKNOWN) {
+    m1 = known_ones;
+  } else if (build) {
     unsigned long long t = 0;
     WTG_CHECK_HIP(hipMemcpyAsync(&t, ones_total, 8, hipMemcpyDeviceToHost, ctx->stream));
     WTG_CHECK_HIP(hipStreamSynchronize(ctx->stream));
@@ -503,13 +506,37 @@ int wtg_rank_select_run(wtg_ctx* ctx, const uint64_t* d_words, uint64_t nbits, u
     if (ra.want[t])
       WTG_TRY(sel_count(ctx, nbits, outs[t], m[t], ra.sel[t].sbend, last + t, ws + o_fin + t * fin_bytes, d_nl + t));
   if (!ra.want[0] && !ra.want[1]) return WTG_OK;
-  uint32_t nl[2] = {0, 0};
-  WTG_CHECK_HIP(hipMemcpyAsync(nl, d_nl, 8, hipMemcpyDeviceToHost, ctx->stream));
-  WTG_CHECK_HIP(hipStreamSynchronize(ctx->stream));
-  for (int t = 0; t < 2; t++)
-    if (ra.want[t])
-      WTG_TRY(sel_fill(ctx, d_words, nbits, outs[t], nl[t], ra.sel[t].sbend, last + t, ws + o_fin + t * fin_bytes));
+  pend->active = true;
+  pend->d_words = d_words;
+  pend->nbits = nbits;
+  pend->d_nl = d_nl;
+  for (int t = 0; t < 2; t++) {
+    pend->outs[t] = ra.want[t] ? outs[t] : nullptr;
+    pend->sbend[t] = ra.sel[t].sbend;
+    pend->last[t] = last + t;
+    pend->fin[t] = ws + o_fin + t * fin_bytes;
+  }
   return WTG_OK;
+}
+
+int wtg_rank_select_end(wtg_ctx* ctx, const wtg_rs_pending* pend, const uint32_t nl[2]) {
+  if (!pend->active) return WTG_OK;
+  for (int t = 0; t < 2; t++)
+    if (pend->outs[t])
+      WTG_TRY(sel_fill(ctx, pend->d_words, pend->nbits, pend->outs[t], nl[t], pend->sbend[t], pend->last[t],
+                       pend->fin[t]));
+  return WTG_OK;
+}
+
+int wtg_rank_select_run(wtg_ctx* ctx, const uint64_t* d_words, uint64_t nbits, uint64_t* d_dir, bool build,
+                        wtg_select* sel1, wtg_select* sel0) {
+  wtg_rs_pending pend;
+  WTG_TRY(wtg_rank_select_begin(ctx, d_words, nbits, d_dir, build, sel1, sel0, WTG_ONES_UNKNOWN, &pend));
+  if (!pend.active) return WTG_OK;
+  uint32_t nl[2] = {0, 0};
+  WTG_CHECK_HIP(hipMemcpyAsync(nl, pend.d_nl, 8, hipMemcpyDeviceToHost, ctx->stream));
+  WTG_CHECK_HIP(hipStreamSynchronize(ctx->stream));
+  return wtg_rank_select_end(ctx, &pend, nl);
 }
 
 extern "C" int wtg_rank9_build(wtg_ctx* ctx, const uint64_t* d_words, uint64_t nbits, uint64_t* d_dir) {
