@@ -61,6 +61,9 @@ struct wtg_huff {
   uint64_t* dir = nullptr;
   wtg_select s1{}, s0{};
   uint64_t* d_hist = nullptr;  // 256 counts
+  // query copy of the bits: 64-byte blocks of (rank1 at the block start, 7 words),
+  // block b covering bits [448b, 448b + 448), so a rank reads one 64-byte sector
+  uint64_t* fmr = nullptr;
 };
 
 // ------------------------------------------------------------------ kernels
@@ -556,6 +559,21 @@ __global__ __launch_bounds__(HF_THREADS) void k_huff_emit_lane(SegArgs a) {
   }
 }
 
+constexpr uint64_t FMR_BITS = 448;
+// one lane per output word: slot 0 of block b = rank1(448 b), slots 1..7 = words 7b..7b+6
+__global__ void k_huff_fmr(const uint64_t* __restrict__ D, const uint64_t* __restrict__ words, uint64_t nw,
+                           uint64_t nblocks, uint64_t* __restrict__ fmr) {
+  const uint64_t k = (uint64_t)blockIdx.x * blockDim.x + threadIdx.x;
+  if (k >= nblocks * 8) return;
+  const uint64_t b = k >> 3, slot = k & 7;
+  if (slot == 0) {
+    fmr[k] = rank9_dev(D, words, b * FMR_BITS);
+  } else {
+    const uint64_t wi = 7 * b + slot - 1;
+    fmr[k] = wi < nw ? words[wi] : 0ull;
+  }
+}
+
 __global__ void k_huff_rbase(HuffTab* tab, const uint64_t* __restrict__ D, const uint64_t* __restrict__ words) {
   const uint32_t v = blockIdx.x * blockDim.x + threadIdx.x;
   if (v < tab->n_inner) tab->rbase[v] = rank9_dev(D, words, tab->bvpos[v]);
@@ -584,10 +602,31 @@ struct HQ {
   const HuffTab* tab;
   const uint64_t* words;
   const uint64_t* D;
+  const uint64_t* fmr;
   SelQ s1, s0;
   uint64_t n;
   uint32_t sigma, single;
 };
+
+// rank1(i) of the tree bits from the 64-byte query blocks (i <= bv_size)
+__device__ __forceinline__ uint64_t fmr_rank(const uint64_t* __restrict__ fmr, uint64_t i) {
+  const uint64_t b = i / FMR_BITS;
+  const uint32_t r = (uint32_t)(i - b * FMR_BITS);
+  const uint4* blk = reinterpret_cast<const uint4*>(fmr + 8 * b);
+  const uint4 q0 = blk[0], q1 = blk[1], q2 = blk[2], q3 = blk[3];
+  const uint64_t w[8] = {((uint64_t)q0.y << 32) | q0.x, ((uint64_t)q0.w << 32) | q0.z,
+                         ((uint64_t)q1.y << 32) | q1.x, ((uint64_t)q1.w << 32) | q1.z,
+                         ((uint64_t)q2.y << 32) | q2.x, ((uint64_t)q2.w << 32) | q2.z,
+                         ((uint64_t)q3.y << 32) | q3.x, ((uint64_t)q3.w << 32) | q3.z};
+  uint64_t cnt = w[0];
+  const uint32_t wq = r >> 6, sh = r & 63;
+#pragma unroll
+  for (uint32_t q = 0; q < 7; q++) {
+    const uint64_t m = q < wq ? ~0ull : (q == wq ? ((1ull << sh) - 1) : 0ull);
+    cnt += popc64(w[1 + q] & m);
+  }
+  return cnt;
+}
 
 // rank_c(i) by walking c's code (wt_pc_par.hpp:462-485)
 __device__ __forceinline__ uint64_t huff_rank(const QLds& Q, const HQ& h, uint64_t i, uint32_t c) {
@@ -596,7 +635,7 @@ __device__ __forceinline__ uint64_t huff_rank(const QLds& Q, const HQ& h, uint64
   uint64_t code = Q.code[c];
   uint32_t v = 0;
   for (uint32_t l = 0; l < len && i; l++, code >>= 1) {
-    const uint64_t r = rank9_dev(h.D, h.words, Q.bvpos[v] + i) - Q.rbase[v];
+    const uint64_t r = fmr_rank(h.fmr, Q.bvpos[v] + i) - Q.rbase[v];
     const uint32_t b = (uint32_t)code & 1u;
     i = b ? r : i - r;
     v = Q.child[v][b];
@@ -637,7 +676,7 @@ __global__ __launch_bounds__(256) void k_huff_access(HQ h, const uint64_t* __res
         uint32_t v = 0;
         for (;;) {
           const uint64_t p = Q.bvpos[v] + i;
-          const uint64_t r = rank9_dev(h.D, h.words, p) - Q.rbase[v];
+          const uint64_t r = fmr_rank(h.fmr, p) - Q.rbase[v];
           const uint32_t b = (uint32_t)((h.words[p >> 6] >> (p & 63)) & 1ull);
           i = b ? r : i - r;
           const uint32_t nx = Q.child[v][b];
@@ -709,8 +748,8 @@ __global__ __launch_bounds__(256) void k_fm_count(HQ h, const uint8_t* __restric
       uint32_t v = 0;
       for (uint32_t l = 0; l < len && a < b; l++, code >>= 1) {
         const uint64_t bp = Q.bvpos[v], rb = Q.rbase[v];
-        const uint64_t ra = rank9_dev(h.D, h.words, bp + a) - rb;
-        const uint64_t rbv = rank9_dev(h.D, h.words, bp + b) - rb;
+        const uint64_t ra = fmr_rank(h.fmr, bp + a) - rb;
+        const uint64_t rbv = fmr_rank(h.fmr, bp + b) - rb;
         const uint32_t bit = (uint32_t)code & 1u;
         if (bit) { a = ra; b = rbv; } else { a -= ra; b -= rbv; }
         v = Q.child[v][bit];
@@ -844,6 +883,7 @@ HQ make_hq(const wtg_huff* wt) {
   h.tab = wt->d_tab;
   h.words = wt->words;
   h.D = wt->dir;
+  h.fmr = wt->fmr;
   const uint64_t W = (wt->shape.bv_size + 63) / 64 + 1;
   h.s1 = SelQ{wt->s1.superblock, wt->s1.mini, wt->s1.long_index, wt->s1.long_pool, wt->words, W, 1};
   h.s0 = SelQ{wt->s0.superblock, wt->s0.mini, wt->s0.long_index, wt->s0.long_pool, wt->words, W, 0};
@@ -866,6 +906,7 @@ extern "C" int wtg_huff_destroy(wtg_ctx* ctx, wtg_huff* wt) {
   wtg_free(ctx, wt->words);
   wtg_free(ctx, wt->dir);
   wtg_free(ctx, wt->d_hist);
+  wtg_free(ctx, wt->fmr);
   wtg_select_free(ctx, &wt->s1);
   wtg_select_free(ctx, &wt->s0);
   delete wt;
@@ -983,6 +1024,14 @@ extern "C" int wtg_huff_build(wtg_ctx* ctx, const uint8_t* d_text, uint64_t n, w
     if ((rc = wtg_rank_select_begin(ctx, wt->words, S.bv_size, wt->dir, true, &wt->s1, &wt->s0, ones, &pend)) !=
         WTG_OK)
       goto fail;
+    // the query blocks (rank9 directory done)
+    if (S.bv_size > 0) {
+      const uint64_t nblocks = S.bv_size / FMR_BITS + 1;
+      if ((rc = wtg_malloc(ctx, (void**)&wt->fmr, nblocks * 64)) != WTG_OK) goto fail;
+      WTG_LAUNCH(ctx, "huff_fmr", k_huff_fmr, dim3((unsigned)wtg_ceil_div(nblocks * 8, 256)), dim3(256), 0,
+                 wt->dir, wt->words, nw, nblocks, wt->fmr);
+      WTG_LAUNCH_CHECK();
+    }
     // one host read for the long-superblock counts and the per-node rank bases
     uint32_t nl[2] = {0, 0};
     uint64_t rb[256];
