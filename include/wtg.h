@@ -167,6 +167,23 @@ int wtg_wtint_select_batch(wtg_ctx* ctx, const wtg_wtint* wt, const uint64_t* d_
 /* inverse_select(i) = (rank(i, wt[i]), wt[i])  (wt_int_par.hpp:588-611) */
 int wtg_wtint_inverse_select_batch(wtg_ctx* ctx, const wtg_wtint* wt, const uint64_t* d_i,
                                    uint64_t nq, uint64_t* d_rank, uint64_t* d_sym);
+/* lex_count(i,j,c) = (rank(i,c), #values < c in [i,j), #values > c in [i,j))
+ * (wt_int_par.hpp:712-741; c >= 2^L gives (0, j-i, 0)).  i > j or j > size: WTG_NONE. */
+int wtg_wtint_lex_count_batch(wtg_ctx* ctx, const wtg_wtint* wt, const uint64_t* d_i,
+                              const uint64_t* d_j, const uint64_t* d_c, uint64_t nq,
+                              uint64_t* d_rank, uint64_t* d_smaller, uint64_t* d_greater);
+/* lex_smaller_count(i,c) = (rank(i,c), #values < c in [0,i))  (wt_int_par.hpp:753-780).
+ * i > size: WTG_NONE. */
+int wtg_wtint_lex_smaller_count_batch(wtg_ctx* ctx, const wtg_wtint* wt, const uint64_t* d_i,
+                                      const uint64_t* d_c, uint64_t nq, uint64_t* d_rank,
+                                      uint64_t* d_smaller);
+/* range_search_2d(lb, rb, vlb, vrb, report=false).first: #positions in [lb,rb] with
+ * values in [vlb,vrb]  (wt_int_par.hpp:801-870).  As the reference: vrb is clamped
+ * to 2^L, vlb > vrb or lb > rb gives 0, and vlb = 2^L counts the value 2^L-1 (its
+ * leaf test does not compare vlb).  rb >= size: WTG_NONE. */
+int wtg_wtint_range_count_batch(wtg_ctx* ctx, const wtg_wtint* wt, const uint64_t* d_lb,
+                                const uint64_t* d_rb, const uint64_t* d_vlb,
+                                const uint64_t* d_vrb, uint64_t nq, uint64_t* d_out);
 
 /* ------------------------------------- wt_huff (Huffman-shaped WT over bytes) */
 /* Tree shape of sdsl::wt_huff<> (wt_huff/wt_huff_par.hpp:92-117: Huffman merge of

@@ -72,6 +72,12 @@ def lib():
         L.orc_select_brute.argtypes = [vp, u64, i32, u64]
         L.orc_wtint_rank.restype = u64
         L.orc_wtint_rank.argtypes = [vp, vp, u64, u32, u64, u64]
+        L.orc_wtint_lex_count.restype = None
+        L.orc_wtint_lex_count.argtypes = [vp, vp, u64, u32, u64, u64, u64, vp]
+        L.orc_wtint_lex_smaller.restype = None
+        L.orc_wtint_lex_smaller.argtypes = [vp, vp, u64, u32, u64, u64, vp]
+        L.orc_wtint_range_count.restype = u64
+        L.orc_wtint_range_count.argtypes = [vp, vp, u64, u32, u64, u64, u64, u64]
         L.orc_wtint_access.restype = u64
         L.orc_wtint_access.argtypes = [vp, vp, u64, u32, u64]
         L.orc_wtint_select.restype = u64
@@ -227,6 +233,20 @@ class WtInt:
     def select(self, i, c):
         return lib().orc_wtint_select(_p(self.D), _p(self.wp), C.byref(self.S1.S),
                                       C.byref(self.S0.S), self.n, self.L, int(i), int(c))
+
+    def lex_count(self, i, j, c):
+        out = np.zeros(3, np.uint64)
+        lib().orc_wtint_lex_count(_p(self.D), _p(self.wp), self.n, self.L, int(i), int(j), int(c), _p(out))
+        return tuple(int(x) for x in out)
+
+    def lex_smaller_count(self, i, c):
+        out = np.zeros(2, np.uint64)
+        lib().orc_wtint_lex_smaller(_p(self.D), _p(self.wp), self.n, self.L, int(i), int(c), _p(out))
+        return tuple(int(x) for x in out)
+
+    def range_count(self, lb, rb, vlb, vrb):
+        return lib().orc_wtint_range_count(_p(self.D), _p(self.wp), self.n, self.L, int(lb), int(rb), int(vlb),
+                                           int(vrb))
 
 
 class Huff:

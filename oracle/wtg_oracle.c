@@ -15,7 +15,9 @@
  *   orc_select_*       -- brute-force position of the i-th argument; structure
  *                         content cross-checked against an independent Python
  *                         restatement of init_slow in the tests.
- *   orc_wtint_*        -- brute force on the text (count / i-th occurrence / value).
+ *   orc_wtint_*        -- brute force on the text (count / i-th occurrence / value;
+ *                         lex counts: values below / above c in [i, j); range
+ *                         count: positions in [lb, rb] with values in [vlb, vrb]).
  *   orc_huff_*         -- answers by brute force on the text; the BFS node order
  *                         of sdsl byte_tree is "parity unpinned" (sdsl absent).
  *   orc_fm_count       -- brute-force substring count and SA binary search.
@@ -406,6 +408,81 @@ uint64_t orc_wtint_select(const uint64_t* D, const uint64_t* words, const orc_se
     mask <<= 1;
   }
   return i - 1;
+}
+
+void orc_wtint_lex_count(const uint64_t* D, const uint64_t* words, uint64_t n, uint32_t L, uint64_t i,
+                         uint64_t j, uint64_t c, uint64_t* out3) {
+  /* wt_int/wt_int_par.hpp:712-741: (rank(i,c), #smaller, #greater in [i,j)) */
+  if (L < 64 && (1ull << L) <= c) { out3[0] = 0; out3[1] = j - i; out3[2] = 0; return; }
+  uint64_t offset = 0, smaller = 0, greater = 0, mask = 1ull << (L - 1), node_size = n;
+  for (uint32_t k = 0; k < L; k++) {
+    uint64_t o = R9(offset);
+    uint64_t oi = R9(offset + i) - o;
+    uint64_t oj = R9(offset + j) - o;
+    uint64_t oe = R9(offset + node_size) - o;
+    if (c & mask) {
+      offset += node_size - oe;
+      node_size = oe;
+      smaller += j - i - oj + oi;
+      i = oi;
+      j = oj;
+    } else {
+      node_size -= oe;
+      greater += oj - oi;
+      i -= oi;
+      j -= oj;
+    }
+    offset += n;
+    mask >>= 1;
+  }
+  out3[0] = i; out3[1] = smaller; out3[2] = greater;
+}
+void orc_wtint_lex_smaller(const uint64_t* D, const uint64_t* words, uint64_t n, uint32_t L, uint64_t i,
+                           uint64_t c, uint64_t* out2) {
+  /* wt_int/wt_int_par.hpp:753-780: (rank(i,c), #smaller in [0,i)) */
+  if (L < 64 && (1ull << L) <= c) { out2[0] = 0; out2[1] = i; return; }
+  uint64_t offset = 0, result = 0, mask = 1ull << (L - 1), node_size = n;
+  for (uint32_t k = 0; k < L && i; k++) {
+    uint64_t o = R9(offset);
+    uint64_t oi = R9(offset + i) - o;
+    uint64_t oe = R9(offset + node_size) - o;
+    if (c & mask) { offset += node_size - oe; node_size = oe; result += i - oi; i = oi; }
+    else { node_size -= oe; i -= oi; }
+    offset += n;
+    mask >>= 1;
+  }
+  out2[0] = i; out2[1] = result;
+}
+/* wt_int/wt_int_par.hpp:816-870 (_range_search_2d, report = false): count of the
+ * positions in [lb, rb] of the node at (level, ilb) whose values lie in [vlb, vrb] */
+static void range2d(const uint64_t* D, const uint64_t* words, uint64_t n, uint32_t L, uint64_t lb, uint64_t rb,
+                    uint64_t vlb, uint64_t vrb, uint32_t level, uint64_t ilb, uint64_t node_size, uint64_t offset,
+                    uint64_t* cnt) {
+  if (lb > rb) return;
+  if (level == L) { *cnt += rb - lb + 1; return; }
+  uint64_t irb = ilb + (1ull << (L - level));
+  uint64_t mid = (irb + ilb) >> 1;
+  uint64_t o = R9(offset), olb = R9(offset + lb), orb = R9(offset + rb + 1), oe = R9(offset + node_size);
+  uint64_t zo = offset - o, zlb = offset + lb - olb, zrb = offset + rb + 1 - orb, ze = offset + node_size - oe;
+  if (vlb < mid && mid) {
+    uint64_t nlb = zlb - zo, nrb = zrb - zo;
+    if (nrb) range2d(D, words, n, L, nlb, nrb - 1, vlb, vrb < mid - 1 ? vrb : mid - 1, level + 1, ilb, ze - zo,
+                     offset + n, cnt);
+  }
+  if (vrb >= mid) {
+    uint64_t nlb = olb - o, nrb = orb - o;
+    if (nrb) range2d(D, words, n, L, nlb, nrb - 1, mid > vlb ? mid : vlb, vrb, level + 1, mid, oe - o,
+                     offset + n + (ze - zo), cnt);
+  }
+}
+uint64_t orc_wtint_range_count(const uint64_t* D, const uint64_t* words, uint64_t n, uint32_t L, uint64_t lb,
+                               uint64_t rb, uint64_t vlb, uint64_t vrb) {
+  /* wt_int/wt_int_par.hpp:801-814 */
+  if (L < 64 && vrb > (1ull << L)) vrb = 1ull << L;
+  if (vlb > vrb) return 0;
+  uint64_t cnt = 0;
+  range2d(D, words, n, L, lb, rb, vlb, vrb, 0, 0, n, 0, &cnt);
+  return cnt;
 }
 
 /* --------------------------------------------------------------- Huffman */

@@ -73,6 +73,12 @@ uint64_t orc_wtint_access(const uint64_t* D, const uint64_t* words, uint64_t n, 
 uint64_t orc_wtint_select(const uint64_t* D, const uint64_t* words,
                           const orc_select* S1, const orc_select* S0,
                           uint64_t n, uint32_t L, uint64_t i, uint64_t c);
+void orc_wtint_lex_count(const uint64_t* D, const uint64_t* words, uint64_t n, uint32_t L,
+                         uint64_t i, uint64_t j, uint64_t c, uint64_t* out3);
+void orc_wtint_lex_smaller(const uint64_t* D, const uint64_t* words, uint64_t n, uint32_t L,
+                           uint64_t i, uint64_t c, uint64_t* out2);
+uint64_t orc_wtint_range_count(const uint64_t* D, const uint64_t* words, uint64_t n, uint32_t L,
+                               uint64_t lb, uint64_t rb, uint64_t vlb, uint64_t vrb);
 
 /* Huffman-shaped wavelet tree over a byte text (wt_huff/wt_huff_par.hpp:92-117,
  * wt_huff/wt_pc_par.hpp:193-257).  Node order is BFS from the root, child[0]

@@ -91,6 +91,28 @@ int main(int argc, char** argv) {
       res.push_back(sel);
     }
     dump(out + "/wtint.bin", res.data(), res.size() * 8);
+    // lex_count / lex_smaller_count / range_search_2d (count and reported points)
+    std::vector<uint64_t> lx;
+    for (uint64_t k = 0; k < 200; k++) {
+      const uint64_t i = (k * 7919) % n, j = i + (k * 104729) % (n - i + 1);
+      const uint64_t c = (k % 9 == 0) ? 8192 + k : s[(k * 31) % n] + (k % 3) - 1;
+      auto t = wt.lex_count(i, j, c);
+      auto u = wt.lex_smaller_count(j, c);
+      lx.push_back(i); lx.push_back(j); lx.push_back(c);
+      lx.push_back(std::get<0>(t)); lx.push_back(std::get<1>(t)); lx.push_back(std::get<2>(t));
+      lx.push_back(std::get<0>(u)); lx.push_back(std::get<1>(u));
+    }
+    dump(out + "/wtint_lex.bin", lx.data(), lx.size() * 8);
+    std::vector<uint64_t> rs;
+    for (uint64_t k = 0; k < 40; k++) {
+      const uint64_t lb = (k * 7919) % n, rb = std::min<uint64_t>(n - 1, lb + (k * 3001) % 20000);
+      const uint64_t vlb = (k * 613) % 5000, vrb = vlb + (k * 97) % 3000;
+      auto r = wt.range_search_2d(lb, rb, vlb, vrb, k % 4 == 0);
+      rs.push_back(lb); rs.push_back(rb); rs.push_back(vlb); rs.push_back(vrb);
+      rs.push_back(r.first); rs.push_back(r.second.size());
+      for (auto& pt : r.second) { rs.push_back(pt.first); rs.push_back(pt.second); }
+    }
+    dump(out + "/wtint_range.bin", rs.data(), rs.size() * 8);
   }
   // ---- FMIndex() + wt_huff<> over the BWT (fm-index/par/WT.C:33-45)
   {

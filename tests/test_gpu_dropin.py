@@ -72,6 +72,32 @@ def test_dropin_driver(wsfm, orc, variant, dt):
             occ = np.nonzero(sv == c)[0]
             j = 1 + (i % 7)
             assert sel == (occ[j - 1] if j <= occ.size else orc.NONE)
+        # lex_count / lex_smaller_count (wt_int_par.hpp:712-780) by brute force
+        lx = load(d + "/wtint_lex.bin").reshape(-1, 8)
+        top = 1 << Lw
+        for i, j, c, rk, sm, gr, rk2, sm2 in (tuple(int(x) for x in r) for r in lx):
+            seg = sv[i:j].astype(np.int64)
+            if c >= top:
+                assert (rk, sm, gr) == (0, j - i, 0)
+                assert (rk2, sm2) == (0, j)
+            else:
+                assert (rk, sm, gr) == (int((sv[:i] == c).sum()), int((seg < c).sum()), int((seg > c).sum()))
+                assert (rk2, sm2) == (int((sv[:j] == c).sum()), int((sv[:j] < c).sum()))
+        # range_search_2d: count, and the reported points in (value, index) order
+        rs = load(d + "/wtint_range.bin")
+        p = 0
+        while p < rs.size:
+            lb, rb, vlb, vrb, cnt, npt = (int(x) for x in rs[p:p + 6])
+            p += 6
+            pts = rs[p:p + 2 * npt].reshape(-1, 2)
+            p += 2 * npt
+            idx = np.arange(lb, rb + 1)
+            val = sv[lb:rb + 1].astype(np.int64)
+            sel = (val >= vlb) & (val <= vrb)
+            assert cnt == int(sel.sum())
+            if npt:
+                exp = sorted(zip(val[sel].tolist(), idx[sel].tolist()))
+                assert [(int(b), int(a)) for a, b in pts] == exp
         # FMIndex() + wt_huff<>
         n = 200003
         t = wsfm.gen_dna(14, n)

@@ -59,5 +59,61 @@ def test_wtint_vs_oracle(ctx, orc, n, hi, max_level, dt):
         for a, b, g in list(zip(jj, cs, got))[:200]:
             occ = np.nonzero(s == b)[0]
             assert g == (occ[int(a) - 1] if int(a) <= occ.size else orc.NONE)
+        lex_and_range(wt, O, s, L, rng, orc)
     finally:
         wt.free(); ds.free()
+
+
+def lex_and_range(wt, O, s, L, rng, orc):
+    """lex_count / lex_smaller_count / range_search_2d counts against the oracle's
+    restatement of the reference traversals and brute force on the text."""
+    n = s.size
+    q = max(8, min(2000, 4 * n))
+    top = 1 << L
+    a = rng.integers(0, n + 1, q)
+    b = rng.integers(0, n + 1, q)
+    i, j = np.minimum(a, b).astype(np.uint64), np.maximum(a, b).astype(np.uint64)
+    i[:3], j[:3] = [0, 0, n], [n, 0, n]
+    c = np.where(rng.random(q) < 0.8, s[rng.integers(0, n, q)].astype(np.int64) + rng.integers(-1, 2, q),
+                 rng.integers(0, top + 3, q))
+    c = np.maximum(c, 0).astype(np.uint64)
+    rk, sm, gr = wt.lex_count(i, j, c)
+    rk2, sm2 = wt.lex_smaller_count(j, c)
+    for k in range(q):
+        e = O.lex_count(int(i[k]), int(j[k]), int(c[k]))
+        assert (int(rk[k]), int(sm[k]), int(gr[k])) == e, k
+        assert (int(rk2[k]), int(sm2[k])) == O.lex_smaller_count(int(j[k]), int(c[k])), k
+    for k in range(min(q, 300)):
+        seg = s[int(i[k]):int(j[k])].astype(np.int64)
+        ck = int(c[k])
+        assert int(sm[k]) == int((seg < ck).sum()) and int(sm2[k]) == int((s[:int(j[k])] < ck).sum())
+        if ck < top:
+            assert int(gr[k]) == int((seg > ck).sum()) and int(rk[k]) == int((s[:int(i[k])] == ck).sum())
+    # invalid intervals: i > j or j > n
+    bad_i = np.array([5 % (n + 1) + 1, 0], np.uint64)
+    bad_j = np.array([5 % (n + 1), n + 1], np.uint64)
+    r = wt.lex_count(bad_i, bad_j, np.zeros(2, np.uint64))
+    assert all(int(x) == orc.NONE for x in np.concatenate(r))
+    # range counts, including lb > rb, vlb > vrb, vrb above 2^L and vlb = 2^L
+    lb = rng.integers(0, n, q).astype(np.uint64)
+    rb = np.minimum(lb + rng.integers(0, max(1, n // 3), q), n - 1).astype(np.uint64)
+    rb[:2] = [lb[0], lb[1] - 1 if lb[1] else 0]
+    v1 = rng.integers(0, top + 2, q)
+    v2 = rng.integers(0, top + 2, q)
+    vlb, vrb = np.minimum(v1, v2).astype(np.uint64), np.maximum(v1, v2).astype(np.uint64)
+    vlb[2], vrb[2] = 7, 3
+    vlb[3], vrb[3] = top, top + 5
+    vlb[4], vrb[4] = 0, top + 9
+    got = wt.range_count(lb, rb, vlb, vrb)
+    for k in range(q):
+        assert int(got[k]) == O.range_count(int(lb[k]), int(rb[k]), int(vlb[k]), int(vrb[k])), k
+    for k in range(min(q, 300)):
+        lo, hi = int(vlb[k]), min(int(vrb[k]), top)
+        if lo > hi or int(lb[k]) > int(rb[k]):
+            assert int(got[k]) == 0
+            continue
+        lo = min(lo, top - 1)   # the reference's leaf test does not compare vlb
+        seg = s[int(lb[k]):int(rb[k]) + 1].astype(np.int64)
+        assert int(got[k]) == int(((seg >= lo) & (seg <= hi)).sum()), k
+    assert int(wt.range_count(np.array([0], np.uint64), np.array([n], np.uint64), np.zeros(1, np.uint64),
+                              np.ones(1, np.uint64))[0]) == orc.NONE

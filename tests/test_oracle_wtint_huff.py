@@ -2,7 +2,8 @@
 
 wt_int (wt_int/wt_int_par.hpp:519-660) and wt_pc (wt_huff/wt_pc_par.hpp:430-554)
 answers are fixed by the text itself: rank(i,c) = #c in s[0,i), select(i,c) =
-position of the i-th c, access(i) = s[i].  The Huffman shape
+position of the i-th c, access(i) = s[i], lex counts and range counts = counts of
+values below / above / inside the bounds.  The Huffman shape
 (wt_huff/wt_huff_par.hpp:92-117) is checked against an independent heapq
 Huffman cost and the Kraft equality; its BFS node order (sdsl byte_tree) is
 parity unpinned.  FM count is checked against naive substring counting and
@@ -34,6 +35,30 @@ def test_wtint_queries_brute(orc, n, hi, L):
             assert W.select(occ.size + 1, c) == orc.NONE
         else:
             assert W.select(1, c) == orc.NONE
+    # lex_count (:712-741), lex_smaller_count (:753-780): counts of values below /
+    # above c; range_search_2d count (:801-870): positions in [lb,rb] with values in
+    # [vlb,vrb], where the reference clamps vrb to 2^L and, its leaf test not
+    # comparing vlb, counts the value 2^L-1 for vlb = 2^L
+    top = 1 << L
+    for _ in range(200):
+        i, j = sorted(int(x) for x in rng.integers(0, n + 1, 2))
+        c = int(rng.integers(0, top + 3))
+        seg = s[i:j].astype(np.int64)
+        inside = c < top
+        assert W.lex_count(i, j, c) == (int((s[:i] == c).sum()) if inside else 0, int((seg < c).sum()),
+                                        int((seg > c).sum()) if inside else 0)
+        assert W.lex_smaller_count(j, c) == (int((s[:j] == c).sum()) if inside else 0, int((s[:j] < c).sum()))
+        lb, rb = (int(x) for x in rng.integers(0, n, 2))
+        vlb, vrb = sorted(int(x) for x in rng.integers(0, top + 3, 2))
+        got = W.range_count(lb, rb, vlb, vrb)
+        hi_ = min(vrb, top)
+        if lb > rb or vlb > hi_:
+            assert got == 0
+        else:
+            seg = s[lb:rb + 1].astype(np.int64)
+            assert got == int(((seg >= min(vlb, top - 1)) & (seg <= hi_)).sum())
+    assert W.range_count(0, n - 1, top, top + 7) == int((s == top - 1).sum())
+    assert W.range_count(0, n - 1, 3, 2) == 0
 
 
 def huff_cost(freqs):

@@ -109,6 +109,9 @@ SIGNATURES = {
     "wtg_wtint_access_batch": (i32, [vp, vp, vp, u64, vp]),
     "wtg_wtint_select_batch": (i32, [vp, vp, vp, vp, u64, vp]),
     "wtg_wtint_inverse_select_batch": (i32, [vp, vp, vp, u64, vp, vp]),
+    "wtg_wtint_lex_count_batch": (i32, [vp, vp, vp, vp, vp, u64, vp, vp, vp]),
+    "wtg_wtint_lex_smaller_count_batch": (i32, [vp, vp, vp, vp, u64, vp, vp]),
+    "wtg_wtint_range_count_batch": (i32, [vp, vp, vp, vp, vp, vp, u64, vp]),
     "wtg_huff_build": (i32, [vp, vp, u64, C.POINTER(vp)]),
     "wtg_huff_destroy": (i32, [vp, vp]),
     "wtg_huff_get_shape": (i32, [vp, C.POINTER(HuffShape)]),
@@ -744,6 +747,18 @@ class WtInt:
 
     def inverse_select(self, i):
         return self._q(lib().wtg_wtint_inverse_select_batch, i, n_out=2)
+
+    def lex_count(self, i, j, c):
+        """(rank(i,c), #values < c in [i,j), #values > c in [i,j)) per query."""
+        return self._q(lib().wtg_wtint_lex_count_batch, i, j, c, n_out=3)
+
+    def lex_smaller_count(self, i, c):
+        """(rank(i,c), #values < c in [0,i)) per query."""
+        return self._q(lib().wtg_wtint_lex_smaller_count_batch, i, c, n_out=2)
+
+    def range_count(self, lb, rb, vlb, vrb):
+        """#positions in [lb,rb] with values in [vlb,vrb] (range_search_2d count)."""
+        return self._q(lib().wtg_wtint_range_count_batch, lb, rb, vlb, vrb)
 
     def free(self):
         if self.h is not None:

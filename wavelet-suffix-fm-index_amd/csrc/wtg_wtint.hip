@@ -44,6 +44,87 @@ __global__ void k_wtint_rank(WtQ w, const uint64_t* __restrict__ qi, const uint6
   out[q] = i;
 }
 
+// lex_count traversal (wt_int/wt_int_par.hpp:712-741): rank(i,c) and the number
+// of values below / above c in [i, j)
+__device__ __forceinline__ void wtint_lex(const WtQ& w, uint64_t i, uint64_t j, uint64_t c, uint64_t& rk,
+                                          uint64_t& sm, uint64_t& gr) {
+  sm = gr = 0;
+  if (w.L < 64 && (1ull << w.L) <= c) { rk = 0; sm = j - i; return; }
+  uint64_t offset = 0, mask = 1ull << (w.L - 1), node_size = w.n;
+  for (uint32_t k = 0; k < w.L; k++) {
+    const uint64_t o = rank9_dev(w.D, w.words, offset);
+    const uint64_t oi = rank9_dev(w.D, w.words, offset + i) - o;
+    const uint64_t oj = rank9_dev(w.D, w.words, offset + j) - o;
+    const uint64_t oe = rank9_dev(w.D, w.words, offset + node_size) - o;
+    if (c & mask) { offset += node_size - oe; node_size = oe; sm += (j - i) - (oj - oi); i = oi; j = oj; }
+    else { node_size -= oe; gr += oj - oi; i -= oi; j -= oj; }
+    offset += w.n;
+    mask >>= 1;
+  }
+  rk = i;
+}
+
+__global__ void k_wtint_lex_count(WtQ w, const uint64_t* __restrict__ qi, const uint64_t* __restrict__ qj,
+                                  const uint64_t* __restrict__ qc, uint64_t nq, uint64_t* __restrict__ out_rank,
+                                  uint64_t* __restrict__ out_smaller, uint64_t* __restrict__ out_greater) {
+  const uint64_t q = (uint64_t)blockIdx.x * blockDim.x + threadIdx.x;
+  if (q >= nq) return;
+  const uint64_t i = qi[q], j = qj[q];
+  uint64_t rk = WTG_NONE, sm = WTG_NONE, gr = WTG_NONE;
+  if (i <= j && j <= w.n) wtint_lex(w, i, j, qc[q], rk, sm, gr);
+  out_rank[q] = rk;
+  out_smaller[q] = sm;
+  out_greater[q] = gr;
+}
+
+// lex_smaller_count (wt_int/wt_int_par.hpp:753-780): rank(i,c) and the number of
+// values below c in [0, i)
+__global__ void k_wtint_lex_smaller(WtQ w, const uint64_t* __restrict__ qi, const uint64_t* __restrict__ qc,
+                                    uint64_t nq, uint64_t* __restrict__ out_rank, uint64_t* __restrict__ out_smaller) {
+  const uint64_t q = (uint64_t)blockIdx.x * blockDim.x + threadIdx.x;
+  if (q >= nq) return;
+  uint64_t i = qi[q];
+  const uint64_t c = qc[q];
+  if (i > w.n) { out_rank[q] = out_smaller[q] = WTG_NONE; return; }
+  if (w.L < 64 && (1ull << w.L) <= c) { out_rank[q] = 0; out_smaller[q] = i; return; }
+  uint64_t offset = 0, mask = 1ull << (w.L - 1), node_size = w.n, res = 0;
+  for (uint32_t k = 0; k < w.L && i; k++) {
+    const uint64_t o = rank9_dev(w.D, w.words, offset);
+    const uint64_t oi = rank9_dev(w.D, w.words, offset + i) - o;
+    const uint64_t oe = rank9_dev(w.D, w.words, offset + node_size) - o;
+    if (c & mask) { offset += node_size - oe; node_size = oe; res += i - oi; i = oi; }
+    else { node_size -= oe; i -= oi; }
+    offset += w.n;
+    mask >>= 1;
+  }
+  out_rank[q] = i;
+  out_smaller[q] = res;
+}
+
+// range_search_2d count (wt_int/wt_int_par.hpp:801-870, report = false): the
+// positions in [lb, rb] with values in [vlb, vrb], as the difference of two
+// lex_count "smaller" counts over [lb, rb + 1).  The reference clamps vrb to 2^L
+// and its leaf test does not compare vlb, so vlb = 2^L counts the value 2^L - 1:
+// vlb is clamped to 2^L - 1 the same way.
+__global__ void k_wtint_range_count(WtQ w, const uint64_t* __restrict__ qlb, const uint64_t* __restrict__ qrb,
+                                    const uint64_t* __restrict__ qvlb, const uint64_t* __restrict__ qvrb,
+                                    uint64_t nq, uint64_t* __restrict__ out) {
+  const uint64_t q = (uint64_t)blockIdx.x * blockDim.x + threadIdx.x;
+  if (q >= nq) return;
+  const uint64_t lb = qlb[q], rb = qrb[q];
+  uint64_t vlb = qvlb[q], vrb = qvrb[q];
+  if (rb >= w.n) { out[q] = WTG_NONE; return; }
+  const uint64_t top = w.L < 64 ? (1ull << w.L) : ~0ull;
+  if (vrb > top) vrb = top;
+  if (vlb > vrb || lb > rb) { out[q] = 0; return; }
+  if (w.L < 64 && vlb > top - 1) vlb = top - 1;
+  uint64_t rk, gr, below_hi, below_lo;
+  if (w.L < 64 ? vrb >= top - 1 : vrb == ~0ull) below_hi = rb + 1 - lb;   // every value is <= vrb
+  else wtint_lex(w, lb, rb + 1, vrb + 1, rk, below_hi, gr);
+  wtint_lex(w, lb, rb + 1, vlb, rk, below_lo, gr);
+  out[q] = below_hi - below_lo;
+}
+
 __global__ void k_wtint_access(WtQ w, const uint64_t* __restrict__ qi, uint64_t nq,
                                uint64_t* __restrict__ out_sym, uint64_t* __restrict__ out_rank) {
   const uint64_t q = (uint64_t)blockIdx.x * blockDim.x + threadIdx.x;
@@ -225,6 +306,42 @@ extern "C" int wtg_wtint_inverse_select_batch(wtg_ctx* ctx, const wtg_wtint* wt,
   WTG_TRY(wtg_set_device(ctx));
   WTG_LAUNCH(ctx, "wtint_access", k_wtint_access, dim3((unsigned)wtg_ceil_div(nq, 256)), dim3(256), 0,
              make_q(wt), d_i, nq, d_sym, d_rank);
+  WTG_LAUNCH_CHECK();
+  return wtg_finish(ctx);
+}
+extern "C" int wtg_wtint_lex_count_batch(wtg_ctx* ctx, const wtg_wtint* wt, const uint64_t* d_i,
+                                         const uint64_t* d_j, const uint64_t* d_c, uint64_t nq, uint64_t* d_rank,
+                                         uint64_t* d_smaller, uint64_t* d_greater) {
+  WTG_TRY(q_check(ctx, wt, nq, d_i, d_rank));
+  WTG_REQUIRE(nq == 0 || (d_j && d_c && d_smaller && d_greater), "null argument");
+  if (!nq) return WTG_OK;
+  WTG_TRY(wtg_set_device(ctx));
+  WTG_LAUNCH(ctx, "wtint_lex_count", k_wtint_lex_count, dim3((unsigned)wtg_ceil_div(nq, 256)), dim3(256), 0,
+             make_q(wt), d_i, d_j, d_c, nq, d_rank, d_smaller, d_greater);
+  WTG_LAUNCH_CHECK();
+  return wtg_finish(ctx);
+}
+extern "C" int wtg_wtint_lex_smaller_count_batch(wtg_ctx* ctx, const wtg_wtint* wt, const uint64_t* d_i,
+                                                 const uint64_t* d_c, uint64_t nq, uint64_t* d_rank,
+                                                 uint64_t* d_smaller) {
+  WTG_TRY(q_check(ctx, wt, nq, d_i, d_rank));
+  WTG_REQUIRE(nq == 0 || (d_c && d_smaller), "null argument");
+  if (!nq) return WTG_OK;
+  WTG_TRY(wtg_set_device(ctx));
+  WTG_LAUNCH(ctx, "wtint_lex_smaller", k_wtint_lex_smaller, dim3((unsigned)wtg_ceil_div(nq, 256)), dim3(256), 0,
+             make_q(wt), d_i, d_c, nq, d_rank, d_smaller);
+  WTG_LAUNCH_CHECK();
+  return wtg_finish(ctx);
+}
+extern "C" int wtg_wtint_range_count_batch(wtg_ctx* ctx, const wtg_wtint* wt, const uint64_t* d_lb,
+                                           const uint64_t* d_rb, const uint64_t* d_vlb, const uint64_t* d_vrb,
+                                           uint64_t nq, uint64_t* d_out) {
+  WTG_TRY(q_check(ctx, wt, nq, d_lb, d_out));
+  WTG_REQUIRE(nq == 0 || (d_rb && d_vlb && d_vrb), "null argument");
+  if (!nq) return WTG_OK;
+  WTG_TRY(wtg_set_device(ctx));
+  WTG_LAUNCH(ctx, "wtint_range_count", k_wtint_range_count, dim3((unsigned)wtg_ceil_div(nq, 256)), dim3(256), 0,
+             make_q(wt), d_lb, d_rb, d_vlb, d_vrb, nq, d_out);
   WTG_LAUNCH_CHECK();
   return wtg_finish(ctx);
 }

@@ -2,7 +2,7 @@
 // (wt_int/wt_int_par.hpp:57-1086): same include guard, constructor
 // wt_int(int_vector<w>& buf, size, max_level=0) and the query surface size /
 // empty / sigma / tree / max_level / operator[] / rank / inverse_select /
-// select.  The tree bits, rank_support_v and select_support_mcl (for 1 and 0)
+// select / lex_count / lex_smaller_count / range_search_2d.  The tree bits, rank_support_v and select_support_mcl (for 1 and 0)
 // are built on the GPU (wtg_wtint_build) and copied into host supports, so the
 // scalar queries run the reference's host algorithms; *_batch() run batches on
 // the device against the device copy.  select() throws std::logic_error when
@@ -10,7 +10,9 @@
 #ifndef INCLUDED_SDSL_INT_WAVELET_TREE_PAR
 #define INCLUDED_SDSL_INT_WAVELET_TREE_PAR
 #include <stdexcept>
+#include <algorithm>
 #include <string>
+#include <tuple>
 #include <utility>
 #include <vector>
 #include "rank_support_v_par.hpp"
@@ -122,6 +124,52 @@ class wt_int {
     return std::make_pair(i, c);
   }
 
+  // (rank(i,c), #values < c in [i,j), #values > c in [i,j))
+  std::tuple<size_type, size_type, size_type> lex_count(size_type i, size_type j, value_type c) const {  // :712-741
+    if (((1ULL) << m_max_level) <= c) return std::make_tuple(size_type(0), j - i, size_type(0));
+    size_type offset = 0, node_size = m_size, smaller = 0, greater = 0;
+    uint64_t mask = 1ULL << (m_max_level - 1);
+    for (uint32_t k = 0; k < m_max_level; ++k) {
+      const size_type o = m_tree_rank(offset), oi = m_tree_rank(offset + i) - o, oj = m_tree_rank(offset + j) - o,
+                      oe = m_tree_rank(offset + node_size) - o;
+      if (c & mask) { offset += node_size - oe; node_size = oe; smaller += (j - i) - (oj - oi); i = oi; j = oj; }
+      else { node_size -= oe; greater += oj - oi; i -= oi; j -= oj; }
+      offset += m_size;
+      mask >>= 1;
+    }
+    return std::make_tuple(i, smaller, greater);
+  }
+
+  // (rank(i,c), #values < c in [0,i))
+  std::tuple<size_type, size_type> lex_smaller_count(size_type i, value_type c) const {  // :753-780
+    if (((1ULL) << m_max_level) <= c) return std::make_tuple(size_type(0), i);
+    size_type offset = 0, node_size = m_size, below = 0;
+    uint64_t mask = 1ULL << (m_max_level - 1);
+    for (uint32_t k = 0; k < m_max_level && i; ++k) {
+      const size_type o = m_tree_rank(offset), oi = m_tree_rank(offset + i) - o,
+                      oe = m_tree_rank(offset + node_size) - o;
+      if (c & mask) { offset += node_size - oe; node_size = oe; below += i - oi; i = oi; }
+      else { node_size -= oe; i -= oi; }
+      offset += m_size;
+      mask >>= 1;
+    }
+    return std::make_tuple(i, below);
+  }
+
+  // points (index, value) with index in [lb, rb] and value in [vlb, vrb]: (count,
+  // the points when report) -- :801-870
+  std::pair<size_type, std::vector<std::pair<value_type, size_type>>> range_search_2d(
+      size_type lb, size_type rb, value_type vlb, value_type vrb, bool report = true) const {
+    std::pair<size_type, std::vector<std::pair<value_type, size_type>>> res;
+    res.first = 0;
+    if (vrb > (1ULL << m_max_level)) vrb = 1ULL << m_max_level;
+    if (vlb > vrb) return res;
+    std::vector<size_type> offs(m_max_level + 1), obs(m_max_level + 1);
+    offs[0] = 0;
+    range2d(lb, rb, vlb, vrb, 0, 0, m_size, offs, obs, 0, report, res);
+    return res;
+  }
+
   size_type select(size_type i, value_type c) const {  // :622-660
     std::vector<size_type> path_off(m_max_level + 1), path_rank(m_max_level + 1);
     size_type offset = 0, node_size = m_size;
@@ -161,9 +209,64 @@ class wt_int {
   void access_batch(const uint64_t* d_i, uint64_t nq, uint64_t* d_out) const {
     wtg::check(wtg_wtint_access_batch(wtg::default_context(), m_dev, d_i, nq, d_out));
   }
+  void lex_count_batch(const uint64_t* d_i, const uint64_t* d_j, const uint64_t* d_c, uint64_t nq,
+                       uint64_t* d_rank, uint64_t* d_smaller, uint64_t* d_greater) const {
+    wtg::check(wtg_wtint_lex_count_batch(wtg::default_context(), m_dev, d_i, d_j, d_c, nq, d_rank, d_smaller,
+                                         d_greater));
+  }
+  void lex_smaller_count_batch(const uint64_t* d_i, const uint64_t* d_c, uint64_t nq, uint64_t* d_rank,
+                               uint64_t* d_smaller) const {
+    wtg::check(wtg_wtint_lex_smaller_count_batch(wtg::default_context(), m_dev, d_i, d_c, nq, d_rank, d_smaller));
+  }
+  void range_count_batch(const uint64_t* d_lb, const uint64_t* d_rb, const uint64_t* d_vlb, const uint64_t* d_vrb,
+                         uint64_t nq, uint64_t* d_out) const {
+    wtg::check(wtg_wtint_range_count_batch(wtg::default_context(), m_dev, d_lb, d_rb, d_vlb, d_vrb, nq, d_out));
+  }
   const wtg_wtint* device() const { return m_dev; }
 
  private:
+  // one node of range_search_2d: [lb, rb] inside the node at (level, value range
+  // starting at ilb); offs / obs: level offsets and ones before them on the path
+  void range2d(size_type lb, size_type rb, value_type vlb, value_type vrb, uint32_t level, size_type ilb,
+               size_type node_size, std::vector<size_type>& offs, std::vector<size_type>& obs, size_type path,
+               bool report, std::pair<size_type, std::vector<std::pair<value_type, size_type>>>& res) const {
+    if (lb > rb) return;
+    if (level == m_max_level) {
+      if (report) {
+        for (size_type p = lb + 1; p <= rb + 1; ++p) {  // back to the text position by select up the path
+          size_type i = p;
+          value_type c = path;
+          for (uint32_t k = m_max_level; k > 0; --k) {
+            const size_type off = offs[k - 1], ob = obs[k - 1];
+            i = (c & 1) ? m_tree_select1(ob + i) - off + 1 : m_tree_select0(off - ob + i) - off + 1;
+            c >>= 1;
+          }
+          res.second.emplace_back(i - 1, path);
+        }
+      }
+      res.first += rb - lb + 1;
+      return;
+    }
+    const size_type mid = (2 * ilb + (1ULL << (m_max_level - level))) >> 1;
+    const size_type off = offs[level];
+    const size_type o = m_tree_rank(off), olb = m_tree_rank(off + lb), orb = m_tree_rank(off + rb + 1),
+                    oe = m_tree_rank(off + node_size);
+    obs[level] = o;
+    const size_type zo = off - o, zlb = off + lb - olb, zrb = off + rb + 1 - orb, ze = off + node_size - oe;
+    if (vlb < mid && mid) {
+      offs[level + 1] = off + m_size;
+      if (zrb - zo)
+        range2d(zlb - zo, zrb - zo - 1, vlb, std::min<value_type>(vrb, mid - 1), level + 1, ilb, ze - zo, offs, obs,
+                path << 1, report, res);
+    }
+    if (vrb >= mid) {
+      offs[level + 1] = off + m_size + (ze - zo);
+      if (orb - o)
+        range2d(olb - o, orb - o - 1, std::max<value_type>(mid, vlb), vrb, level + 1, mid, oe - o, offs, obs,
+                (path << 1) | 1, report, res);
+    }
+  }
+
   void fetch() {
     uint64_t n = 0, sg = 0;
     uint32_t L = 0;
