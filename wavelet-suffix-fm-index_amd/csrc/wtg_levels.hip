@@ -44,11 +44,15 @@ static __host__ __device__ __forceinline__ uint64_t heap_off(uint32_t j) { retur
 // Counter replicas for the next-bit counts of the 2^j groups of level j: a
 // tile adds to replica (tile mod reps).  A skewed input can send nearly every
 // tile's contribution to one group, so the replica count follows the tile
-// count (at most 1024, and at most 2^17 counters per level), not the mean
-// group size.
+// count, not the mean group size: enough replicas that at most 32 tiles add
+// into one counter (at most 1024, and at most 2^17 counters per level).  Up to
+// C1_SUM_REPS replicas of at most 2^C1_SUM_LOG_NODES groups the node-table pass
+// sums them itself (no reduce launch).
+#define C1_SUM_REPS 32
+#define C1_SUM_LOG_NODES 10
 static uint32_t c1_reps(uint32_t j, uint64_t tiles) {
   uint32_t r = 1;
-  while (r < 1024 && r < tiles && ((uint64_t)r << (j + 1)) <= (1ull << 17)) r <<= 1;
+  while (r < 1024 && ((uint64_t)r << 5) < tiles && ((uint64_t)r << (j + 1)) <= (1ull << 17)) r <<= 1;
   return r;
 }
 
@@ -1099,7 +1103,10 @@ int levels_build_t(wtg_ctx* ctx, const T* d_sym, uint64_t n, uint32_t L, uint64_
       WTG_LAUNCH_CHECK();
     }
     // node table of level k (node starts always: cheap, needed by WTnodes)
-    if (k >= 1 && k > first && n > 0 && P.reps[k] > 1) {
+    // (the node-table pass reads a node's replicas strided: only for few nodes)
+    const bool reduce = k >= 1 && k > first && n > 0 && P.reps[k] > 1 &&
+                        (P.reps[k] > C1_SUM_REPS || k > C1_SUM_LOG_NODES);
+    if (reduce) {
       WTG_LAUNCH(ctx, "c1_reduce", k_c1_reduce,
                  dim3((unsigned)wtg_ceil_div(1ull << k, 256), (unsigned)wtg_ceil_div(P.reps[k] - 1, C1_RCH)), dim3(256),
                  0, c1 + P.c1off[k], P.reps[k], (uint64_t)(1ull << k));
@@ -1109,7 +1116,8 @@ int levels_build_t(wtg_ctx* ctx, const T* d_sym, uint64_t n, uint32_t L, uint64_
       NodeTabArgs na{};
       na.cnt = cnt;
       na.c1 = c1 + P.c1off[k];
-      na.reps = 1;
+      // replicas hold counts only when the previous split wrote them (k > first)
+      na.reps = (reduce || k <= first) ? 1u : P.reps[k];
       na.start = start;
       na.tab = split ? tab + heap_off(k) : nullptr;
       na.st_s = ntst + P.nt_base[k];
