@@ -691,8 +691,8 @@ __device__ __forceinline__ void key_store(__amdgpu_buffer_rsrc_t r, uint32_t i, 
 // Split pass over FULL tiles (every tile but a partial last one), in 32-bit
 // index arithmetic (n < 2^32) and without per-element validity tests: the
 // same stable scatter, next-level bit words and counts as k_level<T, 1>.
-// Needs node_shift < 8*sizeof(T) (the host sends level 0 of 32-level uint32
-// keys to k_level).  Chunk zero prefixes by a 16-lane DPP scan; slot-order
+// Needs node_shift < 8*sizeof(T) for 32- and 64-bit keys (the host sends level
+// 0 of 32-level uint32 keys to k_level); narrower keys shift as int.  Chunk zero prefixes by a 16-lane DPP scan; slot-order
 // heads and next-level bits from two LDS reads per element; stores from the
 // node table cached in LDS (a tile's nodes are consecutive).
 template <typename T, bool VEC, bool COUNT>
@@ -952,9 +952,10 @@ int launch_level(wtg_ctx* ctx, const LevelArgs& la, uint64_t tiles, int mode, bo
   constexpr uint64_t TILE = 256ull * LV_STEPS * VecOf<T>::V;
   void (*k)(LevelArgs) = nullptr;
   if (mode == 1) {
-    // full tiles: k_split; a partial last tile (and a node shift as wide as
-    // the key): k_level
-    const uint64_t full = la.node_shift < 8 * sizeof(T) ? (uint64_t)la.n / TILE : 0;
+    // full tiles: k_split; a partial last tile (and, for 32- and 64-bit keys, a
+    // node shift as wide as the key): k_level.  8- and 16-bit keys are promoted
+    // to int before the shift, so their full-width shift is defined.
+    const uint64_t full = (sizeof(T) < 4 || la.node_shift < 8 * sizeof(T)) ? (uint64_t)la.n / TILE : 0;
     if (full) {
       if (vec) pick_split<T, true>(count, k); else pick_split<T, false>(count, k);
       // the last split (no key stores) is timed under its own name
