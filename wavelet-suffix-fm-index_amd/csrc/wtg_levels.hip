@@ -247,6 +247,12 @@ struct LevelArgs {
 };
 
 #define LV_STEPS 4
+#define LV_STEPS_U8 2
+// vector steps per thread per tile (a tile is 256 * steps * V keys): byte keys
+// take 8192-key tiles, twice the blocks of 16384 (C1's 2^24 keys: split 38 ->
+// 30 us per level; one step per tile, 4096 keys, is slower again)
+template <typename T> struct StepsOf { static constexpr int S = LV_STEPS; };
+template <> struct StepsOf<uint8_t> { static constexpr int S = LV_STEPS_U8; };
 
 // wave-uniform copies of a value (readfirstlane) and of one lane's value
 template <typename U>
@@ -497,11 +503,11 @@ __global__ __launch_bounds__(256) void k_level(LevelArgs a) {
   constexpr bool SCATTER = MODE == 1;
   constexpr int V = VecOf<T>::V;
   constexpr int G = 64 / V;              // lanes per bit word
-  constexpr int TILE = 256 * LV_STEPS * V;
+  constexpr int TILE = 256 * StepsOf<T>::S * V;
   constexpr int NW = TILE / 64;          // words per tile
   constexpr int ROUNDS = TILE / 256;     // store rounds (one element per lane)
   constexpr int NPL = (V == 16) ? 5 : (V == 8) ? 4 : (V == 4) ? 3 : 2;  // bit planes of a count <= V
-  constexpr int NCH = LV_STEPS * 4;      // (step, wave) chunks
+  constexpr int NCH = StepsOf<T>::S * 4;      // (step, wave) chunks
   __shared__ uint64_t s_words[SCATTER ? 1 : NW + 1];
   __shared__ uint32_t s_wz[NCH];
   __shared__ uint64_t s_nb[SCATTER ? NW + 1 : 1];
@@ -520,9 +526,9 @@ __global__ __launch_bounds__(256) void k_level(LevelArgs a) {
   uint64_t zprev = 0;
   if (SCATTER) zprev = a.zpre[tile];
 
-  T x[LV_STEPS][V];
+  T x[StepsOf<T>::S][V];
 #pragma unroll
-  for (int s = 0; s < LV_STEPS; s++) {
+  for (int s = 0; s < StepsOf<T>::S; s++) {
     const uint64_t e0 = tile_start + (uint64_t)(s * 256 + tid) * V;
     if (VEC && full) {
       const v4u32 q = __builtin_nontemporal_load((const v4u32*)(src + e0));
@@ -535,10 +541,10 @@ __global__ __launch_bounds__(256) void k_level(LevelArgs a) {
     }
   }
 
-  uint32_t mk[LV_STEPS], pre[LV_STEPS];
+  uint32_t mk[StepsOf<T>::S], pre[StepsOf<T>::S];
   bool bad = false;
 #pragma unroll
-  for (int s = 0; s < LV_STEPS; s++) {
+  for (int s = 0; s < StepsOf<T>::S; s++) {
     const uint64_t e0 = tile_start + (uint64_t)(s * 256 + tid) * V;
     uint32_t m = 0, vm = 0;
 #pragma unroll
@@ -573,12 +579,12 @@ __global__ __launch_bounds__(256) void k_level(LevelArgs a) {
   if (bad) atomicOr(a.err, FLAG_BADSYM);
   __syncthreads();
 
-  uint32_t woff[LV_STEPS];
+  uint32_t woff[StepsOf<T>::S];
   uint32_t zt = 0;
 #pragma unroll
   for (int k = 0; k < NCH; k++) {
 #pragma unroll
-    for (int s = 0; s < LV_STEPS; s++)
+    for (int s = 0; s < StepsOf<T>::S; s++)
       if (k == s * 4 + w) woff[s] = zt;
     zt += s_wz[k];
   }
@@ -612,7 +618,7 @@ __global__ __launch_bounds__(256) void k_level(LevelArgs a) {
     if (cached && (uint64_t)tid <= plast - pfirst) s_tab[tid] = a.tab[pfirst + tid];
     // ---- stable 0/1 partition of the tile into LDS
 #pragma unroll
-    for (int s = 0; s < LV_STEPS; s++) {
+    for (int s = 0; s < StepsOf<T>::S; s++) {
       const uint32_t m = mk[s];
       const uint32_t zb = woff[s] + pre[s];
       const uint32_t j0 = (uint32_t)(s * 256 + tid) * V;
@@ -698,11 +704,11 @@ __device__ __forceinline__ void key_store(__amdgpu_buffer_rsrc_t r, uint32_t i, 
 template <typename T, bool VEC, bool COUNT>
 __global__ __launch_bounds__(256) void k_split(LevelArgs a) {
   constexpr int V = VecOf<T>::V;
-  constexpr int TILE = 256 * LV_STEPS * V;
+  constexpr int TILE = 256 * StepsOf<T>::S * V;
   constexpr int NW = TILE / 64;
   constexpr int ROUNDS = TILE / 256;
   constexpr int NPL = (V == 16) ? 5 : (V == 8) ? 4 : (V == 4) ? 3 : 2;
-  constexpr int NCH = LV_STEPS * 4;  // (step, wave) chunks, chunk s*4+w
+  constexpr int NCH = StepsOf<T>::S * 4;  // (step, wave) chunks, chunk s*4+w
   static_assert(NCH <= 16, "chunk scan covers one DPP row");
   __shared__ uint32_t s_wz[NCH];
   __shared__ uint64_t s_nb[NW + 1];
@@ -717,9 +723,9 @@ __global__ __launch_bounds__(256) void k_split(LevelArgs a) {
   const T* src = (const T*)a.src + ts;
   const uint32_t zprev = a.zpre[tile];
 
-  T x[LV_STEPS][V];
+  T x[StepsOf<T>::S][V];
 #pragma unroll
-  for (int s = 0; s < LV_STEPS; s++) {
+  for (int s = 0; s < StepsOf<T>::S; s++) {
     const uint32_t e0 = (uint32_t)(s * 256 + tid) * V;
     if (VEC) {
       const v4u32 q = __builtin_nontemporal_load((const v4u32*)(src + e0));
@@ -732,9 +738,9 @@ __global__ __launch_bounds__(256) void k_split(LevelArgs a) {
     }
   }
   // this level's bits, the lane's zero rank inside its chunk, chunk zeros
-  uint32_t mk[LV_STEPS], pre[LV_STEPS];
+  uint32_t mk[StepsOf<T>::S], pre[StepsOf<T>::S];
 #pragma unroll
-  for (int s = 0; s < LV_STEPS; s++) {
+  for (int s = 0; s < StepsOf<T>::S; s++) {
     uint32_t m = 0;
 #pragma unroll
     for (int v = 0; v < V; v++) m |= (uint32_t)((x[s][v] >> a.bitpos) & 1u) << v;
@@ -744,7 +750,7 @@ __global__ __launch_bounds__(256) void k_split(LevelArgs a) {
     if (lane == 0) s_wz[s * 4 + w] = tot;
   }
   if (tid == 0) s_ends[0] = x[0][0];
-  if (tid == 255) s_ends[1] = x[LV_STEPS - 1][V - 1];
+  if (tid == 255) s_ends[1] = x[StepsOf<T>::S - 1][V - 1];
   __syncthreads();
 
   // chunk prefixes: lane c < NCH holds chunk c's zeros; inclusive scan in row 0
@@ -757,9 +763,9 @@ __global__ __launch_bounds__(256) void k_split(LevelArgs a) {
   const uint32_t cx = ci - cz;
   const uint32_t zt = (uint32_t)__builtin_amdgcn_readlane((int)ci, NCH - 1);
   const int wu = __builtin_amdgcn_readfirstlane(w);
-  uint32_t woff[LV_STEPS];
+  uint32_t woff[StepsOf<T>::S];
 #pragma unroll
-  for (int s = 0; s < LV_STEPS; s++) woff[s] = (uint32_t)__builtin_amdgcn_readlane((int)cx, s * 4 + wu);
+  for (int s = 0; s < StepsOf<T>::S; s++) woff[s] = (uint32_t)__builtin_amdgcn_readlane((int)cx, s * 4 + wu);
   if (tid == 0) a.zc[tile] = zt;  // unused by the split; keeps zc like k_level's
 
   // node-table entries of the tile's node range into LDS
@@ -770,7 +776,7 @@ __global__ __launch_bounds__(256) void k_split(LevelArgs a) {
 
   // stable 0/1 partition of the tile into LDS
 #pragma unroll
-  for (int s = 0; s < LV_STEPS; s++) {
+  for (int s = 0; s < StepsOf<T>::S; s++) {
     const uint32_t m = mk[s];
     const uint32_t zb = woff[s] + pre[s];                      // slot of the lane's first zero
     const uint32_t ob = zt + (uint32_t)(s * 256 + tid) * V - zb;  // slot of its first one
@@ -899,7 +905,7 @@ struct LevelPlan {
   uint64_t nt_base[26];    // state offset of level k (two arrays: s and z)
 };
 
-int tile_elems_for(int w) { return 256 * LV_STEPS * (16 / w); }
+int tile_elems_for(int w) { return 256 * (w == 1 ? LV_STEPS_U8 : LV_STEPS) * (16 / w); }
 
 LevelPlan plan_levels(uint64_t n, uint32_t L, int w) {
   LevelPlan P{};
@@ -949,7 +955,7 @@ void pick_split(bool count, void (*&k)(LevelArgs)) {
 
 template <typename T>
 int launch_level(wtg_ctx* ctx, const LevelArgs& la, uint64_t tiles, int mode, bool vec, bool count) {
-  constexpr uint64_t TILE = 256ull * LV_STEPS * VecOf<T>::V;
+  constexpr uint64_t TILE = 256ull * StepsOf<T>::S * VecOf<T>::V;
   void (*k)(LevelArgs) = nullptr;
   if (mode == 1) {
     // full tiles: k_split; a partial last tile (and, for 32- and 64-bit keys, a
