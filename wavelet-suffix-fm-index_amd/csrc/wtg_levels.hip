@@ -44,7 +44,7 @@ static __host__ __device__ __forceinline__ uint64_t heap_off(uint32_t j) { retur
 // Counter replicas for the next-bit counts of the 2^j groups of level j: a
 // tile adds to replica (tile mod reps).  A skewed input can send nearly every
 // tile's contribution to one group, so the replica count follows the tile
-// count, not the mean group size: enough replicas that at most 32 tiles add
+// count, not the mean group size: enough replicas that at most 64 tiles add
 // into one counter (at most 1024, and at most 2^17 counters per level).  Up to
 // C1_SUM_REPS replicas of at most 2^C1_SUM_LOG_NODES groups the node-table pass
 // sums them itself (no reduce launch).
@@ -52,7 +52,7 @@ static __host__ __device__ __forceinline__ uint64_t heap_off(uint32_t j) { retur
 #define C1_SUM_LOG_NODES 10
 static uint32_t c1_reps(uint32_t j, uint64_t tiles) {
   uint32_t r = 1;
-  while (r < 1024 && ((uint64_t)r << 5) < tiles && ((uint64_t)r << (j + 1)) <= (1ull << 17)) r <<= 1;
+  while (r < 1024 && ((uint64_t)r << 6) < tiles && ((uint64_t)r << (j + 1)) <= (1ull << 17)) r <<= 1;
   return r;
 }
 
