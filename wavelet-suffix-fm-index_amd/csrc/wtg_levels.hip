@@ -87,11 +87,15 @@ struct NodeTabArgs {
   unsigned long long* distinct;  // optional: += #nonzero CNT_{k+1}
   uint32_t k;
   uint32_t derive;       // 1: CNT_{k+1} = [CNT_k - c1, c1]
-  // optional: block 0 also writes the exclusive scan of zin[0..zn) to zout[0..zn]
-  // (zn <= 4096: the level's zeros per tile, when one scan tile covers them)
+  // optional: the exclusive scan of zin[0..zn) into zout[0..zn] (the level's
+  // zeros per tile), scan tile t (4096 values) done by the block that takes
+  // tile t, with its own look-back states; node tiles and scan tiles share the
+  // tile counter, so every wait is on a block that started earlier
   const uint32_t* zin;
   uint32_t* zout;
-  uint32_t zn;
+  uint64_t zn;
+  uint64_t* zst;
+  uint32_t ztiles, ntiles;
 };
 
 __global__ __launch_bounds__(256) void k_nodetab(NodeTabArgs a) {
@@ -101,8 +105,8 @@ __global__ __launch_bounds__(256) void k_nodetab(NodeTabArgs a) {
   if (threadIdx.x == 0) s_tile = atomicAdd(a.counter, 1u);
   __syncthreads();
   const uint32_t tile = s_tile;
-  if (a.zin && tile == 0) {  // block-uniform
-    const uint32_t b = threadIdx.x * 16;
+  if (a.zin && tile < a.ztiles) {  // block-uniform
+    const uint64_t b = (uint64_t)tile * 4096 + threadIdx.x * 16;
     uint32_t v[16], sum = 0;
 #pragma unroll
     for (int j = 0; j < 16; j++) {
@@ -110,14 +114,22 @@ __global__ __launch_bounds__(256) void k_nodetab(NodeTabArgs a) {
       sum += v[j];
     }
     uint32_t tot;
-    uint32_t run = block_excl_scan_u32(sum, s_wave, tot);
+    const uint32_t ex = block_excl_scan_u32(sum, s_wave, tot);
+    if (threadIdx.x < 64) {
+      const uint64_t pre = lb_exclusive(a.zst, tile, tot, a.err);
+      if (threadIdx.x == 0) s_pre[0] = pre;
+    }
+    __syncthreads();
+    uint32_t run = (uint32_t)(s_pre[0] + ex);
 #pragma unroll
     for (int j = 0; j < 16; j++) {
       if (b + j < a.zn) a.zout[b + j] = run;
       run += v[j];
     }
-    if (threadIdx.x == 0) a.zout[a.zn] = tot;
+    if (b < a.zn && b + 16 >= a.zn) a.zout[a.zn] = run;
+    __syncthreads();  // s_pre and s_wave are reused below
   }
+  if (tile >= a.ntiles) return;  // a scan-only block
   const uint64_t nnodes = 1ull << a.k;
   const uint32_t* ck = a.cnt + heap_off(a.k);
   uint32_t* ck1 = a.cnt + heap_off(a.k + 1);
@@ -1104,8 +1116,8 @@ int levels_build_t(wtg_ctx* ctx, const T* d_sym, uint64_t n, uint32_t L, uint64_
       // first level: bits pass (its bit words + zeros per tile); every later
       // level's bits and tile zeros were produced by the previous split
       if (k == first) WTG_TRY(launch_level<T>(ctx, la, P.tiles, 0, kvec, false));
-      // (after a split, one scan tile's worth is folded into the node-table pass)
-      if ((split || k == first) && !(k > first && P.scan_tiles == 1)) {
+      // (after a split the scan is folded into the node-table pass)
+      if (k == first) {
         WTG_LAUNCH(ctx, "scan_tiles", k_scan_u32<uint32_t>, dim3((unsigned)P.scan_tiles), dim3(256), 0, zck, zpre,
                    P.tiles, scst + (size_t)k * (P.scan_tiles + 1), ctr + k, ctx->d_flag);
         WTG_LAUNCH_CHECK();
@@ -1160,13 +1172,17 @@ int levels_build_t(wtg_ctx* ctx, const T* d_sym, uint64_t n, uint32_t L, uint64_
       na.k = k;
       na.derive = k >= 1 ? 1u : 0u;
       na.distinct = (d_distinct && k + 1 == L && L > 1) ? d_distinct : nullptr;
-      if (split && n > 0 && k > first && P.scan_tiles == 1) {
+      na.ntiles = (uint32_t)P.nt_tiles[k];
+      if (split && n > 0 && k > first) {
         na.zin = zck;
         na.zout = zpre;
-        na.zn = (uint32_t)P.tiles;
+        na.zn = P.tiles;
+        na.zst = scst + (size_t)k * (P.scan_tiles + 1);
+        na.ztiles = (uint32_t)P.scan_tiles;
       }
       if (split || want_nodes || na.distinct) {
-        WTG_LAUNCH(ctx, "nodetab", k_nodetab, dim3((unsigned)P.nt_tiles[k]), dim3(256), 0, na);
+        WTG_LAUNCH(ctx, "nodetab", k_nodetab, dim3((unsigned)std::max<uint64_t>(P.nt_tiles[k], na.ztiles)),
+                   dim3(256), 0, na);
         WTG_LAUNCH_CHECK();
       }
     }
