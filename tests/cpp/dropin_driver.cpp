@@ -113,6 +113,17 @@ int main(int argc, char** argv) {
       for (auto& pt : r.second) { rs.push_back(pt.first); rs.push_back(pt.second); }
     }
     dump(out + "/wtint_range.bin", rs.data(), rs.size() * 8);
+    // interval_symbols: per query (i, j, k, then k triples (c, rank_c_i, rank_c_j))
+    std::vector<uint64_t> is;
+    std::vector<uint64_t> cs(wt.sigma + 1), ri(wt.sigma + 1), rj(wt.sigma + 1);
+    for (uint64_t q = 0; q < 60; q++) {
+      const uint64_t i = (q * 7919) % n, j = std::min<uint64_t>(n, i + (q % 5 == 0 ? q % 3 : (q * 131) % 4000));
+      uint64_t k = 0;
+      wt.interval_symbols(i, j, k, cs, ri, rj);
+      is.push_back(i); is.push_back(j); is.push_back(k);
+      for (uint64_t t = 0; t < k; t++) { is.push_back(cs[t]); is.push_back(ri[t]); is.push_back(rj[t]); }
+    }
+    dump(out + "/wtint_isym.bin", is.data(), is.size() * 8);
   }
   // ---- FMIndex() + wt_huff<> over the BWT (fm-index/par/WT.C:33-45)
   {

@@ -98,6 +98,18 @@ def test_dropin_driver(wsfm, orc, variant, dt):
             if npt:
                 exp = sorted(zip(val[sel].tolist(), idx[sel].tolist()))
                 assert [(int(b), int(a)) for a, b in pts] == exp
+        # interval_symbols (wt_int_par.hpp:684-702): distinct values ascending with ranks
+        isy = load(d + "/wtint_isym.bin")
+        p = 0
+        while p < isy.size:
+            i, j, k = (int(x) for x in isy[p:p + 3])
+            p += 3
+            got = isy[p:p + 3 * k].reshape(-1, 3)
+            p += 3 * k
+            vals = np.unique(sv[i:j])
+            assert k == vals.size
+            for (c, ri, rj), v in zip(got.tolist(), vals.tolist()):
+                assert c == v and ri == int((sv[:i] == v).sum()) and rj == int((sv[:j] == v).sum())
         # FMIndex() + wt_huff<>
         n = 200003
         t = wsfm.gen_dna(14, n)

@@ -2,7 +2,7 @@
 // (wt_int/wt_int_par.hpp:57-1086): same include guard, constructor
 // wt_int(int_vector<w>& buf, size, max_level=0) and the query surface size /
 // empty / sigma / tree / max_level / operator[] / rank / inverse_select /
-// select / lex_count / lex_smaller_count / range_search_2d.  The tree bits, rank_support_v and select_support_mcl (for 1 and 0)
+// select / interval_symbols / lex_count / lex_smaller_count / range_search_2d.  The tree bits, rank_support_v and select_support_mcl (for 1 and 0)
 // are built on the GPU (wtg_wtint_build) and copied into host supports, so the
 // scalar queries run the reference's host algorithms; *_batch() run batches on
 // the device against the device copy.  select() throws std::logic_error when
@@ -124,6 +124,23 @@ class wt_int {
     return std::make_pair(i, c);
   }
 
+  // the k distinct values of [i, j) in ascending order, with rank(i, c) and
+  // rank(j, c) for each (cs / rank_c_i / rank_c_j hold at least sigma entries)
+  void interval_symbols(size_type i, size_type j, size_type& k, std::vector<value_type>& cs,
+                        std::vector<size_type>& rank_c_i, std::vector<size_type>& rank_c_j) const {  // :684-702
+    k = 0;
+    if (i == j) return;
+    if (i + 1 == j) {
+      const auto r = inverse_select(i);
+      cs[0] = r.second;
+      rank_c_i[0] = r.first;
+      rank_c_j[0] = r.first + 1;
+      k = 1;
+      return;
+    }
+    interval_walk(i, j, k, cs, rank_c_i, rank_c_j, 0, 0, m_size, 0);
+  }
+
   // (rank(i,c), #values < c in [i,j), #values > c in [i,j))
   std::tuple<size_type, size_type, size_type> lex_count(size_type i, size_type j, value_type c) const {  // :712-741
     if (((1ULL) << m_max_level) <= c) return std::make_tuple(size_type(0), j - i, size_type(0));
@@ -225,6 +242,26 @@ class wt_int {
   const wtg_wtint* device() const { return m_dev; }
 
  private:
+  // one node of interval_symbols (:119-158): [i, j) of the node at (level, path),
+  // left child before right child so the values come out ascending
+  void interval_walk(size_type i, size_type j, size_type& k, std::vector<value_type>& cs,
+                     std::vector<size_type>& rank_c_i, std::vector<size_type>& rank_c_j, uint32_t level,
+                     value_type path, size_type node_size, size_type offset) const {
+    if (level >= m_max_level) {
+      rank_c_i[k] = i;
+      rank_c_j[k] = j;
+      cs[k++] = path;
+      return;
+    }
+    const size_type o = m_tree_rank(offset), oi = m_tree_rank(offset + i) - o, oj = m_tree_rank(offset + j) - o,
+                    oe = m_tree_rank(offset + node_size) - o;
+    if ((j - i) - (oj - oi) > 0)
+      interval_walk(i - oi, j - oj, k, cs, rank_c_i, rank_c_j, level + 1, path << 1, node_size - oe, offset + m_size);
+    if (oj - oi > 0)
+      interval_walk(oi, oj, k, cs, rank_c_i, rank_c_j, level + 1, (path << 1) | 1, oe,
+                    offset + (node_size - oe) + m_size);
+  }
+
   // one node of range_search_2d: [lb, rb] inside the node at (level, value range
   // starting at ilb); offs / obs: level offsets and ones before them on the path
   void range2d(size_type lb, size_type rb, value_type vlb, value_type vrb, uint32_t level, size_type ilb,
