@@ -151,6 +151,18 @@ int main(int argc, char** argv) {
       res.push_back(sel);
     }
     dump(out + "/huff.bin", res.data(), res.size() * 8);
+    // interval_symbols over the wt_huff: (i, j, k, then k triples (c, rank_c_i, rank_c_j))
+    std::vector<uint64_t> hs;
+    std::vector<uint8_t> hcs(wt.sigma + 1);
+    std::vector<uint64_t> hri(wt.sigma + 1), hrj(wt.sigma + 1);
+    for (uint64_t q = 0; q < 60; q++) {
+      const uint64_t i = (q * 7919) % n, j = std::min<uint64_t>(n, i + (q % 4 == 0 ? q % 4 + (q / 4) % 3 : (q * 131) % 5000));
+      uint64_t k = 0;
+      wt.interval_symbols(i, j, k, hcs, hri, hrj);
+      hs.push_back(i); hs.push_back(j); hs.push_back(k);
+      for (uint64_t t = 0; t < k; t++) { hs.push_back(hcs[t]); hs.push_back(hri[t]); hs.push_back(hrj[t]); }
+    }
+    dump(out + "/huff_isym.bin", hs.data(), hs.size() * 8);
     // host backward-search counts through the fm_index wrapper
     wtg::fm_index idx(t.data(), n);
     const uint32_t m = 12;

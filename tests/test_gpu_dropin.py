@@ -119,6 +119,33 @@ def test_dropin_driver(wsfm, orc, variant, dt):
         res = load(d + "/huff.bin")
         assert res[0] == len(np.unique(bwt))
         rows = res[1:].reshape(-1, 5)
+        # interval_symbols (wt_pc_par.hpp:579-617): tree order (left child first,
+        # from the oracle's shape) from three positions on, text order for two
+        H = orc.Huff(bwt).H
+        order = []
+        stack = [0]
+        while stack:
+            v = stack.pop()
+            if H.child0[v] < 0:
+                order.append(int(H.sym[v]))
+            else:
+                stack += [H.child1[v], H.child0[v]]
+        hsy = load(d + "/huff_isym.bin")
+        p = 0
+        while p < hsy.size:
+            i, j, k = (int(x) for x in hsy[p:p + 3])
+            p += 3
+            got = hsy[p:p + 3 * k].reshape(-1, 3).tolist()
+            p += 3 * k
+            seg = bwt[i:j]
+            if j - i == 2 and seg[0] != seg[1]:
+                exp_syms = [int(seg[0]), int(seg[1])]
+            else:
+                present = set(np.unique(seg).tolist())
+                exp_syms = [c for c in order if c in present]
+            assert [g[0] for g in got] == exp_syms, (i, j)
+            for c, ri, rj in got:
+                assert ri == int((bwt[:i] == c).sum()) and rj == int((bwt[:j] == c).sum())
         cs = [65, 67, 71, 84, 78, 0]
         for k, i in enumerate(range(0, n, 101)):
             c = cs[k % 6]

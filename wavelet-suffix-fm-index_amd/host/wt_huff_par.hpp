@@ -2,7 +2,7 @@
 // (wt_huff/wt_huff_par.hpp:68-78 over wt_huff/wt_pc_par.hpp): same include
 // guard, the constructor wt_huff(int_vector<8>& text, size) (wt_pc_par.hpp:321-361)
 // and the query surface size / empty / sigma / operator[] / rank /
-// inverse_select / select.  Counts, Huffman shape, tree bits, rank_support_v
+// inverse_select / select / interval_symbols.  Counts, Huffman shape, tree bits, rank_support_v
 // and select_support_mcl (1 and 0) are built on the GPU (wtg_huff_build) and
 // copied into host supports, so scalar queries run on the host with the
 // reference's results; rank_batch / select_batch / access_batch / count_batch
@@ -10,6 +10,7 @@
 // not occur i times (the reference asserts).
 #ifndef INCLUDED_SDSL_WT_HUFF
 #define INCLUDED_SDSL_WT_HUFF
+#include <algorithm>
 #include <stdexcept>
 #include <string>
 #include <utility>
@@ -107,6 +108,41 @@ class wt_huff {
     return std::make_pair(i, (value_type)m_shape.sym[v]);
   }
 
+  // the k distinct symbols of [i, j) with rank(i, c) and rank(j, c): in tree order
+  // (left child first) from three positions on, in text order for two, as the
+  // reference (cs / rank_c_i / rank_c_j hold at least sigma entries)
+  void interval_symbols(size_type i, size_type j, size_type& k, std::vector<value_type>& cs,
+                        std::vector<size_type>& rank_c_i, std::vector<size_type>& rank_c_j) const {  // :579-617
+    k = 0;
+    if (i == j) return;
+    if (m_sigma == 1) {
+      k = 1;
+      cs[0] = (value_type)m_shape.sym[0];
+      rank_c_i[0] = std::min(i, m_size);
+      rank_c_j[0] = std::min(j, m_size);
+    } else if (j - i == 1) {
+      k = 1;
+      const auto r = inverse_select(i);
+      rank_c_i[0] = r.first;
+      cs[0] = r.second;
+      rank_c_j[0] = r.first + 1;
+    } else if (j - i == 2) {
+      const auto r0 = inverse_select(i), r1 = inverse_select(i + 1);
+      rank_c_i[0] = r0.first; cs[0] = r0.second;
+      rank_c_i[1] = r1.first; cs[1] = r1.second;
+      if (cs[0] == cs[1]) {
+        k = 1;
+        rank_c_j[0] = rank_c_i[0] + 2;
+      } else {
+        k = 2;
+        rank_c_j[0] = rank_c_i[0] + 1;
+        rank_c_j[1] = rank_c_i[1] + 1;
+      }
+    } else {
+      interval_walk(i, j, k, cs, rank_c_i, rank_c_j, 0);
+    }
+  }
+
   size_type select(size_type i, value_type c) const {  // :527-554
     const int leaf = m_shape.c_to_leaf[c];
     if (leaf < 0 || i == 0 || i > m_shape.C[c + 1] - m_shape.C[c])
@@ -123,6 +159,26 @@ class wt_huff {
       v = p;
     }
     return res;
+  }
+
+  // one inner node of interval_symbols (:149-183): [i, j) of node v, j > i
+  void interval_walk(size_type i, size_type j, size_type& k, std::vector<value_type>& cs,
+                     std::vector<size_type>& rank_c_i, std::vector<size_type>& rank_c_j, int v) const {
+    const size_type bp = m_shape.bv_pos[v], br = m_shape.bv_pos_rank[v];
+    const size_type i1 = m_bv_rank(bp + i) - br, j1 = m_bv_rank(bp + j) - br;
+    const size_type i0 = i - i1, j0 = j - j1;
+    const size_type lo[2] = {i0, i1}, hi[2] = {j0, j1};
+    for (int b = 0; b < 2; b++) {
+      if (lo[b] == hi[b]) continue;
+      const int c = m_shape.child[v][b];
+      if (m_shape.child[c][0] >= 0) {
+        interval_walk(lo[b], hi[b], k, cs, rank_c_i, rank_c_j, c);
+      } else {
+        rank_c_i[k] = lo[b];
+        rank_c_j[k] = hi[b];
+        cs[k++] = (value_type)m_shape.sym[c];
+      }
+    }
   }
 
   // ---- batched device queries (device buffers of nq entries)
