@@ -738,8 +738,16 @@ __global__ __launch_bounds__(256) void k_fm_count(HQ h, const uint8_t* __restric
   for (uint64_t q = (uint64_t)blockIdx.x * blockDim.x + threadIdx.x; q < npat;
        q += (uint64_t)gridDim.x * blockDim.x) {
     const uint8_t* P = pats + q * m;
+    // the first step needs no rank: [0, n) backward-extended by c is [C[c], C[c+1])
     uint64_t sp = 0, ep = h.n;
-    for (int j = (int)m - 1; j >= 0 && sp < ep; j--) {
+    int j = (int)m - 1;
+    if (m > 0) {
+      const uint32_t c = P[j];
+      sp = s_C[c];
+      ep = s_C[c + 1];
+      j--;
+    }
+    for (; j >= 0 && sp < ep; j--) {
       const uint32_t c = P[j];
       const uint32_t len = Q.len[c];
       if (!len && !(h.sigma == 1 && c == h.single)) { sp = ep = 0; break; }
