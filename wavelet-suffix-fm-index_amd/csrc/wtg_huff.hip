@@ -727,46 +727,81 @@ __global__ __launch_bounds__(256) void k_huff_select(HQ h, const uint64_t* __res
 }
 
 // FM-index count: backward search, [sp, ep) <- C[c] + rank_c([sp, ep)) from the
-// last pattern byte; rank_c of both ends walks the same nodes
+// last pattern byte; rank_c of both ends walks the same nodes.  Two patterns per
+// lane (2t, 2t+1), their searches interleaved step by step: both patterns' rank
+// blocks for a level are loaded before either is used, so a lane keeps two
+// dependent chains in flight (1M length-20 DNA patterns: 0.76 -> 0.68 ms)
 __global__ __launch_bounds__(256) void k_fm_count(HQ h, const uint8_t* __restrict__ pats, uint32_t m,
-                                                  uint64_t npat, uint64_t* __restrict__ out) {
+                                                   uint64_t npat, uint64_t* __restrict__ out) {
   __shared__ QLds Q;
   __shared__ uint64_t s_C[257];
   load_q(Q, h.tab);
   for (int k = threadIdx.x; k < 257; k += blockDim.x) s_C[k] = h.tab->C[k];
   __syncthreads();
-  for (uint64_t q = (uint64_t)blockIdx.x * blockDim.x + threadIdx.x; q < npat;
-       q += (uint64_t)gridDim.x * blockDim.x) {
-    const uint8_t* P = pats + q * m;
-    // the first step needs no rank: [0, n) backward-extended by c is [C[c], C[c+1])
-    uint64_t sp = 0, ep = h.n;
-    int j = (int)m - 1;
-    if (m > 0) {
-      const uint32_t c = P[j];
-      sp = s_C[c];
-      ep = s_C[c + 1];
-      j--;
+  const uint64_t npair = (npat + 1) / 2;
+  for (uint64_t t = (uint64_t)blockIdx.x * blockDim.x + threadIdx.x; t < npair;
+       t += (uint64_t)gridDim.x * blockDim.x) {
+    const bool has1 = 2 * t + 1 < npat;
+    const uint8_t* P[2] = {pats + 2 * t * m, pats + (has1 ? 2 * t + 1 : 2 * t) * m};
+    uint64_t sp[2], ep[2];
+#pragma unroll
+    for (int p = 0; p < 2; p++) {  // the first step needs no rank
+      const uint32_t c = P[p][m - 1];
+      sp[p] = s_C[c];
+      ep[p] = s_C[c + 1];
     }
-    for (; j >= 0 && sp < ep; j--) {
-      const uint32_t c = P[j];
-      const uint32_t len = Q.len[c];
-      if (!len && !(h.sigma == 1 && c == h.single)) { sp = ep = 0; break; }
-      uint64_t a = sp, b = ep;
-      uint64_t code = Q.code[c];
-      uint32_t v = 0;
-      for (uint32_t l = 0; l < len && a < b; l++, code >>= 1) {
-        const uint64_t bp = Q.bvpos[v], rb = Q.rbase[v];
-        const uint64_t ra = fmr_rank(h.fmr, bp + a) - rb;
-        const uint64_t rbv = fmr_rank(h.fmr, bp + b) - rb;
-        const uint32_t bit = (uint32_t)code & 1u;
-        if (bit) { a = ra; b = rbv; } else { a -= ra; b -= rbv; }
-        v = Q.child[v][bit];
+    if (!has1) sp[1] = ep[1] = 0;
+    for (int j = (int)m - 2; j >= 0; j--) {
+      if (sp[0] >= ep[0] && sp[1] >= ep[1]) break;
+      uint32_t cc[2], len[2], v[2] = {0, 0};
+      uint64_t code[2], a[2], b[2];
+#pragma unroll
+      for (int p = 0; p < 2; p++) {
+        cc[p] = P[p][j];
+        len[p] = 0;
+        a[p] = sp[p];
+        b[p] = ep[p];
+        code[p] = Q.code[cc[p]];
+        if (sp[p] < ep[p]) {
+          len[p] = Q.len[cc[p]];
+          if (!len[p] && !(h.sigma == 1 && cc[p] == h.single)) a[p] = b[p] = 0;  // absent symbol
+        }
       }
-      if (a >= b) { sp = ep = 0; break; }
-      sp = s_C[c] + a;
-      ep = s_C[c] + b;
+      const uint32_t lmax = len[0] > len[1] ? len[0] : len[1];
+      for (uint32_t l = 0; l < lmax; l++) {
+        bool go[2];
+        uint64_t ia[2], ib[2];
+#pragma unroll
+        for (int p = 0; p < 2; p++) {
+          go[p] = l < len[p] && a[p] < b[p];
+          const uint64_t bp = Q.bvpos[v[p]];
+          ia[p] = go[p] ? bp + a[p] : 0;
+          ib[p] = go[p] ? bp + b[p] : 0;
+        }
+        const uint64_t ra0 = fmr_rank(h.fmr, ia[0]), rb0 = fmr_rank(h.fmr, ib[0]);
+        const uint64_t ra1 = fmr_rank(h.fmr, ia[1]), rb1 = fmr_rank(h.fmr, ib[1]);
+        const uint64_t ra[2] = {ra0, ra1}, rbv[2] = {rb0, rb1};
+#pragma unroll
+        for (int p = 0; p < 2; p++) {
+          if (!go[p]) continue;
+          const uint64_t rb = Q.rbase[v[p]];
+          const uint64_t x = ra[p] - rb, y = rbv[p] - rb;
+          const uint32_t bit = (uint32_t)code[p] & 1u;
+          if (bit) { a[p] = x; b[p] = y; } else { a[p] -= x; b[p] -= y; }
+          v[p] = Q.child[v[p]][bit];
+          code[p] >>= 1;
+        }
+      }
+#pragma unroll
+      for (int p = 0; p < 2; p++) {
+        if (sp[p] >= ep[p]) continue;
+        if (a[p] >= b[p]) { sp[p] = ep[p] = 0; continue; }
+        sp[p] = s_C[cc[p]] + a[p];
+        ep[p] = s_C[cc[p]] + b[p];
+      }
     }
-    out[q] = ep > sp ? ep - sp : 0;
+    out[2 * t] = ep[0] > sp[0] ? ep[0] - sp[0] : 0;
+    if (has1) out[2 * t + 1] = ep[1] > sp[1] ? ep[1] - sp[1] : 0;
   }
 }
 
@@ -1137,8 +1172,8 @@ extern "C" int wtg_fm_count_batch(wtg_ctx* ctx, const wtg_huff* wt, const uint8_
   WTG_REQUIRE(m >= 1, "pattern length 0");
   if (!npat) return WTG_OK;
   WTG_TRY(wtg_set_device(ctx));
-  WTG_LAUNCH(ctx, "fm_count", k_fm_count, dim3(query_grid(npat)), dim3(256), 0, make_hq(wt), d_pats, m, npat,
-             d_out);
+  WTG_LAUNCH(ctx, "fm_count", k_fm_count, dim3(query_grid((npat + 1) / 2)), dim3(256), 0, make_hq(wt), d_pats,
+             m, npat, d_out);
   WTG_LAUNCH_CHECK();
   return wtg_finish(ctx);
 }
