@@ -124,6 +124,41 @@ int main(int argc, char** argv) {
       for (uint64_t t = 0; t < k; t++) { is.push_back(cs[t]); is.push_back(ri[t]); is.push_back(rj[t]); }
     }
     dump(out + "/wtint_isym.bin", is.data(), is.size() * 8);
+    // node interface: leaves reached by expand from the root (sym, size), then
+    // ranges followed down the path of c (the final size is the count of c)
+    std::vector<uint64_t> nd;
+    std::vector<sdsl::wt_int<>::node_type> stack{wt.root()};
+    while (!stack.empty()) {
+      const auto v = stack.back();
+      stack.pop_back();
+      if (wt.empty(v)) continue;
+      if (wt.is_leaf(v)) { nd.push_back(wt.sym(v)); nd.push_back(v.size); continue; }
+      const auto ch = wt.expand(v);
+      stack.push_back(ch.second);
+      stack.push_back(ch.first);
+    }
+    dump(out + "/wtint_leaves.bin", nd.data(), nd.size() * 8);
+    std::vector<uint64_t> ex;
+    for (uint64_t q = 0; q < 40; q++) {
+      const uint64_t i = (q * 7919) % n, j = std::min<uint64_t>(n - 1, i + (q * 977) % 9000);
+      const uint64_t c = s[(q * 31) % n];
+      auto v = wt.root();
+      sdsl::range_type r{{i, j}};
+      sdsl::range_vec_type rv{r, sdsl::range_type{{0, n - 1}}};
+      while (!wt.is_leaf(v)) {
+        const int bit = (c >> (wt.max_level - 1 - v.level)) & 1;
+        const auto rr = wt.expand(v, r);
+        const auto rrv = wt.expand(v, rv);
+        const auto ch = wt.expand(v);
+        r = bit ? rr.second : rr.first;
+        rv = bit ? rrv.second : rrv.first;
+        v = bit ? ch.second : ch.first;
+      }
+      ex.push_back(i); ex.push_back(j); ex.push_back(c);
+      ex.push_back(r[1] + 1 - r[0]); ex.push_back(rv[0][1] + 1 - rv[0][0]); ex.push_back(rv[1][1] + 1 - rv[1][0]);
+      ex.push_back(v.size); ex.push_back(wt.sym(v));
+    }
+    dump(out + "/wtint_expand.bin", ex.data(), ex.size() * 8);
   }
   // ---- FMIndex() + wt_huff<> over the BWT (fm-index/par/WT.C:33-45)
   {

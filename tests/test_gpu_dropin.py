@@ -110,6 +110,15 @@ def test_dropin_driver(wsfm, orc, variant, dt):
             assert k == vals.size
             for (c, ri, rj), v in zip(got.tolist(), vals.tolist()):
                 assert c == v and ri == int((sv[:i] == v).sum()) and rj == int((sv[:j] == v).sum())
+        # node interface (wt_int_par.hpp:915-1085): leaves by expand from the root,
+        # and ranges followed down the path of c
+        lv = load(d + "/wtint_leaves.bin").reshape(-1, 2)
+        vals, cnts = np.unique(sv, return_counts=True)
+        assert np.array_equal(lv[:, 0], vals.astype(np.uint64)) and np.array_equal(lv[:, 1], cnts.astype(np.uint64))
+        for i, j, c, n1, n2, n3, vsz, vsym in load(d + "/wtint_expand.bin").reshape(-1, 8).tolist():
+            assert vsym == c and vsz == int((sv == c).sum())
+            assert n1 == n2 == int((sv[i:j + 1] == c).sum())
+            assert n3 == vsz
         # FMIndex() + wt_huff<>
         n = 200003
         t = wsfm.gen_dna(14, n)

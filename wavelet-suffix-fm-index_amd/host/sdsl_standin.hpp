@@ -7,6 +7,7 @@
 // i/64; capacity() = 64*ceil(size/64); one zero padding word follows the data.
 #pragma once
 #ifndef INCLUDED_SDSL_INT_VECTOR
+#include <array>
 #include <cstdint>
 #include <istream>
 #include <ostream>
@@ -66,6 +67,9 @@ class int_vector<1> {
 };
 
 typedef int_vector<1> bit_vector;
+// closed index ranges [first, second] of the node interface (sdsl's wt_helper)
+typedef std::array<uint64_t, 2> range_type;
+typedef std::vector<range_type> range_vec_type;
 
 }  // namespace sdsl
 #endif

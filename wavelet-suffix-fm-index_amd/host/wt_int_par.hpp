@@ -2,7 +2,8 @@
 // (wt_int/wt_int_par.hpp:57-1086): same include guard, constructor
 // wt_int(int_vector<w>& buf, size, max_level=0) and the query surface size /
 // empty / sigma / tree / max_level / operator[] / rank / inverse_select /
-// select / interval_symbols / lex_count / lex_smaller_count / range_search_2d.  The tree bits, rank_support_v and select_support_mcl (for 1 and 0)
+// select / interval_symbols / lex_count / lex_smaller_count / range_search_2d and
+// the node interface (root / is_leaf / sym / empty / expand / path).  The tree bits, rank_support_v and select_support_mcl (for 1 and 0)
 // are built on the GPU (wtg_wtint_build) and copied into host supports, so the
 // scalar queries run the reference's host algorithms; *_batch() run batches on
 // the device against the device copy.  select() throws std::logic_error when
@@ -186,6 +187,52 @@ class wt_int {
     range2d(lb, rb, vlb, vrb, 0, 0, m_size, offs, obs, 0, report, res);
     return res;
   }
+
+  // ---- node interface (:915-1085): a node is (offset of its bits, size, level,
+  // value prefix)
+  struct node_type {
+    size_type offset = 0, size = 0, level = 0;
+    value_type sym = 0;
+    node_type(size_type o = 0, size_type sz = 0, size_type l = 0, value_type sy = 0)
+        : offset(o), size(sz), level(l), sym(sy) {}
+    bool operator==(const node_type& v) const { return offset == v.offset; }
+    bool operator<(const node_type& v) const { return offset < v.offset; }
+    bool operator>(const node_type& v) const { return offset > v.offset; }
+  };
+  bool is_leaf(const node_type& v) const { return v.level == m_max_level; }
+  value_type sym(const node_type& v) const { return v.sym; }
+  bool empty(const node_type& v) const { return v.size == 0; }
+  node_type root() const { return node_type(0, m_size, 0, 0); }
+  // the two children of an inner node
+  std::pair<node_type, node_type> expand(const node_type& v) const {
+    const size_type ones = m_tree_rank(v.offset + v.size) - m_tree_rank(v.offset);
+    const node_type left(v.offset + m_size, v.size - ones, v.level + 1, v.sym << 1);
+    const node_type right(v.offset + m_size + left.size, ones, v.level + 1, (v.sym << 1) | 1);
+    return std::make_pair(left, right);
+  }
+  // a closed range [s, e] of v mapped into both children (an empty result is
+  // [x, x-1], in size_type arithmetic as the reference)
+  std::pair<range_type, range_type> expand(const node_type& v, const range_type& r) const {
+    const size_type base = m_tree_rank(v.offset);
+    const size_type sr = m_tree_rank(v.offset + r[0]);
+    const size_type right_size = m_tree_rank(v.offset + r[1] + 1) - sr;
+    const size_type left_size = (r[1] - r[0] + 1) - right_size;
+    const size_type right_sp = sr - base, left_sp = r[0] - right_sp;
+    return std::make_pair(range_type{{left_sp, left_sp + left_size - 1}},
+                          range_type{{right_sp, right_sp + right_size - 1}});
+  }
+  std::pair<range_vec_type, range_vec_type> expand(const node_type& v, const range_vec_type& ranges) const {
+    std::pair<range_vec_type, range_vec_type> res;
+    res.first.reserve(ranges.size());
+    res.second.reserve(ranges.size());
+    for (const auto& r : ranges) {
+      const auto lr = expand(v, r);
+      res.first.push_back(lr.first);
+      res.second.push_back(lr.second);
+    }
+    return res;
+  }
+  std::pair<uint64_t, uint64_t> path(value_type c) const { return {m_max_level, c}; }
 
   size_type select(size_type i, value_type c) const {  // :622-660
     std::vector<size_type> path_off(m_max_level + 1), path_rank(m_max_level + 1);
