@@ -222,8 +222,8 @@ struct wtg_batch {
 
 static void batch_release(wtg_ctx* ctx, wtg_batch* b) {
   if (!b) return;
-  if (b->exec) hipGraphExecDestroy(b->exec);
-  if (b->g) hipGraphDestroy(b->g);
+  if (b->exec) (void)hipGraphExecDestroy(b->exec);
+  if (b->g) (void)hipGraphDestroy(b->g);
   for (auto& G : b->groups) {
     wtg_free(ctx, G.d_seqoff);
     wtg_free(ctx, G.d_pref);
@@ -396,8 +396,8 @@ extern "C" int wtg_batch_run(wtg_ctx* ctx, wtg_batch* b) {
     return wtg_finish(ctx);
   }
   if (b->exec && b->graph_ws != ctx->ws) {  // the workspace moved: record again
-    hipGraphExecDestroy(b->exec);
-    hipGraphDestroy(b->g);
+    (void)hipGraphExecDestroy(b->exec);
+    (void)hipGraphDestroy(b->g);
     b->exec = nullptr;
     b->g = nullptr;
   }
@@ -410,7 +410,7 @@ extern "C" int wtg_batch_run(wtg_ctx* ctx, wtg_batch* b) {
     const hipError_t e = hipStreamEndCapture(ctx->stream, &g);
     ctx->ktime = kt;
     if (rc != WTG_OK) {
-      if (g) hipGraphDestroy(g);
+      if (g) (void)hipGraphDestroy(g);
       return rc;
     }
     WTG_CHECK_HIP(e);

@@ -54,7 +54,7 @@ extern "C" int wtg_ctx_create(int device, wtg_ctx** out) {
         hipDeviceGetDefaultMemPool(&pool, device) == hipSuccess &&
         hipMemPoolSetAttribute(pool, hipMemPoolAttrReleaseThreshold, &keep) == hipSuccess)
       c->pooled = true;
-    hipGetLastError();
+    (void)hipGetLastError();
   }
   WTG_CHECK_HIP(hipDeviceSynchronize());
   *out = c;
@@ -63,15 +63,15 @@ extern "C" int wtg_ctx_create(int device, wtg_ctx** out) {
 
 extern "C" int wtg_ctx_destroy(wtg_ctx* ctx) {
   if (!ctx) return WTG_OK;
-  hipSetDevice(ctx->device);
-  hipStreamSynchronize(ctx->stream);
-  for (auto& p : ctx->kpending) { hipEventDestroy(p.a); hipEventDestroy(p.b); }
-  for (auto e : ctx->kpool) hipEventDestroy(e);
-  if (ctx->ws) hipFree(ctx->ws);
-  if (ctx->d_flag) hipFree(ctx->d_flag);
-  if (ctx->t0) hipEventDestroy(ctx->t0);
-  if (ctx->t1) hipEventDestroy(ctx->t1);
-  hipStreamDestroy(ctx->stream);
+  (void)hipSetDevice(ctx->device);
+  (void)hipStreamSynchronize(ctx->stream);
+  for (auto& p : ctx->kpending) { (void)hipEventDestroy(p.a); (void)hipEventDestroy(p.b); }
+  for (auto e : ctx->kpool) (void)hipEventDestroy(e);
+  if (ctx->ws) (void)hipFree(ctx->ws);
+  if (ctx->d_flag) (void)hipFree(ctx->d_flag);
+  if (ctx->t0) (void)hipEventDestroy(ctx->t0);
+  if (ctx->t1) (void)hipEventDestroy(ctx->t1);
+  (void)hipStreamDestroy(ctx->stream);
   delete ctx;
   return WTG_OK;
 }
@@ -115,7 +115,7 @@ int wtg_ws_reserve(wtg_ctx* ctx, size_t bytes) {
   hipError_t e = hipMalloc(&ctx->ws, want);
   if (e != hipSuccess) {
     ctx->ws = nullptr;
-    hipGetLastError();
+    (void)hipGetLastError();
     wtg_set_error("workspace allocation failed");
     return WTG_ERR_NOMEM;
   }
@@ -128,7 +128,7 @@ extern "C" int wtg_malloc(wtg_ctx* ctx, void** dptr, uint64_t bytes) {
   WTG_TRY(wtg_set_device(ctx));
   hipError_t e = ctx->pooled ? hipMallocAsync(dptr, bytes ? bytes : 1, ctx->stream) : hipMalloc(dptr, bytes ? bytes : 1);
   if (e != hipSuccess) {
-    hipGetLastError();
+    (void)hipGetLastError();
     *dptr = nullptr;
     wtg_set_error("hipMalloc failed");
     return WTG_ERR_NOMEM;
@@ -226,19 +226,19 @@ extern "C" int wtg_timer_stop(wtg_ctx* ctx, float* ms) {
 static hipEvent_t kpool_get(wtg_ctx* ctx) {
   if (!ctx->kpool.empty()) { hipEvent_t e = ctx->kpool.back(); ctx->kpool.pop_back(); return e; }
   hipEvent_t e = nullptr;
-  hipEventCreate(&e);
+  (void)hipEventCreate(&e);
   return e;
 }
 wtg_kscope::wtg_kscope(wtg_ctx* c, const char* n) : ctx(c), name(n) {
   if (ctx->ktime && (ctx->kfilter.empty() || std::strncmp(n, ctx->kfilter.c_str(), ctx->kfilter.size()) == 0)) {
     a = kpool_get(ctx);
     b = kpool_get(ctx);
-    hipEventRecord(a, ctx->stream);
+    (void)hipEventRecord(a, ctx->stream);
   }
 }
 wtg_kscope::~wtg_kscope() {
   if (a && b) {
-    hipEventRecord(b, ctx->stream);
+    (void)hipEventRecord(b, ctx->stream);
     ctx->kpending.push_back({std::string(name), a, b});
   }
 }
