@@ -291,6 +291,16 @@ def bench_fm(pkg, args):
     bwt = pkg.bwt_from_sa(t, sa)
     pats = pkg.gen_patterns(SEED + 4, t, 20, args.fm_patterns)
     ctx = pkg.Context(0)
+    # the same suffix array on the GPU (prefix doubling), checked against the host's
+    dt = ctx.to_device(t)
+    pkg.suffix_array_gpu(ctx, dt, n).free()
+    ctx.sync()
+    t0 = time.perf_counter()
+    dsa = pkg.suffix_array_gpu(ctx, dt, n)
+    ctx.sync()
+    gpu_sa_s = time.perf_counter() - t0
+    sa_match = bool(np.array_equal(dsa.get(), sa))
+    dsa.free(); dt.free()
     d = ctx.to_device(bwt)
     for _ in range(max(args.warmup, 1)):
         wt = ctx.huff_build(d)
@@ -332,7 +342,8 @@ def bench_fm(pkg, args):
            "kernels_ms": {k: round(v[0] / max(v[1], 1), 4) for k, v in kt.items()},
            "emit_GBps": round((n + bits / 8.0) / (emit_ms / 1e3) / 1e9, 1) if emit_ms else None,
            "count_ms": round(q_ms, 4), "count_Mpat_s": round(pats.shape[0] / (q_ms / 1e3) / 1e6, 2),
-           "counts_match_sa_binary_search": ok, "host_sa_s": round(sa_s, 2)}
+           "counts_match_sa_binary_search": ok, "host_sa_s": round(sa_s, 2), "gpu_sa_s": round(gpu_sa_s, 4),
+           "gpu_sa_matches_host": sa_match}
     if not args.no_cpu:
         # CPU: the oracle's serial restatement of the Huffman tree bits (wt_huff_par.hpp:92-117,
         # wt_pc_par.hpp:193-257) on a prefix of the BWT
@@ -345,7 +356,7 @@ def bench_fm(pkg, args):
                                          "first %d BWT bytes" % m}
     wt.free(); d.free(); dp.free(); do.free(); ctx.close()
     print(json.dumps(out), flush=True)
-    return 0 if ok else 1
+    return 0 if ok and sa_match else 1
 
 
 def bench_bytes(pkg, args):

@@ -237,6 +237,10 @@ int wtg_huff_select_batch(wtg_ctx* ctx, const wtg_huff* wt, const uint64_t* d_i,
  * fm-index/par/WT.C:35-36): suffixes in lexicographic order, a proper prefix
  * before its extensions.  Host memory, n < 2^40. */
 int wtg_suffix_array(const uint8_t* text, uint64_t n, int64_t* sa);
+/* The same suffix array on the GPU (device text and output, n < 2^31): prefix
+ * doubling with device radix sorts, in place of the reference's divsufsort call
+ * (fm-index/par/WT.C:35-36). */
+int wtg_suffix_array_gpu(wtg_ctx* ctx, const uint8_t* d_text, uint64_t n, int64_t* d_sa);
 /* bwt[i] = text[sa[i]-1], or text[n-1] where sa[i] = 0  (fm-index/par/WT.C:38-41) */
 int wtg_bwt_from_sa(const uint8_t* text, const int64_t* sa, uint64_t n, uint8_t* bwt);
 /* Backward-search count of npat patterns of m bytes each (d_pats, pattern q at

@@ -1,0 +1,46 @@
+"""Suffix array on the GPU (prefix doubling, wtg_suffix_array_gpu) against the
+host SA-IS (wtg_suffix_array, itself pinned by a comparison sort in the CPU
+tests) and a direct sort of the suffixes on small texts: identical arrays, and
+the BWT built from it (fm-index/par/WT.C:38-41)."""
+import numpy as np
+import pytest
+
+pytestmark = pytest.mark.gpu
+
+
+def texts(wsfm):
+    rng = np.random.default_rng(17)
+    yield "dna", wsfm.gen_dna(5, 1 << 20)
+    yield "random256", rng.integers(0, 256, 300001).astype(np.uint8)
+    yield "zeros_inside", rng.integers(0, 3, 100003).astype(np.uint8)   # real 0 bytes, no sentinel
+    yield "all_a", np.full(70000, 97, np.uint8)
+    yield "periodic", np.tile(np.frombuffer(b"abcab", np.uint8), 20001)
+    yield "repeats", np.concatenate([wsfm.gen_dna(6, 40000)[:-1]] * 3 + [np.zeros(1, np.uint8)])
+    for n in (1, 2, 7, 8, 64, 4095, 4096, 4097):
+        yield "tiny%d" % n, rng.integers(0, 4, n).astype(np.uint8) + 65
+
+
+def brute(t):
+    s = t.tobytes()
+    return np.array(sorted(range(len(s)), key=lambda i: s[i:]), np.int64)
+
+
+@pytest.mark.parametrize("case", range(14))
+def test_gpu_suffix_array(wsfm, orc, ctx, case):
+    cases = list(texts(wsfm))
+    assert len(cases) == 14
+    name, t = cases[case]
+    print("sa case", name, t.size, flush=True)
+    d = ctx.to_device(t)
+    dsa = wsfm.suffix_array_gpu(ctx, d, t.size)
+    try:
+        got = dsa.get()
+        exp = wsfm.suffix_array(t)
+        assert np.array_equal(got, exp), name
+        if t.size <= 5000:
+            assert np.array_equal(got, brute(t)), name
+        if name == "dna":
+            assert np.array_equal(orc.bwt_from_sa(t, got), wsfm.bwt_from_sa(t, exp))
+    finally:
+        dsa.free()
+        d.free()

@@ -122,6 +122,7 @@ SIGNATURES = {
     "wtg_huff_access_batch": (i32, [vp, vp, vp, u64, vp, vp]),
     "wtg_huff_select_batch": (i32, [vp, vp, vp, vp, u64, vp]),
     "wtg_suffix_array": (i32, [vp, u64, vp]),
+    "wtg_suffix_array_gpu": (i32, [vp, vp, u64, vp]),
     "wtg_bwt_from_sa": (i32, [vp, vp, u64, vp]),
     "wtg_fm_count_batch": (i32, [vp, vp, vp, u32, u64, vp]),
     "wtg_lsel_build": (i32, [vp, u32, C.POINTER(vp), U64P, C.POINTER(LselParams), C.POINTER(vp)]),
@@ -252,6 +253,18 @@ def suffix_array(text):
     sa = np.empty(text.size, dtype=np.int64)
     _check(lib().wtg_suffix_array(_p(text), text.size, _p(sa)))
     return sa
+
+
+def suffix_array_gpu(ctx, d_text, n):
+    """Suffix array of a device byte text on the GPU (prefix doubling); returns
+    the device array (int64)."""
+    d_sa = ctx.empty(n, np.int64)
+    try:
+        _check(lib().wtg_suffix_array_gpu(ctx.h, d_text.ptr, n, d_sa.ptr))
+    except Exception:
+        d_sa.free()
+        raise
+    return d_sa
 
 
 def bwt_from_sa(text, sa):
