@@ -1,9 +1,10 @@
 // fm_index_par.hpp -- drop-in for the FM-index construction of the reference
 // (fm-index/par/WT.h:52, fm-index/par/WT.C:33-45):
 //   void FMIndex(unsigned char* s, long n, sdsl::int_vector<8>& bwt)
-// computes the suffix array (host SA-IS in place of divsufsort; the suffix
-// array is unique, so the BWT is the same), the BWT bwt[i] = s[SA[i]-1] (s[n-1]
-// where SA[i] = 0) and the Huffman-shaped wavelet tree over it on the GPU.
+// computes the suffix array (on the GPU by prefix doubling below 2^31 bytes,
+// host SA-IS above, in place of divsufsort; the suffix array is unique, so the
+// BWT is the same), the BWT bwt[i] = s[SA[i]-1] (s[n-1] where SA[i] = 0) and the
+// Huffman-shaped wavelet tree over it on the GPU.
 // The reference discards the tree; fm_index below keeps it and answers
 // backward-search counts on the host (count) or in device batches.
 #ifndef WTG_FM_INDEX_PAR_HPP
@@ -22,7 +23,14 @@ class fm_index {
     m_bwt.resize(n);
     if (n) {
       std::vector<int64_t> sa(n);
-      check(wtg_suffix_array(s, n, sa.data()));
+      if (n < (1ull << 31)) {  // suffix array on the GPU (prefix doubling)
+        dbuf dt(n), dsa(n * 8);
+        dt.upload(s, n);
+        check(wtg_suffix_array_gpu(default_context(), dt.as<uint8_t>(), n, dsa.as<int64_t>()));
+        dsa.download(sa.data(), n * 8);
+      } else {
+        check(wtg_suffix_array(s, n, sa.data()));
+      }
       check(wtg_bwt_from_sa(s, sa.data(), n, m_bwt.data()));
     }
     m_wt = sdsl::wt_huff<>(m_bwt.data(), n);
