@@ -241,6 +241,9 @@ int wtg_suffix_array(const uint8_t* text, uint64_t n, int64_t* sa);
  * doubling with device radix sorts, in place of the reference's divsufsort call
  * (fm-index/par/WT.C:35-36). */
 int wtg_suffix_array_gpu(wtg_ctx* ctx, const uint8_t* d_text, uint64_t n, int64_t* d_sa);
+/* wtg_bwt_from_sa on the GPU (device buffers); an entry of d_sa outside [0, n)
+ * is reported as WTG_ERR_ARG. */
+int wtg_bwt_from_sa_gpu(wtg_ctx* ctx, const uint8_t* d_text, const int64_t* d_sa, uint64_t n, uint8_t* d_bwt);
 /* bwt[i] = text[sa[i]-1], or text[n-1] where sa[i] = 0  (fm-index/par/WT.C:38-41) */
 int wtg_bwt_from_sa(const uint8_t* text, const int64_t* sa, uint64_t n, uint8_t* bwt);
 /* Backward-search count of npat patterns of m bytes each (d_pats, pattern q at

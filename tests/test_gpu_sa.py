@@ -39,8 +39,11 @@ def test_gpu_suffix_array(wsfm, orc, ctx, case):
         assert np.array_equal(got, exp), name
         if t.size <= 5000:
             assert np.array_equal(got, brute(t)), name
-        if name == "dna":
-            assert np.array_equal(orc.bwt_from_sa(t, got), wsfm.bwt_from_sa(t, exp))
+        dbwt = wsfm.bwt_from_sa_gpu(ctx, d, dsa, t.size)
+        try:
+            assert np.array_equal(dbwt.get(), orc.bwt_from_sa(t, exp)), name
+        finally:
+            dbwt.free()
     finally:
         dsa.free()
         d.free()

@@ -123,6 +123,7 @@ SIGNATURES = {
     "wtg_huff_select_batch": (i32, [vp, vp, vp, vp, u64, vp]),
     "wtg_suffix_array": (i32, [vp, u64, vp]),
     "wtg_suffix_array_gpu": (i32, [vp, vp, u64, vp]),
+    "wtg_bwt_from_sa_gpu": (i32, [vp, vp, vp, u64, vp]),
     "wtg_bwt_from_sa": (i32, [vp, vp, u64, vp]),
     "wtg_fm_count_batch": (i32, [vp, vp, vp, u32, u64, vp]),
     "wtg_lsel_build": (i32, [vp, u32, C.POINTER(vp), U64P, C.POINTER(LselParams), C.POINTER(vp)]),
@@ -265,6 +266,17 @@ def suffix_array_gpu(ctx, d_text, n):
         d_sa.free()
         raise
     return d_sa
+
+
+def bwt_from_sa_gpu(ctx, d_text, d_sa, n):
+    """BWT from a device text and suffix array on the GPU; returns the device array."""
+    d_bwt = ctx.empty(n, np.uint8)
+    try:
+        _check(lib().wtg_bwt_from_sa_gpu(ctx.h, d_text.ptr, d_sa.ptr, n, d_bwt.ptr))
+    except Exception:
+        d_bwt.free()
+        raise
+    return d_bwt
 
 
 def bwt_from_sa(text, sa):

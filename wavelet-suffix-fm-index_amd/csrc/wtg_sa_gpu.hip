@@ -144,7 +144,32 @@ __global__ void k_sa_out(const uint32_t* __restrict__ idx, uint64_t n, int64_t* 
   if (i < n) sa[i] = (int64_t)idx[i];
 }
 
+// bwt[i] = t[sa[i]-1], or t[n-1] where sa[i] = 0 (fm-index/par/WT.C:38-41)
+__global__ void k_bwt(const uint8_t* __restrict__ t, const int64_t* __restrict__ sa, uint64_t n,
+                      uint8_t* __restrict__ bwt, uint32_t* __restrict__ err) {
+  const uint64_t i = (uint64_t)blockIdx.x * blockDim.x + threadIdx.x;
+  if (i >= n) return;
+  const int64_t p = sa[i];
+  if (p < 0 || (uint64_t)p >= n) {  // not a suffix array of this text
+    atomicOr(err, 2u);
+    bwt[i] = 0;
+    return;
+  }
+  bwt[i] = p ? t[p - 1] : t[n - 1];
+}
+
 }  // namespace
+
+extern "C" int wtg_bwt_from_sa_gpu(wtg_ctx* ctx, const uint8_t* d_text, const int64_t* d_sa, uint64_t n,
+                                   uint8_t* d_bwt) {
+  WTG_REQUIRE(ctx && (n == 0 || (d_text && d_sa && d_bwt)), "wtg_bwt_from_sa_gpu: null argument");
+  if (n == 0) return WTG_OK;
+  WTG_TRY(wtg_set_device(ctx));
+  WTG_LAUNCH(ctx, "bwt", k_bwt, dim3((unsigned)wtg_ceil_div(n, 256)), dim3(256), 0, d_text, d_sa, n, d_bwt,
+             ctx->d_flag);
+  WTG_LAUNCH_CHECK();
+  return wtg_finish(ctx);
+}
 
 extern "C" int wtg_suffix_array_gpu(wtg_ctx* ctx, const uint8_t* d_text, uint64_t n, int64_t* d_sa) {
   WTG_REQUIRE(ctx && (n == 0 || (d_text && d_sa)), "wtg_suffix_array_gpu: null argument");

@@ -22,16 +22,17 @@ class fm_index {
   fm_index(const unsigned char* s, uint64_t n) : m_n(n) {
     m_bwt.resize(n);
     if (n) {
-      std::vector<int64_t> sa(n);
-      if (n < (1ull << 31)) {  // suffix array on the GPU (prefix doubling)
-        dbuf dt(n), dsa(n * 8);
+      if (n < (1ull << 31)) {  // suffix array and BWT on the GPU (prefix doubling)
+        dbuf dt(n), dsa(n * 8), db(n);
         dt.upload(s, n);
         check(wtg_suffix_array_gpu(default_context(), dt.as<uint8_t>(), n, dsa.as<int64_t>()));
-        dsa.download(sa.data(), n * 8);
+        check(wtg_bwt_from_sa_gpu(default_context(), dt.as<uint8_t>(), dsa.as<int64_t>(), n, db.as<uint8_t>()));
+        db.download(m_bwt.data(), n);
       } else {
+        std::vector<int64_t> sa(n);
         check(wtg_suffix_array(s, n, sa.data()));
+        check(wtg_bwt_from_sa(s, sa.data(), n, m_bwt.data()));
       }
-      check(wtg_bwt_from_sa(s, sa.data(), n, m_bwt.data()));
     }
     m_wt = sdsl::wt_huff<>(m_bwt.data(), n);
   }
