@@ -1,12 +1,15 @@
 // wtg_sa_gpu.hip -- suffix array of a byte text on the GPU by prefix doubling
 // (the reference calls divsufsort, fm-index/par/WT.C:35-36; any exact suffix
-// sorter gives the same array).  Round 0 sorts the suffixes by their first 7
+// sorter gives the same array).  Round 0 sorts all suffixes by their first 7
 // bytes, each coded as byte+1 in 9 bits with 0 past the text end, so a suffix
-// that ends sorts before its extensions.  Each later round h = 7, 14, 28, ...
-// sorts by (rank of the h-prefix, rank of the h-prefix h bytes on, +1; 0 past
-// the end) with a device radix sort; a suffix's rank is the position of its
-// group's first suffix in the sorted order, so ranks fit 32 bits and the key
-// 64.  It stops once every rank is distinct.
+// that ends sorts before its extensions.  A suffix's rank is the position of
+// its group's first suffix in the sorted order (32 bits).  Each later round
+// h = 7, 14, 28, ... re-sorts only the suffixes still in groups of two or more,
+// by (group rank, rank h bytes on + 1; 0 past the end), and splits each group
+// in place: new rank = group rank + offset of the subgroup's first suffix in
+// the group.  Suffixes left alone in a group keep their final rank and drop out
+// of later rounds (the ranks stay consistent, as in Larsson-Sadakane doubling).
+// It stops when no group of two or more is left; sa[rank[i]] = i.
 #include "wtg_internal.hpp"
 #include <rocprim/device/device_radix_sort.hpp>
 
@@ -23,16 +26,9 @@ __global__ void k_sa_init(const uint8_t* __restrict__ t, uint64_t n, uint64_t* _
   idx[i] = (uint32_t)i;
 }
 
-// rank of sorted position p = the first position of its key group: heads
-// scattered as (p if key[p] != key[p-1] else 0), then an inclusive max-scan;
-// here the head values, scanned by the caller
-__global__ void k_sa_heads(const uint64_t* __restrict__ key, uint64_t n, uint32_t* __restrict__ head) {
-  const uint64_t p = (uint64_t)blockIdx.x * blockDim.x + threadIdx.x;
-  if (p >= n) return;
-  head[p] = (p == 0 || key[p] != key[p - 1]) ? (uint32_t)p : 0u;
-}
-
-// inclusive max-scan of head[] within 4096-value tiles (tile maxima to tmax)
+// inclusive max-scan of head flags (p where a run starts, else 0) within 4096-value
+// tiles (tile maxima to tmax); with the exclusive scan of the tile maxima this
+// gives each sorted position the first position of its run (tile maxima to tmax)
 __global__ __launch_bounds__(256) void k_sa_maxscan_tiles(uint32_t* __restrict__ head, uint64_t n,
                                                           uint32_t* __restrict__ tmax) {
   __shared__ uint32_t s_w[4];
@@ -112,36 +108,59 @@ __global__ __launch_bounds__(256) void k_sa_tile_prefix(uint32_t* __restrict__ t
   }
 }
 
-// ranks by suffix position: rank[idx[p]] = max(head[p], earlier tiles' maximum);
-// also counts the distinct groups (heads) into *groups
-__global__ void k_sa_rank(const uint32_t* __restrict__ head, const uint32_t* __restrict__ tmax,
-                          const uint32_t* __restrict__ idx, const uint64_t* __restrict__ key, uint64_t n,
-                          uint32_t* __restrict__ rank, unsigned long long* __restrict__ groups) {
-  const uint64_t p = (uint64_t)blockIdx.x * blockDim.x + threadIdx.x;
-  const bool in = p < n;
-  uint32_t r = 0;
-  bool h = false;
-  if (in) {
-    const uint32_t pre = tmax[p / 4096];  // exclusive max over the earlier tiles
-    r = head[p] > pre ? head[p] : pre;
-    rank[idx[p]] = r;
-    h = p == 0 || key[p] != key[p - 1];
-  }
-  const uint64_t b = wave_ballot(h);
-  if (lane_id() == 0 && b) atomicAdd(groups, (unsigned long long)popc64(b));
+// keys of the unresolved suffixes: (current rank, rank h bytes on + 1; 0 past the end)
+__global__ void k_sa_keys_act(const uint32_t* __restrict__ rank, const uint32_t* __restrict__ act, uint64_t m,
+                              uint64_t n, uint64_t h, uint64_t* __restrict__ key, uint32_t* __restrict__ val) {
+  const uint64_t k = (uint64_t)blockIdx.x * blockDim.x + threadIdx.x;
+  if (k >= m) return;
+  const uint32_t i = act[k];
+  key[k] = ((uint64_t)rank[i] << 32) | (i + h < n ? (uint64_t)rank[i + h] + 1 : 0ull);
+  val[k] = i;
 }
 
-__global__ void k_sa_keys(const uint32_t* __restrict__ rank, uint64_t n, uint64_t h, uint64_t* __restrict__ key,
-                          uint32_t* __restrict__ idx) {
+// after a sort of m keys: fa = k where the rank part (key >> 32) starts a run,
+// fb = k where the whole key starts a run (0 elsewhere; max-scanned next)
+__global__ void k_sa_flags(const uint64_t* __restrict__ key, uint64_t m, int whole_group, uint32_t* __restrict__ fa,
+                           uint32_t* __restrict__ fb) {
+  const uint64_t k = (uint64_t)blockIdx.x * blockDim.x + threadIdx.x;
+  if (k >= m) return;
+  const bool b = k == 0 || key[k] != key[k - 1];
+  const bool a = !whole_group && (k == 0 || (key[k] >> 32) != (key[k - 1] >> 32));
+  fa[k] = a ? (uint32_t)k : 0u;
+  fb[k] = b ? (uint32_t)k : 0u;
+}
+
+// new rank of sorted position k: the group's old rank (0 in round 0) plus the
+// offset of k's subgroup head from the group's first position; keep[k] = 1 when
+// k's subgroup has more than one suffix
+__global__ void k_sa_update(const uint64_t* __restrict__ key, const uint32_t* __restrict__ val, uint64_t m,
+                            int whole_group, const uint32_t* __restrict__ fa, const uint32_t* __restrict__ ta,
+                            const uint32_t* __restrict__ fb, const uint32_t* __restrict__ tb,
+                            uint32_t* __restrict__ rank, uint32_t* __restrict__ keep) {
+  const uint64_t k = (uint64_t)blockIdx.x * blockDim.x + threadIdx.x;
+  if (k >= m) return;
+  const uint64_t t = k / 4096;
+  const uint32_t first = whole_group ? 0u : (fa[k] > ta[t] ? fa[k] : ta[t]);
+  const uint32_t head = fb[k] > tb[t] ? fb[k] : tb[t];
+  const uint32_t g = whole_group ? 0u : (uint32_t)(key[k] >> 32);
+  rank[val[k]] = g + (head - first);
+  const bool single = head == (uint32_t)k && (k + 1 == m || key[k + 1] != key[k]);
+  keep[k] = single ? 0u : 1u;
+}
+
+__global__ void k_sa_compact(const uint32_t* __restrict__ val, const uint32_t* __restrict__ keep,
+                             const uint32_t* __restrict__ pos, uint64_t m, uint32_t* __restrict__ act) {
+  const uint64_t k = (uint64_t)blockIdx.x * blockDim.x + threadIdx.x;
+  if (k < m && keep[k]) act[pos[k]] = val[k];
+}
+
+__global__ void k_sa_final(const uint32_t* __restrict__ rank, uint64_t n, int64_t* __restrict__ sa,
+                           uint32_t* __restrict__ err) {
   const uint64_t i = (uint64_t)blockIdx.x * blockDim.x + threadIdx.x;
   if (i >= n) return;
-  key[i] = ((uint64_t)rank[i] << 32) | (i + h < n ? (uint64_t)rank[i + h] + 1 : 0ull);
-  idx[i] = (uint32_t)i;
-}
-
-__global__ void k_sa_out(const uint32_t* __restrict__ idx, uint64_t n, int64_t* __restrict__ sa) {
-  const uint64_t i = (uint64_t)blockIdx.x * blockDim.x + threadIdx.x;
-  if (i < n) sa[i] = (int64_t)idx[i];
+  const uint32_t r = rank[i];
+  if (r < n) sa[r] = (int64_t)i;
+  else atomicOr(err, 2u);
 }
 
 // bwt[i] = t[sa[i]-1], or t[n-1] where sa[i] = 0 (fm-index/par/WT.C:38-41)
@@ -176,25 +195,30 @@ extern "C" int wtg_suffix_array_gpu(wtg_ctx* ctx, const uint8_t* d_text, uint64_
   WTG_REQUIRE(n < (1ull << 31), "wtg_suffix_array_gpu: n >= 2^31");
   if (n == 0) return WTG_OK;
   WTG_TRY(wtg_set_device(ctx));
+  // Each round sorts only the suffixes whose group is not yet a singleton
+  // (ranks of resolved suffixes are final and stay consistent with the order).
   const uint64_t tiles = wtg_ceil_div(n, 4096);
-  uint64_t *k0 = nullptr, *k1 = nullptr;
-  uint32_t *i0 = nullptr, *i1 = nullptr, *head = nullptr, *rank = nullptr, *tmax = nullptr;
-  unsigned long long* groups = nullptr;
+  uint64_t *k0 = nullptr, *k1 = nullptr, *st = nullptr;
+  uint32_t *v0 = nullptr, *v1 = nullptr, *act = nullptr, *rank = nullptr, *fa = nullptr, *fb = nullptr;
+  uint32_t *ta = nullptr, *tb = nullptr, *keep = nullptr, *pos = nullptr, *ctr = nullptr;
   void* tmp = nullptr;
   size_t tmp_bytes = 0;
   int rc = WTG_OK;
-  const unsigned grid = (unsigned)wtg_ceil_div(n, 256);
-  // sort keys of at most 63 bits (round 0) / 32 + bits(n) bits (later rounds)
   if ((rc = wtg_malloc(ctx, (void**)&k0, n * 8)) != WTG_OK || (rc = wtg_malloc(ctx, (void**)&k1, n * 8)) != WTG_OK ||
-      (rc = wtg_malloc(ctx, (void**)&i0, n * 4)) != WTG_OK || (rc = wtg_malloc(ctx, (void**)&i1, n * 4)) != WTG_OK ||
-      (rc = wtg_malloc(ctx, (void**)&head, n * 4)) != WTG_OK ||
+      (rc = wtg_malloc(ctx, (void**)&v0, n * 4)) != WTG_OK || (rc = wtg_malloc(ctx, (void**)&v1, n * 4)) != WTG_OK ||
+      (rc = wtg_malloc(ctx, (void**)&act, n * 4)) != WTG_OK ||
       (rc = wtg_malloc(ctx, (void**)&rank, n * 4)) != WTG_OK ||
-      (rc = wtg_malloc(ctx, (void**)&tmax, tiles * 4)) != WTG_OK ||
-      (rc = wtg_malloc(ctx, (void**)&groups, 8)) != WTG_OK)
+      (rc = wtg_malloc(ctx, (void**)&fa, n * 4)) != WTG_OK || (rc = wtg_malloc(ctx, (void**)&fb, n * 4)) != WTG_OK ||
+      (rc = wtg_malloc(ctx, (void**)&ta, tiles * 4)) != WTG_OK ||
+      (rc = wtg_malloc(ctx, (void**)&tb, tiles * 4)) != WTG_OK ||
+      (rc = wtg_malloc(ctx, (void**)&keep, n * 4)) != WTG_OK ||
+      (rc = wtg_malloc(ctx, (void**)&pos, (n + 1) * 4)) != WTG_OK ||
+      (rc = wtg_malloc(ctx, (void**)&st, (tiles + 1) * 8)) != WTG_OK ||
+      (rc = wtg_malloc(ctx, (void**)&ctr, 256)) != WTG_OK)
     goto out;
   for (unsigned eb : {63u, 64u}) {  // temporary storage for the widest sort
     size_t b = 0;
-    if (rocprim::radix_sort_pairs(nullptr, b, k0, k1, i0, i1, (uint32_t)n, 0, eb, ctx->stream) != hipSuccess) {
+    if (rocprim::radix_sort_pairs(nullptr, b, k0, k1, v0, v1, (uint32_t)n, 0, eb, ctx->stream) != hipSuccess) {
       wtg_set_error("wtg_suffix_array_gpu: sort size query failed");
       rc = WTG_ERR_HIP;
       goto out;
@@ -205,51 +229,71 @@ extern "C" int wtg_suffix_array_gpu(wtg_ctx* ctx, const uint8_t* d_text, uint64_
   {
     uint32_t nbits = 1;
     while (nbits < 32 && (1ull << nbits) <= n) nbits++;
-    uint64_t h = 0;
-    for (int round = 0; round < 40; round++) {
+    uint64_t m = n, h = 0;
+    for (int round = 0; round < 40 && m > 0; round++) {
+      const unsigned grid = (unsigned)wtg_ceil_div(m, 256);
+      const uint64_t mt = wtg_ceil_div(m, 4096);
       if (round == 0) {
-        WTG_LAUNCH(ctx, "sa_init", k_sa_init, dim3(grid), dim3(256), 0, d_text, n, k0, i0);
+        WTG_LAUNCH(ctx, "sa_init", k_sa_init, dim3(grid), dim3(256), 0, d_text, n, k0, v0);
       } else {
-        WTG_LAUNCH(ctx, "sa_keys", k_sa_keys, dim3(grid), dim3(256), 0, rank, n, h, k0, i0);
+        WTG_LAUNCH(ctx, "sa_keys", k_sa_keys_act, dim3(grid), dim3(256), 0, rank, act, m, n, h, k0, v0);
       }
       WTG_LAUNCH_CHECK();
       const unsigned end_bit = round == 0 ? 63u : 32u + nbits;
-      if (rocprim::radix_sort_pairs(tmp, tmp_bytes, k0, k1, i0, i1, (uint32_t)n, 0, end_bit, ctx->stream) !=
+      if (rocprim::radix_sort_pairs(tmp, tmp_bytes, k0, k1, v0, v1, (uint32_t)m, 0, end_bit, ctx->stream) !=
           hipSuccess) {
         wtg_set_error("wtg_suffix_array_gpu: sort failed");
         rc = WTG_ERR_HIP;
         goto out;
       }
-      if ((rc = wtg_fill_u64(ctx, groups, 1, 0)) != WTG_OK) goto out;
-      WTG_LAUNCH(ctx, "sa_heads", k_sa_heads, dim3(grid), dim3(256), 0, k1, n, head);
+      const int whole = round == 0 ? 1 : 0;
+      WTG_LAUNCH(ctx, "sa_flags", k_sa_flags, dim3(grid), dim3(256), 0, k1, m, whole, fa, fb);
       WTG_LAUNCH_CHECK();
-      WTG_LAUNCH(ctx, "sa_maxscan", k_sa_maxscan_tiles, dim3((unsigned)tiles), dim3(256), 0, head, n, tmax);
+      WTG_LAUNCH(ctx, "sa_maxscan", k_sa_maxscan_tiles, dim3((unsigned)mt), dim3(256), 0, fb, m, tb);
       WTG_LAUNCH_CHECK();
-      WTG_LAUNCH(ctx, "sa_tile_prefix", k_sa_tile_prefix, dim3(1), dim3(256), 0, tmax, tiles);
+      WTG_LAUNCH(ctx, "sa_tile_prefix", k_sa_tile_prefix, dim3(1), dim3(256), 0, tb, mt);
       WTG_LAUNCH_CHECK();
-      WTG_LAUNCH(ctx, "sa_rank", k_sa_rank, dim3(grid), dim3(256), 0, head, tmax, i1, k1, n, rank, groups);
+      if (!whole) {
+        WTG_LAUNCH(ctx, "sa_maxscan", k_sa_maxscan_tiles, dim3((unsigned)mt), dim3(256), 0, fa, m, ta);
+        WTG_LAUNCH_CHECK();
+        WTG_LAUNCH(ctx, "sa_tile_prefix", k_sa_tile_prefix, dim3(1), dim3(256), 0, ta, mt);
+        WTG_LAUNCH_CHECK();
+      }
+      WTG_LAUNCH(ctx, "sa_update", k_sa_update, dim3(grid), dim3(256), 0, k1, v1, m, whole, fa, ta, fb, tb, rank,
+                 keep);
       WTG_LAUNCH_CHECK();
-      unsigned long long g = 0;
-      if (hipMemcpyAsync(&g, groups, 8, hipMemcpyDeviceToHost, ctx->stream) != hipSuccess ||
+      // compact the unresolved suffixes (exclusive scan of keep)
+      const uint64_t sct = wtg_ceil_div(m, 4096);
+      if ((rc = wtg_fill_bytes_zero(ctx, st, (sct + 1) * 8)) != WTG_OK) goto out;
+      if ((rc = wtg_fill_bytes_zero(ctx, ctr, 4)) != WTG_OK) goto out;
+      WTG_LAUNCH(ctx, "sa_keep_scan", k_scan_u32<uint32_t>, dim3((unsigned)sct), dim3(256), 0, keep, pos, m, st, ctr,
+                 ctx->d_flag);
+      WTG_LAUNCH_CHECK();
+      WTG_LAUNCH(ctx, "sa_compact", k_sa_compact, dim3(grid), dim3(256), 0, v1, keep, pos, m, act);
+      WTG_LAUNCH_CHECK();
+      uint32_t mnext = 0;
+      if (hipMemcpyAsync(&mnext, pos + m, 4, hipMemcpyDeviceToHost, ctx->stream) != hipSuccess ||
           hipStreamSynchronize(ctx->stream) != hipSuccess) {
-        wtg_set_error("wtg_suffix_array_gpu: group count copy failed");
+        wtg_set_error("wtg_suffix_array_gpu: count copy failed");
         rc = WTG_ERR_HIP;
         goto out;
       }
+      m = mnext;
       h = h == 0 ? 7 : 2 * h;
-      if (g == n) break;
-      if (round == 39 || h >= 2 * n) {
+      if (m > 0 && (round == 39 || h >= 2 * n)) {
         wtg_set_error("wtg_suffix_array_gpu: ranks did not separate");
         rc = WTG_ERR_DEVICE;
         goto out;
       }
     }
-    WTG_LAUNCH(ctx, "sa_out", k_sa_out, dim3(grid), dim3(256), 0, i1, n, d_sa);
+    WTG_LAUNCH(ctx, "sa_final", k_sa_final, dim3((unsigned)wtg_ceil_div(n, 256)), dim3(256), 0, rank, n, d_sa,
+               ctx->d_flag);
     WTG_LAUNCH_CHECK();
   }
   rc = wtg_finish(ctx);
 out:
-  wtg_free(ctx, k0); wtg_free(ctx, k1); wtg_free(ctx, i0); wtg_free(ctx, i1);
-  wtg_free(ctx, head); wtg_free(ctx, rank); wtg_free(ctx, tmax); wtg_free(ctx, groups); wtg_free(ctx, tmp);
+  wtg_free(ctx, k0); wtg_free(ctx, k1); wtg_free(ctx, v0); wtg_free(ctx, v1); wtg_free(ctx, act);
+  wtg_free(ctx, rank); wtg_free(ctx, fa); wtg_free(ctx, fb); wtg_free(ctx, ta); wtg_free(ctx, tb);
+  wtg_free(ctx, keep); wtg_free(ctx, pos); wtg_free(ctx, st); wtg_free(ctx, ctr); wtg_free(ctx, tmp);
   return rc;
 }
