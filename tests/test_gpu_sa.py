@@ -16,6 +16,11 @@ def texts(wsfm):
     yield "all_a", np.full(70000, 97, np.uint8)
     yield "periodic", np.tile(np.frombuffer(b"abcab", np.uint8), 20001)
     yield "repeats", np.concatenate([wsfm.gen_dna(6, 40000)[:-1]] * 3 + [np.zeros(1, np.uint8)])
+    # 255 values (8-bit codes, 7 bytes in round 0); 16 values spread over the
+    # byte range (5-bit codes, 12 bytes) with long copied stretches
+    yield "alpha255", rng.integers(1, 256, 200003).astype(np.uint8)
+    a16 = rng.choice(np.arange(0, 256, 17, dtype=np.uint8)[:16], 60000)
+    yield "alpha16_repeats", np.concatenate([a16, a16[5000:45000], a16[:20000]])
     for n in (1, 2, 7, 8, 64, 4095, 4096, 4097):
         yield "tiny%d" % n, rng.integers(0, 4, n).astype(np.uint8) + 65
 
@@ -25,10 +30,10 @@ def brute(t):
     return np.array(sorted(range(len(s)), key=lambda i: s[i:]), np.int64)
 
 
-@pytest.mark.parametrize("case", range(14))
+@pytest.mark.parametrize("case", range(16))
 def test_gpu_suffix_array(wsfm, orc, ctx, case):
     cases = list(texts(wsfm))
-    assert len(cases) == 14
+    assert len(cases) == 16
     name, t = cases[case]
     print("sa case", name, t.size, flush=True)
     d = ctx.to_device(t)
