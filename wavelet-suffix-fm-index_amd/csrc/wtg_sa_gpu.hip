@@ -32,16 +32,25 @@ __global__ __launch_bounds__(256) void k_sa_present(const uint8_t* __restrict__ 
 }
 
 // round-0 key of suffix i: its first q bytes, each as code[byte] (1..sigma, in
-// byte order) in c bits, 0 past the text end; q*c <= 63
-__global__ void k_sa_init(const uint8_t* __restrict__ t, uint64_t n, const uint16_t* __restrict__ code, uint32_t c,
-                          uint32_t q, uint64_t* __restrict__ key, uint32_t* __restrict__ idx) {
+// byte order) in c bits, 0 past the text end; q*c <= 63, so q <= 63.  The
+// block's 256 + 63 codes are staged in LDS.
+__global__ __launch_bounds__(256) void k_sa_init(const uint8_t* __restrict__ t, uint64_t n,
+                                                 const uint16_t* __restrict__ code, uint32_t c, uint32_t q,
+                                                 uint64_t* __restrict__ key, uint32_t* __restrict__ idx) {
   __shared__ uint16_t s_code[256];
+  __shared__ uint16_t s_win[256 + 64];
   s_code[threadIdx.x] = code[threadIdx.x];
   __syncthreads();
-  const uint64_t i = (uint64_t)blockIdx.x * blockDim.x + threadIdx.x;
+  const uint64_t b0 = (uint64_t)blockIdx.x * 256;
+  for (uint32_t j = threadIdx.x; j < 256 + 64; j += 256) {
+    const uint64_t p = b0 + j;
+    s_win[j] = p < n ? s_code[t[p]] : (uint16_t)0;
+  }
+  __syncthreads();
+  const uint64_t i = b0 + threadIdx.x;
   if (i >= n) return;
   uint64_t k = 0;
-  for (uint32_t j = 0; j < q; j++) k = (k << c) | (i + j < n ? (uint64_t)s_code[t[i + j]] : 0ull);
+  for (uint32_t j = 0; j < q; j++) k = (k << c) | (uint64_t)s_win[threadIdx.x + j];
   key[i] = k;
   idx[i] = (uint32_t)i;
 }
