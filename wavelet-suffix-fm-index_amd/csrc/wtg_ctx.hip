@@ -88,6 +88,10 @@ extern "C" int wtg_ctx_sync(wtg_ctx* ctx) {
       wtg_set_error("input symbol >= 2^levels (or inconsistent with the tree); results invalid");
       return WTG_ERR_ARG;
     }
+    if (f & 4u) {
+      wtg_set_error("device suffix ranks inconsistent; results invalid");
+      return WTG_ERR_DEVICE;
+    }
     wtg_set_error("device-side bounded wait expired (look-back scan); results invalid");
     return WTG_ERR_DEVICE;
   }

@@ -208,7 +208,7 @@ __global__ void k_sa_update(const uint64_t* __restrict__ key, const uint32_t* __
   const bool was_single = !whole_group && first == (uint32_t)k && (k + 1 == m || (key[k + 1] >> sh) != g);
   if (single && !was_single) {
     if (r < n) sa[r] = (int64_t)i;
-    else atomicOr(err, 2u);  // ranks inconsistent: not expected
+    else atomicOr(err, 4u);  // ranks inconsistent: not expected
   }
 }
 
