@@ -64,16 +64,34 @@ __global__ __launch_bounds__(256) void k_sa_runs_tiles(const uint64_t* __restric
                                                        int whole_group, uint32_t sh, uint32_t* __restrict__ fa,
                                                        uint32_t* __restrict__ fb, uint32_t* __restrict__ ta,
                                                        uint32_t* __restrict__ tb) {
+  // the tile's keys go through LDS (coalesced loads; thread t then takes keys
+  // 16t..16t+15), padded one slot per 16 against bank conflicts; the same
+  // space then holds the two scanned outputs for coalesced stores
+  constexpr uint32_t PAD = 4096 + 4096 / 16;
+  __shared__ uint64_t s_k[PAD];
   __shared__ uint32_t s_a[4], s_b[4];
-  const uint64_t base = (uint64_t)blockIdx.x * 4096 + threadIdx.x * 16;
-  uint32_t va[16], vb[16], ma = 0, mb = 0;
-  uint64_t prev = base > 0 && base - 1 < m ? key[base - 1] : 0ull;
+  uint32_t* s_o = (uint32_t*)s_k;  // [0, PAD) fa, [PAD, 2 PAD) fb
+  const uint64_t tile0 = (uint64_t)blockIdx.x * 4096;
 #pragma unroll
-  for (int j = 0; j < 16; j++) {
+  for (uint32_t j = 0; j < 16; j++) {
+    const uint32_t p = j * 256 + threadIdx.x;
+    if (tile0 + p < m) s_k[p + (p >> 4)] = key[tile0 + p];
+  }
+  __syncthreads();
+  const uint32_t l0 = threadIdx.x * 16;
+  const uint64_t base = tile0 + l0;
+  uint32_t va[16], vb[16], ma = 0, mb = 0;
+  uint64_t prev = 0;
+  if (base < m) {
+    if (l0 > 0) prev = s_k[(l0 - 1) + ((l0 - 1) >> 4)];
+    else if (tile0 > 0) prev = key[tile0 - 1];
+  }
+#pragma unroll
+  for (uint32_t j = 0; j < 16; j++) {
     const uint64_t p = base + j;
     uint32_t a = 0, b = 0;
     if (p < m) {
-      const uint64_t kc = key[p];
+      const uint64_t kc = s_k[threadIdx.x * 17 + j];
       b = (p == 0 || kc != prev) ? (uint32_t)p : 0u;
       a = (!whole_group && (p == 0 || (kc >> sh) != (prev >> sh))) ? (uint32_t)p : 0u;
       prev = kc;
@@ -98,7 +116,7 @@ __global__ __launch_bounds__(256) void k_sa_runs_tiles(const uint64_t* __restric
     s_a[w] = xa;
     s_b[w] = xb;
   }
-  __syncthreads();
+  __syncthreads();  // also: every thread is done reading s_k
   uint32_t pa = 0, pb = 0;
   for (int k = 0; k < w; k++) {
     pa = s_a[k] > pa ? s_a[k] : pa;
@@ -109,10 +127,17 @@ __global__ __launch_bounds__(256) void k_sa_runs_tiles(const uint64_t* __restric
   ea = ea > pa ? ea : pa;
   eb = eb > pb ? eb : pb;
 #pragma unroll
-  for (int j = 0; j < 16; j++) {
-    if (base + j < m) {
-      if (!whole_group) fa[base + j] = va[j] > ea ? va[j] : ea;
-      fb[base + j] = vb[j] > eb ? vb[j] : eb;
+  for (uint32_t j = 0; j < 16; j++) {
+    s_o[threadIdx.x * 17 + j] = va[j] > ea ? va[j] : ea;
+    s_o[PAD + threadIdx.x * 17 + j] = vb[j] > eb ? vb[j] : eb;
+  }
+  __syncthreads();
+#pragma unroll
+  for (uint32_t j = 0; j < 16; j++) {
+    const uint32_t p = j * 256 + threadIdx.x;
+    if (tile0 + p < m) {
+      if (!whole_group) fa[tile0 + p] = s_o[p + (p >> 4)];
+      fb[tile0 + p] = s_o[PAD + p + (p >> 4)];
     }
   }
   if (threadIdx.x == 0) {
@@ -245,6 +270,17 @@ extern "C" int wtg_bwt_from_sa_gpu(wtg_ctx* ctx, const uint8_t* d_text, const in
   return wtg_finish(ctx);
 }
 
+// a failed launch releases the buffers (goto out) instead of returning
+#define SA_LAUNCH_CHECK()                                                  \
+  do {                                                                     \
+    const hipError_t _e = hipGetLastError();                               \
+    if (_e != hipSuccess) {                                                \
+      wtg_set_error(hipGetErrorString(_e));                                \
+      rc = WTG_ERR_HIP;                                                    \
+      goto out;                                                            \
+    }                                                                      \
+  } while (0)
+
 extern "C" int wtg_suffix_array_gpu(wtg_ctx* ctx, const uint8_t* d_text, uint64_t n, int64_t* d_sa) {
   WTG_REQUIRE(ctx && (n == 0 || (d_text && d_sa)), "wtg_suffix_array_gpu: null argument");
   WTG_REQUIRE(n < (1ull << 31), "wtg_suffix_array_gpu: n >= 2^31");
@@ -293,7 +329,7 @@ extern "C" int wtg_suffix_array_gpu(wtg_ctx* ctx, const uint8_t* d_text, uint64_
     if ((rc = wtg_fill_bytes_zero(ctx, present, 256 * 4)) != WTG_OK) goto out;
     WTG_LAUNCH(ctx, "sa_present", k_sa_present, dim3((unsigned)wtg_ceil_div(n, 4096)), dim3(256), 0, d_text, n,
                present);
-    WTG_LAUNCH_CHECK();
+    SA_LAUNCH_CHECK();
     if (hipMemcpyAsync(h_present, present, sizeof(h_present), hipMemcpyDeviceToHost, ctx->stream) != hipSuccess ||
         hipStreamSynchronize(ctx->stream) != hipSuccess) {
       wtg_set_error("wtg_suffix_array_gpu: alphabet copy failed");
@@ -322,7 +358,7 @@ extern "C" int wtg_suffix_array_gpu(wtg_ctx* ctx, const uint8_t* d_text, uint64_
         WTG_LAUNCH(ctx, "sa_keys", k_sa_keys, dim3((unsigned)wtg_ceil_div(m, 256)), dim3(256), 0, rank,
                    all ? (const uint32_t*)nullptr : act, m, n, h, nbits, k0, v0);
       }
-      WTG_LAUNCH_CHECK();
+      SA_LAUNCH_CHECK();
       const unsigned end_bit = round == 0 ? q * cbits : 2u * nbits;
       const unsigned grid = (unsigned)wtg_ceil_div(m, 256);
       const uint64_t mt = wtg_ceil_div(m, 4096);
@@ -335,25 +371,25 @@ extern "C" int wtg_suffix_array_gpu(wtg_ctx* ctx, const uint8_t* d_text, uint64_
       const int whole = round == 0 ? 1 : 0;
       WTG_LAUNCH(ctx, "sa_runs", k_sa_runs_tiles, dim3((unsigned)mt), dim3(256), 0, k1, m, whole, nbits, fa, fb, ta,
                  tb);
-      WTG_LAUNCH_CHECK();
+      SA_LAUNCH_CHECK();
       WTG_LAUNCH(ctx, "sa_tile_prefix", k_sa_tile_prefix, dim3(1), dim3(256), 0, tb, mt);
-      WTG_LAUNCH_CHECK();
+      SA_LAUNCH_CHECK();
       if (!whole) {
         WTG_LAUNCH(ctx, "sa_tile_prefix", k_sa_tile_prefix, dim3(1), dim3(256), 0, ta, mt);
-        WTG_LAUNCH_CHECK();
+        SA_LAUNCH_CHECK();
       }
       WTG_LAUNCH(ctx, "sa_update", k_sa_update, dim3(grid), dim3(256), 0, k1, v1, m, n, whole, nbits, fa, ta, fb, tb,
                  rank, keep, d_sa, ctx->d_flag);
-      WTG_LAUNCH_CHECK();
+      SA_LAUNCH_CHECK();
       // compact the unresolved suffixes (exclusive scan of keep)
       const uint64_t sct = wtg_ceil_div(m, 4096);
       if ((rc = wtg_fill_bytes_zero(ctx, st, (sct + 1) * 8)) != WTG_OK) goto out;
       if ((rc = wtg_fill_bytes_zero(ctx, ctr, 4)) != WTG_OK) goto out;
       WTG_LAUNCH(ctx, "sa_keep_scan", k_scan_u32<uint32_t>, dim3((unsigned)sct), dim3(256), 0, keep, pos, m, st, ctr,
                  ctx->d_flag);
-      WTG_LAUNCH_CHECK();
+      SA_LAUNCH_CHECK();
       WTG_LAUNCH(ctx, "sa_compact", k_sa_compact, dim3(grid), dim3(256), 0, v1, keep, pos, m, act);
-      WTG_LAUNCH_CHECK();
+      SA_LAUNCH_CHECK();
       uint32_t mnext = 0;
       if (hipMemcpyAsync(&mnext, pos + m, 4, hipMemcpyDeviceToHost, ctx->stream) != hipSuccess ||
           hipStreamSynchronize(ctx->stream) != hipSuccess) {
@@ -378,3 +414,4 @@ out:
   wtg_free(ctx, present);
   return rc;
 }
+#undef SA_LAUNCH_CHECK
