@@ -45,3 +45,41 @@ def test_full_size_matches_reference_digest(wsfm, ctx, name):
     assert r.size == spec["nodes_reached"]
     rec = np.ascontiguousarray(np.concatenate([r.astype(np.uint64)[:, None], nodes[r]], axis=1))
     assert hashlib.sha256(rec.tobytes()).hexdigest() == spec["nodes_reached_sha256"]
+
+
+SUPPORT_CASES = sorted(k for k in FIX if "supports" in FIX[k])
+
+
+@pytest.mark.parametrize("name", SUPPORT_CASES)
+def test_full_size_supports_match_oracle_digest(wsfm, ctx, name):
+    """rank_support_v<1,1> + select_support_mcl<1,1> / <0,1> over the whole
+    C2 tree (5,368,709,120 bits: positions, superblock starts and mini offsets
+    past 2^32), built on the GPU from the device tree, hash to the oracle's
+    digests of the reference levelWT's words (rank/rank_support_v_par.hpp:69-123,
+    select/select_support_mcl_par.hpp:227-276; wt_int builds them over the
+    concatenated tree, wt_int/wt_int_par.hpp:402-404)."""
+    spec = FIX[name]
+    sup = spec["supports"]
+    print("full-size supports", name, sup["nbits"], flush=True)
+    s = make_input(wsfm, name)
+    n, L = spec["n"], spec["levels"]
+    ds = ctx.to_device(s)
+    del s
+    dw = ctx.empty(wsfm.lib().wtg_levels_words(n, L), np.uint64)
+    ctx.levels_build(ds, n, L, dw)
+    ds.free()
+    assert n * L == sup["nbits"]
+    D, s1, s0 = ctx.rank9_select_build(dw, n * L)
+    try:
+        d = D.get()
+        assert d.size == sup["rank9_words"]
+        assert hashlib.sha256(d.tobytes()).hexdigest() == sup["rank9_sha256"]
+        del d
+        for bit, sel in ((1, s1), (0, s0)):
+            c = sel.content()
+            exp = sup["select%d" % bit]
+            assert (c["arg_cnt"], c["sb"], c["n_long"], c["logn"]) == \
+                (exp["arg_cnt"], exp["sb"], exp["n_long"], exp["logn"])
+            assert wsfm.select_content_sha256(c) == exp["sha256"], "select%d content" % bit
+    finally:
+        s1.free(); s0.free(); D.free(); dw.free()

@@ -96,6 +96,7 @@ SIGNATURES = {
     "wtg_rank9_words": (u64, [u64]),
     "wtg_rank9_build": (i32, [vp, vp, u64, vp]),
     "wtg_rank9_batch": (i32, [vp, vp, vp, u64, vp, u64, vp]),
+    "wtg_rank9_build_noalloc": (i32, [vp, vp, u64, vp]),
     "wtg_select_build": (i32, [vp, vp, vp, u64, i32, C.POINTER(Select)]),
     "wtg_rank9_select_build": (i32, [vp, vp, u64, vp, C.POINTER(Select), C.POINTER(Select)]),
     "wtg_select_free": (i32, [vp, C.POINTER(Select)]),
@@ -189,6 +190,25 @@ def levels_of_sigma(sigma):
     """L = max(1, log2Up(sigma))  (wavelet/levelWT/WT.C:53)."""
     sigma = int(sigma)
     return max(1, (sigma - 1).bit_length()) if sigma > 1 else 1
+
+
+def select_content_sha256(c):
+    """SHA-256 of a select_support_mcl content in one fixed byte order
+    (select/select_support_mcl_par.hpp:483-561 fields): the header
+    (arg_cnt, sb, n_long, logn, logn4 as little-endian uint64), then
+    superblock[sb] (uint64), mini[sb*64] (uint32), mini_width[sb] (uint8),
+    long_width[sb] (uint8), long_index[sb] (int64) and the long pool
+    [n_long*4096] (uint64).  `c` is a dict with those keys (SelectHandle.content)."""
+    import hashlib
+    h = hashlib.sha256()
+    sb, nl = int(c["sb"]), int(c["n_long"])
+    h.update(np.array([c["arg_cnt"], sb, nl, c["logn"], c["logn4"]], "<u8").tobytes())
+    for key, dt, cnt in (("superblock", "<u8", sb), ("mini", "<u4", sb * 64), ("mini_width", "u1", sb),
+                         ("long_width", "u1", sb), ("long_index", "<i8", sb), ("long_pool", "<u8", nl * 4096)):
+        a = np.ascontiguousarray(np.asarray(c[key])[:cnt], dtype=dt)
+        assert a.size == cnt, key
+        h.update(a.tobytes())
+    return h.hexdigest()
 
 
 # ------------------------------------------------------------ generators
