@@ -114,6 +114,18 @@ int wtg_rank9_build(wtg_ctx* ctx, const uint64_t* d_words, uint64_t nbits, uint6
 int wtg_rank9_batch(wtg_ctx* ctx, const uint64_t* d_words, const uint64_t* d_dir,
                     uint64_t nbits, const uint64_t* d_idx, uint64_t nq, uint64_t* d_out);
 
+/* Allocation-free variant of wtg_rank9_build.  No counterpart in the reference:
+ * it is for callers that cannot allocate device memory (code under HIP graph
+ * capture, tight serving loops, harnesses that forbid hipMalloc).  Same
+ * directory, same kernels; their scratch (tile sums, their scan and its
+ * look-back state, 13 MiB at most) lives in one static device array per GPU
+ * instead of the context's workspace, so the call makes no hipMalloc / hipFree.
+ * Limit: nbits <= WTG_RANK9_NOALLOC_MAX_BITS (larger -> WTG_ERR_UNSUPPORTED).
+ * Every call on one GPU shares that array: calls that overlap on one GPU (two
+ * contexts, or two host threads) race and must be serialized by the caller. */
+#define WTG_RANK9_NOALLOC_MAX_BITS (1ull << 35)
+int wtg_rank9_build_noalloc(wtg_ctx* ctx, const uint64_t* d_words, uint64_t nbits, uint64_t* d_dir);
+
 /* ------------------------------------------------ (b') select samples */
 /* Content of sdsl::select_support_mcl<bit,1>  (select/select_support_mcl_par.hpp:72-117,
  * :227-276 init_slow, :483-561 init_fast).  All arrays are device arrays owned

@@ -57,6 +57,10 @@ extern "C" int wtg_ctx_create(int device, wtg_ctx** out) {
     (void)hipGetLastError();
   }
   WTG_CHECK_HIP(hipDeviceSynchronize());
+  {
+    const int rc = wtg_noalloc_init_device(device);
+    if (rc != WTG_OK) { wtg_ctx_destroy(c); return rc; }
+  }
   *out = c;
   return WTG_OK;
 }

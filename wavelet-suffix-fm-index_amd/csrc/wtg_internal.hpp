@@ -103,6 +103,8 @@ struct wtg_rs_pending {
 int wtg_rank_select_begin(wtg_ctx* ctx, const uint64_t* d_words, uint64_t nbits, uint64_t* d_dir, bool build,
                           wtg_select* sel1, wtg_select* sel0, uint64_t known_ones, wtg_rs_pending* pend);
 int wtg_rank_select_end(wtg_ctx* ctx, const wtg_rs_pending* pend, const uint32_t nl[2]);
+// resolves the per-device address of the allocation-free rank9 scratch (wtg_ctx_create)
+int wtg_noalloc_init_device(int device);
 int wtg_max_value(wtg_ctx* ctx, const void* d_sym, int sym_bytes, uint64_t n, uint64_t* out);
 int wtg_fill_bytes_zero(wtg_ctx* ctx, void* d, uint64_t bytes);
 

@@ -22,6 +22,7 @@
 #include "wtg_internal.hpp"
 #include <algorithm>
 #include <cstring>
+#include <mutex>
 #include <vector>
 
 #define FLAG_LOOKBACK 1u
@@ -596,6 +597,104 @@ extern "C" int wtg_select_batch(wtg_ctx* ctx, const wtg_select* s, const uint64_
          (s->nbits + 63) / 64, s->bit};
   WTG_LAUNCH(ctx, "select_query", k_select_query, dim3((unsigned)wtg_ceil_div(nq, 256)), dim3(256), 0,
              q, d_i, nq, s->arg_cnt, d_out);
+  WTG_LAUNCH_CHECK();
+  return wtg_finish(ctx);
+}
+
+// ------------------------------------------------- allocation-free rank9 build
+// wtg_rank9_build_noalloc runs the kernels of wtg_rank9_build (tile sums, their
+// exclusive scan, the directory pass) with their scratch in one file-scope
+// device array instead of the context's workspace, so the call makes no device
+// allocation.  Layout for a vector of at most WTG_RANK9_NOALLOC_MAX_BITS bits:
+// [scan counter + ones total: 256 B][scan look-back states][tile sums][tile prefixes].
+namespace {
+constexpr uint64_t na_words(uint64_t nbits) { return (nbits + 63) / 64; }
+constexpr uint64_t na_wtiles(uint64_t nbits) {
+  return (8 * ((na_words(nbits) * 64 >> 9) + 1) + RS_TILE_WORDS - 1) / RS_TILE_WORDS;
+}
+constexpr uint64_t na_stiles(uint64_t nbits) { return (na_wtiles(nbits) + 4095) / 4096; }
+constexpr uint64_t na_al(uint64_t x) { return (x + 255) & ~uint64_t(255); }
+constexpr uint64_t NA_O_STATES = 256;
+constexpr uint64_t na_o_tsum(uint64_t nbits) { return NA_O_STATES + na_al((na_stiles(nbits) + 1) * 8); }
+constexpr uint64_t na_o_tpre(uint64_t nbits) { return na_o_tsum(nbits) + na_al((na_wtiles(nbits) + 1) * 4); }
+constexpr uint64_t na_bytes(uint64_t nbits) { return na_o_tpre(nbits) + na_al((na_wtiles(nbits) + 1) * 8); }
+constexpr uint64_t NA_SCRATCH_BYTES = na_bytes(WTG_RANK9_NOALLOC_MAX_BITS);
+static_assert(NA_SCRATCH_BYTES <= (13ull << 20), "scratch of the allocation-free rank9 build is 13 MiB at most");
+}  // namespace
+
+static __device__ __align__(256) uint8_t scratch[NA_SCRATCH_BYTES];
+
+// device addresses of `scratch`, one per device (each device holds its own copy)
+static std::mutex g_na_mu;
+static std::map<int, uint8_t*> g_na_ptr;
+
+// Called by wtg_ctx_create with `device` current: resolves the device address
+// of `scratch` once per device.
+int wtg_noalloc_init_device(int device) {
+  std::lock_guard<std::mutex> lk(g_na_mu);
+  if (g_na_ptr.count(device)) return WTG_OK;
+  void* ptr = nullptr;
+  WTG_CHECK_HIP(hipGetSymbolAddress(&ptr, HIP_SYMBOL(scratch)));
+  if (!ptr) {
+    wtg_set_error("hipGetSymbolAddress(scratch) returned a null address");
+    return WTG_ERR_HIP;
+  }
+  g_na_ptr[device] = (uint8_t*)ptr;
+  return WTG_OK;
+}
+
+__global__ void k_na_zero(uint64_t* __restrict__ d, uint64_t n) {
+  const uint64_t i = (uint64_t)blockIdx.x * blockDim.x + threadIdx.x;
+  if (i < n) d[i] = 0;
+}
+
+extern "C" int wtg_rank9_build_noalloc(wtg_ctx* ctx, const uint64_t* d_words, uint64_t nbits, uint64_t* d_dir) {
+  WTG_REQUIRE(ctx && d_dir && (nbits == 0 || d_words), "wtg_rank9_build_noalloc: null argument");
+  if (nbits > WTG_RANK9_NOALLOC_MAX_BITS) {
+    wtg_set_error("wtg_rank9_build_noalloc: nbits above WTG_RANK9_NOALLOC_MAX_BITS (2^35); use wtg_rank9_build");
+    return WTG_ERR_UNSUPPORTED;
+  }
+  WTG_TRY(wtg_set_device(ctx));
+  uint8_t* ws = nullptr;
+  {
+    std::lock_guard<std::mutex> lk(g_na_mu);
+    auto it = g_na_ptr.find(ctx->device);
+    if (it != g_na_ptr.end()) ws = it->second;
+  }
+  if (!ws) {
+    wtg_set_error("wtg_rank9_build_noalloc: scratch address of this device not resolved (context not created)");
+    return WTG_ERR_HIP;
+  }
+  if (nbits == 0) {
+    hipLaunchKernelGGL(k_na_zero, dim3(1), dim3(64), 0, ctx->stream, d_dir, (uint64_t)2);
+    WTG_LAUNCH_CHECK();
+    return wtg_finish(ctx);
+  }
+  const uint64_t cap = (nbits + 63) / 64 * 64;
+  const uint64_t W = cap >> 6, S = cap >> 9;
+  const uint64_t wtiles = na_wtiles(nbits), stiles = na_stiles(nbits);
+  uint32_t* ctr = (uint32_t*)ws;
+  unsigned long long* ones_total = (unsigned long long*)(ws + 64);
+  uint64_t* scst = (uint64_t*)(ws + NA_O_STATES);
+  uint32_t* tsum = (uint32_t*)(ws + na_o_tsum(nbits));
+  uint64_t* tpre = (uint64_t*)(ws + na_o_tpre(nbits));
+  // the scan's counter and look-back states start at zero (the previous call's
+  // values stay in the array)
+  const uint64_t nz = na_o_tsum(nbits) / 8;
+  hipLaunchKernelGGL(k_na_zero, dim3((unsigned)wtg_ceil_div(nz, 256)), dim3(256), 0, ctx->stream, (uint64_t*)ws, nz);
+  WTG_LAUNCH_CHECK();
+  const bool aligned = ((uintptr_t)d_words & 15) == 0;
+  const unsigned g4 = (unsigned)wtg_ceil_div(wtiles, 4);
+  hipLaunchKernelGGL(k_rank_tile_sums, dim3(g4), dim3(256), 0, ctx->stream, d_words, W, wtiles, tsum, aligned);
+  WTG_LAUNCH_CHECK();
+  hipLaunchKernelGGL(k_scan_u32<uint64_t>, dim3((unsigned)stiles), dim3(256), 0, ctx->stream,
+                     (const uint32_t*)tsum, tpre, wtiles, scst, ctr, ctx->d_flag);
+  WTG_LAUNCH_CHECK();
+  RankSelArgs ra{};
+  ra.words = d_words; ra.nbits = nbits; ra.W = W; ra.S = S; ra.D = d_dir;
+  ra.err = ctx->d_flag; ra.tpre = tpre; ra.ones_total = ones_total;
+  ra.want[0] = ra.want[1] = 0;
+  hipLaunchKernelGGL(k_rank_entries, dim3(g4), dim3(256), 0, ctx->stream, ra, wtiles, true, aligned);
   WTG_LAUNCH_CHECK();
   return wtg_finish(ctx);
 }
