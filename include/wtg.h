@@ -113,6 +113,11 @@ int wtg_rank9_build(wtg_ctx* ctx, const uint64_t* d_words, uint64_t nbits, uint6
  * (rank_support_v_par.hpp:131-142). */
 int wtg_rank9_batch(wtg_ctx* ctx, const uint64_t* d_words, const uint64_t* d_dir,
                     uint64_t nbits, const uint64_t* d_idx, uint64_t nq, uint64_t* d_out);
+/* The same for rank_support_v<bit,1> (bit 0: rank0(idx) = idx - rank1(idx), the
+ * count rank_support_v<0,1>::rank returns, rank_support_v_par.hpp:131-142), from
+ * the rank1 directory d_dir. */
+int wtg_rank9_batch_bit(wtg_ctx* ctx, const uint64_t* d_words, const uint64_t* d_dir, uint64_t nbits,
+                        int bit, const uint64_t* d_idx, uint64_t nq, uint64_t* d_out);
 
 /* Allocation-free variant of wtg_rank9_build.  No counterpart in the reference:
  * it is for callers that cannot allocate device memory (code under HIP graph
@@ -161,6 +166,11 @@ typedef struct wtg_wtint wtg_wtint;
 int wtg_wtint_build(wtg_ctx* ctx, const void* d_sym, int sym_bytes, uint64_t n,
                     uint32_t max_level, wtg_wtint** out);
 int wtg_wtint_destroy(wtg_ctx* ctx, wtg_wtint* wt);
+/* A wt_int over an existing tree (d_tree: wtg_levels_words(n, levels) device
+ * words, copied): builds rank_support_v + select_support_mcl<1> + <0> over it,
+ * as wt_int::load (wt_int/wt_int_par.hpp:901-911) restores them; levels <= 64. */
+int wtg_wtint_from_tree(wtg_ctx* ctx, const uint64_t* d_tree, uint64_t n, uint32_t levels, uint64_t sigma,
+                        wtg_wtint** out);
 /* size, max_level, sigma (#distinct values), device tree words and rank directory */
 int wtg_wtint_info(const wtg_wtint* wt, uint64_t* n, uint32_t* levels, uint64_t* sigma,
                    const uint64_t** d_words, const uint64_t** d_dir);
@@ -227,6 +237,13 @@ typedef struct wtg_huff wtg_huff;
  * select1, select0.  d_text: n bytes in device memory. */
 int wtg_huff_build(wtg_ctx* ctx, const uint8_t* d_text, uint64_t n, wtg_huff** out);
 int wtg_huff_destroy(wtg_ctx* ctx, wtg_huff* wt);
+/* A wt_huff from its byte counts (counts[256]) and existing tree bits (d_words:
+ * nbits bits in ceil(nbits/64) device words, copied), as wt_pc::load restores
+ * one (wt_huff/wt_pc_par.hpp:766-775): the Huffman shape of the counts, then
+ * rank9 + select1 + select0 and the query blocks over the bits.  nbits must be
+ * the bit count of that shape (else WTG_ERR_ARG). */
+int wtg_huff_from_bits(wtg_ctx* ctx, const uint64_t* counts, const uint64_t* d_words, uint64_t nbits,
+                       wtg_huff** out);
 int wtg_huff_get_shape(const wtg_huff* wt, wtg_huff_shape* out);
 /* device tree words (ceil(bv_size/64)+1, last = 0) and rank9 directory */
 int wtg_huff_data(const wtg_huff* wt, const uint64_t** d_words, const uint64_t** d_dir);
@@ -358,6 +375,13 @@ int wtg_lsel_select_batch(wtg_ctx* ctx, const wtg_lsel* ls, const uint32_t* d_ve
 /* the same queries answered by the plain sampled select only */
 int wtg_lsel_plain_batch(wtg_ctx* ctx, const wtg_lsel* ls, const uint32_t* d_vec,
                          const uint64_t* d_j, uint64_t nq, uint64_t* d_out);
+/* Access report: out[q] = the 64-byte HBM sectors query q reads on the learned
+ * (learned = 1) or the plain path (select_support_mcl_par.hpp:564-611: sample
+ * lookups + forward word scan), counted along the same addresses; d_ans = the
+ * answers of a previous wtg_lsel_select_batch / wtg_lsel_plain_batch call on
+ * the same queries.  0 for an invalid query. */
+int wtg_lsel_sectors(wtg_ctx* ctx, const wtg_lsel* ls, const uint32_t* d_vec, const uint64_t* d_j,
+                     const uint64_t* d_ans, uint64_t nq, int learned, uint32_t* d_out);
 /* Training loss (mean squared residual) and its gradient for one model, by the
  * training kernels: d_knots = nknots pairs {uint32 j; float y} (j increasing),
  * samples d_pos[t] = select((t+1)*stride) (those outside [j_0, j_last] unseen),

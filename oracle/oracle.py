@@ -68,6 +68,10 @@ def lib():
         L.orc_select_fill.argtypes = [vp, u64, i32, C.POINTER(OrcSelect)]
         L.orc_select_query.restype = u64
         L.orc_select_query.argtypes = [C.POINTER(OrcSelect), vp, i32, u64]
+        L.orc_rank9_rank_batch.restype = None
+        L.orc_rank9_rank_batch.argtypes = [vp, vp, u64, vp, u64, vp]
+        L.orc_select_query_batch.restype = None
+        L.orc_select_query_batch.argtypes = [C.POINTER(OrcSelect), vp, i32, vp, u64, vp]
         L.orc_select_brute.restype = u64
         L.orc_select_brute.argtypes = [vp, u64, i32, u64]
         L.orc_wtint_rank.restype = u64
@@ -164,10 +168,14 @@ def rank9_build(words, nbits):
     return D
 
 
-def rank9(D, words, idx):
+def rank9(D, words, idx, nbits=None):
+    """rank(idx) per rank_support_v<1,1>; with nbits, idx > nbits gives NONE."""
     w = padded(words)
-    L = lib()
-    return np.array([L.orc_rank9_rank(_p(D), _p(w), int(i)) for i in idx], dtype=np.uint64)
+    idx = np.ascontiguousarray(idx, dtype=np.uint64)
+    out = np.empty(idx.size, np.uint64)
+    nb = w.size * 64 if nbits is None else int(nbits)
+    lib().orc_rank9_rank_batch(_p(D), _p(w), nb, _p(idx), idx.size, _p(out))
+    return out
 
 
 class Select:
@@ -201,9 +209,10 @@ class Select:
         return lib().orc_select_query(C.byref(self.S), _p(self.words), self.bit, int(i))
 
     def queries(self, idx):
-        f = lib().orc_select_query
-        ref, w, b = C.byref(self.S), _p(self.words), self.bit
-        return np.array([f(ref, w, b, int(i)) for i in idx], dtype=np.uint64)
+        idx = np.ascontiguousarray(idx, dtype=np.uint64)
+        out = np.empty(idx.size, np.uint64)
+        lib().orc_select_query_batch(C.byref(self.S), _p(self.words), self.bit, _p(idx), idx.size, _p(out))
+        return out
 
 
 def select_brute(words, nbits, bit, i):
@@ -343,6 +352,31 @@ class RefWT:
 
     def workers(self):
         return self.L.ref_workers()
+
+
+class RefSA:
+    """The reference's PBBS DC3 suffix array (suffixArray/parallelKS/pks.C,
+    compiled into oracle/_ref/libref_pks.so), or ok = False."""
+
+    def __init__(self):
+        path = os.path.join(HERE, "_ref", "libref_pks.so")
+        self.ok = os.path.exists(path)
+        if not self.ok:
+            return
+        self.L = C.CDLL(path)
+        self.L.ref_suffix_array.restype = C.c_int
+        self.L.ref_suffix_array.argtypes = [C.c_void_p, C.c_uint64, C.c_void_p]
+        self.L.ref_sa_workers.restype = C.c_int
+
+    def suffix_array(self, t):
+        t = np.ascontiguousarray(t, dtype=np.uint8)
+        sa = np.empty(t.size, np.int64)
+        rc = self.L.ref_suffix_array(_p(t), t.size, _p(sa))
+        assert rc == 0, "ref_suffix_array failed (%d)" % rc
+        return sa
+
+    def workers(self):
+        return self.L.ref_sa_workers()
 
 
 # ----------------------------------------------------------- learned select

@@ -18,8 +18,13 @@
  *   orc_wtint_*        -- brute force on the text (count / i-th occurrence / value;
  *                         lex counts: values below / above c in [i, j); range
  *                         count: positions in [lb, rb] with values in [vlb, vrb]).
- *   orc_huff_*         -- answers by brute force on the text; the BFS node order
- *                         of sdsl byte_tree is "parity unpinned" (sdsl absent).
+ *   orc_huff_*         -- answers by brute force on the text; the shape against an
+ *                         independent (freq, id) heap merge on tie-heavy
+ *                         alphabets, the BFS numbering and bv_pos against the
+ *                         reference's own statement (wt_pc_par.hpp:134-135) and
+ *                         the bits against the path rule (tests); sdsl's
+ *                         byte_tree internals beyond that statement are
+ *                         "parity unpinned" (sdsl absent).
  *   orc_fm_count       -- brute-force substring count and SA binary search.
  */
 #include "wtg_oracle.h"
@@ -243,6 +248,11 @@ uint64_t orc_rank9_rank(const uint64_t* D, const uint64_t* words, uint64_t idx) 
   return r;
 }
 
+void orc_rank9_rank_batch(const uint64_t* D, const uint64_t* words, uint64_t nbits, const uint64_t* idx,
+                          uint64_t nq, uint64_t* out) {
+  for (uint64_t q = 0; q < nq; q++) out[q] = idx[q] <= nbits ? orc_rank9_rank(D, words, idx[q]) : ORC_NONE;
+}
+
 /* ----------------------------------------------------------------- select */
 static inline uint64_t argword(const uint64_t* words, uint64_t j, int bit) {
   return bit ? words[j] : ~words[j];
@@ -339,6 +349,11 @@ uint64_t orc_select_query(const orc_select* S, const uint64_t* words, int bit, u
     if (sum + args >= i) return (wp << 6) + sel_in_word(x, i - sum);
     sum += args;
   }
+}
+void orc_select_query_batch(const orc_select* S, const uint64_t* words, int bit, const uint64_t* idx,
+                            uint64_t nq, uint64_t* out) {
+  for (uint64_t q = 0; q < nq; q++)
+    out[q] = (idx[q] >= 1 && idx[q] <= S->arg_cnt) ? orc_select_query(S, words, bit, idx[q]) : ORC_NONE;
 }
 uint64_t orc_select_brute(const uint64_t* words, uint64_t nbits, int bit, uint64_t i) {
   uint64_t c = 0, res = ORC_NONE;

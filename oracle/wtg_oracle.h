@@ -44,6 +44,9 @@ uint64_t orc_wt_query(const orc_node* nodes, const uint64_t* words, uint64_t ind
 uint64_t orc_rank9_entries(uint64_t nbits); /* number of uint64 words in D */
 void orc_rank9_build(const uint64_t* words, uint64_t nbits, uint64_t* D);
 uint64_t orc_rank9_rank(const uint64_t* D, const uint64_t* words, uint64_t idx); /* :131-142 */
+/* orc_rank9_rank for idx[0..nq): ORC_NONE where idx > nbits */
+void orc_rank9_rank_batch(const uint64_t* D, const uint64_t* words, uint64_t nbits, const uint64_t* idx,
+                          uint64_t nq, uint64_t* out);
 
 /* select_support_mcl<b,1> content, select/select_support_mcl_par.hpp:227-276 (init_slow)
  * and :483-561 (init_fast) produce the same content. */
@@ -63,6 +66,9 @@ void orc_select_plan(const uint64_t* words, uint64_t nbits, int bit, orc_select*
 void orc_select_fill(const uint64_t* words, uint64_t nbits, int bit, orc_select* S);
 uint64_t orc_select_query(const orc_select* S, const uint64_t* words, int bit, uint64_t i); /* :564-611 */
 uint64_t orc_select_brute(const uint64_t* words, uint64_t nbits, int bit, uint64_t i);
+/* orc_select_query for idx[0..nq): out[q] = select(idx[q]), ORC_NONE outside [1, arg_cnt] */
+void orc_select_query_batch(const orc_select* S, const uint64_t* words, int bit, const uint64_t* idx,
+                            uint64_t nq, uint64_t* out);
 
 /* wt_int queries over the concatenated tree, wt_int/wt_int_par.hpp:519-660 */
 uint64_t orc_wtint_rank(const uint64_t* D, const uint64_t* words, uint64_t n, uint32_t L,

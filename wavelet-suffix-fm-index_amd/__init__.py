@@ -97,12 +97,14 @@ SIGNATURES = {
     "wtg_rank9_build": (i32, [vp, vp, u64, vp]),
     "wtg_rank9_batch": (i32, [vp, vp, vp, u64, vp, u64, vp]),
     "wtg_rank9_build_noalloc": (i32, [vp, vp, u64, vp]),
+    "wtg_rank9_batch_bit": (i32, [vp, vp, vp, u64, i32, vp, u64, vp]),
     "wtg_select_build": (i32, [vp, vp, vp, u64, i32, C.POINTER(Select)]),
     "wtg_rank9_select_build": (i32, [vp, vp, u64, vp, C.POINTER(Select), C.POINTER(Select)]),
     "wtg_select_free": (i32, [vp, C.POINTER(Select)]),
     "wtg_select_batch": (i32, [vp, C.POINTER(Select), vp, vp, u64, vp]),
     "wtg_wtint_build": (i32, [vp, vp, i32, u64, u32, C.POINTER(vp)]),
     "wtg_wtint_destroy": (i32, [vp, vp]),
+    "wtg_wtint_from_tree": (i32, [vp, vp, u64, u32, u64, C.POINTER(vp)]),
     "wtg_wtint_info": (i32, [vp, U64P, C.POINTER(u32), U64P, C.POINTER(vp), C.POINTER(vp)]),
     "wtg_wtint_select1": (C.POINTER(Select), [vp]),
     "wtg_wtint_select0": (C.POINTER(Select), [vp]),
@@ -115,6 +117,7 @@ SIGNATURES = {
     "wtg_wtint_range_count_batch": (i32, [vp, vp, vp, vp, vp, vp, u64, vp]),
     "wtg_huff_build": (i32, [vp, vp, u64, C.POINTER(vp)]),
     "wtg_huff_destroy": (i32, [vp, vp]),
+    "wtg_huff_from_bits": (i32, [vp, vp, vp, u64, C.POINTER(vp)]),
     "wtg_huff_get_shape": (i32, [vp, C.POINTER(HuffShape)]),
     "wtg_huff_data": (i32, [vp, C.POINTER(vp), C.POINTER(vp)]),
     "wtg_huff_select1": (C.POINTER(Select), [vp]),
@@ -133,6 +136,7 @@ SIGNATURES = {
     "wtg_lsel_model": (i32, [vp, u32, C.POINTER(LselModelView)]),
     "wtg_lsel_select_batch": (i32, [vp, vp, vp, vp, u64, vp]),
     "wtg_lsel_plain_batch": (i32, [vp, vp, vp, vp, u64, vp]),
+    "wtg_lsel_sectors": (i32, [vp, vp, vp, vp, vp, u64, i32, vp]),
     "wtg_lsel_loss_grad": (i32, [vp, vp, u32, u32, C.c_double, C.c_double, vp, u64,
                                  C.POINTER(C.c_double), vp]),
     "wtg_gen_select_vectors": (i32, [u64, u32, u64, vp, vp]),
@@ -556,6 +560,14 @@ class LearnedSelect:
         _check(fn(self.ctx.h, self.h, d_vec.ptr, d_j.ptr, d_j.size, d_out.ptr))
         return d_out
 
+    def sectors(self, d_vec, d_j, d_ans, learned=True):
+        """Device uint32 array: 64-byte HBM sectors each query reads on the learned
+        or the plain path (d_ans: that path's answers to the same queries)."""
+        out = self.ctx.empty(d_j.size, np.uint32)
+        _check(lib().wtg_lsel_sectors(self.ctx.h, self.h, d_vec.ptr, d_j.ptr, d_ans.ptr, d_j.size,
+                                      1 if learned else 0, out.ptr))
+        return out
+
     def select(self, vec, j, learned=True):
         ctx = self.ctx
         dv = ctx.to_device(np.ascontiguousarray(vec, np.uint32))
@@ -809,3 +821,6 @@ class WtInt:
         if self.h is not None:
             _check(lib().wtg_wtint_destroy(self.ctx.h, self.h))
             self.h = None
+
+
+from . import workloads  # noqa: E402,F401  (benchmark inputs and queries)

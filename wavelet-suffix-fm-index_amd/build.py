@@ -84,6 +84,15 @@ def build_dropin_drivers(verbose=False):
         if r.returncode != 0:
             raise RuntimeError("driver build failed: %s\n%s" % (" ".join(cmd), r.stderr))
         outs.append(exe)
+    # serialize / load round trips of the C++ drop-ins
+    exe = os.path.join(outdir, "serialize")
+    cmd = ["g++", "-O2", "-std=c++17", "-Wall", "-I" + os.path.join(ROOT, "include"),
+           "-I" + os.path.join(HERE, "host"), os.path.join(ROOT, "tests", "cpp", "serialize_driver.cpp"),
+           "-o", exe, "-L" + HERE, "-lwtg", "-Wl,-rpath," + HERE, "-Wl,-rpath,/opt/rocm/lib"]
+    r = subprocess.run(cmd, capture_output=True, text=True)
+    if r.returncode != 0:
+        raise RuntimeError("driver build failed: %s\n%s" % (" ".join(cmd), r.stderr))
+    outs.append(exe)
     if verbose:
         print("built", ", ".join(outs))
     return outs

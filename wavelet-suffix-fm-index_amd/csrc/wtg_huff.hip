@@ -1041,6 +1041,53 @@ extern "C" int wtg_huff_destroy(wtg_ctx* ctx, wtg_huff* wt) {
   return WTG_OK;
 }
 
+// After the tree bits are in wt->words (and the shape in wt->shape / wt->tab,
+// the table on the device when there are inner nodes): rank9 + select1 +
+// select0, the query blocks and the per-node rank bases.
+static int huff_finish(wtg_ctx* ctx, wtg_huff* wt, const uint64_t* hist) {
+  const wtg_huff_shape& S = wt->shape;
+  const uint64_t nw = (S.bv_size + 63) / 64 + 1;
+  // the ones of the tree bits: each byte contributes the ones of its code
+  uint64_t ones = 0;
+  for (int c = 0; c < 256; c++) ones += hist[c] * (uint64_t)__builtin_popcountll(wt->tab.code[c]);
+  wtg_rs_pending pend;
+  WTG_TRY(wtg_rank_select_begin(ctx, wt->words, S.bv_size, wt->dir, true, &wt->s1, &wt->s0, ones, &pend));
+  // the query blocks (rank9 directory done)
+  if (S.bv_size > 0) {
+    const uint64_t nblocks = S.bv_size / FMR_BITS + 1;
+    WTG_TRY(wtg_malloc(ctx, (void**)&wt->fmr, nblocks * 64));
+    WTG_LAUNCH(ctx, "huff_fmr", k_huff_fmr, dim3((unsigned)wtg_ceil_div(nblocks * 8, 256)), dim3(256), 0,
+               wt->dir, wt->words, nw, nblocks, wt->fmr);
+    WTG_LAUNCH_CHECK();
+  }
+  // one host read for the long-superblock counts and the per-node rank bases
+  uint32_t nl[2] = {0, 0};
+  uint64_t rb[256];
+  if (S.n_inner > 0) {
+    WTG_LAUNCH(ctx, "huff_rbase", k_huff_rbase, dim3(1), dim3(256), 0, wt->d_tab, wt->dir, wt->words);
+    WTG_LAUNCH_CHECK();
+  } else if (hipMemcpyAsync(wt->d_tab, &wt->tab, sizeof(HuffTab), hipMemcpyHostToDevice, ctx->stream) !=
+             hipSuccess) {
+    wtg_set_error("wt_huff: table copy failed");
+    return WTG_ERR_HIP;
+  }
+  if ((pend.active && hipMemcpyAsync(nl, pend.d_nl, 8, hipMemcpyDeviceToHost, ctx->stream) != hipSuccess) ||
+      (S.n_inner > 0 &&
+       hipMemcpyAsync(rb, wt->d_tab->rbase, sizeof(rb), hipMemcpyDeviceToHost, ctx->stream) != hipSuccess) ||
+      hipStreamSynchronize(ctx->stream) != hipSuccess) {
+    wtg_set_error("wt_huff: rank base copy failed");
+    return WTG_ERR_HIP;
+  }
+  WTG_TRY(wtg_rank_select_end(ctx, &pend, nl));
+  if (S.n_inner > 0) {
+    std::memcpy(wt->tab.rbase, rb, sizeof(rb));
+    uint32_t ni = 0;
+    for (uint32_t v = 0; v < wt->shape.n_nodes; v++)
+      if (wt->shape.child[v][0] >= 0) wt->shape.bv_pos_rank[v] = rb[ni++];
+  }
+  return WTG_OK;
+}
+
 extern "C" int wtg_huff_build(wtg_ctx* ctx, const uint8_t* d_text, uint64_t n, wtg_huff** out) {
   WTG_REQUIRE(ctx && out && (n == 0 || d_text), "wtg_huff_build: null argument");
   WTG_REQUIRE(n < (1ull << 40), "wtg_huff_build: n >= 2^40");
@@ -1147,48 +1194,62 @@ extern "C" int wtg_huff_build(wtg_ctx* ctx, const uint8_t* d_text, uint64_t n, w
       }
       WTG_LAUNCH_CHECK();
     }
-    // the ones of the tree bits: each byte contributes the ones of its code
-    uint64_t ones = 0;
-    for (int c = 0; c < 256; c++) ones += hist[c] * (uint64_t)__builtin_popcountll(wt->tab.code[c]);
-    wtg_rs_pending pend;
-    if ((rc = wtg_rank_select_begin(ctx, wt->words, S.bv_size, wt->dir, true, &wt->s1, &wt->s0, ones, &pend)) !=
-        WTG_OK)
-      goto fail;
-    // the query blocks (rank9 directory done)
-    if (S.bv_size > 0) {
-      const uint64_t nblocks = S.bv_size / FMR_BITS + 1;
-      if ((rc = wtg_malloc(ctx, (void**)&wt->fmr, nblocks * 64)) != WTG_OK) goto fail;
-      WTG_LAUNCH(ctx, "huff_fmr", k_huff_fmr, dim3((unsigned)wtg_ceil_div(nblocks * 8, 256)), dim3(256), 0,
-                 wt->dir, wt->words, nw, nblocks, wt->fmr);
-      WTG_LAUNCH_CHECK();
+    if ((rc = huff_finish(ctx, wt, hist)) != WTG_OK) goto fail;
+  }
+  if ((rc = wtg_finish(ctx)) != WTG_OK) goto fail;
+  *out = wt;
+  return WTG_OK;
+fail:
+  wtg_huff_destroy(ctx, wt);
+  return rc;
+}
+
+extern "C" int wtg_huff_from_bits(wtg_ctx* ctx, const uint64_t* counts, const uint64_t* d_words, uint64_t nbits,
+                                  wtg_huff** out) {
+  WTG_REQUIRE(ctx && out && counts, "wtg_huff_from_bits: null argument");
+  WTG_TRY(wtg_set_device(ctx));
+  *out = nullptr;
+  uint64_t hist[256], n = 0;
+  for (int c = 0; c < 256; c++) n += (hist[c] = counts[c]);
+  WTG_REQUIRE(n < (1ull << 40), "wtg_huff_from_bits: n >= 2^40");
+  wtg_huff* wt = new wtg_huff();
+  int rc = WTG_OK;
+  wt->shape.n = n;
+  if (n == 0) {
+    for (int c = 0; c < 256; c++) wt->shape.c_to_leaf[c] = -1;
+    *out = wt;
+    return WTG_OK;
+  }
+  build_shape(hist, n, wt->shape, wt->tab);
+  if (wt->shape.max_depth > 64) {
+    wtg_set_error("wtg_huff_from_bits: Huffman code longer than 64 bits");
+    wtg_huff_destroy(ctx, wt);
+    return WTG_ERR_UNSUPPORTED;
+  }
+  {
+    const uint64_t nb = wt->shape.bv_size, nw = (nb + 63) / 64 + 1;
+    if (nb != nbits || (nb && !d_words)) {
+      wtg_set_error("wtg_huff_from_bits: the bits do not match the Huffman tree of the counts");
+      wtg_huff_destroy(ctx, wt);
+      return WTG_ERR_ARG;
     }
-    // one host read for the long-superblock counts and the per-node rank bases
-    uint32_t nl[2] = {0, 0};
-    uint64_t rb[256];
-    if (S.n_inner > 0) {
-      WTG_LAUNCH(ctx, "huff_rbase", k_huff_rbase, dim3(1), dim3(256), 0, wt->d_tab, wt->dir, wt->words);
-      WTG_LAUNCH_CHECK();
-    } else if (hipMemcpyAsync(wt->d_tab, &wt->tab, sizeof(HuffTab), hipMemcpyHostToDevice, ctx->stream) !=
-               hipSuccess) {
-      wtg_set_error("wtg_huff_build: table copy failed");
+    if ((rc = wtg_malloc(ctx, (void**)&wt->d_tab, sizeof(HuffTab))) != WTG_OK) goto fail;
+    if ((rc = wtg_malloc(ctx, (void**)&wt->words, nw * 8)) != WTG_OK) goto fail;
+    if ((rc = wtg_malloc(ctx, (void**)&wt->dir, wtg_rank9_words(nb) * 8)) != WTG_OK) goto fail;
+    if ((rc = wtg_fill_u64(ctx, wt->words, nw, 0)) != WTG_OK) goto fail;
+    if (nb && hipMemcpyAsync(wt->words, d_words, (nb + 63) / 64 * 8, hipMemcpyDeviceToDevice, ctx->stream) !=
+                  hipSuccess) {
+      wtg_set_error("wtg_huff_from_bits: copy failed");
       rc = WTG_ERR_HIP;
       goto fail;
     }
-    if ((pend.active && hipMemcpyAsync(nl, pend.d_nl, 8, hipMemcpyDeviceToHost, ctx->stream) != hipSuccess) ||
-        (S.n_inner > 0 &&
-         hipMemcpyAsync(rb, wt->d_tab->rbase, sizeof(rb), hipMemcpyDeviceToHost, ctx->stream) != hipSuccess) ||
-        hipStreamSynchronize(ctx->stream) != hipSuccess) {
-      wtg_set_error("wtg_huff_build: rank base copy failed");
+    if (nb && hipMemcpyAsync(wt->d_tab, &wt->tab, sizeof(HuffTab), hipMemcpyHostToDevice, ctx->stream) !=
+                  hipSuccess) {
+      wtg_set_error("wtg_huff_from_bits: table copy failed");
       rc = WTG_ERR_HIP;
       goto fail;
     }
-    if ((rc = wtg_rank_select_end(ctx, &pend, nl)) != WTG_OK) goto fail;
-    if (S.n_inner > 0) {
-      std::memcpy(wt->tab.rbase, rb, sizeof(rb));
-      uint32_t ni = 0;
-      for (uint32_t v = 0; v < wt->shape.n_nodes; v++)
-        if (wt->shape.child[v][0] >= 0) wt->shape.bv_pos_rank[v] = rb[ni++];
-    }
+    if ((rc = huff_finish(ctx, wt, hist)) != WTG_OK) goto fail;
   }
   if ((rc = wtg_finish(ctx)) != WTG_OK) goto fail;
   *out = wt;

@@ -160,6 +160,58 @@ __global__ __launch_bounds__(256) void k_lsel_query(const LselDesc* __restrict__
   }
 }
 
+// ------------------------------------------------------ access counting
+// The 64-byte sectors of HBM one query reads, counted by walking the same
+// addresses as k_lsel_query<LEARNED> (a report, not on the query path).
+// Plain (select_support_mcl_par.hpp:564-611): long_index[sb], then the long
+// pool entry, or superblock[sb] + the mini sample + the scanned words.
+// Learned: the lookup table entry, the knots the probe reads, the rank9
+// entries of the three-block window and the answer's word.
+__device__ __forceinline__ uint32_t sectors_span(const void* first, const void* last) {
+  return (uint32_t)(((uintptr_t)last >> 6) - ((uintptr_t)first >> 6) + 1);
+}
+
+__device__ uint32_t plain_sectors(const SelQ& s, uint64_t j, uint64_t ans) {
+  const uint64_t i = j - 1, sb = i >> 12, off = i & 0xFFF;
+  if (s.long_index[sb] >= 0) return 2;
+  if ((off & 63) == 0) return 3;
+  const uint64_t base = s.superblock[sb] + s.mini[sb * 64 + (off >> 6)];
+  return 3 + sectors_span(s.words + ((base + 1) >> 6), s.words + (ans >> 6));
+}
+
+__device__ uint32_t learned_sectors(const LselDesc& d, uint64_t j, uint64_t ans) {
+  uint32_t k = d.lut[j >> d.lut_log];
+  const uint32_t k0 = k;
+  while (j >= d.knots[k + 1].j && k + 2 < d.nknots) ++k;
+  uint32_t n = 1 + sectors_span(&d.knots[k0], &d.knots[k + 1]);
+  const uint64_t b = lsel_predict(d, j) >> 9;
+  const uint64_t b0 = b ? b - 1 : 0;
+  const uint64_t last = b0 + 3 <= d.nblk ? b0 + 3 : d.nblk;
+  n += b0 <= d.nblk ? sectors_span(d.dir + 2 * b0, d.dir + 2 * last) : 0;
+  const uint64_t s0 = b0 <= d.nblk ? d.dir[2 * b0] : d.ones;
+  const uint64_t s3 = b0 + 3 <= d.nblk ? d.dir[2 * (b0 + 3)] : d.ones;
+  if (s0 >= j || s3 < j) return n + plain_sectors(d.s1, j, ans);  // outside the window
+  return n + 1;
+}
+
+template <bool LEARNED>
+__global__ __launch_bounds__(256) void k_lsel_sectors(const LselDesc* __restrict__ descs, uint32_t nvec,
+                                                      const uint32_t* __restrict__ qv,
+                                                      const uint64_t* __restrict__ qj,
+                                                      const uint64_t* __restrict__ ans, uint64_t nq,
+                                                      uint32_t* __restrict__ out) {
+  const uint64_t q = (uint64_t)blockIdx.x * blockDim.x + threadIdx.x;
+  if (q >= nq) return;
+  const uint32_t b = qv[q];
+  const uint64_t j = qj[q];
+  uint32_t n = 0;
+  if (b < nvec && j >= 1 && j <= descs[b].ones) {
+    const LselDesc& d = descs[b];
+    n = (LEARNED && d.learned) ? learned_sectors(d, j, ans[q]) : plain_sectors(d.s1, j, ans[q]);
+  }
+  out[q] = n;
+}
+
 // ------------------------------------------------------------- placement
 struct PlaceVec {
   SelQ s1;
@@ -717,6 +769,22 @@ extern "C" int wtg_lsel_select_batch(wtg_ctx* ctx, const wtg_lsel* ls, const uin
 extern "C" int wtg_lsel_plain_batch(wtg_ctx* ctx, const wtg_lsel* ls, const uint32_t* d_vec,
                                     const uint64_t* d_j, uint64_t nq, uint64_t* d_out) {
   return lsel_query(ctx, ls, d_vec, d_j, nq, d_out, false);
+}
+
+extern "C" int wtg_lsel_sectors(wtg_ctx* ctx, const wtg_lsel* ls, const uint32_t* d_vec, const uint64_t* d_j,
+                                const uint64_t* d_ans, uint64_t nq, int learned, uint32_t* d_out) {
+  WTG_REQUIRE(ctx && ls && (nq == 0 || (d_vec && d_j && d_ans && d_out)), "wtg_lsel_sectors: null argument");
+  if (!nq) return WTG_OK;
+  WTG_TRY(wtg_set_device(ctx));
+  const dim3 g((unsigned)wtg_ceil_div(nq, 256));
+  if (learned)
+    WTG_LAUNCH(ctx, "lsel_sectors", k_lsel_sectors<true>, g, dim3(256), 0, ls->d_desc, ls->nvec, d_vec, d_j, d_ans,
+               nq, d_out);
+  else
+    WTG_LAUNCH(ctx, "lsel_sectors", k_lsel_sectors<false>, g, dim3(256), 0, ls->d_desc, ls->nvec, d_vec, d_j, d_ans,
+               nq, d_out);
+  WTG_LAUNCH_CHECK();
+  return wtg_finish(ctx);
 }
 
 extern "C" int wtg_lsel_loss_grad(wtg_ctx* ctx, const void* d_knots, uint32_t nknots, uint32_t stride,

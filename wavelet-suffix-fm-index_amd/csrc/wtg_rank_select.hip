@@ -288,6 +288,8 @@ __global__ __launch_bounds__(LF_THREADS) void k_long_fill(const uint64_t* __rest
 }
 
 // ------------------------------------------------------------- queries
+// rank1(i), or rank0(i) = i - rank1(i) for BIT = 0, from the rank1 directory
+template <int BIT>
 __global__ void k_rank9_query(const uint64_t* __restrict__ words, const uint64_t* __restrict__ D,
                               uint64_t nbits, const uint64_t* __restrict__ idx, uint64_t nq,
                               uint64_t* __restrict__ out) {
@@ -298,7 +300,7 @@ __global__ void k_rank9_query(const uint64_t* __restrict__ words, const uint64_t
   const uint64_t b = i >> 9;
   uint64_t r = D[2 * b] + ((D[2 * b + 1] >> (63 - 9 * ((i & 511) >> 6))) & 0x1FF);
   if (i & 63) r += popc64(words[i >> 6] & ((1ull << (i & 63)) - 1));
-  out[q] = r;
+  out[q] = BIT ? r : i - r;
 }
 
 __global__ void k_select_query(SelQ s, const uint64_t* __restrict__ iv, uint64_t nq, uint64_t m,
@@ -579,11 +581,20 @@ extern "C" int wtg_select_free(wtg_ctx* ctx, wtg_select* s) {
 
 extern "C" int wtg_rank9_batch(wtg_ctx* ctx, const uint64_t* d_words, const uint64_t* d_dir,
                                uint64_t nbits, const uint64_t* d_idx, uint64_t nq, uint64_t* d_out) {
+  return wtg_rank9_batch_bit(ctx, d_words, d_dir, nbits, 1, d_idx, nq, d_out);
+}
+
+extern "C" int wtg_rank9_batch_bit(wtg_ctx* ctx, const uint64_t* d_words, const uint64_t* d_dir, uint64_t nbits,
+                                   int bit, const uint64_t* d_idx, uint64_t nq, uint64_t* d_out) {
   WTG_REQUIRE(ctx && d_dir && (nq == 0 || (d_idx && d_out)), "wtg_rank9_batch: null argument");
+  WTG_REQUIRE(bit == 0 || bit == 1, "wtg_rank9_batch_bit: bit must be 0 or 1");
   if (nq == 0) return WTG_OK;
   WTG_TRY(wtg_set_device(ctx));
-  WTG_LAUNCH(ctx, "rank9_query", k_rank9_query, dim3((unsigned)wtg_ceil_div(nq, 256)), dim3(256), 0,
-             d_words, d_dir, nbits, d_idx, nq, d_out);
+  const dim3 g((unsigned)wtg_ceil_div(nq, 256));
+  if (bit)
+    WTG_LAUNCH(ctx, "rank9_query", k_rank9_query<1>, g, dim3(256), 0, d_words, d_dir, nbits, d_idx, nq, d_out);
+  else
+    WTG_LAUNCH(ctx, "rank9_query", k_rank9_query<0>, g, dim3(256), 0, d_words, d_dir, nbits, d_idx, nq, d_out);
   WTG_LAUNCH_CHECK();
   return wtg_finish(ctx);
 }

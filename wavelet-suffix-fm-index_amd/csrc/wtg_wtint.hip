@@ -244,6 +244,41 @@ fail:
   return rc;
 }
 
+extern "C" int wtg_wtint_from_tree(wtg_ctx* ctx, const uint64_t* d_tree, uint64_t n, uint32_t levels,
+                                   uint64_t sigma, wtg_wtint** out) {
+  WTG_REQUIRE(ctx && out && (n == 0 || d_tree), "wtg_wtint_from_tree: null argument");
+  WTG_REQUIRE(levels <= 64, "wtg_wtint_from_tree: levels > 64");
+  WTG_TRY(wtg_set_device(ctx));
+  *out = nullptr;
+  wtg_wtint* wt = new wtg_wtint();
+  wt->n = n;
+  wt->L = levels;
+  wt->sigma = sigma;
+  if (n == 0 || levels == 0) {
+    *out = wt;
+    return WTG_OK;
+  }
+  wt->nbits = n * levels;
+  const uint64_t nw = wtg_levels_words(n, levels);
+  int rc = WTG_OK;
+  if ((rc = wtg_malloc(ctx, (void**)&wt->words, (nw + 1) * 8)) != WTG_OK) goto fail;
+  if ((rc = wtg_malloc(ctx, (void**)&wt->dir, wtg_rank9_words(wt->nbits) * 8)) != WTG_OK) goto fail;
+  if ((rc = wtg_fill_u64(ctx, wt->words + nw, 1, 0)) != WTG_OK) goto fail;
+  if (hipMemcpyAsync(wt->words, d_tree, nw * 8, hipMemcpyDeviceToDevice, ctx->stream) != hipSuccess) {
+    wtg_set_error("wtg_wtint_from_tree: copy failed");
+    rc = WTG_ERR_HIP;
+    goto fail;
+  }
+  if ((rc = wtg_rank_select_run(ctx, wt->words, wt->nbits, wt->dir, true, &wt->s1, &wt->s0)) != WTG_OK)
+    goto fail;
+  if ((rc = wtg_finish(ctx)) != WTG_OK) goto fail;
+  *out = wt;
+  return WTG_OK;
+fail:
+  wtg_wtint_destroy(ctx, wt);
+  return rc;
+}
+
 extern "C" int wtg_wtint_destroy(wtg_ctx* ctx, wtg_wtint* wt) {
   if (!wt) return WTG_OK;
   WTG_REQUIRE(ctx, "null ctx");

@@ -1,10 +1,16 @@
-// rank_support_v_par.hpp -- drop-in for sdsl::rank_support_v<1,1> of the
+// rank_support_v_par.hpp -- drop-in for sdsl::rank_support_v<b,1> of the
 // reference (rank/rank_support_v_par.hpp:21-183): same include guard, class
-// name, constructor, rank()/operator()/size()/set_vector()/swap() and the
-// interleaved directory m_basic_block (superblock count + seven 9-bit block
-// counts per 512 bits).  The directory is built on the GPU (wtg_rank9_build)
-// and kept on the host, so rank() is a const, thread-safe host call exactly as
-// in the reference; rank_batch() answers many queries on the device.
+// name, constructor, rank()/operator()/size()/serialize()/load()/set_vector()/
+// swap() and the interleaved directory m_basic_block (superblock count + seven
+// 9-bit block counts per 512 bits).  The directory is built on the GPU
+// (wtg_rank9_build) and kept on the host, so rank() is a const, thread-safe
+// host call exactly as in the reference; rank_batch() answers many queries on
+// the device.
+//
+// Bit patterns: `1` (rank_1_type) and `0` (bit_vector::rank_0_type, counting
+// zeros over whole words, so the directory is 64*words - the ones' directory).
+// The two-bit patterns `10` and `01` of the reference (:57-58) are not built
+// here; instantiating them stops compilation with a message.
 #ifndef INCLUDED_SDSL_RANK_SUPPORT_V
 #define INCLUDED_SDSL_RANK_SUPPORT_V
 #include <cstdint>
@@ -12,6 +18,7 @@
 #include <ostream>
 #include <string>
 #include <vector>
+#include "sdsl_io.hpp"
 #include "sdsl_standin.hpp"
 #include "wtg_runtime.hpp"
 
@@ -19,8 +26,9 @@ namespace sdsl {
 
 template <uint8_t t_b = 1, uint8_t t_pat_len = 1>
 class rank_support_v {
-  static_assert(t_b == 1 && t_pat_len == 1,
-                "the GPU rank directory is built for the bit pattern `1` (rank_1_type)");
+  static_assert((t_b == 1 || t_b == 0) && t_pat_len == 1,
+                "rank_support_v: the GPU directory is built for the bit patterns `1` and `0`; "
+                "the patterns `10` and `01` are not supported by this drop-in");
 
  public:
   typedef uint64_t size_type;
@@ -31,6 +39,27 @@ class rank_support_v {
  private:
   const bit_vector* m_v = nullptr;
   std::vector<uint64_t> m_basic_block;
+
+  // pattern `0`: arguments are the zeros of whole words (bits past size()
+  // included, as the reference's trait counts ~word)
+  static uint64_t argword(uint64_t w) { return t_b ? w : ~w; }
+
+  // the ones' directory (GPU) -> the zeros' directory: every count c over j
+  // whole words becomes 64*j - c
+  void to_pattern(uint64_t nbits) {
+    if (t_b || nbits == 0) return;
+    const uint64_t W = (nbits + 63) / 64, S = m_basic_block.size() / 2 - 1;
+    for (uint64_t b = 0; b <= S; b++) {
+      m_basic_block[2 * b] = 512 * b - m_basic_block[2 * b];
+      const uint64_t m = W > 8 * b ? (W - 8 * b < 8 ? W - 8 * b : 8) : 0;
+      uint64_t F = m_basic_block[2 * b + 1], G = 0;
+      for (uint64_t j = 1; j < 8 && j <= m; j++) {
+        const uint64_t c = (F >> (63 - 9 * j)) & 0x1FF;
+        G |= (64 * j - c) << (63 - 9 * j);
+      }
+      m_basic_block[2 * b + 1] = G;
+    }
+  }
 
  public:
   explicit rank_support_v(const bit_vector* v = nullptr) {  // :69-123
@@ -57,35 +86,32 @@ class rank_support_v {
     wtg::check(wtg_rank9_build(wtg::default_context(), d_words, nbits, dd.as<uint64_t>()));
     m_basic_block.resize(nd);
     dd.download(m_basic_block.data(), nd * 8);
+    to_pattern(nbits);
   }
+  // the ones' directory of nbits bits, already built on the device
   void assign_directory(const uint64_t* d_dir, uint64_t nbits) {
     const uint64_t nd = wtg_rank9_words(nbits);
     m_basic_block.resize(nd);
     wtg::check(wtg_memcpy_d2h(wtg::default_context(), m_basic_block.data(), d_dir, nd * 8));
+    to_pattern(nbits);
   }
 
   size_type rank(size_type idx) const {  // :131-142
     const uint64_t* p = m_basic_block.data() + ((idx >> 8) & 0xFFFFFFFFFFFFFFFEULL);
     size_type r = *p + ((*(p + 1) >> (63 - 9 * ((idx & 0x1FF) >> 6))) & 0x1FF);
-    if (idx & 0x3F) r += __builtin_popcountll(m_v->data()[idx >> 6] & ((1ULL << (idx & 63)) - 1));
+    if (idx & 0x3F) r += __builtin_popcountll(argword(m_v->data()[idx >> 6]) & ((1ULL << (idx & 63)) - 1));
     return r;
   }
   size_type operator()(size_type idx) const { return rank(idx); }
   size_type size() const { return m_v->size(); }
 
-  // sdsl int_vector<64> layout of m_basic_block: bit size (uint64), then the words
+  // m_basic_block as an sdsl int_vector<64> image (rank_support_v_par.hpp:154-164)
   size_type serialize(std::ostream& out, structure_tree_node* = nullptr, std::string = "") const {
-    const uint64_t bits = m_basic_block.size() * 64;
-    out.write((const char*)&bits, 8);
-    out.write((const char*)m_basic_block.data(), m_basic_block.size() * 8);
-    return 8 + m_basic_block.size() * 8;
+    return wtg::sdslio::write_u64s(out, m_basic_block.data(), m_basic_block.size());
   }
-  void load(std::istream& in, const bit_vector* v = nullptr) {
+  void load(std::istream& in, const bit_vector* v = nullptr) {  // :166-170
     set_vector(v);
-    uint64_t bits = 0;
-    in.read((char*)&bits, 8);
-    m_basic_block.resize(bits / 64);
-    in.read((char*)m_basic_block.data(), m_basic_block.size() * 8);
+    wtg::sdslio::read_u64s(in, m_basic_block);
   }
   void set_vector(const bit_vector* v = nullptr) { m_v = v; }
   void swap(rank_support_v& rs) {
@@ -93,12 +119,25 @@ class rank_support_v {
   }
   const std::vector<uint64_t>& basic_block() const { return m_basic_block; }
 
-  // rank(idx) for a batch of device indices, answered on the device
+  // rank(idx) for a batch of device indices, answered on the device from the
+  // ones' directory
   void rank_batch(const uint64_t* d_words, const uint64_t* d_idx, uint64_t nq, uint64_t* d_out) const {
-    wtg::dbuf dd(m_basic_block.size() * 8);
-    dd.upload(m_basic_block.data(), m_basic_block.size() * 8);
-    wtg::check(wtg_rank9_batch(wtg::default_context(), d_words, dd.as<uint64_t>(),
-                               m_v ? m_v->size() : 0, d_idx, nq, d_out));
+    std::vector<uint64_t> ones_dir(m_basic_block);
+    const uint64_t nbits = m_v ? m_v->size() : 0;
+    if (!t_b) {  // back to the ones' directory
+      const uint64_t W = (nbits + 63) / 64, S = ones_dir.size() / 2 - 1;
+      for (uint64_t b = 0; b <= S && nbits; b++) {
+        ones_dir[2 * b] = 512 * b - ones_dir[2 * b];
+        const uint64_t m = W > 8 * b ? (W - 8 * b < 8 ? W - 8 * b : 8) : 0;
+        uint64_t F = ones_dir[2 * b + 1], G = 0;
+        for (uint64_t j = 1; j < 8 && j <= m; j++) G |= (64 * j - ((F >> (63 - 9 * j)) & 0x1FF)) << (63 - 9 * j);
+        ones_dir[2 * b + 1] = G;
+      }
+    }
+    wtg::dbuf dd(ones_dir.size() * 8);
+    dd.upload(ones_dir.data(), ones_dir.size() * 8);
+    wtg::check(wtg_rank9_batch_bit(wtg::default_context(), d_words, dd.as<uint64_t>(), nbits, t_b, d_idx, nq,
+                                   d_out));
   }
 };
 

@@ -197,6 +197,127 @@ class wt_huff {
   }
   const wtg_huff* device() const { return m_dev; }
 
+  // The reference's stream layout (wt_pc_par.hpp:750-764): size and sigma
+  // (uint64), the tree bit_vector, rank_support_v, select_support_mcl<1>, <0>,
+  // then the tree: the node count (uint64); per node in BFS order bv_pos and
+  // bv_pos_rank (uint64), parent, child[0], child[1] (uint16, 0xFFFF = none);
+  // the leaf of each byte (256 x uint16, 0xFFFF = absent); each byte's code
+  // (256 x uint64: branch bits from the root in bits 0..55, length in 56..63).
+  size_type serialize(std::ostream& out, structure_tree_node* = nullptr, std::string = "") const {
+    using namespace wtg::sdslio;
+    size_type n = write_member((uint64_t)m_size, out) + write_member((uint64_t)m_sigma, out);
+    n += write_bits(out, m_bv.data(), m_bv.size());
+    n += m_bv_rank.serialize(out);
+    n += m_bv_select1.serialize(out);
+    n += m_bv_select0.serialize(out);
+    const uint64_t nn = m_size ? m_shape.n_nodes : 0;
+    n += write_member(nn, out);
+    auto u16 = [](int32_t x) { return (uint16_t)(x < 0 ? 0xFFFF : x); };
+    for (uint64_t v = 0; v < nn; v++) {
+      const uint64_t pos = m_shape.bv_pos[v];
+      const uint64_t prk = m_bv.size() ? m_bv_rank(pos) : 0;
+      n += write_member(pos, out) + write_member(prk, out);
+      n += write_member(u16(m_shape.parent[v]), out) + write_member(u16(m_shape.child[v][0]), out) +
+           write_member(u16(m_shape.child[v][1]), out);
+    }
+    for (int c = 0; c < 256; c++) n += write_member(u16(m_size ? m_shape.c_to_leaf[c] : -1), out);
+    for (int c = 0; c < 256; c++) {
+      const uint64_t len = m_size && m_shape.c_to_leaf[c] >= 0 ? m_shape.path_len[c] : 0;
+      n += write_member((uint64_t)((len ? m_shape.path[c] : 0) | (len << 56)), out);
+    }
+    return n;
+  }
+  // wt_pc_par.hpp:766-775.  The device copy for the batched queries is rebuilt
+  // from the byte counts (the frequencies of the leaves) and the bits; a tree
+  // that is not the Huffman tree of its counts is rejected.
+  void load(std::istream& in) {
+    using namespace wtg::sdslio;
+    release();
+    uint64_t size = 0, sigma = 0, nbits = 0, nn = 0;
+    read_member(size, in);
+    read_member(sigma, in);
+    std::vector<uint64_t> words;
+    read_bits(in, words, nbits);
+    m_size = size;
+    m_sigma = sigma;
+    m_bv = bit_vector_type(nbits);
+    if (nbits) std::copy(words.begin(), words.begin() + (nbits + 63) / 64, m_bv.data());
+    m_bv_rank.load(in, &m_bv);
+    m_bv_select1.load(in, &m_bv);
+    m_bv_select0.load(in, &m_bv);
+    read_member(nn, in);
+    if (nn > WTG_HUFF_MAX_NODES) throw std::runtime_error("wt_huff::load: more than 511 tree nodes");
+    m_shape = wtg_huff_shape{};
+    m_shape.n = m_size;
+    m_shape.bv_size = nbits;
+    m_shape.n_nodes = (uint32_t)nn;
+    auto i32 = [](uint16_t x) { return x == 0xFFFF ? -1 : (int32_t)x; };
+    for (uint64_t v = 0; v < nn; v++) {
+      uint64_t pos = 0, prk = 0;
+      uint16_t p = 0, c0 = 0, c1 = 0;
+      read_member(pos, in); read_member(prk, in);
+      read_member(p, in); read_member(c0, in); read_member(c1, in);
+      m_shape.bv_pos[v] = pos;
+      m_shape.bv_pos_rank[v] = prk;
+      m_shape.parent[v] = i32(p);
+      m_shape.child[v][0] = i32(c0);
+      m_shape.child[v][1] = i32(c1);
+      m_shape.sym[v] = -1;
+      if (c0 != 0xFFFF) m_shape.n_inner++;
+    }
+    for (int c = 0; c < 256; c++) {
+      uint16_t x = 0;
+      read_member(x, in);
+      m_shape.c_to_leaf[c] = i32(x);
+      if (x != 0xFFFF) {
+        if (x >= nn) throw std::runtime_error("wt_huff::load: leaf index out of range");
+        m_shape.sym[x] = c;
+      }
+    }
+    for (int c = 0; c < 256; c++) {
+      uint64_t p = 0;
+      read_member(p, in);
+      m_shape.path_len[c] = (uint32_t)(p >> 56);
+      m_shape.path[c] = p & ((1ull << 56) - 1);
+      m_shape.max_depth = std::max(m_shape.max_depth, m_shape.path_len[c]);
+    }
+    m_shape.sigma = (uint32_t)m_sigma;
+    if (!m_size) return;
+    // leaf frequencies: the root holds every position; an inner node's ones go
+    // to child[1], its zeros to child[0] (BFS order: parents come first)
+    std::vector<uint64_t> freq(nn, 0);
+    if (nn == 0) throw std::runtime_error("wt_huff::load: a nonempty text without tree nodes");
+    freq[0] = m_size;
+    for (uint64_t v = 0; v < nn; v++) {
+      const int32_t c0 = m_shape.child[v][0], c1 = m_shape.child[v][1];
+      if (c0 < 0) continue;
+      if (c0 >= (int32_t)nn || c1 < 0 || c1 >= (int32_t)nn || m_shape.bv_pos[v] + freq[v] > nbits)
+        throw std::runtime_error("wt_huff::load: inconsistent tree");
+      const uint64_t a = m_shape.bv_pos[v], ones = m_bv_rank(a + freq[v]) - m_bv_rank(a);
+      freq[m_shape.child[v][1]] = ones;
+      freq[m_shape.child[v][0]] = freq[v] - ones;
+    }
+    uint64_t counts[256] = {0};
+    for (int c = 0; c < 256; c++)
+      if (m_shape.c_to_leaf[c] >= 0) counts[c] = freq[m_shape.c_to_leaf[c]];
+    m_shape.C[0] = 0;
+    for (int c = 0; c < 256; c++) m_shape.C[c + 1] = m_shape.C[c] + counts[c];
+    wtg::dbuf dw(words.size() * 8);
+    dw.upload(words.data(), words.size() * 8);
+    wtg::check(wtg_huff_from_bits(wtg::default_context(), counts, dw.as<uint64_t>(), nbits, &m_dev));
+    wtg_huff_shape ds{};
+    wtg::check(wtg_huff_get_shape(m_dev, &ds));
+    bool same = ds.n_nodes == m_shape.n_nodes && ds.bv_size == m_shape.bv_size;
+    for (uint32_t v = 0; same && v < ds.n_nodes; v++)
+      same = ds.child[v][0] == m_shape.child[v][0] && ds.child[v][1] == m_shape.child[v][1] &&
+             ds.sym[v] == m_shape.sym[v] && (ds.child[v][0] < 0 || ds.bv_pos[v] == m_shape.bv_pos[v]);
+    if (!same) {
+      release();
+      throw std::runtime_error("wt_huff::load: the stored tree is not the Huffman tree of its symbol counts");
+    }
+    m_shape = ds;
+  }
+
  private:
   void build(const unsigned char* host_text) {
     wtg::dbuf dt(m_size);

@@ -288,6 +288,45 @@ class wt_int {
   }
   const wtg_wtint* device() const { return m_dev; }
 
+  // The reference's stream layout (:886-899): size and sigma (uint64), the tree
+  // bit_vector, rank_support_v, select_support_mcl<1>, <0>, max_level (uint32).
+  size_type serialize(std::ostream& out, structure_tree_node* = nullptr, std::string = "") const {
+    using namespace wtg::sdslio;
+    size_type n = write_member((uint64_t)m_size, out) + write_member((uint64_t)m_sigma, out);
+    n += write_bits(out, m_tree.data(), m_tree.size());
+    n += m_tree_rank.serialize(out);
+    n += m_tree_select1.serialize(out);
+    n += m_tree_select0.serialize(out);
+    n += write_member(m_max_level, out);
+    return n;
+  }
+  // :901-911; the device copy for the batched queries is rebuilt from the tree
+  void load(std::istream& in) {
+    using namespace wtg::sdslio;
+    if (m_dev) wtg_wtint_destroy(wtg::default_context(), m_dev);
+    m_dev = nullptr;
+    uint64_t size = 0, sigma = 0, nbits = 0;
+    read_member(size, in);
+    read_member(sigma, in);
+    std::vector<uint64_t> words;
+    read_bits(in, words, nbits);
+    m_size = size;
+    m_sigma = sigma;
+    m_tree = bit_vector_type(nbits);
+    if (nbits) std::copy(words.begin(), words.begin() + (nbits + 63) / 64, m_tree.data());
+    m_tree_rank.load(in, &m_tree);
+    m_tree_select1.load(in, &m_tree);
+    m_tree_select0.load(in, &m_tree);
+    read_member(m_max_level, in);
+    if (m_size && m_max_level) {
+      if (nbits != m_size * m_max_level) throw std::runtime_error("wt_int::load: tree size != size * max_level");
+      wtg::dbuf dw(words.size() * 8);
+      dw.upload(words.data(), words.size() * 8);
+      wtg::check(wtg_wtint_from_tree(wtg::default_context(), dw.as<uint64_t>(), m_size, m_max_level, m_sigma,
+                                     &m_dev));
+    }
+  }
+
  private:
   // one node of interval_symbols (:119-158): [i, j) of the node at (level, path),
   // left child before right child so the values come out ascending
