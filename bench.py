@@ -109,14 +109,14 @@ def cpu_baseline_levels(s, L, sample_n, rounds, sym):
 
 
 def measured_traffic(log2n, L):
-    """HBM bytes per level-split launch from the committed rocprofv3 PMC summary
-    (FETCH_SIZE + WRITE_SIZE, each scaled by a 1 GiB copy), for the workload it
-    was measured on; None otherwise."""
-    path = os.path.join(ROOT, "profiles", "level_split_u32_n268M_L20.md")
+    """HBM bytes per multi-level emit launch from the committed rocprofv3 PMC
+    summary (FETCH_SIZE + WRITE_SIZE, each scaled by a 1 GiB copy), for the
+    workload it was measured on; None otherwise."""
+    path = os.path.join(ROOT, "profiles", "mld_emit_u32_n268M_L20.md")
     if (log2n, L) != (28, 20) or not os.path.exists(path):
         return None
     for line in open(path):
-        if line.startswith("traffic_bytes_per_launch(level_split) = "):
+        if line.startswith("traffic_bytes_per_launch(mld_emit) = "):
             return float(line.split("=")[1])
     return None
 
@@ -184,10 +184,6 @@ def bench_wt_int(pkg, args, full):
     for _ in range(args.warmup):
         ctx.levels_build(d, n, L, dw)
     ctx.sync()
-    ctx.kernel_timing_reset()
-    ctx.kernel_timing_only("level_")  # events around the level kernels only
-    ctx.kernel_timing(True)
-    ctx.sync()
     log("timing %d steps" % args.steps)
     t0 = time.perf_counter()
     ctx.timer_start()
@@ -196,9 +192,23 @@ def bench_wt_int(pkg, args, full):
     ms = ctx.timer_stop()
     ctx.sync()
     wall = time.perf_counter() - t0
-    split_ms, split_n = ctx.kernel_time("level_split")
-    bits_ms, bits_n = ctx.kernel_time("level_bits")
+    # per-kernel times in separate builds (events around the multi-level kernels only)
+    ksteps = 5
+    ctx.kernel_timing_reset()
+    ctx.kernel_timing_only("mld_")
+    ctx.kernel_timing(True)
+    for _ in range(ksteps):
+        ctx.levels_build(d, n, L, dw)
+    kt = {k: ctx.kernel_time("mld_" + k) for k in ("count", "scan", "hist", "cum", "emit", "ntiles", "tiles")}
     ctx.kernel_timing(False)
+    ctx.kernel_timing_only(None)
+    # the level-by-level split pipeline on the same input, for comparison
+    ctx.set_level_method(1)
+    ctx.levels_build(d, n, L, dw)
+    split_ms_step = timed(ctx, lambda: ctx.levels_build(d, n, L, dw), 3)
+    ctx.set_level_method(0)
+    ctx.levels_build(d, n, L, dw)
+    ctx.sync()
     ctx.set_blocking(True)
 
     parity = {}
@@ -255,25 +265,46 @@ def bench_wt_int(pkg, args, full):
     ctx.close()
 
     step_ms = ms / args.steps
-    split_bytes = 2.0 * n * w + n / 8.0
-    split_avg_s = split_ms / max(split_n, 1) / 1e3
-    achieved = split_bytes / split_avg_s / 1e9
+    # the multi-level direct passes: groups of up to 8 levels; per group the
+    # count kernel reads the keys, the emit kernel reads them, writes the group's
+    # level bits and (all but the last group) the keys in the next order (uint32)
+    groups, s_ = [], 0
+    while s_ < L:
+        g_ = min(8, L - s_)
+        groups.append(g_)
+        s_ += g_
+    kw = [w] + [4] * (len(groups) - 1)
+    emit_bytes = sum(n * kw[i] + (4 * n if i + 1 < len(groups) else 0) + g_ * n / 8.0
+                     for i, g_ in enumerate(groups))
+    count_bytes = sum(n * kw[i] for i in range(len(groups)))
+    emit_ms, emit_n = kt["emit"]
+    emit_s_step = emit_ms / ksteps / 1e3
+    achieved = emit_bytes / emit_s_step / 1e9
+    kms = {k: round(v[0] / ksteps, 4) for k, v in kt.items()}
     out = {
         "metric": METRIC, "value": round(n / (step_ms / 1e3) / 1e9, 4), "unit": "Gsym/s", "n_gpus": 1,
         "steps": args.steps, "warmup": args.warmup, "ms_per_step": round(step_ms, 4),
         "higher_is_better": True, "scaling": "weak", "vs_baseline": None, "dtype": "u32",
         "data": "synthetic",
-        "config": {"workload": "wt_int tree build (levelWT level split), n=2^%d uint32 uniform "
+        "config": {"workload": "wt_int tree build (levelWT levels), n=2^%d uint32 uniform "
                                "in [0,2^%d), max_level=%d" % (args.log2n, L, L),
                    "n": n, "levels": L, "sym_bytes": w, "seed": SEED},
         "roofline": {"bound": "hbm", "achieved": round(achieved, 1), "peak": HBM_PEAK_GBS,
                      "unit": "GB/s", "frac": round(achieved / HBM_PEAK_GBS, 4),
                      "traffic": measured_traffic(args.log2n, L),
-                     "kernel": "k_split<uint32> (level_split)",
-                     "bytes_per_launch": split_bytes, "avg_launch_ms": round(split_avg_s * 1e3, 4),
-                     "launches": split_n},
+                     "kernel": "k_mld_emit (multi-level pass: %s levels per launch)" % "+".join(map(str, groups)),
+                     "bytes_per_launch": round(emit_bytes / len(groups), 1),
+                     "avg_launch_ms": round(emit_ms / max(emit_n, 1), 4), "launches": emit_n},
         "build_roofline": {"B_a_bytes": B_a, "frac": frac(B_a, step_ms),
-                           "level_bits_avg_ms": round(bits_ms / max(bits_n, 1), 4)},
+                           "bytes_moved_model": emit_bytes + count_bytes,
+                           "frac_of_moved": frac(emit_bytes + count_bytes, step_ms),
+                           "note": "B_a = n*(2*L*w + L/8) is the level-by-level model; the multi-level passes "
+                                   "move bytes_moved_model (keys read twice and written once per group of up "
+                                   "to 8 levels, plus the bits)",
+                           "kernels_ms_per_step": kms},
+        "level_split_pipeline": {"ms": round(split_ms_step, 4),
+                                 "Gsym_s": round(n / (split_ms_step / 1e3) / 1e9, 4),
+                                 "frac_B_a": frac(B_a, split_ms_step)},
         "parity": parity,
         "wt_int_build": wt_int,
         "rank_select": sup,

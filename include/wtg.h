@@ -47,6 +47,13 @@ int wtg_ctx_sync(wtg_ctx* ctx);
 /* 1 (default): every entry point returns after its work finished.
  * 0: entry points only enqueue; errors surface at the next wtg_ctx_sync. */
 int wtg_ctx_set_blocking(wtg_ctx* ctx, int blocking);
+/* How the level-wise trees of this context are built (same output either way):
+ * 0 (default): multi-level direct passes, up to 8 levels per pass over the keys;
+ * 1: one stable split pass per level (the reference's own schedule,
+ *    wavelet/levelWT/WT.C:43-227).  The environment variable WTG_LEVEL_SPLIT=1
+ *    selects 1 for every context.  No counterpart in the reference, which has
+ *    one schedule. */
+int wtg_ctx_set_level_method(wtg_ctx* ctx, int method);
 /* The HIP stream (hipStream_t) the context launches on, as an opaque pointer. */
 void* wtg_ctx_stream(wtg_ctx* ctx);
 

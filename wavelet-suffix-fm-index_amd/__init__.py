@@ -76,6 +76,7 @@ SIGNATURES = {
     "wtg_ctx_destroy": (i32, [vp]),
     "wtg_ctx_sync": (i32, [vp]),
     "wtg_ctx_set_blocking": (i32, [vp, i32]),
+    "wtg_ctx_set_level_method": (i32, [vp, i32]),
     "wtg_ctx_stream": (vp, [vp]),
     "wtg_malloc": (i32, [vp, C.POINTER(vp), u64]),
     "wtg_free": (i32, [vp, vp]),
@@ -401,6 +402,10 @@ class Context:
 
     def set_blocking(self, b):
         _check(lib().wtg_ctx_set_blocking(self.h, 1 if b else 0))
+
+    def set_level_method(self, method):
+        """0: multi-level direct passes (default); 1: one split pass per level."""
+        _check(lib().wtg_ctx_set_level_method(self.h, int(method)))
 
     # ---- memory
     def empty(self, n, dtype=np.uint64):

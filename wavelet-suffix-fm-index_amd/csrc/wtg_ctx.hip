@@ -108,6 +108,13 @@ int wtg_finish(wtg_ctx* ctx) {
   return WTG_OK;
 }
 
+extern "C" int wtg_ctx_set_level_method(wtg_ctx* ctx, int method) {
+  WTG_REQUIRE(ctx, "null ctx");
+  WTG_REQUIRE(method == 0 || method == 1, "wtg_ctx_set_level_method: method must be 0 or 1");
+  ctx->level_method = method;
+  return WTG_OK;
+}
+
 extern "C" int wtg_ctx_set_blocking(wtg_ctx* ctx, int blocking) {
   WTG_REQUIRE(ctx, "null ctx");
   ctx->blocking = blocking ? 1 : 0;

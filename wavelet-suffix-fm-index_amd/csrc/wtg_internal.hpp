@@ -33,6 +33,7 @@ struct wtg_ctx {
   int device = 0;
   hipStream_t stream = nullptr;
   int blocking = 1;
+  int level_method = 0;  // 0: multi-level direct passes, 1: level-by-level split
   // growable workspace (device); regions carved per call
   void* ws = nullptr;
   size_t ws_bytes = 0;
@@ -103,6 +104,11 @@ struct wtg_rs_pending {
 int wtg_rank_select_begin(wtg_ctx* ctx, const uint64_t* d_words, uint64_t nbits, uint64_t* d_dir, bool build,
                           wtg_select* sel1, wtg_select* sel0, uint64_t known_ones, wtg_rs_pending* pend);
 int wtg_rank_select_end(wtg_ctx* ctx, const wtg_rs_pending* pend, const uint32_t nl[2]);
+// multi-level direct build (up to 8 levels per pass; n >= 1, L <= 24, n < 2^32)
+// and whether it is on (WTG_LEVEL_SPLIT=1 selects the level-by-level split)
+int wtg_mld_enabled();
+int wtg_mld_build(wtg_ctx* ctx, const void* d_sym, int sym_bytes, uint64_t n, uint32_t L, uint64_t* d_words,
+                  wtg_node* d_nodes, uint64_t node_slots, unsigned long long* d_distinct);
 // resolves the per-device address of the allocation-free rank9 scratch (wtg_ctx_create)
 int wtg_noalloc_init_device(int device);
 int wtg_max_value(wtg_ctx* ctx, const void* d_sym, int sym_bytes, uint64_t n, uint64_t* out);

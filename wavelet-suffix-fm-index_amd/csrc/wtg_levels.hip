@@ -1229,6 +1229,8 @@ int wtg_levels_build_async(wtg_ctx* ctx, const void* d_sym, int sym_bytes, uint6
   WTG_TRY(levels_check(ctx, sym_bytes, n, levels));
   WTG_REQUIRE(n == 0 || (d_sym && d_words), "null device buffer");
   WTG_TRY(wtg_set_device(ctx));
+  if (n > 0 && ctx->level_method == 0 && wtg_mld_enabled())
+    return wtg_mld_build(ctx, d_sym, sym_bytes, n, levels, d_words, d_nodes, node_slots, d_distinct);
   switch (sym_bytes) {
     case 1: return levels_build_t<uint8_t>(ctx, (const uint8_t*)d_sym, n, levels, d_words, d_nodes, node_slots, d_distinct);
     case 2: return levels_build_t<uint16_t>(ctx, (const uint16_t*)d_sym, n, levels, d_words, d_nodes, node_slots, d_distinct);
