@@ -44,8 +44,6 @@
 namespace {
 
 constexpr uint32_t MLD_T = 2048;            // keys per tile
-constexpr uint32_t MLD_E = MLD_T / 256;     // keys per thread
-constexpr uint32_t MLD_WK = MLD_T / 4;      // keys per wave
 constexpr uint32_t MLD_GMAX = 8;            // levels per group
 constexpr uint32_t MLD_BW = MLD_T / 32 + 2; // staged bit words per level (+2 read-ahead pad)
 constexpr uint32_t MLD_CPW = MLD_T / 64 / 4; // 64-key chunks per wave
@@ -104,44 +102,53 @@ __device__ __forceinline__ uint64_t mld_key(const void* keys, uint64_t i) {
   return (uint64_t)reinterpret_cast<const K*>(keys)[i];
 }
 
-// leaf counts of one tile: LDS atomics per key; a chunk whose keys all share
+// leaf counts of the tiles, one wave per tile (all of its loads in flight at
+// once, no block barrier): LDS atomics per key; a chunk whose keys all share
 // one leaf adds its size once
+constexpr uint32_t MLD_CNT_C = MLD_T / 64;  // 64-key chunks per tile
 template <typename K>
 __global__ __launch_bounds__(256) void k_mld_count(MldArgs a) {
-  __shared__ uint32_t s_cnt[1u << MLD_GMAX];
+  __shared__ uint32_t s_cnt_all[4][1u << MLD_GMAX];
+  const uint32_t w = (uint32_t)__builtin_amdgcn_readfirstlane((int)(threadIdx.x >> 6));
+  const uint32_t t = blockIdx.x * 4 + w;
+  uint32_t* s_cnt = s_cnt_all[w];
   uint32_t seg, len, first;
   uint64_t base;
-  if (!mld_tile(a, blockIdx.x, seg, base, len, first)) return;
-  const uint32_t G = 1u << a.g, w = (uint32_t)__builtin_amdgcn_readfirstlane((int)(threadIdx.x >> 6));
-  for (uint32_t i = threadIdx.x; i < G; i += 256) s_cnt[i] = 0;
-  __syncthreads();
+  if (!mld_tile(a, t, seg, base, len, first)) return;
+  const uint32_t G = 1u << a.g, lane = (uint32_t)lane_id();
   const uint32_t sh = a.L - a.s - a.g;
   bool bad = false;
-  uint32_t dd[MLD_E];  // the wave's leaves, all loads issued first
+  uint32_t dd[MLD_CNT_C];  // the tile's leaves, all loads issued first
 #pragma unroll
-  for (uint32_t m = 0; m < MLD_E; m++) {
-    const uint32_t k = w * MLD_WK + m * 64 + (uint32_t)lane_id();
+  for (uint32_t m = 0; m < MLD_CNT_C; m++) {
+    const uint32_t k = m * 64 + lane;
     const uint64_t x = k < len ? mld_key<K>(a.keys, base + k) : 0;
     if (a.s == 0 && (x >> a.L) != 0) bad = true;
     dd[m] = (uint32_t)(x >> sh) & (G - 1);
   }
+  for (uint32_t i = lane; i < G; i += 64) s_cnt[i] = 0;
+  __builtin_amdgcn_fence(__ATOMIC_RELEASE, "wavefront");
+  __builtin_amdgcn_wave_barrier();
+  __builtin_amdgcn_fence(__ATOMIC_ACQUIRE, "wavefront");
 #pragma unroll
-  for (uint32_t m = 0; m < MLD_E; m++) {
-    const uint32_t k = w * MLD_WK + m * 64 + (uint32_t)lane_id();
-    if (w * MLD_WK + m * 64 >= len) break;  // wave-uniform
+  for (uint32_t m = 0; m < MLD_CNT_C; m++) {
+    const uint32_t k = m * 64 + lane;
+    if (m * 64 >= len) break;  // wave-uniform
     const bool act = k < len;
     const uint32_t d = dd[m];
     const uint32_t d0 = (uint32_t)__builtin_amdgcn_readfirstlane((int)d);  // lane 0's leaf (lane 0 is active)
     const uint32_t nact = popc64(wave_ballot(act));
     if (wave_ballot(act && d != d0) == 0) {
-      if (lane_id() == 0) atomicAdd(&s_cnt[d0], nact);
+      if (lane == 0) atomicAdd(&s_cnt[d0], nact);
     } else if (act) {
       atomicAdd(&s_cnt[d], 1u);
     }
   }
   if (bad) atomicOr(a.err, 2u);
-  __syncthreads();
-  for (uint32_t i = threadIdx.x; i < G; i += 256) a.tcnt_w[(uint64_t)blockIdx.x * G + i] = s_cnt[i];
+  __builtin_amdgcn_fence(__ATOMIC_RELEASE, "wavefront");
+  __builtin_amdgcn_wave_barrier();
+  __builtin_amdgcn_fence(__ATOMIC_ACQUIRE, "wavefront");
+  for (uint32_t i = lane; i < G; i += 64) a.tcnt_w[(uint64_t)t * G + i] = s_cnt[i];
 }
 
 // column-wise exclusive prefix of the leaf counts over the tiles, blocks of
@@ -268,7 +275,7 @@ __device__ __forceinline__ uint64_t lds_bits64(const uint32_t* row, uint32_t p) 
 }
 
 template <typename K, bool SCATTER>
-__global__ __launch_bounds__(256) void k_mld_emit(MldArgs a) {
+__global__ __launch_bounds__(256) __attribute__((amdgpu_waves_per_eu(8))) void k_mld_emit(MldArgs a) {
   // LDS of the level pipeline: keys (ping-pong), tile prefix and staged bits
   // (under 20 KiB at 2048-key tiles)
   __shared__ __align__(16) uint32_t s_k[2][MLD_T];   // the tile's keys in its level order
@@ -276,6 +283,8 @@ __global__ __launch_bounds__(256) void k_mld_emit(MldArgs a) {
   __shared__ uint16_t s_tpre[(1u << MLD_GMAX) + 1];   // exclusive prefix over d of the tile's counts
   __shared__ __align__(8) uint32_t s_bits[MLD_GMAX][MLD_BW];  // level bits in the tile's level order
   static_assert(MLD_BW % 2 == 0, "64-bit rows");
+  __shared__ uint32_t s_nxy[(1u << MLD_GMAX) - 1];    // node tables of the levels (see below)
+  __shared__ uint32_t s_lvl0[MLD_GMAX];
   __shared__ uint32_t s_wave[4];
   uint32_t seg, len, first;
   uint64_t base;
@@ -313,6 +322,31 @@ __global__ __launch_bounds__(256) void k_mld_emit(MldArgs a) {
     if (d == 0) s_tpre[G] = (uint16_t)tot;
   }
   __syncthreads();
+  // node tables of all levels, from the tile's leaf counts alone.  Entry
+  // r = 2^j - 1 + q is node q of level s+j: start S, ZN keys with bit 0 (its
+  // first child), ZS keys with bit 0 in the nodes before it.  The split sends
+  // the node's k-th zero to S + k and its k-th one to S + ZN + k: for a key at
+  // p with zc zeros before it in the tile that is X + zc (zero) or Y + p - zc
+  // (one), with X = S - ZS and Y = ZS + ZN, packed X | Y << 16
+  {
+    const uint32_t r = threadIdx.x;
+    const uint32_t j = 31 - __clz(r + 1);
+    uint32_t S = 0, zn = 0;
+    if (r < G - 1) {
+      const uint32_t q = r + 1 - (1u << j), sgj = g - j;
+      S = s_tpre[q << sgj];
+      zn = s_tpre[(2 * q + 1) << (sgj - 1)] - S;
+    }
+    uint32_t tot;
+    const uint32_t zx = block_excl_scan_u32(zn, s_wave, tot);  // over all entries; each level rebased below
+    if (r < G - 1 && r + 1 == (1u << j)) s_lvl0[j] = zx;
+    __syncthreads();
+    if (r < G - 1) {
+      const uint32_t zs = zx - s_lvl0[j];
+      s_nxy[r] = (S - zs) | ((zs + zn) << 16);
+    }
+  }
+  __syncthreads();
   // the tile's own level pipeline: at level s+j the keys are stably sorted by
   // their top j bits of d (the tile's part of each level-(s+j) node is
   // contiguous); the ballots of the level bit in that order are the level's
@@ -335,51 +369,19 @@ __global__ __launch_bounds__(256) void k_mld_emit(MldArgs a) {
     }
     if (j + 1 == g && !SCATTER) break;  // the last level's order is not needed
     __syncthreads();
-    // every wave: exclusive prefix of the chunk zeros (lane = chunk), then the
-    // node table of level s+j in registers.  Zeros before tile position P =
-    // chunk prefix of P >> 6 + zeros of that chunk below P.  A node [S, E)
-    // with ZS zeros before it and ZN in it sends its k-th zero to S + k and
-    // its k-th one to S + ZN + k; for a key at p with zc zeros before it in
-    // the tile that is X + zc (zero) or Y + p - zc (one), X = S - ZS and
-    // Y = ZS + ZN, packed X | Y << 16 on lane q (node q) and q - 64.
+    // every wave: exclusive prefix of the chunk zeros (lane = chunk); zeros
+    // before tile position p = that prefix + zeros of p's chunk below p
     const uint32_t v = lane < nch ? 64u - popc64(reinterpret_cast<const uint64_t*>(s_bits[j])[lane]) : 0u;
     const uint32_t incl = wave_incl_scan_dpp(v);
     const uint32_t ex = incl - v;
-    const uint32_t total = (uint32_t)__builtin_amdgcn_readlane((int)incl, 63);
-    const uint32_t sgj = g - j;
-    uint32_t nXY[2] = {0u, 0u};
-#pragma unroll
-    for (uint32_t h = 0; h < 2; h++) {
-      if (h == 1 && j < 7) break;  // nodes 64..127 exist at j = 7 only
-      const uint32_t q = lane + 64 * h;
-      const bool valid = q < (1u << j);
-      const uint32_t S = valid ? s_tpre[q << sgj] : 0u, E = valid ? s_tpre[(q + 1) << sgj] : 0u;
-      const uint32_t cs = S >> 6, ce = E >> 6;
-      const uint32_t ps = (uint32_t)__shfl((int)ex, (int)(cs & 63)), pe = (uint32_t)__shfl((int)ex, (int)(ce & 63));
-      const uint64_t ws = reinterpret_cast<const uint64_t*>(s_bits[j])[cs];
-      const uint64_t we = reinterpret_cast<const uint64_t*>(s_bits[j])[ce];
-      const uint32_t zs = (cs < 64 ? ps : total) + popc64(~ws & ((1ull << (S & 63)) - 1));
-      const uint32_t ze = (ce < 64 ? pe : total) + popc64(~we & ((1ull << (E & 63)) - 1));
-      nXY[h] = (S - zs) | (ze << 16);
-    }
     // stable split of each node: its zeros first, then its ones.  Every
     // chunk of the tile takes part (chunks past the keys hold pad keys,
     // which stay in place), so the node lookups of all chunks are issued
     // before the first is used
+    const uint32_t* nt = s_nxy + (1u << j) - 1;
     uint32_t xy[MLD_CPW];
 #pragma unroll
-    for (uint32_t m = 0; m < MLD_CPW; m++) {
-      const uint32_t q4 = __builtin_amdgcn_ubfe(xk[m], bp + 1, j) << 2;  // node of level s+j, as a lane byte address
-      xy[m] = (uint32_t)__builtin_amdgcn_ds_bpermute((int)(q4 & 0xFCu), (int)nXY[0]);
-    }
-    if (j == 7) {  // nodes 64..127
-#pragma unroll
-      for (uint32_t m = 0; m < MLD_CPW; m++) {
-        const uint32_t q4 = __builtin_amdgcn_ubfe(xk[m], bp + 1, j) << 2;
-        const uint32_t xy1 = (uint32_t)__builtin_amdgcn_ds_bpermute((int)(q4 & 0xFCu), (int)nXY[1]);
-        if (q4 >= 256) xy[m] = xy1;
-      }
-    }
+    for (uint32_t m = 0; m < MLD_CPW; m++) xy[m] = nt[__builtin_amdgcn_ubfe(xk[m], bp + 1, j)];
 #pragma unroll
     for (uint32_t m = 0; m < MLD_CPW; m++) {
       const uint32_t c = w + 4 * m;
@@ -599,9 +601,9 @@ int mld_build_t(wtg_ctx* ctx, const K* d_sym, uint64_t n, uint32_t L, uint64_t* 
     const uint32_t nrows = (uint32_t)grid + 1;
     const uint32_t nblocks = (uint32_t)wtg_ceil_div(nrows, MLD_SR);
     if (gi == 0) {
-      WTG_LAUNCH(ctx, "mld_count", k_mld_count<K>, dim3((unsigned)grid), dim3(256), 0, a);
+      WTG_LAUNCH(ctx, "mld_count", k_mld_count<K>, dim3((unsigned)wtg_ceil_div(grid, 4)), dim3(256), 0, a);
     } else {
-      WTG_LAUNCH(ctx, "mld_count", k_mld_count<uint32_t>, dim3((unsigned)grid), dim3(256), 0, a);
+      WTG_LAUNCH(ctx, "mld_count", k_mld_count<uint32_t>, dim3((unsigned)wtg_ceil_div(grid, 4)), dim3(256), 0, a);
     }
     WTG_LAUNCH_CHECK();
     WTG_LAUNCH(ctx, "mld_scan", k_mld_scan1, dim3(nblocks), dim3(256), 0, tcnt, ploc, bsum,
