@@ -244,12 +244,18 @@ __global__ __launch_bounds__(256) void k_mld_ntiles(const uint32_t* __restrict__
   nt[i] = (cum[i + 1] - cum[i] + MLD_T - 1) / MLD_T;
 }
 
-// tile -> segment map: the thread of segment k writes k for its tiles
+// tile -> segment map, one thread per tile: the last segment k with
+// tstart[k] <= t (empty segments share their start with the next one)
 __global__ __launch_bounds__(256) void k_mld_tseg(const uint32_t* __restrict__ tstart, uint64_t nseg,
-                                                  uint32_t* __restrict__ tseg) {
-  const uint64_t k = (uint64_t)blockIdx.x * blockDim.x + threadIdx.x;
-  if (k >= nseg) return;
-  for (uint32_t t = tstart[k]; t < tstart[k + 1]; t++) tseg[t] = (uint32_t)k;
+                                                  uint32_t* __restrict__ tseg, uint64_t maxtiles) {
+  const uint64_t t = (uint64_t)blockIdx.x * blockDim.x + threadIdx.x;
+  if (t >= maxtiles || t >= tstart[nseg]) return;
+  uint64_t lo = 0, hi = nseg;  // tstart[lo] <= t < tstart[hi]
+  while (hi - lo > 1) {
+    const uint64_t mid = (lo + hi) >> 1;
+    if (tstart[mid] <= t) lo = mid; else hi = mid;
+  }
+  tseg[t] = (uint32_t)lo;
 }
 
 // inclusive prefix sum over the 64 lanes with DPP moves (no LDS round trip):
@@ -288,11 +294,11 @@ __global__ __launch_bounds__(256) __attribute__((amdgpu_waves_per_eu(8))) void k
   __shared__ uint32_t s_wave[4];
   uint32_t seg, len, first;
   uint64_t base;
-  if (!mld_tile(a, blockIdx.x, seg, base, len, first)) return;
+  const uint32_t t = blockIdx.x;
+  if (!mld_tile(a, t, seg, base, len, first)) return;
   const uint32_t g = a.g, G = 1u << g, lane = (uint32_t)lane_id();
   const uint32_t w = (uint32_t)__builtin_amdgcn_readfirstlane((int)(threadIdx.x >> 6));
   const uint32_t sh = a.L - a.s - g;
-  const uint32_t t = blockIdx.x;
   // carry and tile counts per leaf, their exclusive prefixes over d; thread d
   // keeps its leaf's node start and carry prefix in registers until the
   // write-out (they then go to the free key buffer)
@@ -642,8 +648,9 @@ int mld_build_t(wtg_ctx* ctx, const K* d_sym, uint64_t n, uint32_t L, uint64_t* 
       WTG_LAUNCH(ctx, "mld_tiles", k_scan_u32<uint32_t>, dim3((unsigned)stiles), dim3(256), 0, nt, tst, nbins, scst,
                  ctr, ctx->d_flag);
       WTG_LAUNCH_CHECK();
-      WTG_LAUNCH(ctx, "mld_tseg", k_mld_tseg, dim3((unsigned)wtg_ceil_div(nbins, 256)), dim3(256), 0, tst, nbins,
-                 tseg);
+      const uint64_t ntiles_next = wtg_ceil_div(n, MLD_T) + std::min<uint64_t>(n, nbins);
+      WTG_LAUNCH(ctx, "mld_tseg", k_mld_tseg, dim3((unsigned)wtg_ceil_div(ntiles_next, 256)), dim3(256), 0, tst,
+                 nbins, tseg, (uint64_t)P.maxtiles);
       WTG_LAUNCH_CHECK();
       src = kbuf[gi & 1];
       cum_prev = cum[gi & 1];
