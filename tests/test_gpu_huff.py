@@ -185,3 +185,34 @@ def test_fm_count_single_char_patterns(wsfm, orc, ctx):
     finally:
         wt.free()
         d.free()
+
+
+def test_huff_past_2e25_bytes(wsfm, orc, ctx):
+    """A text over 2^25 bytes: the histogram's waves then stride over several
+    8 KiB chunks each (1024-block grid), so the multi-chunk accumulation runs.
+    Skewed bytes (one dominant value) plus a sparse tail."""
+    rng = np.random.default_rng(25)
+    n = (1 << 25) + 4099
+    p = 1.0 / np.arange(1, 41) ** 1.6
+    t = (rng.choice(40, n, p=p / p.sum()) * 5 + 3).astype(np.uint8)
+    print("huff case past2e25", n, flush=True)
+    d = ctx.to_device(t)
+    wt = ctx.huff_build(d)
+    try:
+        cnt = np.bincount(t, minlength=256)
+        assert list(wt.shape.C) == [0] + list(np.cumsum(cnt))
+        oh = orc.Huff(t)
+        check_shape(wt, oh)
+        assert np.array_equal(wt.words()[:(oh.nbits + 63) // 64], oh.words)
+        syms = [int(t[0]), int(t[n // 2]), int(np.argmax(cnt)), int(np.nonzero(cnt)[0][-1])]
+        for s in syms:
+            pref = np.concatenate([[0], np.cumsum(t == s, dtype=np.uint64)])
+            i = rng.integers(0, n + 1, 2000).astype(np.uint64)
+            i[:3] = [0, n, (1 << 25) + 17]
+            got = wt.rank(i, np.full(i.size, s, np.uint32))
+            assert np.array_equal(got, pref[i.astype(np.int64)]), s
+        ia = rng.integers(0, n, 4000).astype(np.uint64)
+        assert np.array_equal(wt.access(ia), t[ia].astype(np.uint64))
+    finally:
+        wt.free()
+        d.free()

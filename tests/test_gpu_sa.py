@@ -52,3 +52,45 @@ def test_gpu_suffix_array(wsfm, orc, ctx, case):
     finally:
         dsa.free()
         d.free()
+
+
+def test_gpu_suffix_array_past_2e24(wsfm, ctx):
+    """n = 2^24 + 4097: more than 4096 tiles of 4096 suffixes, so the tile
+    prefix carries across its 4096-tile chunks.  DNA with copied stretches."""
+    rng = np.random.default_rng(24)
+    base = wsfm.gen_dna(8, (1 << 24) + 4097)
+    t = base.copy()
+    for _ in range(200):  # copied stretches of up to 10^4 bytes
+        L = int(rng.integers(100, 10000))
+        src, dst = (int(x) for x in rng.integers(0, t.size - L, 2))
+        t[dst:dst + L] = base[src:src + L]
+    print("sa case past2e24", t.size, flush=True)
+    d = ctx.to_device(t)
+    dsa = wsfm.suffix_array_gpu(ctx, d, t.size)
+    try:
+        assert np.array_equal(dsa.get(), wsfm.suffix_array(t))
+    finally:
+        dsa.free()
+        d.free()
+
+
+def test_bwt_rejects_a_non_suffix_array(wsfm, ctx):
+    """An SA entry outside [0, n) is reported with its own message, and the
+    flag it raised does not fail the next call."""
+    t = wsfm.gen_dna(9, 5000)
+    d = ctx.to_device(t)
+    sa = wsfm.suffix_array(t)
+    bad = sa.copy()
+    bad[123] = t.size + 5
+    dbad = ctx.to_device(bad)
+    try:
+        with pytest.raises(wsfm.WtgError, match="not a suffix array"):
+            wsfm.bwt_from_sa_gpu(ctx, d, dbad, t.size)
+        dsa = wsfm.suffix_array_gpu(ctx, d, t.size)   # a clean call afterwards
+        try:
+            assert np.array_equal(dsa.get(), sa)
+        finally:
+            dsa.free()
+    finally:
+        dbad.free()
+        d.free()

@@ -152,7 +152,7 @@ __global__ __launch_bounds__(256) void k_batch_keys(const uint8_t* __restrict__ 
       add_ones(s, run);
     }
   }
-  if (bad) atomicOr(err, 2u);  // reported as an input symbol >= 2^levels
+  if (bad) atomicOr(err, WTG_FLAG_BADSYM);
   if (cached) {
     __syncthreads();
     for (uint32_t k = threadIdx.x; k < cnt; k += blockDim.x)

@@ -88,18 +88,27 @@ extern "C" int wtg_ctx_sync(wtg_ctx* ctx) {
   WTG_CHECK_HIP(hipMemcpy(&f, ctx->d_flag, 4, hipMemcpyDeviceToHost));
   if (f) {
     WTG_CHECK_HIP(hipMemset(ctx->d_flag, 0, 4));
-    if (f & 2u) {
+    if (f & WTG_FLAG_BADSYM) {
       wtg_set_error("input symbol >= 2^levels (or inconsistent with the tree); results invalid");
       return WTG_ERR_ARG;
     }
-    if (f & 4u) {
+    if (f & WTG_FLAG_SA_RANKS) {
       wtg_set_error("device suffix ranks inconsistent; results invalid");
       return WTG_ERR_DEVICE;
+    }
+    if (f & WTG_FLAG_BAD_SA) {
+      wtg_set_error("suffix array entry outside [0, n): not a suffix array of this text; BWT invalid");
+      return WTG_ERR_ARG;
     }
     wtg_set_error("device-side bounded wait expired (look-back scan); results invalid");
     return WTG_ERR_DEVICE;
   }
   return WTG_OK;
+}
+
+void wtg_abandon(wtg_ctx* ctx) {
+  (void)hipGetLastError();
+  if (hipStreamSynchronize(ctx->stream) == hipSuccess) (void)hipMemset(ctx->d_flag, 0, 4);
 }
 
 int wtg_finish(wtg_ctx* ctx) {

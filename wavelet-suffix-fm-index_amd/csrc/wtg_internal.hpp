@@ -52,8 +52,18 @@ struct wtg_ctx {
 
 // workspace of at least `bytes`, 256-byte aligned regions carved by the caller
 int wtg_ws_reserve(wtg_ctx* ctx, size_t bytes);
+// bits of the device error flag (ctx->d_flag), decoded by wtg_ctx_sync
+constexpr uint32_t WTG_FLAG_WAIT = 1u;      // a bounded look-back wait expired
+constexpr uint32_t WTG_FLAG_BADSYM = 2u;    // input symbol >= 2^levels / tables inconsistent with the keys
+constexpr uint32_t WTG_FLAG_SA_RANKS = 4u;  // suffix ranks inconsistent
+constexpr uint32_t WTG_FLAG_BAD_SA = 8u;    // a suffix array entry outside [0, n)
+
 // called at the end of every public entry point: sync if blocking, check flag
 int wtg_finish(wtg_ctx* ctx);
+// error path of an entry point that stops early: wait for the stream and clear
+// the device flag, so a flag raised by this call does not fail the next one
+// (the error already set is kept)
+void wtg_abandon(wtg_ctx* ctx);
 int wtg_set_device(wtg_ctx* ctx);
 
 // kernel launch with optional per-kernel event timing
@@ -207,7 +217,7 @@ __device__ __forceinline__ uint64_t lb_wait(uint64_t* states, uint64_t tile, uin
       bool notready = (s >> 62) == 0;
       if (wave_ballot(notready) == 0) break;
       if (++spins > WTG_SPIN_LIMIT) {
-        if (l == 0) atomicOr(err, 1u);
+        if (l == 0) atomicOr(err, WTG_FLAG_WAIT);
         s = (LB_PRE << 62);  // give up: treat as prefix 0 (results invalid, flag raised)
         break;
       }

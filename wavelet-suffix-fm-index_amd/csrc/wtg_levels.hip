@@ -35,8 +35,8 @@
 // because its device-coherent state loads sit on the critical path.
 #include "wtg_internal.hpp"
 
-#define FLAG_LOOKBACK 1u
-#define FLAG_BADSYM 2u
+#define FLAG_LOOKBACK WTG_FLAG_WAIT
+#define FLAG_BADSYM WTG_FLAG_BADSYM
 #define TAB_CACHE 256     // node-table entries cached in LDS per tile
 typedef uint32_t v4u32 __attribute__((ext_vector_type(4)));
 

@@ -144,7 +144,7 @@ __global__ __launch_bounds__(256) void k_mld_count(MldArgs a) {
       atomicAdd(&s_cnt[d], 1u);
     }
   }
-  if (bad) atomicOr(a.err, 2u);
+  if (bad) atomicOr(a.err, WTG_FLAG_BADSYM);
   __builtin_amdgcn_fence(__ATOMIC_RELEASE, "wavefront");
   __builtin_amdgcn_wave_barrier();
   __builtin_amdgcn_fence(__ATOMIC_ACQUIRE, "wavefront");
@@ -433,7 +433,7 @@ __global__ __launch_bounds__(256) __attribute__((amdgpu_waves_per_eu(8))) void k
       const uint32_t cnt = s_tpre[hi] - s_tpre[lo];
       const uint64_t gs = (uint64_t)(a.s + j) * n + s_cum[lo] + (s_cpre[hi] - s_cpre[lo]);
       if (gs < (uint64_t)(a.s + j) * n || gs + cnt > (uint64_t)(a.s + j + 1) * n) {
-        atomicOr(a.err, 2u);  // tables inconsistent with the keys: write nothing
+        atomicOr(a.err, WTG_FLAG_BADSYM);  // tables inconsistent with the keys: write nothing
       } else if (cnt) {
         nw = (uint32_t)(((gs + cnt - 1) >> 6) - (gs >> 6) + 1);
       }
@@ -475,7 +475,7 @@ __global__ __launch_bounds__(256) __attribute__((amdgpu_waves_per_eu(8))) void k
         const uint32_t x = s_k[g & 1][i];
         const uint32_t dest = s_lb[(x >> sh) & (G - 1)] + i;
         if (dest < n) a.kout[dest] = x;
-        else atomicOr(a.err, 2u);
+        else atomicOr(a.err, WTG_FLAG_BADSYM);
       }
     }
   }

@@ -233,7 +233,7 @@ __global__ void k_sa_update(const uint64_t* __restrict__ key, const uint32_t* __
   const bool was_single = !whole_group && first == (uint32_t)k && (k + 1 == m || (key[k + 1] >> sh) != g);
   if (single && !was_single) {
     if (r < n) sa[r] = (int64_t)i;
-    else atomicOr(err, 4u);  // ranks inconsistent: not expected
+    else atomicOr(err, WTG_FLAG_SA_RANKS);  // ranks inconsistent: not expected
   }
 }
 
@@ -250,7 +250,7 @@ __global__ void k_bwt(const uint8_t* __restrict__ t, const int64_t* __restrict__
   if (i >= n) return;
   const int64_t p = sa[i];
   if (p < 0 || (uint64_t)p >= n) {  // not a suffix array of this text
-    atomicOr(err, 2u);
+    atomicOr(err, WTG_FLAG_BAD_SA);
     bwt[i] = 0;
     return;
   }
@@ -408,6 +408,7 @@ extern "C" int wtg_suffix_array_gpu(wtg_ctx* ctx, const uint8_t* d_text, uint64_
   }
   rc = wtg_finish(ctx);
 out:
+  if (rc != WTG_OK) wtg_abandon(ctx);
   wtg_free(ctx, k0); wtg_free(ctx, k1); wtg_free(ctx, v0); wtg_free(ctx, v1); wtg_free(ctx, act);
   wtg_free(ctx, rank); wtg_free(ctx, fa); wtg_free(ctx, fb); wtg_free(ctx, ta); wtg_free(ctx, tb);
   wtg_free(ctx, keep); wtg_free(ctx, pos); wtg_free(ctx, st); wtg_free(ctx, ctr); wtg_free(ctx, tmp);

@@ -22,16 +22,16 @@ class fm_index {
   fm_index(const unsigned char* s, uint64_t n) : m_n(n) {
     m_bwt.resize(n);
     if (n) {
-      if (n < (1ull << 31)) {  // suffix array and BWT on the GPU (prefix doubling)
-        dbuf dt(n), dsa(n * 8), db(n);
-        dt.upload(s, n);
-        check(wtg_suffix_array_gpu(default_context(), dt.as<uint8_t>(), n, dsa.as<int64_t>()));
-        check(wtg_bwt_from_sa_gpu(default_context(), dt.as<uint8_t>(), dsa.as<int64_t>(), n, db.as<uint8_t>()));
-        db.download(m_bwt.data(), n);
-      } else {
+      // suffix array and BWT on the GPU (prefix doubling) below 2^31 bytes; the
+      // host SA-IS above, or when the device cannot hold the working arrays
+      // (about 48n bytes), where the reference's divsufsort would still succeed
+      const int rc = n < (1ull << 31) ? gpu_bwt(s, n, m_bwt.data()) : WTG_ERR_NOMEM;
+      if (rc == WTG_ERR_NOMEM) {
         std::vector<int64_t> sa(n);
         check(wtg_suffix_array(s, n, sa.data()));
         check(wtg_bwt_from_sa(s, sa.data(), n, m_bwt.data()));
+      } else {
+        check(rc);
       }
     }
     m_wt = sdsl::wt_huff<>(m_bwt.data(), n);
@@ -58,6 +58,22 @@ class fm_index {
   }
 
  private:
+  static int gpu_bwt(const unsigned char* s, uint64_t n, unsigned char* out) {
+    wtg_ctx* ctx = default_context();
+    void *dt = nullptr, *dsa = nullptr, *db = nullptr;
+    int rc = wtg_malloc(ctx, &dt, n);
+    if (rc == WTG_OK) rc = wtg_malloc(ctx, &dsa, n * 8);
+    if (rc == WTG_OK) rc = wtg_malloc(ctx, &db, n);
+    if (rc == WTG_OK) rc = wtg_memcpy_h2d(ctx, dt, s, n);
+    if (rc == WTG_OK) rc = wtg_suffix_array_gpu(ctx, (const uint8_t*)dt, n, (int64_t*)dsa);
+    if (rc == WTG_OK) rc = wtg_bwt_from_sa_gpu(ctx, (const uint8_t*)dt, (const int64_t*)dsa, n, (uint8_t*)db);
+    if (rc == WTG_OK) rc = wtg_memcpy_d2h(ctx, out, db, n);
+    if (db) wtg_free(ctx, db);
+    if (dsa) wtg_free(ctx, dsa);
+    if (dt) wtg_free(ctx, dt);
+    return rc;
+  }
+
   uint64_t m_n = 0;
   std::vector<unsigned char> m_bwt;
   sdsl::wt_huff<> m_wt;
