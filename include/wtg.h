@@ -340,7 +340,7 @@ typedef struct {
   uint32_t train_stride;  /* training samples select(T), select(2T), ...; power of two (16) */
   uint32_t max_gap;       /* longest knot interval in arguments; power of two, 2..2^24 (4096) */
   uint32_t tol;           /* knot placement tolerance in bits, 1 <= tol < eps_max - 1 (480) */
-  uint32_t epochs;        /* Adam steps (200) */
+  uint32_t epochs;        /* Adam steps (50) */
   float lr;               /* Adam step in bit positions (4) */
   uint32_t eps_max;       /* accepted error window in bits, <= 512 (512) */
 } wtg_lsel_params;
@@ -350,7 +350,8 @@ typedef struct {
   uint32_t nknots;
   uint64_t eps;           /* max |prediction - select(j)| over every j (exhaustive), model in use */
   uint64_t eps_init;      /* the same for the placed, untrained knots */
-  double loss_init, loss_final;  /* mean squared residual on the training samples, placed / in use */
+  double loss_init, loss_final;  /* mean squared residual on the training samples: placed knots /
+                                    after training (the knots in use when trained = 1) */
   uint32_t trained;       /* 1: the Adam-trained knots are in use; 0: the placed ones (training
                              raised eps above eps_max) */
   uint32_t lut_log;       /* knot lookup table: one entry per 2^lut_log arguments */

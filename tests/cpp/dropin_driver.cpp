@@ -159,6 +159,10 @@ int main(int argc, char** argv) {
       ex.push_back(v.size); ex.push_back(wt.sym(v));
     }
     dump(out + "/wtint_expand.bin", ex.data(), ex.size() * 8);
+    // iterators (wt_int_par.hpp:877-886)
+    std::vector<uint64_t> it(wt.begin(), wt.end());
+    it.push_back((uint64_t)(wt.end() - wt.begin()));
+    dump(out + "/wtint_iter.bin", it.data(), it.size() * 8);
   }
   // ---- FMIndex() + wt_huff<> over the BWT (fm-index/par/WT.C:33-45)
   {
@@ -198,6 +202,50 @@ int main(int argc, char** argv) {
       for (uint64_t t = 0; t < k; t++) { hs.push_back(hcs[t]); hs.push_back(hri[t]); hs.push_back(hrj[t]); }
     }
     dump(out + "/huff_isym.bin", hs.data(), hs.size() * 8);
+    // iterators (wt_pc_par.hpp:740-746)
+    std::vector<unsigned char> hit(wt.begin(), wt.end());
+    dump(out + "/huff_iter.bin", hit.data(), hit.size());
+    // node interface (wt_pc_par.hpp:778-909): leaves by expand from the root with
+    // the root range [0, n-1] mapped down (sym, range size); then, per symbol
+    // c, the range [i, j] followed down c's path (path length, code, final
+    // range size, leaf sym); symbol_gte / symbol_lte for every byte
+    std::vector<uint64_t> hl;
+    std::vector<std::pair<sdsl::wt_huff<>::node_type, sdsl::range_type>> st{{wt.root(), sdsl::range_type{{0, n - 1}}}};
+    while (!st.empty()) {
+      const auto e = st.back();
+      st.pop_back();
+      const uint64_t sz = e.second[1] + 1 - e.second[0];
+      if (!wt.empty(e.first) || sz == 0) continue;
+      if (wt.is_leaf(e.first)) { hl.push_back(wt.sym(e.first)); hl.push_back(sz); continue; }
+      const auto ch = wt.expand(e.first);
+      const auto rr = wt.expand(e.first, e.second);
+      st.push_back({ch.second, rr.second});
+      st.push_back({ch.first, rr.first});
+    }
+    dump(out + "/huff_leaves.bin", hl.data(), hl.size() * 8);
+    std::vector<uint64_t> hp;
+    for (int c = 0; c < 256; c++) {
+      const auto g = wt.symbol_gte((unsigned char)c), l = wt.symbol_lte((unsigned char)c);
+      hp.push_back(g.first); hp.push_back(g.second); hp.push_back(l.first); hp.push_back(l.second);
+      if (!g.first || g.second != c) continue;
+      const auto pl = wt.path((unsigned char)c);
+      const uint64_t i = (c * 7919) % n, j = std::min<uint64_t>(n - 1, i + 20000);
+      sdsl::range_type r{{i, j}};
+      sdsl::range_vec_type rv{r, sdsl::range_type{{0, n - 1}}};
+      auto v = wt.root();
+      for (uint64_t d = 0; d < pl.first; d++) {
+        const int bit = (pl.second >> (pl.first - 1 - d)) & 1;
+        const auto rr = wt.expand(v, r);
+        const auto rrv = wt.expand(v, rv);
+        r = bit ? rr.second : rr.first;
+        rv = bit ? rrv.second : rrv.first;
+        v = bit ? wt.expand(v).second : wt.expand(v).first;
+      }
+      hp.push_back(c); hp.push_back(pl.first); hp.push_back(pl.second); hp.push_back(i); hp.push_back(j);
+      hp.push_back(r[1] + 1 - r[0]); hp.push_back(rv[1][1] + 1 - rv[1][0]); hp.push_back(wt.is_leaf(v));
+      hp.push_back(wt.sym(v));
+    }
+    dump(out + "/huff_nodes.bin", hp.data(), hp.size() * 8);
     // host backward-search counts through the fm_index wrapper
     wtg::fm_index idx(t.data(), n);
     const uint32_t m = 12;

@@ -119,6 +119,8 @@ def test_dropin_driver(wsfm, orc, variant, dt):
             assert vsym == c and vsz == int((sv == c).sum())
             assert n1 == n2 == int((sv[i:j + 1] == c).sum())
             assert n3 == vsz
+        it = load(d + "/wtint_iter.bin")
+        assert it[-1] == m and np.array_equal(it[:-1], sv.astype(np.uint64))
         # FMIndex() + wt_huff<>
         n = 200003
         t = wsfm.gen_dna(14, n)
@@ -166,5 +168,29 @@ def test_dropin_driver(wsfm, orc, variant, dt):
                 occ = np.nonzero(bwt == c)[0]
                 j = 1 + (i % 5000)
                 assert sel == (occ[j - 1] if j <= occ.size else orc.NONE)
+        # iterators, node interface, symbol_gte / symbol_lte (wt_pc_par.hpp:740-909)
+        assert np.array_equal(load(d + "/huff_iter.bin", np.uint8), bwt)
+        hl = load(d + "/huff_leaves.bin").reshape(-1, 2)
+        vals, cnts = np.unique(bwt, return_counts=True)
+        assert sorted(map(tuple, hl.tolist())) == list(zip(vals.tolist(), cnts.tolist()))
+        assert [int(x) for x in hl[:, 0]] == order   # leaves in tree order, left child first
+        hp = load(d + "/huff_nodes.bin")
+        present = set(vals.tolist())
+        p = 0
+        for c in range(256):
+            g1, g, l1, lo = (int(x) for x in hp[p:p + 4])
+            p += 4
+            ge = [x for x in range(c, 256) if x in present]
+            le = [x for x in range(c, -1, -1) if x in present]
+            assert (g1, g) == ((1, ge[0]) if ge else (0, 0))
+            assert (l1, lo) == ((1, le[0]) if le else (0, 0))
+            if c in present:
+                cc, plen, code, i, j, rsz, fsz, leaf, sym = (int(x) for x in hp[p:p + 9])
+                p += 9
+                assert cc == c and leaf == 1 and sym == c
+                assert plen == H.path_len[c]
+                assert code == int("".join(str((H.path[c] >> k) & 1) for k in range(plen)) or "0", 2)
+                assert rsz == int((bwt[i:j + 1] == c).sum()) and fsz == int((bwt == c).sum())
+        assert p == hp.size
         pats = wsfm.gen_patterns(15, t, 12, 300)
         assert np.array_equal(load(d + "/fm_count.bin"), orc.sa_count(t, sa, pats))

@@ -79,6 +79,12 @@ def test_placement_matches_oracle(wsfm, orc, ctx):
             p += ln
             one = not one
         vecs.append(bits)
+    # very sparse (64-word windows with no one or a few), and one long run of
+    # ones between sparse stretches (windows of 64 ones per word)
+    vecs.append(rng.random(nb) < 0.0015)
+    dense = rng.random(nb) < 0.01
+    dense[70000:170000] = True
+    vecs.append(dense)
     words = [pack(b.astype(np.uint8)) for b in vecs]
     dv = make_vectors(ctx, words)
     tol, gap = 64, 1024
@@ -181,3 +187,76 @@ def test_bad_params(wsfm, ctx):
     with pytest.raises(wsfm.WtgError):   # max_gap not a power of two
         wsfm.LearnedSelect(ctx, [d], [128], wsfm.LselParams(16, 100, 100, 10, 1.0, 512))
     d.free()
+
+
+@pytest.mark.parametrize("tol,gap", [(480, 4096), (200, 16384)])
+def test_placement_matches_oracle_wide_chunks(wsfm, orc, ctx, tol, gap):
+    """The default chunk (4096 arguments) and a wider one, tolerances as used."""
+    rng = np.random.default_rng(tol)
+    nb = (1 << 19) + 5
+    vecs = [rng.random(nb) < 0.05, rng.random(nb) < 0.6]
+    words = [pack(b.astype(np.uint8)) for b in vecs]
+    dv = make_vectors(ctx, words)
+    ls = wsfm.LearnedSelect(ctx, dv, [nb] * len(vecs), wsfm.LselParams(16, gap, tol, 0, 4.0, 512))
+    try:
+        for b, bits in enumerate(vecs):
+            print("placement vector", b, tol, gap, flush=True)
+            ones = np.flatnonzero(bits).astype(np.float64)
+            knots, lut, lut_log, alpha, beta = ls.model(b)
+            J, V = orc.place_knots(ones, float(tol), gap)
+            assert np.array_equal(knots["j"].astype(np.uint64), J)
+            res = (V - orc.model_line(alpha, beta, J.astype(np.float64))).astype(np.float32)
+            assert np.allclose(knots["y"], res, rtol=0, atol=1e-3 + 1e-6 * np.abs(res).max())
+            assert ls.info(b)["eps"] <= tol + 1
+    finally:
+        ls.free()
+        for d in dv:
+            d.free()
+
+
+def test_training_gain(wsfm, ctx):
+    """What training gains over the placed model, on the same vectors and
+    queries: Adam lowers the mean squared error of every trained model, while
+    the query cost stays the same: once eps <= 512 the correction reads the
+    same three-block window, so the 64-byte sectors per query agree within
+    alignment noise.  Answers are identical on the plain, placed and trained
+    paths.  (The gain over the plain sampled select is its query rate, which
+    the C5 bench measures at full size.)"""
+    nvec, nb = 8, 1 << 22
+    w, dens = wsfm.gen_select_vectors(41, nvec, nb)
+    dv = make_vectors(ctx, [w[b] for b in range(nvec)])
+    placed = wsfm.LearnedSelect(ctx, dv, [nb] * nvec, wsfm.LselParams(16, 4096, 480, 0, 4.0, 512))
+    trained = wsfm.LearnedSelect(ctx, dv, [nb] * nvec, wsfm.LselParams(16, 4096, 480, 200, 4.0, 512))
+    rng = np.random.default_rng(3)
+    try:
+        vec, j = [], []
+        for b in range(nvec):
+            ones = trained.info(b)["ones"]
+            vec.append(np.full(50000, b, np.uint32))
+            j.append(rng.integers(1, ones + 1, 50000).astype(np.uint64))
+        dvec, dj = ctx.to_device(np.concatenate(vec)), ctx.to_device(np.concatenate(j))
+        outs, secs = {}, {}
+        for name, ls, learned in (("plain", placed, False), ("placed", placed, True), ("trained", trained, True)):
+            do = ctx.empty(dj.size, np.uint64)
+            ls.query(dvec, dj, do, learned)
+            outs[name] = do.get()
+            ds = ls.sectors(dvec, dj, do, learned)
+            secs[name] = float(ds.get().mean())
+            ds.free()
+            do.free()
+        ntrained = sum(trained.info(b)["trained"] for b in range(nvec))
+        print("sectors per query", secs, "trained vectors", ntrained, flush=True)
+        assert np.array_equal(outs["placed"], outs["plain"])
+        assert np.array_equal(outs["trained"], outs["plain"])
+        assert abs(secs["trained"] - secs["placed"]) < 0.05
+        for b in range(nvec):
+            info = trained.info(b)
+            if info["trained"]:
+                assert info["loss_final"] < info["loss_init"] and info["eps"] <= 512
+        dvec.free()
+        dj.free()
+    finally:
+        placed.free()
+        trained.free()
+        for d in dv:
+            d.free()

@@ -216,6 +216,7 @@ __global__ __launch_bounds__(256) void k_lsel_sectors(const LselDesc* __restrict
 struct PlaceVec {
   SelQ s1;
   const uint64_t* words;
+  uint64_t nwords;
   uint64_t ones;
   uint64_t cbase;        // first flat chunk index of this vector
   double alpha, beta;
@@ -230,26 +231,65 @@ __device__ __forceinline__ uint32_t find_base(const uint64_t* base, uint32_t nv,
   return lo;
 }
 
-// Greedy knot placement over one chunk of arguments [j0, j1] (thread per
-// chunk).  Count pass (off == nullptr): cnt[flat] = knots of the chunk.
-// Emit pass: the knots go to knots[off[flat] ..], at most off[flat+1]-off[flat]
-// of them (a different count sets *err).
+__device__ __forceinline__ double readlane_f64(double v, int l) {
+  const uint64_t u = __double_as_longlong(v);
+  const uint32_t lo = (uint32_t)__builtin_amdgcn_readlane((int)(uint32_t)u, l);
+  const uint32_t hi = (uint32_t)__builtin_amdgcn_readlane((int)(uint32_t)(u >> 32), l);
+  return __longlong_as_double((long long)(((uint64_t)hi << 32) | lo));
+}
+__device__ __forceinline__ uint64_t shfl_u64(uint64_t v, int l) {
+  const uint32_t lo = (uint32_t)__shfl((int)(uint32_t)v, l);
+  const uint32_t hi = (uint32_t)__shfl((int)(uint32_t)(v >> 32), l);
+  return ((uint64_t)hi << 32) | lo;
+}
+__device__ __forceinline__ double shfl_up_f64(double v, int d) {
+  const uint64_t u = __double_as_longlong(v);
+  const uint32_t lo = (uint32_t)__shfl_up((int)(uint32_t)u, d);
+  const uint32_t hi = (uint32_t)__shfl_up((int)(uint32_t)(u >> 32), d);
+  return __longlong_as_double((long long)(((uint64_t)hi << 32) | lo));
+}
+
+// Greedy knot placement over one chunk of arguments [j0, j1], one wave per
+// chunk.  The sequential walk it reproduces: from the last knot (x0, y0) keep
+// the cone [lo, hi] of slopes whose line passes within tol of every one seen,
+// lo = max over ones of (P - tol - y0) / (j - x0), hi = min of (P + tol - y0) /
+// (j - x0); the first argument j that empties it puts a knot at the previous
+// argument jp on the cone's middle slope, y0 + (lo + hi)/2 * (jp - x0), and
+// the walk restarts from there with j.  Here the ones come 64 at a time (a
+// window of 64 words: popcount scan, lane i finds its one by a binary search
+// over the scan), lane i takes argument jn + i, and a running max / min over
+// the lanes (a prefix scan: max and min are exact, so the values are the
+// sequential ones) finds the first lane whose cone is empty; the knot goes in,
+// and the lanes from that one on are redone from the new knot.  Same double
+// arithmetic as the walk (no contraction), so the same knots.
+// Count pass (off == nullptr): cnt[flat] = the chunk's knots, and the first
+// LSEL_CAP of them to scratch[flat * LSEL_CAP ..].  Emit pass: a chunk with at
+// most LSEL_CAP knots copies them from scratch; a larger one walks again and
+// writes knots[off[flat] ..] (a different count sets *err).
+constexpr uint32_t LSEL_CAP = 16;
 __global__ __launch_bounds__(256) void k_lsel_place(const PlaceVec* __restrict__ pv, const uint64_t* __restrict__ cbase,
                                                     uint32_t nv, uint64_t nflat, uint32_t gap_log, double tol,
                                                     uint32_t* __restrict__ cnt, const uint32_t* __restrict__ off,
-                                                    Knot* __restrict__ knots, uint32_t* __restrict__ err) {
+                                                    Knot* __restrict__ knots, Knot* __restrict__ scratch,
+                                                    uint32_t* __restrict__ err) {
 #pragma clang fp contract(off)
-  const uint64_t flat = (uint64_t)blockIdx.x * blockDim.x + threadIdx.x;
+  const uint32_t lane = (uint32_t)lane_id();
+  const uint64_t flat = (uint64_t)blockIdx.x * 4 + (uint32_t)__builtin_amdgcn_readfirstlane((int)(threadIdx.x >> 6));
   if (flat >= nflat) return;
   const PlaceVec& v = pv[find_base(cbase, nv, flat)];
   const uint64_t c = flat - v.cbase;
   const uint64_t j0 = (c << gap_log) + 1;
   const uint64_t j1 = ((c + 1) << gap_log) < v.ones ? ((c + 1) << gap_log) : v.ones;
   const uint32_t lim = off ? off[flat + 1] - off[flat] : 0u;
-  Knot* out = off ? knots + off[flat] : nullptr;
-  uint32_t n = 0;
+  if (off && lim <= LSEL_CAP) {  // the count pass kept them
+    if (lane < lim) knots[off[flat] + lane] = scratch[flat * LSEL_CAP + lane];
+    return;
+  }
+  Knot* out = off ? knots + off[flat] : scratch + flat * LSEL_CAP;
+  const uint32_t cap = off ? lim : LSEL_CAP;
+  uint32_t n = 0;  // wave-uniform
   auto put = [&](uint64_t jj, double val) {
-    if (out && n < lim) out[n] = Knot{(uint32_t)jj, (float)(val - lsel_line(v.alpha, v.beta, (double)jj))};
+    if (n < cap && lane == 0) out[n] = Knot{(uint32_t)jj, (float)(val - lsel_line(v.alpha, v.beta, (double)jj))};
     n++;
   };
   if (j0 <= j1) {
@@ -258,29 +298,93 @@ __global__ __launch_bounds__(256) void k_lsel_place(const PlaceVec* __restrict__
     uint64_t x = v.words[w] & (~0ull << (pos & 63));
     x &= x - 1;  // the one at pos itself
     double x0 = (double)j0, y0 = (double)pos, lo = -1e300, hi = 1e300;
-    uint64_t jp = j0;
+    uint64_t jp = j0, jn = j0 + 1;
     put(j0, y0);
-    for (uint64_t jj = j0 + 1; jj <= j1; jj++) {
-      while (!x) x = v.words[++w];  // the jj-th one exists: jj <= ones
-      const double P = (double)((w << 6) + (uint64_t)__builtin_ctzll(x));
-      x &= x - 1;
-      double dx = (double)jj - x0;
-      double nlo = (P - tol - y0) / dx, nhi = (P + tol - y0) / dx;
-      nlo = nlo > lo ? nlo : lo;
-      nhi = nhi < hi ? nhi : hi;
-      if (nlo > nhi) {
-        // no line from (x0, y0) fits: knot at the previous argument
-        const double ye = y0 + 0.5 * (lo + hi) * ((double)jp - x0);
-        put(jp, ye);
-        x0 = (double)jp;
-        y0 = ye;
-        dx = (double)jj - x0;
-        nlo = (P - tol - y0) / dx;
-        nhi = (P + tol - y0) / dx;
+    while (jn <= j1) {
+      // the window: word w (its unconsumed ones x) and the 63 words after it
+      const uint64_t wl = w + lane;
+      const uint64_t wd = lane == 0 ? x : (wl < v.nwords ? v.words[wl] : 0ull);
+      const uint32_t pc = popc64(wd);
+      const uint32_t incl = wave_incl_scan(pc);
+      const uint32_t T = (uint32_t)__builtin_amdgcn_readlane((int)incl, 63);
+      const uint64_t need = j1 - jn + 1;
+      const uint32_t m = (uint32_t)(need < (uint64_t)(T < 64u ? T : 64u) ? need : (T < 64u ? T : 64u));
+      if (m == 0) {  // 64 words without a one
+        if (w + 64 >= v.nwords) {  // no ones left: inconsistent with the count
+          if (lane == 0) *err = 1u;
+          break;
+        }
+        w += 64;
+        x = v.words[w];
+        continue;
       }
-      lo = nlo;
-      hi = nhi;
-      jp = jj;
+      // lane i (< m): the i-th one of the window, in word k = the first lane
+      // whose inclusive count exceeds i
+      uint32_t k = 0;
+#pragma unroll
+      for (uint32_t st = 32; st >= 1; st >>= 1)
+        if ((uint32_t)__shfl((int)incl, (int)(k + st - 1)) <= lane) k += st;
+      const uint32_t ks = k < 63 ? k : 63;
+      const uint32_t ex_k = (uint32_t)__shfl((int)incl, (int)ks) - (uint32_t)__shfl((int)pc, (int)ks);
+      const uint64_t wk = shfl_u64(wd, (int)ks);
+      const uint32_t bit = lane < m ? select_in_word(wk, lane - ex_k) : 0u;
+      const double P = (double)(((w + ks) << 6) + bit);
+      const double jj = (double)(jn + lane);
+      // the cone over lanes [start, m)
+      uint32_t start = 0;
+      for (uint32_t it = 0;; it++) {
+        if (it > 64) {  // each round moves start forward: cannot happen
+          if (lane == 0) *err = 1u;
+          break;
+        }
+        const bool act = lane >= start && lane < m;
+        double a = -1e300, b = 1e300;
+        if (act) {
+          const double dx = jj - x0;
+          a = (P - tol - y0) / dx;
+          b = (P + tol - y0) / dx;
+        }
+#pragma unroll
+        for (int d = 1; d < 64; d <<= 1) {
+          const double ta = shfl_up_f64(a, d), tb = shfl_up_f64(b, d);
+          if ((int)lane >= d) {
+            a = ta > a ? ta : a;
+            b = tb < b ? tb : b;
+          }
+        }
+        a = lo > a ? lo : a;
+        b = hi < b ? hi : b;
+        const uint64_t fail = wave_ballot(act && a > b);
+        if (fail == 0) {
+          lo = readlane_f64(a, (int)(m - 1));
+          hi = readlane_f64(b, (int)(m - 1));
+          jp = jn + m - 1;
+          break;
+        }
+        const uint32_t f = (uint32_t)__builtin_ctzll(fail);
+        double plo = lo, phi = hi;
+        uint64_t jq = jp;
+        if (f > start) {
+          plo = readlane_f64(a, (int)(f - 1));
+          phi = readlane_f64(b, (int)(f - 1));
+          jq = jn + f - 1;
+        }
+        const double ye = y0 + 0.5 * (plo + phi) * ((double)jq - x0);
+        put(jq, ye);
+        x0 = (double)jq;
+        y0 = ye;
+        lo = -1e300;
+        hi = 1e300;
+        jp = jq;
+        start = f;
+      }
+      // consume the window's first m ones
+      const uint32_t kl = (uint32_t)__builtin_amdgcn_readlane((int)ks, (int)(m - 1));
+      const uint32_t bl = (uint32_t)__builtin_amdgcn_readlane((int)bit, (int)(m - 1));
+      const uint64_t wlast = shfl_u64(wd, (int)kl);
+      w += kl;
+      x = wlast & ~((2ull << bl) - 1);
+      jn += m;
     }
     if ((double)jp > x0) put(jp, y0 + 0.5 * (lo + hi) * ((double)jp - x0));
   }
@@ -389,8 +493,9 @@ __global__ void k_lsel_grad(const TrainVec* __restrict__ tv, TrainBufs tb, float
 }
 
 // sum of squared residuals of model b (training loss numerator)
-__global__ __launch_bounds__(256) void k_lsel_loss(const TrainVec* __restrict__ tv, uint32_t b, TrainBufs tb,
+__global__ __launch_bounds__(256) void k_lsel_loss(const TrainVec* __restrict__ tv, TrainBufs tb,
                                                    double* __restrict__ out) {
+  const uint32_t b = blockIdx.x;  // one block per model
   const TrainVec& v = tv[b];
   double acc = 0;
   for (uint64_t k = threadIdx.x; k + 1 < v.nknots; k += blockDim.x) acc += (double)tb.le[v.kfirst + k];
@@ -404,9 +509,12 @@ __global__ __launch_bounds__(256) void k_lsel_loss(const TrainVec* __restrict__ 
   if (threadIdx.x == 0) out[b] = s[0];
 }
 
-// exhaustive error window of vector b: thread per word, every one of the word
-__global__ __launch_bounds__(256) void k_lsel_eps(const LselDesc* __restrict__ descs, uint32_t b,
+// exhaustive error window of vector vids[blockIdx.y]: thread per word, every
+// one of the word
+__global__ __launch_bounds__(256) void k_lsel_eps(const LselDesc* __restrict__ descs,
+                                                  const uint32_t* __restrict__ vids,
                                                   unsigned long long* __restrict__ eps) {
+  const uint32_t b = vids[blockIdx.y];
   const LselDesc d = descs[b];
   const uint64_t W = (d.nbits + 63) >> 6;
   uint64_t mx = 0;
@@ -498,7 +606,7 @@ static int lsel_models(wtg_ctx* ctx, wtg_lsel* ls, const wtg_lsel_params& P, con
   std::vector<uint64_t> cbase(na + 1, 0);
   for (uint32_t a = 0; a < na; a++) {
     const LselDesc& d = ls->desc[act[a]];
-    pv[a] = PlaceVec{d.s1, d.words, d.ones, cbase[a], d.alpha, d.beta};
+    pv[a] = PlaceVec{d.s1, d.words, (d.nbits + 63) >> 6, d.ones, cbase[a], d.alpha, d.beta};
     cbase[a + 1] = cbase[a] + wtg_ceil_div(d.ones, P.max_gap);
   }
   const uint64_t nflat = cbase[na];
@@ -512,9 +620,11 @@ static int lsel_models(wtg_ctx* ctx, wtg_lsel* ls, const wtg_lsel_params& P, con
   WTG_CHECK_HIP(hipMemcpyAsync(d_pv, pv.data(), na * sizeof(PlaceVec), hipMemcpyHostToDevice, ctx->stream));
   WTG_CHECK_HIP(hipMemcpyAsync(d_cbase, cbase.data(), (na + 1) * 8, hipMemcpyHostToDevice, ctx->stream));
   WTG_TRY(wtg_fill_u64(ctx, d_err, 1, 0));
-  const unsigned gp = (unsigned)wtg_ceil_div(nflat, 256);
+  Knot* d_scr = nullptr;  // the count pass keeps each chunk's first LSEL_CAP knots
+  WTG_TRY(wtg_malloc(ctx, (void**)&d_scr, nflat * LSEL_CAP * sizeof(Knot) + 256));
+  const unsigned gp = (unsigned)wtg_ceil_div(nflat, 4);  // a wave per chunk
   WTG_LAUNCH(ctx, "lsel_place", k_lsel_place, dim3(gp), dim3(256), 0, d_pv, d_cbase, na, nflat, gap_log,
-             (double)P.tol, d_cnt, (const uint32_t*)nullptr, (Knot*)nullptr, d_err);
+             (double)P.tol, d_cnt, (const uint32_t*)nullptr, (Knot*)nullptr, d_scr, d_err);
   WTG_LAUNCH_CHECK();
   std::vector<uint32_t> off(nflat + 1, 0);
   WTG_TRY(sync_copy(ctx, off.data(), d_cnt, nflat * 4, hipMemcpyDeviceToHost));
@@ -540,10 +650,12 @@ static int lsel_models(wtg_ctx* ctx, wtg_lsel* ls, const wtg_lsel_params& P, con
   WTG_TRY(alloc_block(ctx, ls, (na + 1) * 8, (void**)&d_kfirst));
   WTG_CHECK_HIP(hipMemcpyAsync(d_cnt, off.data(), (nflat + 1) * 4, hipMemcpyHostToDevice, ctx->stream));
   WTG_LAUNCH(ctx, "lsel_place", k_lsel_place, dim3(gp), dim3(256), 0, d_pv, d_cbase, na, nflat, gap_log,
-             (double)P.tol, (uint32_t*)nullptr, (const uint32_t*)d_cnt, d_K, d_err);
+             (double)P.tol, (uint32_t*)nullptr, (const uint32_t*)d_cnt, d_K, d_scr, d_err);
   WTG_LAUNCH_CHECK();
   uint32_t perr = 0;
-  WTG_TRY(sync_copy(ctx, &perr, d_err, 4, hipMemcpyDeviceToHost));
+  const int prc = sync_copy(ctx, &perr, d_err, 4, hipMemcpyDeviceToHost);
+  wtg_free(ctx, d_scr);
+  WTG_TRY(prc);
   WTG_REQUIRE(perr == 0, "wtg_lsel_build: knot placement passes disagree");
   WTG_CHECK_HIP(hipMemcpyAsync(d_K0, d_K, nk_total * sizeof(Knot), hipMemcpyDeviceToDevice, ctx->stream));
   WTG_TRY(wtg_fill_u64(ctx, m1, nk_total * 5 / 2 + 1, 0));
@@ -587,18 +699,22 @@ static int lsel_models(wtg_ctx* ctx, wtg_lsel* ls, const wtg_lsel_params& P, con
   TrainBufs tb{d_K, m1, m2, gl, gr, le, d_kfirst};
   const unsigned gk = (unsigned)wtg_ceil_div(nk_total, 256);
   // eps (exhaustive) and training loss of the knots now in d_K
+  uint32_t* d_act = nullptr;
+  WTG_TRY(alloc_block(ctx, ls, na * 4 + 256, (void**)&d_act));
+  WTG_CHECK_HIP(hipMemcpyAsync(d_act, act.data(), na * 4, hipMemcpyHostToDevice, ctx->stream));
+  uint64_t wmax = 1;
+  for (uint32_t a = 0; a < na; a++) wmax = std::max<uint64_t>(wmax, (nbits[act[a]] + 63) / 64);
+  const unsigned ge_x = (unsigned)std::max<uint64_t>(1, std::min<uint64_t>(wtg_ceil_div(wmax, 256),
+                                                                          std::max<uint64_t>(4096 / na, 32)));
   auto measure = [&](std::vector<uint64_t>& eps, std::vector<double>& loss) -> int {
     WTG_TRY(wtg_fill_u64(ctx, d_eps, nvec, 0));
     WTG_LAUNCH(ctx, "lsel_fwd_bwd", k_lsel_fwd_bwd, dim3(gk), dim3(256), 0, d_tv, na, tb, nk_total);
     WTG_LAUNCH_CHECK();
-    for (uint32_t a = 0; a < na; a++) {
-      const uint32_t b = act[a];
-      const unsigned ge = (unsigned)std::min<uint64_t>(wtg_ceil_div((nbits[b] + 63) / 64, 256), 2048);
-      WTG_LAUNCH(ctx, "lsel_eps", k_lsel_eps, dim3(ge), dim3(256), 0, ls->d_desc, b, d_eps);
-      WTG_LAUNCH_CHECK();
-      WTG_LAUNCH(ctx, "lsel_loss", k_lsel_loss, dim3(1), dim3(256), 0, d_tv, a, tb, d_loss);
-      WTG_LAUNCH_CHECK();
-    }
+    // every model in one launch each: blocks of a vector stride over its words
+    WTG_LAUNCH(ctx, "lsel_eps", k_lsel_eps, dim3(ge_x, na), dim3(256), 0, ls->d_desc, d_act, d_eps);
+    WTG_LAUNCH_CHECK();
+    WTG_LAUNCH(ctx, "lsel_loss", k_lsel_loss, dim3(na), dim3(256), 0, d_tv, tb, d_loss);
+    WTG_LAUNCH_CHECK();
     std::vector<unsigned long long> e(nvec);
     std::vector<double> l(na);
     WTG_CHECK_HIP(hipMemcpyAsync(e.data(), d_eps, nvec * 8, hipMemcpyDeviceToHost, ctx->stream));
@@ -649,7 +765,7 @@ static int lsel_models(wtg_ctx* ctx, wtg_lsel* ls, const wtg_lsel_params& P, con
 extern "C" int wtg_lsel_build(wtg_ctx* ctx, uint32_t nvec, const uint64_t* const* d_words,
                               const uint64_t* nbits, const wtg_lsel_params* prm, wtg_lsel** out) {
   WTG_REQUIRE(ctx && out && d_words && nbits && nvec >= 1 && nvec <= 256, "wtg_lsel_build: bad argument (1 <= nvec <= 256)");
-  wtg_lsel_params P{16, 4096, 480, 200, 4.0f, 512};
+  wtg_lsel_params P{16, 4096, 480, 50, 4.0f, 512};
   if (prm) P = *prm;
   auto pow2 = [](uint32_t x) { return x && !(x & (x - 1)); };
   WTG_REQUIRE(pow2(P.train_stride) && pow2(P.max_gap) && P.max_gap >= 2 && P.max_gap <= (1u << 24),
@@ -812,7 +928,7 @@ extern "C" int wtg_lsel_loss_grad(wtg_ctx* ctx, const void* d_knots, uint32_t nk
   WTG_LAUNCH_CHECK();
   WTG_LAUNCH(ctx, "lsel_grad", k_lsel_grad, dim3(g), dim3(256), 0, d_tv, tb, d_grad);
   WTG_LAUNCH_CHECK();
-  WTG_LAUNCH(ctx, "lsel_loss", k_lsel_loss, dim3(1), dim3(256), 0, d_tv, 0u, tb, (double*)(ws + o_l));
+  WTG_LAUNCH(ctx, "lsel_loss", k_lsel_loss, dim3(1), dim3(256), 0, d_tv, tb, (double*)(ws + o_l));
   WTG_LAUNCH_CHECK();
   double l = 0;
   WTG_CHECK_HIP(hipMemcpyAsync(&l, ws + o_l, 8, hipMemcpyDeviceToHost, ctx->stream));

@@ -9,7 +9,9 @@
 #ifndef INCLUDED_SDSL_INT_VECTOR
 #include <array>
 #include <cstdint>
+#include <cstddef>
 #include <istream>
+#include <iterator>
 #include <ostream>
 #include <vector>
 
@@ -70,6 +72,48 @@ typedef int_vector<1> bit_vector;
 // closed index ranges [first, second] of the node interface (sdsl's wt_helper)
 typedef std::array<uint64_t, 2> range_type;
 typedef std::vector<range_type> range_vec_type;
+
+// Read-only random-access iterator over any container with size() and a const
+// operator[] (sdsl's random_access_const_iterator, which the wavelet trees'
+// begin() / end() return): an index into the container, dereferenced by
+// operator[].
+template <class t_rac>
+class random_access_const_iterator {
+ public:
+  typedef std::random_access_iterator_tag iterator_category;
+  typedef typename t_rac::value_type value_type;
+  typedef value_type const_reference;
+  typedef value_type reference;
+  typedef const value_type* pointer;
+  typedef typename t_rac::size_type size_type;
+  typedef std::ptrdiff_t difference_type;
+
+  random_access_const_iterator() = default;
+  random_access_const_iterator(const t_rac* rac, size_type idx) : m_rac(rac), m_idx(idx) {}
+  const_reference operator*() const { return (*m_rac)[m_idx]; }
+  const_reference operator[](difference_type i) const { return (*m_rac)[m_idx + i]; }
+  random_access_const_iterator& operator++() { ++m_idx; return *this; }
+  random_access_const_iterator operator++(int) { auto t = *this; ++m_idx; return t; }
+  random_access_const_iterator& operator--() { --m_idx; return *this; }
+  random_access_const_iterator operator--(int) { auto t = *this; --m_idx; return t; }
+  random_access_const_iterator& operator+=(difference_type i) { m_idx += i; return *this; }
+  random_access_const_iterator& operator-=(difference_type i) { m_idx -= i; return *this; }
+  random_access_const_iterator operator+(difference_type i) const { auto t = *this; return t += i; }
+  random_access_const_iterator operator-(difference_type i) const { auto t = *this; return t -= i; }
+  difference_type operator-(const random_access_const_iterator& o) const {
+    return (difference_type)m_idx - (difference_type)o.m_idx;
+  }
+  bool operator==(const random_access_const_iterator& o) const { return m_rac == o.m_rac && m_idx == o.m_idx; }
+  bool operator!=(const random_access_const_iterator& o) const { return !(*this == o); }
+  bool operator<(const random_access_const_iterator& o) const { return m_idx < o.m_idx; }
+  bool operator>(const random_access_const_iterator& o) const { return m_idx > o.m_idx; }
+  bool operator<=(const random_access_const_iterator& o) const { return m_idx <= o.m_idx; }
+  bool operator>=(const random_access_const_iterator& o) const { return m_idx >= o.m_idx; }
+
+ private:
+  const t_rac* m_rac = nullptr;
+  size_type m_idx = 0;
+};
 
 }  // namespace sdsl
 #endif

@@ -2,7 +2,8 @@
 // (wt_huff/wt_huff_par.hpp:68-78 over wt_huff/wt_pc_par.hpp): same include
 // guard, the constructor wt_huff(int_vector<8>& text, size) (wt_pc_par.hpp:321-361)
 // and the query surface size / empty / sigma / operator[] / rank /
-// inverse_select / select / interval_symbols.  Counts, Huffman shape, tree bits, rank_support_v
+// inverse_select / select / interval_symbols, begin / end, the node interface
+// (root / is_leaf / sym / empty / expand / path) and symbol_gte / symbol_lte.  Counts, Huffman shape, tree bits, rank_support_v
 // and select_support_mcl (1 and 0) are built on the GPU (wtg_huff_build) and
 // copied into host supports, so scalar queries run on the host with the
 // reference's results; rank_batch / select_batch / access_batch / count_batch
@@ -179,6 +180,66 @@ class wt_huff {
         cs[k++] = (value_type)m_shape.sym[c];
       }
     }
+  }
+
+  // ---- iterators (wt_pc_par.hpp:740-746): over operator[]
+  typedef random_access_const_iterator<wt_huff> const_iterator;
+  typedef const_iterator iterator;
+  const_iterator begin() const { return const_iterator(this, 0); }
+  const_iterator end() const { return const_iterator(this, size()); }
+
+  // ---- node interface (wt_pc_par.hpp:778-909).  A node is its index in the
+  // tree's BFS order (root 0); a leaf holds one symbol.
+  typedef uint64_t node_type;
+  bool is_leaf(const node_type& v) const { return m_shape.child[v][0] < 0; }
+  value_type sym(const node_type& v) const { return (value_type)m_shape.sym[v]; }
+  // the reference answers true for every node (:786-788)
+  bool empty(const node_type&) const { return true; }
+  node_type root() const { return 0; }
+  std::pair<node_type, node_type> expand(const node_type& v) const {  // pre: !is_leaf(v)
+    return std::make_pair((node_type)m_shape.child[v][0], (node_type)m_shape.child[v][1]);
+  }
+  // a closed range [s, e] of v mapped into both children (an empty result is
+  // [x, x-1] in size_type arithmetic, as the reference)
+  std::pair<range_type, range_type> expand(const node_type& v, const range_type& r) const {
+    const size_type bp = m_shape.bv_pos[v], v_sp_rank = m_shape.bv_pos_rank[v];
+    const size_type sp_rank = m_bv_rank(bp + r[0]);
+    const size_type right_size = m_bv_rank(bp + r[1] + 1) - sp_rank;
+    const size_type left_size = (r[1] - r[0] + 1) - right_size;
+    const size_type right_sp = sp_rank - v_sp_rank, left_sp = r[0] - right_sp;
+    return std::make_pair(range_type{{left_sp, left_sp + left_size - 1}},
+                          range_type{{right_sp, right_sp + right_size - 1}});
+  }
+  std::pair<range_vec_type, range_vec_type> expand(const node_type& v, const range_vec_type& ranges) const {
+    std::pair<range_vec_type, range_vec_type> res;
+    res.first.reserve(ranges.size());
+    res.second.reserve(ranges.size());
+    for (const auto& r : ranges) {
+      const auto lr = expand(v, r);
+      res.first.push_back(lr.first);
+      res.second.push_back(lr.second);
+    }
+    return res;
+  }
+  // {code length, code} of symbol c, the root's branch as the code's highest
+  // bit (:864-871)
+  std::pair<uint64_t, uint64_t> path(value_type c) const {
+    const uint32_t len = m_shape.c_to_leaf[c] >= 0 ? m_shape.path_len[c] : 0u;
+    uint64_t code = 0;
+    for (uint32_t l = 0; l < len; l++) code = (code << 1) | ((m_shape.path[c] >> l) & 1u);
+    return {len, code};
+  }
+  // the smallest symbol >= c / the largest symbol <= c that occurs; first =
+  // false when there is none (:880-899)
+  std::pair<bool, value_type> symbol_gte(value_type c) const {
+    for (uint32_t x = c; x < 256; x++)
+      if (m_size && m_shape.c_to_leaf[x] >= 0) return {true, (value_type)x};
+    return {false, 0};
+  }
+  std::pair<bool, value_type> symbol_lte(value_type c) const {
+    for (int x = c; x >= 0; x--)
+      if (m_size && m_shape.c_to_leaf[x] >= 0) return {true, (value_type)x};
+    return {false, 0};
   }
 
   // ---- batched device queries (device buffers of nq entries)

@@ -188,6 +188,12 @@ class wt_int {
     return res;
   }
 
+  // ---- iterators (:877-886): over operator[]
+  typedef random_access_const_iterator<wt_int> const_iterator;
+  typedef const_iterator iterator;
+  const_iterator begin() const { return const_iterator(this, 0); }
+  const_iterator end() const { return const_iterator(this, size()); }
+
   // ---- node interface (:915-1085): a node is (offset of its bits, size, level,
   // value prefix)
   struct node_type {
