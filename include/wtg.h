@@ -96,7 +96,9 @@ uint64_t wtg_levels_words(uint64_t n, uint32_t levels);
  * d_nodes: NULL, or node_slots device records indexed by heap id (root 0,
  *   children 2v+1 / 2v+2); reached nodes as in the reference, other slots
  *   {0, 0, WTG_NONE, WTG_NONE, WTG_NONE}.
- * Limits: 1 <= levels <= 24 (dense per-level node tables), n < 2^32. */
+ * Limits: 1 <= levels <= 64, n < 2^32; d_nodes only up to 24 levels (above, pass
+ * NULL: WTG_ERR_UNSUPPORTED otherwise).  Above 24 levels nodes are numbered by
+ * their rank within the level (linear work per level) instead of by prefix. */
 int wtg_levels_build(wtg_ctx* ctx, const void* d_sym, int sym_bytes, uint64_t n,
                      uint32_t levels, uint64_t* d_words, wtg_node* d_nodes,
                      uint64_t node_slots);

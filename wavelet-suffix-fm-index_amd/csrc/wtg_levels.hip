@@ -1212,8 +1212,8 @@ static int levels_check(wtg_ctx* ctx, int sym_bytes, uint64_t n, uint32_t L) {
   WTG_REQUIRE(sym_bytes == 1 || sym_bytes == 2 || sym_bytes == 4 || sym_bytes == 8,
               "sym_bytes must be 1, 2, 4 or 8");
   WTG_REQUIRE(L >= 1, "levels must be >= 1");
-  if (L > 24) {
-    wtg_set_error("levels > 24 not supported (dense per-level node tables)");
+  if (L > 64) {
+    wtg_set_error("levels > 64 not supported");
     return WTG_ERR_UNSUPPORTED;
   }
   if (n >= (1ull << 32)) {
@@ -1229,6 +1229,13 @@ int wtg_levels_build_async(wtg_ctx* ctx, const void* d_sym, int sym_bytes, uint6
   WTG_TRY(levels_check(ctx, sym_bytes, n, levels));
   WTG_REQUIRE(n == 0 || (d_sym && d_words), "null device buffer");
   WTG_TRY(wtg_set_device(ctx));
+  if (levels > 24) {  // node ids by rank in the level instead of dense prefix tables
+    if (d_nodes && node_slots) {
+      wtg_set_error("levels > 24: WTnode records are limited to 24 levels (pass no node array)");
+      return WTG_ERR_UNSUPPORTED;
+    }
+    return n ? wtg_deep_build(ctx, d_sym, sym_bytes, n, levels, d_words, d_distinct) : WTG_OK;
+  }
   if (n > 0 && ctx->level_method == 0 && wtg_mld_enabled())
     return wtg_mld_build(ctx, d_sym, sym_bytes, n, levels, d_words, d_nodes, node_slots, d_distinct);
   switch (sym_bytes) {

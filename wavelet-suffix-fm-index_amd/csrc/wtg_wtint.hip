@@ -149,13 +149,14 @@ __global__ void k_wtint_access(WtQ w, const uint64_t* __restrict__ qi, uint64_t 
   if (out_rank) out_rank[q] = i;
 }
 
+template <uint32_t MAXL>  // the tree's levels are at most MAXL
 __global__ void k_wtint_select(WtQ w, const uint64_t* __restrict__ qi, const uint64_t* __restrict__ qc,
                                uint64_t nq, uint64_t* __restrict__ out) {
   const uint64_t q = (uint64_t)blockIdx.x * blockDim.x + threadIdx.x;
   if (q >= nq) return;
   uint64_t i = qi[q];
   const uint64_t c = qc[q];
-  uint64_t path_off[25], path_rank[25];  // levels <= 24
+  uint64_t path_off[MAXL + 1], path_rank[MAXL + 1];
   uint64_t offset = 0, mask = 1ull << (w.L - 1), node_size = w.n;
   path_off[0] = path_rank[0] = 0;
   for (uint32_t k = 0; k < w.L && node_size; k++) {
@@ -386,8 +387,13 @@ extern "C" int wtg_wtint_select_batch(wtg_ctx* ctx, const wtg_wtint* wt, const u
   WTG_REQUIRE(nq == 0 || d_c, "null d_c");
   if (!nq) return WTG_OK;
   WTG_TRY(wtg_set_device(ctx));
-  WTG_LAUNCH(ctx, "wtint_select", k_wtint_select, dim3((unsigned)wtg_ceil_div(nq, 256)), dim3(256), 0,
-             make_q(wt), d_i, d_c, nq, d_out);
+  if (wt->L <= 24) {
+    WTG_LAUNCH(ctx, "wtint_select", k_wtint_select<24>, dim3((unsigned)wtg_ceil_div(nq, 256)), dim3(256), 0,
+               make_q(wt), d_i, d_c, nq, d_out);
+  } else {
+    WTG_LAUNCH(ctx, "wtint_select", k_wtint_select<64>, dim3((unsigned)wtg_ceil_div(nq, 256)), dim3(256), 0,
+               make_q(wt), d_i, d_c, nq, d_out);
+  }
   WTG_LAUNCH_CHECK();
   return wtg_finish(ctx);
 }
