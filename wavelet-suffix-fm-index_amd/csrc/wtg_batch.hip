@@ -215,6 +215,7 @@ struct wtg_batch {
   uint64_t nwords = 0;
   size_t ws_need = 0;
   bool graph = true;
+  bool mld = true;              // multi-level passes (else the level-by-level split)
   hipGraph_t g = nullptr;
   hipGraphExec_t exec = nullptr;
   void* graph_ws = nullptr;     // workspace the recorded graph addresses
@@ -259,6 +260,7 @@ extern "C" int wtg_batch_create_planned(wtg_ctx* ctx, const uint8_t* d_sym, uint
   WTG_TRY(wtg_set_device(ctx));
   *out = nullptr;
   wtg_batch* b = new wtg_batch();
+  b->mld = ctx->level_method == 0 && wtg_mld_enabled();
   b->d_sym = d_sym;
   b->sym_fast = ((uintptr_t)d_sym & 3) == 0;
   b->nseq = nseq;
@@ -329,7 +331,7 @@ extern "C" int wtg_batch_create_planned(wtg_ctx* ctx, const uint8_t* d_sym, uint
         (rc = wtg_malloc(ctx, (void**)&G.d_pref, (m + 1) * 8)) != WTG_OK ||
         (rc = wtg_malloc(ctx, (void**)&G.d_cnt, cnt.size() * 4)) != WTG_OK ||
         (rc = wtg_malloc(ctx, (void**)&G.d_ones, cnt.size() * 4)) != WTG_OK ||
-        (rc = wtg_malloc(ctx, &G.d_keys, std::max<uint64_t>(G.n, 1) * G.key_bytes)) != WTG_OK) {
+        (!b->mld && (rc = wtg_malloc(ctx, &G.d_keys, std::max<uint64_t>(G.n, 1) * G.key_bytes)) != WTG_OK)) {
       b->groups.push_back(G);
       batch_release(ctx, b);
       return rc;
@@ -341,7 +343,8 @@ extern "C" int wtg_batch_create_planned(wtg_ctx* ctx, const uint8_t* d_sym, uint
       batch_release(ctx, b);
       return rc;
     }
-    b->ws_need = std::max(b->ws_need, wtg_levels_ws_bytes(G.n, G.sbits + L, G.key_bytes));
+    b->ws_need = std::max(b->ws_need, b->mld ? wtg_mld_ws_bytes(G.n, G.sbits + L, G.sbits)
+                                             : wtg_levels_ws_bytes(G.n, G.sbits + L, G.key_bytes));
     b->groups.push_back(G);
   }
   if ((rc = wtg_malloc(ctx, (void**)&b->words, std::max<uint64_t>(b->nwords, 1) * 8)) != WTG_OK) {
@@ -361,6 +364,18 @@ extern "C" int wtg_batch_create_planned(wtg_ctx* ctx, const uint8_t* d_sym, uint
 static int batch_enqueue(wtg_ctx* ctx, wtg_batch* b) {
   for (auto& G : b->groups) {
     if (G.n == 0) continue;
+    if (b->mld) {
+      // up to 8 levels per pass from level sbits, straight from the byte symbols:
+      // a sequence is a level-sbits node, its keys its own bytes (their low L bits)
+      if (G.sbits) {
+        WTG_TRY(wtg_mld_build_from(ctx, b->d_sym, 1, G.n, G.sbits + G.L, G.sbits, G.d_cnt, G.d_seqoff,
+                                   b->words + G.word_base));
+      } else {
+        WTG_TRY(wtg_mld_build_from(ctx, b->d_sym + b->off[G.seqs[0]], 1, G.n, G.L, 0, nullptr, nullptr,
+                                   b->words + G.word_base));
+      }
+      continue;
+    }
     const unsigned grid = (unsigned)wtg_ceil_div(G.n, 4096);
     const uint32_t m = (uint32_t)G.seqs.size();
     WTG_TRY(wtg_fill_bytes_zero(ctx, G.d_ones, (1ull << G.sbits) * 4));

@@ -117,6 +117,9 @@ int wtg_rank_select_end(wtg_ctx* ctx, const wtg_rs_pending* pend, const uint32_t
 // multi-level direct build (up to 8 levels per pass; n >= 1, L <= 24, n < 2^32)
 // and whether it is on (WTG_LEVEL_SPLIT=1 selects the level-by-level split)
 int wtg_mld_enabled();
+int wtg_mld_build_from(wtg_ctx* ctx, const void* d_keys, int key_bytes, uint64_t n, uint32_t L, uint32_t s0,
+                       const uint32_t* d_cnt0, const uint64_t* d_src_off, uint64_t* d_words);
+size_t wtg_mld_ws_bytes(uint64_t n, uint32_t L, uint32_t s0);
 // levels 25..64: nodes numbered by rank within the level (wtg_deep.hip)
 int wtg_deep_build(wtg_ctx* ctx, const void* d_sym, int sym_bytes, uint64_t n, uint32_t L, uint64_t* d_words,
                    unsigned long long* d_distinct);

@@ -64,6 +64,10 @@ struct MldArgs {
   const uint32_t* bpre;      // [block][2^g] exclusive prefix of the block sums
   const uint32_t* cum;       // 2^(s+g) + 1 entries
   uint64_t* words;
+  uint32_t lvl0;             // the first level of the build: level l's bits go to (l - lvl0) * n
+  uint32_t check;            // 1: flag input keys with bits at or above chk_shift
+  uint32_t chk_shift;
+  const uint64_t* src_off;   // null, or (first group) where each level-s node's keys start in `keys`
   uint32_t* kout;            // P_{s+g} keys as uint32 (emit with SCATTER)
   uint32_t* err;
 };
@@ -97,6 +101,13 @@ __device__ __forceinline__ bool mld_tile(const MldArgs& a, uint32_t t, uint32_t&
   return true;
 }
 
+// where tile keys start in a.keys: their P_s position, or, when the level-s
+// nodes' keys lie apart (src_off), that node's source offset + the tile's
+// offset inside the node
+__device__ __forceinline__ uint64_t mld_src(const MldArgs& a, uint32_t seg, uint64_t base) {
+  return a.src_off ? a.src_off[seg] + (base - a.cum_s[seg]) : base;
+}
+
 template <typename K>
 __device__ __forceinline__ uint64_t mld_key(const void* keys, uint64_t i) {
   return (uint64_t)reinterpret_cast<const K*>(keys)[i];
@@ -115,6 +126,7 @@ __global__ __launch_bounds__(256) void k_mld_count(MldArgs a) {
   uint32_t seg, len, first;
   uint64_t base;
   if (!mld_tile(a, t, seg, base, len, first)) return;
+  const uint64_t src = mld_src(a, seg, base);
   const uint32_t G = 1u << a.g, lane = (uint32_t)lane_id();
   const uint32_t sh = a.L - a.s - a.g;
   bool bad = false;
@@ -122,8 +134,8 @@ __global__ __launch_bounds__(256) void k_mld_count(MldArgs a) {
 #pragma unroll
   for (uint32_t m = 0; m < MLD_CNT_C; m++) {
     const uint32_t k = m * 64 + lane;
-    const uint64_t x = k < len ? mld_key<K>(a.keys, base + k) : 0;
-    if (a.s == 0 && (x >> a.L) != 0) bad = true;
+    const uint64_t x = k < len ? mld_key<K>(a.keys, src + k) : 0;
+    if (a.check && (x >> a.chk_shift) != 0) bad = true;
     dd[m] = (uint32_t)(x >> sh) & (G - 1);
   }
   for (uint32_t i = lane; i < G; i += 64) s_cnt[i] = 0;
@@ -296,6 +308,7 @@ __global__ __launch_bounds__(256) __attribute__((amdgpu_waves_per_eu(8))) void k
   uint64_t base;
   const uint32_t t = blockIdx.x;
   if (!mld_tile(a, t, seg, base, len, first)) return;
+  const uint64_t src = mld_src(a, seg, base);
   const uint32_t g = a.g, G = 1u << g, lane = (uint32_t)lane_id();
   const uint32_t w = (uint32_t)__builtin_amdgcn_readfirstlane((int)(threadIdx.x >> 6));
   const uint32_t sh = a.L - a.s - g;
@@ -319,7 +332,7 @@ __global__ __launch_bounds__(256) __attribute__((amdgpu_waves_per_eu(8))) void k
 #pragma unroll
     for (uint32_t m = 0; m < MLD_CPW; m++) {
       const uint32_t p = (w + 4 * m) * 64 + lane;
-      xk[m] = p < len ? (uint32_t)mld_key<K>(a.keys, base + p) : ~0u;
+      xk[m] = p < len ? (uint32_t)mld_key<K>(a.keys, src + p) : ~0u;
     }
     cpre_d = block_excl_scan_u32(cv, s_wave, ctot);
     uint32_t tot;
@@ -431,8 +444,9 @@ __global__ __launch_bounds__(256) __attribute__((amdgpu_waves_per_eu(8))) void k
       const uint32_t j = 31 - __clz(r + 1), q = r + 1 - (1u << j);
       const uint32_t lo = q << (g - j), hi = (q + 1) << (g - j);
       const uint32_t cnt = s_tpre[hi] - s_tpre[lo];
-      const uint64_t gs = (uint64_t)(a.s + j) * n + s_cum[lo] + (s_cpre[hi] - s_cpre[lo]);
-      if (gs < (uint64_t)(a.s + j) * n || gs + cnt > (uint64_t)(a.s + j + 1) * n) {
+      const uint64_t lb = (uint64_t)(a.s + j - a.lvl0) * n;  // the level's first bit
+      const uint64_t gs = lb + s_cum[lo] + (s_cpre[hi] - s_cpre[lo]);
+      if (gs < lb || gs + cnt > lb + n) {
         atomicOr(a.err, WTG_FLAG_BADSYM);  // tables inconsistent with the keys: write nothing
       } else if (cnt) {
         nw = (uint32_t)(((gs + cnt - 1) >> 6) - (gs >> 6) + 1);
@@ -524,9 +538,9 @@ struct MldPlan {
       off_scst, off_ctr, total;
 };
 
-MldPlan mld_plan(uint64_t n, uint32_t L) {
+MldPlan mld_plan(uint64_t n, uint32_t L, uint32_t s0) {
   MldPlan P{};
-  uint32_t s = 0;
+  uint32_t s = s0;
   while (s < L) {
     const uint32_t g = L - s < MLD_GMAX ? L - s : MLD_GMAX;
     P.gs[P.ngroups] = s;
@@ -558,10 +572,14 @@ MldPlan mld_plan(uint64_t n, uint32_t L) {
   return P;
 }
 
+// Levels [s0, L) of the keys d_sym (in P_{s0} order: sorted by their top s0
+// bits, d_cnt0 = the 2^s0 node sizes; s0 = 0: any order, d_cnt0 unused);
+// level l's bits go to bit (l - s0) * n of d_words.
 template <typename K>
-int mld_build_t(wtg_ctx* ctx, const K* d_sym, uint64_t n, uint32_t L, uint64_t* d_words, wtg_node* d_nodes,
-                uint64_t node_slots, unsigned long long* d_distinct) {
-  const MldPlan P = mld_plan(n, L);
+int mld_build_t(wtg_ctx* ctx, const K* d_sym, uint64_t n, uint32_t L, uint32_t s0, const uint32_t* d_cnt0,
+                const uint64_t* d_src_off, uint64_t* d_words, wtg_node* d_nodes, uint64_t node_slots,
+                unsigned long long* d_distinct) {
+  const MldPlan P = mld_plan(n, L, s0);
   WTG_TRY(wtg_ws_reserve(ctx, P.total));
   char* ws = (char*)ctx->ws;
   uint32_t* tcnt = (uint32_t*)(ws + P.off_tcnt);
@@ -576,11 +594,32 @@ int mld_build_t(wtg_ctx* ctx, const K* d_sym, uint64_t n, uint32_t L, uint64_t* 
   uint64_t* scst = (uint64_t*)(ws + P.off_scst);
   uint32_t* ctr = (uint32_t*)(ws + P.off_ctr);
   uint32_t* kbuf[2] = {(uint32_t*)(ws + P.off_keys[0]), (uint32_t*)(ws + P.off_keys[1])};
-  WTG_TRY(wtg_fill_bytes_zero(ctx, d_words, wtg_levels_words(n, L) * 8));
+  WTG_TRY(wtg_fill_bytes_zero(ctx, d_words, wtg_levels_words(n, L - s0) * 8));
   if (d_distinct) WTG_TRY(wtg_fill_u64(ctx, d_distinct, 1, 0));
   const uint32_t ntiles0 = (uint32_t)wtg_ceil_div(n, MLD_T);
   const void* src = d_sym;
   const uint32_t* cum_prev = nullptr;
+  if (s0 > 0) {
+    // the level-s0 nodes: their starts (cum[1]; group 0 writes cum[0]), tiles
+    // per node, first tiles and the tile -> node map
+    const uint64_t nb0 = 1ull << s0, st0 = wtg_ceil_div(nb0, 4096);
+    WTG_TRY(wtg_fill_bytes_zero(ctx, scst, (char*)ctr + 256 - (char*)scst));
+    WTG_LAUNCH(ctx, "mld_cum", k_scan_u32<uint32_t>, dim3((unsigned)st0), dim3(256), 0, d_cnt0, cum[1], nb0, scst,
+               ctr, ctx->d_flag);
+    WTG_LAUNCH_CHECK();
+    WTG_LAUNCH(ctx, "mld_ntiles", k_mld_ntiles, dim3((unsigned)wtg_ceil_div(nb0, 256)), dim3(256), 0, cum[1], nb0,
+               nt);
+    WTG_LAUNCH_CHECK();
+    WTG_TRY(wtg_fill_bytes_zero(ctx, scst, (char*)ctr + 256 - (char*)scst));
+    WTG_LAUNCH(ctx, "mld_tiles", k_scan_u32<uint32_t>, dim3((unsigned)st0), dim3(256), 0, nt, tst, nb0, scst, ctr,
+               ctx->d_flag);
+    WTG_LAUNCH_CHECK();
+    const uint64_t nt0 = wtg_ceil_div(n, MLD_T) + std::min<uint64_t>(n, nb0);
+    WTG_LAUNCH(ctx, "mld_tseg", k_mld_tseg, dim3((unsigned)wtg_ceil_div(nt0, 256)), dim3(256), 0, tst, nb0, tseg,
+               (uint64_t)P.maxtiles);
+    WTG_LAUNCH_CHECK();
+    cum_prev = cum[1];
+  }
   for (uint32_t gi = 0; gi < P.ngroups; gi++) {
     const uint32_t s = P.gs[gi], g = P.gg[gi], G = 1u << g;
     const bool last = gi + 1 == P.ngroups;
@@ -600,6 +639,10 @@ int mld_build_t(wtg_ctx* ctx, const K* d_sym, uint64_t n, uint32_t L, uint64_t* 
     a.bpre = bpre;
     a.cum = cum[gi & 1];
     a.words = d_words;
+    a.lvl0 = s0;
+    a.check = gi == 0 ? 1u : 0u;  // the input: no bits at or above L - s0 below the top s0
+    a.chk_shift = d_src_off ? L - s0 : L;
+    a.src_off = gi == 0 ? d_src_off : nullptr;
     a.kout = last ? nullptr : kbuf[gi & 1];
     a.err = ctx->d_flag;
     // grid: an upper bound of the tiles (blocks past the real count exit)
@@ -656,7 +699,7 @@ int mld_build_t(wtg_ctx* ctx, const K* d_sym, uint64_t n, uint32_t L, uint64_t* 
       cum_prev = cum[gi & 1];
     }
   }
-  if (d_nodes && node_slots) {
+  if (d_nodes && node_slots && s0 == 0) {
     WTG_LAUNCH(ctx, "nodes", k_mld_nodes, dim3((unsigned)wtg_ceil_div(node_slots, 256)), dim3(256), 0,
                cum[(P.ngroups - 1) & 1], L, n, d_nodes, node_slots);
     WTG_LAUNCH_CHECK();
@@ -676,13 +719,35 @@ int wtg_mld_enabled() {
   return on;
 }
 
+static int mld_dispatch(wtg_ctx* ctx, const void* d_sym, int sym_bytes, uint64_t n, uint32_t L, uint32_t s0,
+                        const uint32_t* d_cnt0, const uint64_t* d_src_off, uint64_t* d_words, wtg_node* d_nodes,
+                        uint64_t node_slots, unsigned long long* d_distinct) {
+  switch (sym_bytes) {
+    case 1: return mld_build_t<uint8_t>(ctx, (const uint8_t*)d_sym, n, L, s0, d_cnt0, d_src_off, d_words, d_nodes,
+                                        node_slots, d_distinct);
+    case 2: return mld_build_t<uint16_t>(ctx, (const uint16_t*)d_sym, n, L, s0, d_cnt0, d_src_off, d_words, d_nodes,
+                                         node_slots, d_distinct);
+    case 4: return mld_build_t<uint32_t>(ctx, (const uint32_t*)d_sym, n, L, s0, d_cnt0, d_src_off, d_words, d_nodes,
+                                         node_slots, d_distinct);
+    default: return mld_build_t<uint64_t>(ctx, (const uint64_t*)d_sym, n, L, s0, d_cnt0, d_src_off, d_words,
+                                          d_nodes, node_slots, d_distinct);
+  }
+}
+
 // n >= 1, 1 <= L <= 24, n < 2^32
 int wtg_mld_build(wtg_ctx* ctx, const void* d_sym, int sym_bytes, uint64_t n, uint32_t L, uint64_t* d_words,
                   wtg_node* d_nodes, uint64_t node_slots, unsigned long long* d_distinct) {
-  switch (sym_bytes) {
-    case 1: return mld_build_t<uint8_t>(ctx, (const uint8_t*)d_sym, n, L, d_words, d_nodes, node_slots, d_distinct);
-    case 2: return mld_build_t<uint16_t>(ctx, (const uint16_t*)d_sym, n, L, d_words, d_nodes, node_slots, d_distinct);
-    case 4: return mld_build_t<uint32_t>(ctx, (const uint32_t*)d_sym, n, L, d_words, d_nodes, node_slots, d_distinct);
-    default: return mld_build_t<uint64_t>(ctx, (const uint64_t*)d_sym, n, L, d_words, d_nodes, node_slots, d_distinct);
-  }
+  return mld_dispatch(ctx, d_sym, sym_bytes, n, L, 0, nullptr, nullptr, d_words, d_nodes, node_slots, d_distinct);
 }
+
+// levels [s0, L) of keys already sorted by their top s0 bits (d_cnt0: the 2^s0
+// node sizes), level l at bit (l - s0) * n of d_words; 0 <= s0 < L <= 24.
+// d_src_off (nullable, s0 > 0): node v's keys are d_keys[d_src_off[v] ..] and
+// hold only their low L - s0 bits (the node gives the top s0)
+int wtg_mld_build_from(wtg_ctx* ctx, const void* d_keys, int key_bytes, uint64_t n, uint32_t L, uint32_t s0,
+                       const uint32_t* d_cnt0, const uint64_t* d_src_off, uint64_t* d_words) {
+  return mld_dispatch(ctx, d_keys, key_bytes, n, L, s0, d_cnt0, s0 ? d_src_off : nullptr, d_words, nullptr, 0,
+                      nullptr);
+}
+
+size_t wtg_mld_ws_bytes(uint64_t n, uint32_t L, uint32_t s0) { return mld_plan(n, L, s0).total; }
