@@ -439,7 +439,10 @@ def bench_batch(pkg, args, full):
     ctx = pkg.Context(0)
     d = ctx.to_device(sym)
     t0 = time.perf_counter()
-    plan, tune_ms = pkg.Batch.tune(ctx, d, n, sigma, off, reps=2)
+    if args.batch_graph_only:  # one group per alphabet width, no tuning builds (for profiles)
+        plan, tune_ms = np.array([0xFF] + [24 - w for w in range(1, 9)], np.uint8), None
+    else:
+        plan, tune_ms = pkg.Batch.tune(ctx, d, n, sigma, off, reps=2)
     tune_s = time.perf_counter() - t0
     B = pkg.Batch(ctx, d, n, sigma, off, plan=plan)
     L = np.array([pkg.levels_of_sigma(int(x)) for x in sigma])
@@ -455,6 +458,10 @@ def bench_batch(pkg, args, full):
     wo = np.concatenate([[0], np.cumsum([lib.wtg_levels_words(int(k), int(l)) + 1
                                          for k, l in zip(n, L)])]).astype(np.uint64)
     dw = ctx.empty(int(wo[-1]), np.uint64)
+
+    if args.batch_graph_only:
+        return {"value": round(total / (ms / 1e3) / 1e9, 4), "unit": "Gsym/s", "ms_per_step": round(ms, 4),
+                "symbols": total, "note": "graph replays only: no tuner, per-sequence builds or digests"}, True
 
     def per_seq():
         for i in range(nseq):
@@ -492,7 +499,7 @@ def bench_batch(pkg, args, full):
                         "tune_s": round(tune_s, 3)},
            "config": {"workload": "batched levelWT, %d sequences, n_i=floor(2^U(10,%d)), alphabets {4,20,256}"
                                   % (nseq, args.batch_log2max)}}
-    if not full:
+    if not full and tune_ms is not None:
         out["autotune"]["candidates_ms"] = {int(w): [round(float(x), 3) for x in tune_ms[w]]
                                             for w in sorted(set(L.tolist()))}
     if not args.no_cpu:
@@ -694,6 +701,9 @@ def main():
     ap.add_argument("--fm-log2n", type=int, default=27)
     ap.add_argument("--fm-patterns", type=int, default=1_000_000)
     ap.add_argument("--batch-log2max", type=int, default=22)
+    ap.add_argument("--batch-graph-only", action="store_true",
+                    help="C4: time the graph replays only (one group per alphabet width, no tuner, "
+                         "no per-sequence builds or digests), for kernel profiles")
     ap.add_argument("--lsel-queries", type=int, default=100_000_000)
     ap.add_argument("--lsel-tol", type=int, default=480, help="learned select: knot placement tolerance (bits)")
     ap.add_argument("--lsel-gap", type=int, default=4096, help="learned select: longest knot interval (ones)")
