@@ -1,6 +1,7 @@
 """GPU parity of (a), the level-wise wavelet tree (wavelet/levelWT/WT.C:43-227),
 against the oracle (itself pinned to the compiled reference) and the golden
 fixtures written from the compiled reference.  Bit-exact words and nodes."""
+import ctypes
 import os
 
 import numpy as np
@@ -142,3 +143,40 @@ def test_levels_geometric_twenty_levels(wsfm, orc, ctx):
     words, _ = ctx.levels(s, L)
     exp = orc.levels_closed(s, L)
     assert np.array_equal(words, exp)
+
+
+class _At:
+    """A device pointer into another array (an input that starts at any byte)."""
+
+    def __init__(self, arr, first):
+        self.dtype = np.dtype(arr.dtype)
+        self.ptr = ctypes.c_void_p(arr.ptr.value + first * self.dtype.itemsize)
+
+
+@pytest.mark.parametrize("dt,L", [(np.uint8, 8), (np.uint8, 5), (np.uint16, 13), (np.uint32, 20), (np.uint64, 22)])
+def test_levels_unaligned_input(ctx, wsfm, orc, dt, L):
+    """Inputs that start and end inside 16-byte words: the key loads cover the
+    words around them, and symbols outside [first, first + n) (here invalid
+    ones, >= 2^L) are neither counted nor reported."""
+    rng = np.random.default_rng(L)
+    N = 3 * 4096 + 77
+    full = rng.integers(0, 1 << L, N).astype(dt)
+    lw = wsfm.lib().wtg_levels_words
+    for first, n in ((1, 5000), (3, 2048), (5, 9000), (13, 1), (7, 15), (1, N - 2)):
+        s = full[first:first + n].copy()
+        bad = full.copy()
+        if (1 << L) + 1 <= np.iinfo(dt).max:  # invalid symbols right before and after the range
+            bad[first - 1] = (1 << L) + 1
+            if first + n < N:
+                bad[first + n] = (1 << L) + 1
+        say("unaligned", np.dtype(dt).name, L, first, n)
+        db = ctx.to_device(bad)
+        dw = ctx.empty(max(1, lw(n, L)), np.uint64)
+        try:
+            ctx.levels_build(_At(db, first), n, L, dw)
+            got = dw.get()[:lw(n, L)]
+            ow, _ = orc.levels_serial(s, L, with_nodes=False)
+            assert np.array_equal(got, ow), (first, n)
+        finally:
+            dw.free()
+            db.free()

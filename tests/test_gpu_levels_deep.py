@@ -37,28 +37,64 @@ def cases():
     yield "u8_L28", rng.integers(0, 256, 9001).astype(np.uint8), 28
     for n in (1, 2, 63, 64, 65, 257):
         yield "tiny%d" % n, rng.integers(0, 1 << 26, n, dtype=np.uint64).astype(np.uint32), 26
+    # full-width keys (the P_24 sort's bit range ends at the key's top bit), one
+    # block and many
+    yield "u32_L32_small", rng.integers(0, 1 << 32, 300, dtype=np.uint64).astype(np.uint32), 32
+    yield "u64_L64_small", rng.integers(0, np.iinfo(np.uint64).max, 700, dtype=np.uint64), 64
+    yield "u16_L32", rng.integers(0, 1 << 16, 40001).astype(np.uint16), 32
+    yield "u32_L32_2e20", rng.integers(0, 1 << 32, (1 << 20) + 3, dtype=np.uint64).astype(np.uint32), 32
 
 
 CASES = list(cases())
 
 
+@pytest.mark.parametrize("method", [0, 1], ids=["top24_multilevel", "per_level"])
 @pytest.mark.parametrize("case", range(len(CASES)))
-def test_deep_levels(ctx, orc, case):
+def test_deep_levels(ctx, orc, case, method):
+    """method 0: the top 24 levels by the multi-level passes, P_24 by a radix
+    sort, the rest level by level; method 1: every level by the per-level steps."""
     name, s, L = CASES[case]
-    print("deep case", name, s.size, L, flush=True)
+    print("deep case", name, s.size, L, method, flush=True)
     n = s.size
     ds = ctx.to_device(s)
     dw = ctx.empty(max(1, (n * L + 63) // 64), np.uint64)
     try:
+        ctx.set_level_method(method)
         ctx.levels_build(ds, n, L, dw)
         got = dw.get()[:(n * L + 63) // 64]
-        assert np.array_equal(got, closed_form(s, L)[:got.size]), name
+        exp = closed_form(s, L)[:got.size]
+        bad = np.nonzero(got != exp)[0]
+        assert bad.size == 0, "%s: %d words differ, first at level %d" % (name, bad.size, bad[0] * 64 // n if bad.size else -1)
         if L <= 63:
             ow, _ = orc.levels_serial(s, L, with_nodes=False)
             assert np.array_equal(got, ow), name
     finally:
+        ctx.set_level_method(0)
         dw.free()
         ds.free()
+
+
+@pytest.mark.parametrize("method", [0, 1])
+@pytest.mark.parametrize("dt,L", [(np.uint32, 30), (np.uint64, 41)])
+def test_deep_levels_bad_symbol(ctx, wsfm, method, dt, L):
+    """A symbol >= 2^L is reported (by the multi-level count of the top levels,
+    or by the first per-level step) and the context stays usable."""
+    rng = np.random.default_rng(L)
+    n = 50001
+    s = rng.integers(0, 1 << L, n, dtype=np.uint64).astype(dt)
+    s[[7, 30000]] = (1 << L) + 5
+    print("deep bad symbol", np.dtype(dt).name, L, method, flush=True)
+    ctx.set_level_method(method)
+    try:
+        with pytest.raises(wsfm.WtgError):
+            ctx.levels(s, L)
+        ctx.sync()
+        good = s.copy()
+        good[[7, 30000]] = 3
+        w, _ = ctx.levels(good, L)
+        assert np.array_equal(w[:(n * L + 63) // 64], closed_form(good, L)[:(n * L + 63) // 64])
+    finally:
+        ctx.set_level_method(0)
 
 
 def test_deep_levels_reject_nodes(wsfm, ctx):

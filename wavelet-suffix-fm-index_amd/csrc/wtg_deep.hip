@@ -21,7 +21,13 @@
 //                 S + (Ze - Zs) + (i - S) - (zi - Zs) (bit 1): P_{l+1}.
 // Work and memory are linear in n per level whatever the alphabet; this path
 // is chosen only above 24 levels (the multi-level passes are faster below).
+// With the multi-level passes on, the top 24 levels come from them (levels
+// 0..23 depend only on the keys' top 24 bits), P_24 from a stable radix sort
+// of the keys by those 24 bits, and only the levels from 24 on take the
+// per-level steps above.
 #include "wtg_internal.hpp"
+#include <algorithm>
+#include <rocprim/device/device_radix_sort.hpp>
 #include <rocprim/device/device_scan.hpp>
 
 namespace {
@@ -100,15 +106,34 @@ __global__ __launch_bounds__(256) void k_deep_distinct(const K* __restrict__ key
   if (lane_id() == 0 && m) atomicAdd(out, (unsigned long long)popc64(m));
 }
 
+constexpr uint32_t DEEP_TOP = 24;  // levels built by the multi-level passes first
+// the P_24 sort: onesweep (or one block) at every size.  rocPRIM's merge-sort
+// path, which it otherwise takes up to 2^20 keys, masks a bit range that ends
+// at the key's top bit with 1 << width (undefined: here an empty mask)
+using DeepSortCfg = rocprim::radix_sort_config<rocprim::default_config, rocprim::default_config,
+                                               rocprim::default_config, 0>;
+
 template <typename K>
 int deep_build_t(wtg_ctx* ctx, const K* d_sym, uint64_t n, uint32_t L, uint64_t* d_words,
                  unsigned long long* d_distinct) {
-  size_t scan_tmp = 0;
+  const bool top = ctx->level_method == 0 && wtg_mld_enabled();
+  constexpr uint32_t KB = 8 * sizeof(K);
+  // P_24 = the keys stably sorted by bits [L - 24, L) (bits past the key type are 0)
+  const uint32_t sort_lo = L - DEEP_TOP, sort_hi = L < KB ? L : KB;
+  const bool sort = top && sort_lo < sort_hi;
+  size_t scan_tmp = 0, sort_tmp = 0;
   {
     uint64_t* p = nullptr;
     if (rocprim::exclusive_scan(nullptr, scan_tmp, p, p, (uint64_t)0, (size_t)n, rocprim::plus<uint64_t>(),
                                 ctx->stream) != hipSuccess) {
       wtg_set_error("levels (deep): scan size query failed");
+      return WTG_ERR_HIP;
+    }
+    K* q = nullptr;
+    if (sort && rocprim::radix_sort_keys<DeepSortCfg>(nullptr, sort_tmp, (const K*)q, q, (size_t)n, sort_lo,
+                                                      sort_hi, ctx->stream) !=
+                    hipSuccess) {
+      wtg_set_error("levels (deep): sort size query failed");
       return WTG_ERR_HIP;
     }
   }
@@ -120,7 +145,19 @@ int deep_build_t(wtg_ctx* ctx, const K* d_sym, uint64_t n, uint32_t L, uint64_t*
   const size_t o_Zs = wtg_align256(o_S + n * 4);
   const size_t o_Ze = wtg_align256(o_Zs + n * 4);
   const size_t o_tmp = wtg_align256(o_Ze + n * 4);
-  WTG_TRY(wtg_ws_reserve(ctx, o_tmp + scan_tmp + 256));
+  const size_t need = o_tmp + std::max(scan_tmp, sort_tmp) + 256;
+  // one reservation for both steps (the multi-level passes then find it large enough)
+  WTG_TRY(wtg_ws_reserve(ctx, top ? std::max(need, wtg_mld_ws_bytes(n, DEEP_TOP, 0)) : need));
+  uint32_t l0 = 0;
+  if (top) {
+    // levels 0..23 (and their words cleared), then the remaining words cleared
+    WTG_TRY(wtg_mld_build_top(ctx, d_sym, (int)sizeof(K), n, L, DEEP_TOP, d_words));
+    const uint64_t wt = wtg_levels_words(n, DEEP_TOP);
+    WTG_TRY(wtg_fill_bytes_zero(ctx, d_words + wt, (wtg_levels_words(n, L) - wt) * 8));
+    l0 = DEEP_TOP;
+  } else {
+    WTG_TRY(wtg_fill_bytes_zero(ctx, d_words, wtg_levels_words(n, L) * 8));
+  }
   char* ws = (char*)ctx->ws;
   K* kb[2] = {(K*)(ws + o_k0), (K*)(ws + o_k1)};
   uint64_t* fl = (uint64_t*)(ws + o_fl);
@@ -129,10 +166,20 @@ int deep_build_t(wtg_ctx* ctx, const K* d_sym, uint64_t n, uint32_t L, uint64_t*
   uint32_t* Zs = (uint32_t*)(ws + o_Zs);
   uint32_t* Ze = (uint32_t*)(ws + o_Ze);
   void* tmp = ws + o_tmp;
-  WTG_TRY(wtg_fill_bytes_zero(ctx, d_words, wtg_levels_words(n, L) * 8));
   const unsigned grid = (unsigned)wtg_ceil_div(n, 256);
   const K* cur = d_sym;
-  for (uint32_t l = 0; l < L; l++) {
+  if (sort) {
+    // into the buffer the first split below does not write
+    K* out = kb[(l0 & 1) ^ 1];
+    size_t tb = sort_tmp;
+    if (rocprim::radix_sort_keys<DeepSortCfg>(tmp, tb, d_sym, out, (size_t)n, sort_lo, sort_hi, ctx->stream) !=
+        hipSuccess) {
+      wtg_set_error("levels (deep): sort failed");
+      return WTG_ERR_HIP;
+    }
+    cur = out;
+  }
+  for (uint32_t l = l0; l < L; l++) {
     WTG_LAUNCH(ctx, "deep_flags", k_deep_flags<K>, dim3(grid), dim3(256), 0, cur, n, L, l, fl,
                (unsigned long long*)d_words, ctx->d_flag);
     WTG_LAUNCH_CHECK();

@@ -120,6 +120,9 @@ int wtg_mld_enabled();
 int wtg_mld_build_from(wtg_ctx* ctx, const void* d_keys, int key_bytes, uint64_t n, uint32_t L, uint32_t s0,
                        const uint32_t* d_cnt0, const uint64_t* d_src_off, uint64_t* d_words);
 size_t wtg_mld_ws_bytes(uint64_t n, uint32_t L, uint32_t s0);
+// the top Lt <= 24 levels of an L-level tree (keys d_sym >> (L - Lt)), level l at bit l * n
+int wtg_mld_build_top(wtg_ctx* ctx, const void* d_sym, int sym_bytes, uint64_t n, uint32_t L, uint32_t Lt,
+                      uint64_t* d_words);
 // levels 25..64: nodes numbered by rank within the level (wtg_deep.hip)
 int wtg_deep_build(wtg_ctx* ctx, const void* d_sym, int sym_bytes, uint64_t n, uint32_t L, uint64_t* d_words,
                    unsigned long long* d_distinct);

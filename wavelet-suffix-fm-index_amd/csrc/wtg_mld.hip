@@ -68,6 +68,8 @@ struct MldArgs {
   uint32_t check;            // 1: flag input keys with bits at or above chk_shift
   uint32_t chk_shift;
   const uint64_t* src_off;   // null, or (first group) where each level-s node's keys start in `keys`
+  uint32_t kshift;           // first group: the keys are the input values >> kshift (the top L
+                             // levels of a deeper tree); 0 otherwise
   uint32_t* kout;            // P_{s+g} keys as uint32 (emit with SCATTER)
   uint32_t* err;
 };
@@ -113,54 +115,156 @@ __device__ __forceinline__ uint64_t mld_key(const void* keys, uint64_t i) {
   return (uint64_t)reinterpret_cast<const K*>(keys)[i];
 }
 
-// leaf counts of the tiles, one wave per tile (all of its loads in flight at
-// once, no block barrier): LDS atomics per key; a chunk whose keys all share
-// one leaf adds its size once
-constexpr uint32_t MLD_CNT_C = MLD_T / 64;  // 64-key chunks per tile
+// a 16-byte word as a native vector (loads through any address space)
+typedef uint32_t mld_u4 __attribute__((ext_vector_type(4)));
+
+// element e of a 16-byte word of K-sized keys (little endian)
+template <typename K>
+__device__ __forceinline__ uint64_t word_elem(const mld_u4& q, uint32_t e) {
+  const uint32_t c[4] = {q.x, q.y, q.z, q.w};
+  if constexpr (sizeof(K) == 8) {
+    return (uint64_t)c[2 * e] | ((uint64_t)c[2 * e + 1] << 32);
+  } else {
+    constexpr uint32_t B = 8 * sizeof(K), PER = 32 / B;
+    const uint32_t dw = c[e / PER];
+    return B == 32 ? dw : (dw >> (B * (e % PER))) & ((1u << (B & 31)) - 1u);
+  }
+}
+
+// the g-bit leaf of element e: bits [tsh, tsh + g) of the key (gw = bits of
+// that field inside the key's width, for keys narrower than a dword)
+template <typename K>
+__device__ __forceinline__ uint32_t word_leaf(const mld_u4& q, uint32_t e, uint32_t tsh, uint32_t gw, uint32_t gmask) {
+  if constexpr (sizeof(K) == 8) {
+    return (uint32_t)(word_elem<K>(q, e) >> tsh) & gmask;
+  } else {
+    constexpr uint32_t B = 8 * sizeof(K), PER = 32 / B;
+    const uint32_t c[4] = {q.x, q.y, q.z, q.w};
+    return __builtin_amdgcn_ubfe(c[e / PER], B * (e % PER) + tsh, gw);
+  }
+}
+
+// leaf counts of the tiles, one wave per MLD_CNT_TPW<K> consecutive tiles (no
+// block barrier; byte and 16-bit keys take several tiles per wave, so each
+// wave reads 8 KiB).  A tile's keys are read as the aligned 16-byte words
+// that cover them, the loads of all of the wave's tiles in flight at once;
+// counting needs no key order, so every element of every word is counted (LDS
+// atomics; a slot whose 64 elements share one leaf adds once) and the
+// elements of the two edge words that lie outside the tile are taken back
+// afterwards.  The aligned words never leave the pages of the tile's own bytes.
+template <typename K>
+constexpr uint32_t mld_cnt_tpw() { return sizeof(K) >= 4 ? 1u : 4u / (uint32_t)sizeof(K); }
+// a device address as a global-memory pointer (global, not flat, loads)
+typedef const __attribute__((address_space(1))) mld_u4* mld_gword;
+
 template <typename K>
 __global__ __launch_bounds__(256) void k_mld_count(MldArgs a) {
-  __shared__ uint32_t s_cnt_all[4][1u << MLD_GMAX];
+  constexpr uint32_t KW = sizeof(K), E = 16 / KW, TPW = mld_cnt_tpw<K>();
+  constexpr uint32_t NW = (MLD_T * KW / 16 + 1 + 63) / 64;  // words per lane (a tile covers <= T*KW/16 + 1)
+  __shared__ uint32_t s_cnt_all[4][TPW][1u << MLD_GMAX];
   const uint32_t w = (uint32_t)__builtin_amdgcn_readfirstlane((int)(threadIdx.x >> 6));
-  const uint32_t t = blockIdx.x * 4 + w;
-  uint32_t* s_cnt = s_cnt_all[w];
-  uint32_t seg, len, first;
-  uint64_t base;
-  if (!mld_tile(a, t, seg, base, len, first)) return;
-  const uint64_t src = mld_src(a, seg, base);
   const uint32_t G = 1u << a.g, lane = (uint32_t)lane_id();
-  const uint32_t sh = a.L - a.s - a.g;
-  bool bad = false;
-  uint32_t dd[MLD_CNT_C];  // the tile's leaves, all loads issued first
+  uintptr_t b0[TPW], b1[TPW], w0[TPW];
+  uint32_t nw[TPW], tt[TPW];
+  bool ok[TPW];
+  mld_u4 v[TPW][NW];
 #pragma unroll
-  for (uint32_t m = 0; m < MLD_CNT_C; m++) {
-    const uint32_t k = m * 64 + lane;
-    const uint64_t x = k < len ? mld_key<K>(a.keys, src + k) : 0;
-    if (a.check && (x >> a.chk_shift) != 0) bad = true;
-    dd[m] = (uint32_t)(x >> sh) & (G - 1);
-  }
-  for (uint32_t i = lane; i < G; i += 64) s_cnt[i] = 0;
-  __builtin_amdgcn_fence(__ATOMIC_RELEASE, "wavefront");
-  __builtin_amdgcn_wave_barrier();
-  __builtin_amdgcn_fence(__ATOMIC_ACQUIRE, "wavefront");
+  for (uint32_t u = 0; u < TPW; u++) {
+    uint32_t seg, len, first;
+    uint64_t base;
+    tt[u] = (blockIdx.x * 4 + w) * TPW + u;
+    ok[u] = mld_tile(a, tt[u], seg, base, len, first);  // wave-uniform
+    b0[u] = b1[u] = w0[u] = 0;
+    nw[u] = 0;
+    if (ok[u]) {
+      b0[u] = (uintptr_t)a.keys + mld_src(a, seg, base) * KW;
+      b1[u] = b0[u] + (uintptr_t)len * KW;
+      w0[u] = b0[u] & ~(uintptr_t)15;
+      nw[u] = (uint32_t)((b1[u] - w0[u] + 15) >> 4);
+    }
+    const mld_gword wp = (mld_gword)w0[u];
 #pragma unroll
-  for (uint32_t m = 0; m < MLD_CNT_C; m++) {
-    const uint32_t k = m * 64 + lane;
-    if (m * 64 >= len) break;  // wave-uniform
-    const bool act = k < len;
-    const uint32_t d = dd[m];
-    const uint32_t d0 = (uint32_t)__builtin_amdgcn_readfirstlane((int)d);  // lane 0's leaf (lane 0 is active)
-    const uint32_t nact = popc64(wave_ballot(act));
-    if (wave_ballot(act && d != d0) == 0) {
-      if (lane == 0) atomicAdd(&s_cnt[d0], nact);
-    } else if (act) {
-      atomicAdd(&s_cnt[d], 1u);
+    for (uint32_t m = 0; m < NW; m++) {
+      const uint32_t k = m * 64 + lane;
+      v[u][m] = k < nw[u] ? wp[k] : mld_u4{0u, 0u, 0u, 0u};
     }
   }
-  if (bad) atomicOr(a.err, WTG_FLAG_BADSYM);
+  if (!ok[0]) return;
+  const uint32_t tsh = a.L - a.s - a.g + a.kshift;  // the leaf's first bit in the key
+  constexpr uint32_t B = 8 * KW;
+  const uint32_t gw = KW == 8 ? a.g : (tsh >= B ? 0u : (B - tsh < a.g ? B - tsh : a.g));
+  const uint32_t gmask = G - 1;
+  // the bad-key test over whole dwords: bits at or above chk_shift of a key
+  uint32_t cm_lo = 0, cm_hi = 0;
+  if (a.check) {
+    const uint32_t c = a.chk_shift;
+    if constexpr (KW == 8) {
+      cm_lo = c >= 32 ? 0u : ~((1u << c) - 1u);
+      cm_hi = c >= 32 ? ~((1u << (c - 32)) - 1u) : ~0u;
+    } else {
+      const uint32_t km = B == 32 ? ~0u : (1u << (B & 31)) - 1u;
+      const uint32_t hm = c >= B ? 0u : km & ~((1u << c) - 1u);
+      cm_lo = cm_hi = hm * (B == 8 ? 0x01010101u : B == 16 ? 0x00010001u : 1u);
+    }
+  }
+#pragma unroll
+  for (uint32_t u = 0; u < TPW; u++)
+    for (uint32_t i = lane; i < G; i += 64) s_cnt_all[w][u][i] = 0;
   __builtin_amdgcn_fence(__ATOMIC_RELEASE, "wavefront");
   __builtin_amdgcn_wave_barrier();
   __builtin_amdgcn_fence(__ATOMIC_ACQUIRE, "wavefront");
-  for (uint32_t i = lane; i < G; i += 64) a.tcnt_w[(uint64_t)t * G + i] = s_cnt[i];
+  uint32_t acc_lo = 0, acc_hi = 0;  // OR of the interior words (edge words are tested per key below)
+  bool bad = false;
+#pragma unroll
+  for (uint32_t u = 0; u < TPW; u++) {
+    if (!ok[u]) break;  // wave-uniform (later tiles are past the end too)
+    uint32_t* s_cnt = s_cnt_all[w][u];
+#pragma unroll
+    for (uint32_t m = 0; m < NW; m++) {
+      if (m * 64 >= nw[u]) break;  // wave-uniform
+      const uint32_t k = m * 64 + lane;
+      const bool act = k < nw[u];
+      const uint32_t nact = popc64(wave_ballot(act));
+      if (act && k > 0 && k + 1 < nw[u]) {
+        acc_lo |= v[u][m].x | v[u][m].z;
+        acc_hi |= v[u][m].y | v[u][m].w;
+      }
+#pragma unroll
+      for (uint32_t e = 0; e < E; e++) {
+        const uint32_t d = word_leaf<K>(v[u][m], e, tsh, gw, gmask);
+        const uint32_t d0 = (uint32_t)__builtin_amdgcn_readfirstlane((int)d);  // lane 0 is active
+        if (wave_ballot(act && d != d0) == 0) {
+          if (lane == 0) atomicAdd(&s_cnt[d0], nact);
+        } else if (act) {
+          atomicAdd(&s_cnt[d], 1u);
+        }
+      }
+    }
+    // edge words: elements outside [b0, b1) were counted above; the others
+    // get the per-key bad-key test
+    if (lane < 2 && (lane == 0 || nw[u] > 1)) {
+      const uint32_t k = lane == 0 ? 0u : nw[u] - 1;
+      const mld_u4 q = ((mld_gword)w0[u])[k];
+#pragma unroll
+      for (uint32_t e = 0; e < E; e++) {
+        const uintptr_t ea = w0[u] + 16 * (uintptr_t)k + e * KW;
+        if (ea < b0[u] || ea >= b1[u]) {
+          atomicSub(&s_cnt[word_leaf<K>(q, e, tsh, gw, gmask)], 1u);
+        } else if (a.check && (word_elem<K>(q, e) >> a.chk_shift) != 0) {
+          bad = true;
+        }
+      }
+    }
+  }
+  if (bad || ((acc_lo & cm_lo) | (acc_hi & cm_hi)) != 0) atomicOr(a.err, WTG_FLAG_BADSYM);
+  __builtin_amdgcn_fence(__ATOMIC_RELEASE, "wavefront");
+  __builtin_amdgcn_wave_barrier();
+  __builtin_amdgcn_fence(__ATOMIC_ACQUIRE, "wavefront");
+#pragma unroll
+  for (uint32_t u = 0; u < TPW; u++) {
+    if (!ok[u]) break;
+    for (uint32_t i = lane; i < G; i += 64) a.tcnt_w[(uint64_t)tt[u] * G + i] = s_cnt_all[w][u][i];
+  }
 }
 
 // column-wise exclusive prefix of the leaf counts over the tiles, blocks of
@@ -326,19 +430,26 @@ __global__ __launch_bounds__(256) __attribute__((amdgpu_waves_per_eu(8))) void k
       cum_d = a.cum[((uint64_t)seg << g) + d];
     }
     // the tile's keys go straight to registers (level s needs no LDS copy);
-    // all loads are in flight together with the tables above.  Positions
-    // past the tile hold the pad key ~0: a one at every level, in the last
-    // node, so each split leaves it where it is (behind the real keys)
+    // all loads are in flight together with the tables above (unconditional:
+    // positions past the tile load the tile's first key, replaced below)
+    K kr[MLD_CPW];
 #pragma unroll
     for (uint32_t m = 0; m < MLD_CPW; m++) {
       const uint32_t p = (w + 4 * m) * 64 + lane;
-      xk[m] = p < len ? (uint32_t)mld_key<K>(a.keys, src + p) : ~0u;
+      kr[m] = reinterpret_cast<const K*>(a.keys)[src + (p < len ? p : 0u)];
     }
     cpre_d = block_excl_scan_u32(cv, s_wave, ctot);
     uint32_t tot;
     const uint32_t te = block_excl_scan_u32(tv, s_wave, tot);
     if (d < G) s_tpre[d] = (uint16_t)te;
     if (d == 0) s_tpre[G] = (uint16_t)tot;
+    // positions past the tile hold the pad key ~0: a one at every level, in
+    // the last node, so each split leaves it where it is (behind the real keys)
+#pragma unroll
+    for (uint32_t m = 0; m < MLD_CPW; m++) {
+      const uint32_t p = (w + 4 * m) * 64 + lane;
+      xk[m] = p < len ? (uint32_t)((uint64_t)kr[m] >> a.kshift) : ~0u;
+    }
   }
   __syncthreads();
   // node tables of all levels, from the tile's leaf counts alone.  Entry
@@ -574,11 +685,13 @@ MldPlan mld_plan(uint64_t n, uint32_t L, uint32_t s0) {
 
 // Levels [s0, L) of the keys d_sym (in P_{s0} order: sorted by their top s0
 // bits, d_cnt0 = the 2^s0 node sizes; s0 = 0: any order, d_cnt0 unused);
-// level l's bits go to bit (l - s0) * n of d_words.
+// level l's bits go to bit (l - s0) * n of d_words.  kshift > 0 (s0 = 0): the
+// keys are d_sym >> kshift, i.e. these are the top L levels of an
+// (L + kshift)-level tree.
 template <typename K>
 int mld_build_t(wtg_ctx* ctx, const K* d_sym, uint64_t n, uint32_t L, uint32_t s0, const uint32_t* d_cnt0,
                 const uint64_t* d_src_off, uint64_t* d_words, wtg_node* d_nodes, uint64_t node_slots,
-                unsigned long long* d_distinct) {
+                unsigned long long* d_distinct, uint32_t kshift = 0) {
   const MldPlan P = mld_plan(n, L, s0);
   WTG_TRY(wtg_ws_reserve(ctx, P.total));
   char* ws = (char*)ctx->ws;
@@ -640,9 +753,11 @@ int mld_build_t(wtg_ctx* ctx, const K* d_sym, uint64_t n, uint32_t L, uint32_t s
     a.cum = cum[gi & 1];
     a.words = d_words;
     a.lvl0 = s0;
-    a.check = gi == 0 ? 1u : 0u;  // the input: no bits at or above L - s0 below the top s0
-    a.chk_shift = d_src_off ? L - s0 : L;
+    // the input: no bits at or above L - s0 below the top s0 (none above bit 63)
+    a.check = gi == 0 && L + kshift < 64 ? 1u : 0u;
+    a.chk_shift = d_src_off ? L - s0 : L + kshift;
     a.src_off = gi == 0 ? d_src_off : nullptr;
+    a.kshift = gi == 0 ? kshift : 0u;
     a.kout = last ? nullptr : kbuf[gi & 1];
     a.err = ctx->d_flag;
     // grid: an upper bound of the tiles (blocks past the real count exit)
@@ -650,7 +765,8 @@ int mld_build_t(wtg_ctx* ctx, const K* d_sym, uint64_t n, uint32_t L, uint32_t s
     const uint32_t nrows = (uint32_t)grid + 1;
     const uint32_t nblocks = (uint32_t)wtg_ceil_div(nrows, MLD_SR);
     if (gi == 0) {
-      WTG_LAUNCH(ctx, "mld_count", k_mld_count<K>, dim3((unsigned)wtg_ceil_div(grid, 4)), dim3(256), 0, a);
+      WTG_LAUNCH(ctx, "mld_count", k_mld_count<K>, dim3((unsigned)wtg_ceil_div(grid, 4 * mld_cnt_tpw<K>())),
+                 dim3(256), 0, a);
     } else {
       WTG_LAUNCH(ctx, "mld_count", k_mld_count<uint32_t>, dim3((unsigned)wtg_ceil_div(grid, 4)), dim3(256), 0, a);
     }
@@ -721,16 +837,16 @@ int wtg_mld_enabled() {
 
 static int mld_dispatch(wtg_ctx* ctx, const void* d_sym, int sym_bytes, uint64_t n, uint32_t L, uint32_t s0,
                         const uint32_t* d_cnt0, const uint64_t* d_src_off, uint64_t* d_words, wtg_node* d_nodes,
-                        uint64_t node_slots, unsigned long long* d_distinct) {
+                        uint64_t node_slots, unsigned long long* d_distinct, uint32_t kshift = 0) {
   switch (sym_bytes) {
     case 1: return mld_build_t<uint8_t>(ctx, (const uint8_t*)d_sym, n, L, s0, d_cnt0, d_src_off, d_words, d_nodes,
-                                        node_slots, d_distinct);
+                                        node_slots, d_distinct, kshift);
     case 2: return mld_build_t<uint16_t>(ctx, (const uint16_t*)d_sym, n, L, s0, d_cnt0, d_src_off, d_words, d_nodes,
-                                         node_slots, d_distinct);
+                                         node_slots, d_distinct, kshift);
     case 4: return mld_build_t<uint32_t>(ctx, (const uint32_t*)d_sym, n, L, s0, d_cnt0, d_src_off, d_words, d_nodes,
-                                         node_slots, d_distinct);
+                                         node_slots, d_distinct, kshift);
     default: return mld_build_t<uint64_t>(ctx, (const uint64_t*)d_sym, n, L, s0, d_cnt0, d_src_off, d_words,
-                                          d_nodes, node_slots, d_distinct);
+                                          d_nodes, node_slots, d_distinct, kshift);
   }
 }
 
@@ -751,3 +867,10 @@ int wtg_mld_build_from(wtg_ctx* ctx, const void* d_keys, int key_bytes, uint64_t
 }
 
 size_t wtg_mld_ws_bytes(uint64_t n, uint32_t L, uint32_t s0) { return mld_plan(n, L, s0).total; }
+
+// the top Lt levels (Lt <= 24) of an L-level tree over d_sym (L > Lt): the
+// multi-level passes over the keys' top Lt bits; level l at bit l * n
+int wtg_mld_build_top(wtg_ctx* ctx, const void* d_sym, int sym_bytes, uint64_t n, uint32_t L, uint32_t Lt,
+                      uint64_t* d_words) {
+  return mld_dispatch(ctx, d_sym, sym_bytes, n, Lt, 0, nullptr, nullptr, d_words, nullptr, 0, nullptr, L - Lt);
+}
