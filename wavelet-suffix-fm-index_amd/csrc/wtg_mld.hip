@@ -152,12 +152,22 @@ __device__ __forceinline__ uint32_t word_leaf(const mld_u4& q, uint32_t e, uint3
 // atomics; a slot whose 64 elements share one leaf adds once) and the
 // elements of the two edge words that lie outside the tile are taken back
 // afterwards.  The aligned words never leave the pages of the tile's own bytes.
+// Trees of at most SG leaves per tile (SG > 0) count by ballots instead: per
+// slot and leaf one ballot of the lanes holding it, the counts in scalar
+// registers (a few leaves would otherwise put every lane's atomic on one of a
+// few LDS words).
 template <typename K>
 constexpr uint32_t mld_cnt_tpw() { return sizeof(K) >= 4 ? 1u : 4u / (uint32_t)sizeof(K); }
+
+__device__ __forceinline__ void wave_sync_lds_mld() {
+  __builtin_amdgcn_fence(__ATOMIC_RELEASE, "wavefront");
+  __builtin_amdgcn_wave_barrier();
+  __builtin_amdgcn_fence(__ATOMIC_ACQUIRE, "wavefront");
+}
 // a device address as a global-memory pointer (global, not flat, loads)
 typedef const __attribute__((address_space(1))) mld_u4* mld_gword;
 
-template <typename K>
+template <typename K, uint32_t SG>
 __global__ __launch_bounds__(256) void k_mld_count(MldArgs a) {
   constexpr uint32_t KW = sizeof(K), E = 16 / KW, TPW = mld_cnt_tpw<K>();
   constexpr uint32_t NW = (MLD_T * KW / 16 + 1 + 63) / 64;  // words per lane (a tile covers <= T*KW/16 + 1)
@@ -219,6 +229,9 @@ __global__ __launch_bounds__(256) void k_mld_count(MldArgs a) {
   for (uint32_t u = 0; u < TPW; u++) {
     if (!ok[u]) break;  // wave-uniform (later tiles are past the end too)
     uint32_t* s_cnt = s_cnt_all[w][u];
+    uint32_t sc[SG > 0 ? SG : 1];  // SG > 0: the counts per leaf (wave-uniform)
+#pragma unroll
+    for (uint32_t d = 0; d < (SG > 0 ? SG : 1); d++) sc[d] = 0;
 #pragma unroll
     for (uint32_t m = 0; m < NW; m++) {
       if (m * 64 >= nw[u]) break;  // wave-uniform
@@ -232,13 +245,24 @@ __global__ __launch_bounds__(256) void k_mld_count(MldArgs a) {
 #pragma unroll
       for (uint32_t e = 0; e < E; e++) {
         const uint32_t d = word_leaf<K>(v[u][m], e, tsh, gw, gmask);
-        const uint32_t d0 = (uint32_t)__builtin_amdgcn_readfirstlane((int)d);  // lane 0 is active
-        if (wave_ballot(act && d != d0) == 0) {
-          if (lane == 0) atomicAdd(&s_cnt[d0], nact);
-        } else if (act) {
-          atomicAdd(&s_cnt[d], 1u);
+        if constexpr (SG > 0) {
+#pragma unroll
+          for (uint32_t q = 0; q < SG; q++) sc[q] += popc64(wave_ballot(act && d == q));
+        } else {
+          const uint32_t d0 = (uint32_t)__builtin_amdgcn_readfirstlane((int)d);  // lane 0 is active
+          if (wave_ballot(act && d != d0) == 0) {
+            if (lane == 0) atomicAdd(&s_cnt[d0], nact);
+          } else if (act) {
+            atomicAdd(&s_cnt[d], 1u);
+          }
         }
       }
+    }
+    if constexpr (SG > 0) {
+#pragma unroll
+      for (uint32_t q = 0; q < SG; q++)
+        if (lane == q && q < G) s_cnt[q] = sc[q];
+      wave_sync_lds_mld();
     }
     // edge words: elements outside [b0, b1) were counted above; the others
     // get the per-key bad-key test
@@ -765,10 +789,15 @@ int mld_build_t(wtg_ctx* ctx, const K* d_sym, uint64_t n, uint32_t L, uint32_t s
     const uint32_t nrows = (uint32_t)grid + 1;
     const uint32_t nblocks = (uint32_t)wtg_ceil_div(nrows, MLD_SR);
     if (gi == 0) {
-      WTG_LAUNCH(ctx, "mld_count", k_mld_count<K>, dim3((unsigned)wtg_ceil_div(grid, 4 * mld_cnt_tpw<K>())),
-                 dim3(256), 0, a);
+      const dim3 cg((unsigned)wtg_ceil_div(grid, 4 * mld_cnt_tpw<K>()));
+      if (G <= 4) WTG_LAUNCH(ctx, "mld_count", (k_mld_count<K, 4>), cg, dim3(256), 0, a);
+      else if (G <= 8) WTG_LAUNCH(ctx, "mld_count", (k_mld_count<K, 8>), cg, dim3(256), 0, a);
+      else WTG_LAUNCH(ctx, "mld_count", (k_mld_count<K, 0>), cg, dim3(256), 0, a);
     } else {
-      WTG_LAUNCH(ctx, "mld_count", k_mld_count<uint32_t>, dim3((unsigned)wtg_ceil_div(grid, 4)), dim3(256), 0, a);
+      const dim3 cg((unsigned)wtg_ceil_div(grid, 4));
+      if (G <= 4) WTG_LAUNCH(ctx, "mld_count", (k_mld_count<uint32_t, 4>), cg, dim3(256), 0, a);
+      else if (G <= 8) WTG_LAUNCH(ctx, "mld_count", (k_mld_count<uint32_t, 8>), cg, dim3(256), 0, a);
+      else WTG_LAUNCH(ctx, "mld_count", (k_mld_count<uint32_t, 0>), cg, dim3(256), 0, a);
     }
     WTG_LAUNCH_CHECK();
     WTG_LAUNCH(ctx, "mld_scan", k_mld_scan1, dim3(nblocks), dim3(256), 0, tcnt, ploc, bsum,
