@@ -432,6 +432,7 @@ __global__ __launch_bounds__(256) __attribute__((amdgpu_waves_per_eu(8))) void k
   __shared__ uint32_t s_nxy[(1u << MLD_GMAX) - 1];    // node tables of the levels (see below)
   __shared__ uint32_t s_lvl0[MLD_GMAX];
   __shared__ uint32_t s_wave[4];
+  __shared__ uint16_t s_ex[4][MLD_T / 64];  // per wave: zeros before each chunk of the level
   uint32_t seg, len, first;
   uint64_t base;
   const uint32_t t = blockIdx.x;
@@ -523,11 +524,16 @@ __global__ __launch_bounds__(256) __attribute__((amdgpu_waves_per_eu(8))) void k
     }
     if (j + 1 == g && !SCATTER) break;  // the last level's order is not needed
     __syncthreads();
-    // every wave: exclusive prefix of the chunk zeros (lane = chunk); zeros
-    // before tile position p = that prefix + zeros of p's chunk below p
+    // every wave: exclusive prefix of the chunk zeros (lane = chunk), into the
+    // wave's own row so that each chunk reads its base with one broadcast
+    // load; zeros before tile position p = that prefix + zeros of p's chunk
+    // below p
     const uint32_t v = lane < nch ? 64u - popc64(reinterpret_cast<const uint64_t*>(s_bits[j])[lane]) : 0u;
     const uint32_t incl = wave_incl_scan_dpp(v);
-    const uint32_t ex = incl - v;
+    if (lane < MLD_T / 64) s_ex[w][lane] = (uint16_t)(incl - v);
+    __builtin_amdgcn_fence(__ATOMIC_RELEASE, "wavefront");
+    __builtin_amdgcn_wave_barrier();
+    __builtin_amdgcn_fence(__ATOMIC_ACQUIRE, "wavefront");
     // stable split of each node: its zeros first, then its ones.  Every
     // chunk of the tile takes part (chunks past the keys hold pad keys,
     // which stay in place), so the node lookups of all chunks are issued
@@ -543,9 +549,10 @@ __global__ __launch_bounds__(256) __attribute__((amdgpu_waves_per_eu(8))) void k
       const uint32_t x = xk[m];
       const bool zero = ((x >> bp) & 1u) == 0;
       const uint64_t zm = wave_ballot(zero);
-      const uint32_t cz = (uint32_t)__builtin_amdgcn_readlane((int)ex, (int)c);
+      const uint32_t cz = s_ex[w][c];
       const uint32_t zc = __builtin_amdgcn_mbcnt_hi((uint32_t)(zm >> 32), __builtin_amdgcn_mbcnt_lo((uint32_t)zm, cz));
-      B[zero ? (xy[m] & 0xFFFFu) + zc : (xy[m] >> 16) + p - zc] = x;
+      const uint32_t d0 = (xy[m] & 0xFFFFu) + zc, d1 = (xy[m] >> 16) + p - zc;
+      B[zero ? d0 : d1] = x;
     }
     __syncthreads();
   }
