@@ -73,20 +73,31 @@ __device__ __forceinline__ double lsel_line(double alpha, double beta, double j)
   return alpha * j + beta;
 }
 
-// the model's prediction for argument j (1 <= j <= ones), clamped to the vector
-__device__ __forceinline__ uint64_t lsel_predict(const LselDesc& d, uint64_t j) {
-  uint32_t k = d.lut[j >> d.lut_log];
-  Knot a = d.knots[k], b = d.knots[k + 1];
-  while (j >= b.j && k + 2 < d.nknots) {
-    ++k;
-    a = b;
-    b = d.knots[k + 1];
-  }
+// the model's prediction for argument j inside knot interval [a.j, b.j) (the
+// last interval also takes j = b.j), clamped to the vector
+__device__ __forceinline__ uint64_t lsel_eval(const LselDesc& d, const Knot& a, const Knot& b, uint64_t j) {
   const float f = (float)(j - a.j) / (float)(b.j - a.j);
   const float s = (1.0f - f) * a.y + f * b.y;
   const double p = lsel_line(d.alpha, d.beta, (double)j) + (double)s;
   const double hi = (double)(d.nbits - 1);
   return p <= 0.0 ? 0ull : p >= hi ? d.nbits - 1 : (uint64_t)llrint(p);
+}
+
+// move interval k (knots a, b) forward until it holds argument j
+__device__ __forceinline__ void lsel_advance(const LselDesc& d, uint64_t j, uint32_t& k, Knot& a, Knot& b) {
+  while (j >= b.j && k + 2 < d.nknots) {
+    ++k;
+    a = b;
+    b = d.knots[k + 1];
+  }
+}
+
+// the model's prediction for argument j (1 <= j <= ones), clamped to the vector
+__device__ __forceinline__ uint64_t lsel_predict(const LselDesc& d, uint64_t j) {
+  uint32_t k = d.lut[j >> d.lut_log];
+  Knot a = d.knots[k], b = d.knots[k + 1];
+  lsel_advance(d, j, k, a, b);
+  return lsel_eval(d, a, b, j);
 }
 
 // exact select1(j) from a predicted position p: the rank9 entries of blocks
@@ -428,7 +439,9 @@ struct TrainBufs {
 };
 
 // forward + backward: samples j = T*u in [J_k, J_{k+1}) (the last interval
-// includes J_{K-1}); samples outside [J_0, J_{K-1}] are not seen
+// includes J_{K-1}); samples outside [J_0, J_{K-1}] are not seen.  Thread per
+// interval; the sample positions are loaded four at a time before they are
+// used (the sums run in sample order).
 __global__ __launch_bounds__(256) void k_lsel_fwd_bwd(const TrainVec* __restrict__ tv, uint32_t nv, TrainBufs tb,
                                                       uint64_t nk_total) {
   const uint64_t g = (uint64_t)blockIdx.x * blockDim.x + threadIdx.x;
@@ -449,14 +462,21 @@ __global__ __launch_bounds__(256) void k_lsel_fwd_bwd(const TrainVec* __restrict
   if (t2 > v.ns) t2 = v.ns;
   const float span = (float)(b.j - a.j);
   float sa = 0.f, sc = 0.f, e2 = 0.f;
-  for (uint64_t u = t1; u <= t2; u++) {
-    const uint64_t j = u * T;
-    const float f = (float)(j - a.j) / span;
-    const float target = (float)((double)v.pos[u - 1] - lsel_line(v.alpha, v.beta, (double)j));
-    const float e = (1.0f - f) * a.y + f * b.y - target;
-    sa += e * (1.0f - f);
-    sc += e * f;
-    e2 += e * e;
+  for (uint64_t u0 = t1; u0 <= t2; u0 += 4) {
+    uint64_t pv[4];
+#pragma unroll
+    for (uint32_t q = 0; q < 4; q++) pv[q] = u0 + q <= t2 ? v.pos[u0 + q - 1] : 0ull;
+#pragma unroll
+    for (uint32_t q = 0; q < 4; q++) {
+      if (u0 + q > t2) break;
+      const uint64_t j = (u0 + q) * T;
+      const float f = (float)(j - a.j) / span;
+      const float target = (float)((double)pv[q] - lsel_line(v.alpha, v.beta, (double)j));
+      const float e = (1.0f - f) * a.y + f * b.y - target;
+      sa += e * (1.0f - f);
+      sc += e * f;
+      e2 += e * e;
+    }
   }
   tb.gl[g] = sa;
   tb.gr[g] = sc;
@@ -509,8 +529,12 @@ __global__ __launch_bounds__(256) void k_lsel_loss(const TrainVec* __restrict__ 
   if (threadIdx.x == 0) out[b] = s[0];
 }
 
-// exhaustive error window of vector vids[blockIdx.y]: thread per word, every
-// one of the word
+// exhaustive error window of vector vids[blockIdx.y]: thread per run of
+// LSEL_EPS_W consecutive words, every one of them.  The ones of a run have
+// consecutive arguments, so the run looks its knot interval up once (at its
+// first one) and moves it forward from there: the interval lsel_predict finds
+// for each argument, and the same arithmetic (lsel_eval).
+constexpr uint32_t LSEL_EPS_W = 16;
 __global__ __launch_bounds__(256) void k_lsel_eps(const LselDesc* __restrict__ descs,
                                                   const uint32_t* __restrict__ vids,
                                                   unsigned long long* __restrict__ eps) {
@@ -518,32 +542,70 @@ __global__ __launch_bounds__(256) void k_lsel_eps(const LselDesc* __restrict__ d
   const LselDesc d = descs[b];
   const uint64_t W = (d.nbits + 63) >> 6;
   uint64_t mx = 0;
-  for (uint64_t w = (uint64_t)blockIdx.x * blockDim.x + threadIdx.x; w < W;
-       w += (uint64_t)gridDim.x * blockDim.x) {
-    uint64_t x = d.words[w];
-    if (!x) continue;
-    uint64_t j = rank9_dev(d.dir, d.words, w << 6);
-    while (x) {
-      const uint32_t o = (uint32_t)__builtin_ctzll(x);
-      x &= x - 1;
-      j++;
-      const uint64_t p = (w << 6) + o;
-      const uint64_t q = lsel_predict(d, j);
-      const uint64_t e = q > p ? q - p : p - q;
-      mx = e > mx ? e : mx;
+  for (uint64_t w0 = ((uint64_t)blockIdx.x * blockDim.x + threadIdx.x) * LSEL_EPS_W; w0 < W;
+       w0 += (uint64_t)gridDim.x * blockDim.x * LSEL_EPS_W) {
+    const uint64_t w1 = w0 + LSEL_EPS_W < W ? w0 + LSEL_EPS_W : W;
+    uint64_t j = rank9_dev(d.dir, d.words, w0 << 6);  // ones before the run
+    uint32_t k = 0;
+    Knot ka{0u, 0.f}, kb{0u, 0.f};
+    bool have = false;
+    for (uint64_t w = w0; w < w1; w++) {
+      uint64_t x = d.words[w];
+      while (x) {
+        const uint32_t o = (uint32_t)__builtin_ctzll(x);
+        x &= x - 1;
+        j++;
+        if (!have) {
+          k = d.lut[j >> d.lut_log];
+          ka = d.knots[k];
+          kb = d.knots[k + 1];
+          have = true;
+        }
+        lsel_advance(d, j, k, ka, kb);
+        const uint64_t p = (w << 6) + o;
+        const uint64_t q = lsel_eval(d, ka, kb, j);
+        const uint64_t e = q > p ? q - p : p - q;
+        mx = e > mx ? e : mx;
+      }
     }
   }
   mx = wave_max_u64(mx);
   if (lane_id() == 0 && mx) atomicMax(&eps[b], (unsigned long long)mx);
 }
 
-// training samples: pos[t] = select1(T*(t+1)) by the plain sampled select
-__global__ __launch_bounds__(256) void k_lsel_samples(const LselDesc* __restrict__ descs, uint32_t b,
-                                                      uint32_t tlog, uint64_t ns, uint64_t* __restrict__ pos) {
-  const LselDesc& d = descs[b];
-  for (uint64_t t = (uint64_t)blockIdx.x * blockDim.x + threadIdx.x; t < ns;
-       t += (uint64_t)gridDim.x * blockDim.x)
-    pos[t] = select_plain(d.s1, (t + 1) << tlog);
+// training samples of every vector in one launch: pos[u-1] = select1(u*T)
+// for u = 1..ns, found by walking the words (thread per run of LSEL_EPS_W
+// words from its rank9 count: the samples whose argument falls in a word are
+// selected inside it); the first and last also to ends[2b], ends[2b+1]
+struct SampleVec {
+  uint64_t* pos;
+  uint64_t ns;
+};
+__global__ __launch_bounds__(256) void k_lsel_samples(const LselDesc* __restrict__ descs,
+                                                      const uint32_t* __restrict__ vids,
+                                                      const SampleVec* __restrict__ sv, uint32_t tlog,
+                                                      uint64_t* __restrict__ ends) {
+  const uint32_t b = vids[blockIdx.y];
+  const LselDesc d = descs[b];
+  const SampleVec s = sv[blockIdx.y];
+  const uint64_t W = (d.nbits + 63) >> 6, T = 1ull << tlog;
+  for (uint64_t w0 = ((uint64_t)blockIdx.x * blockDim.x + threadIdx.x) * LSEL_EPS_W; w0 < W;
+       w0 += (uint64_t)gridDim.x * blockDim.x * LSEL_EPS_W) {
+    const uint64_t w1 = w0 + LSEL_EPS_W < W ? w0 + LSEL_EPS_W : W;
+    uint64_t j = rank9_dev(d.dir, d.words, w0 << 6);  // ones before the word
+    for (uint64_t w = w0; w < w1; w++) {
+      const uint64_t x = d.words[w];
+      const uint64_t c = (uint64_t)__builtin_popcountll(x);
+      // arguments u*T in (j, j + c]
+      for (uint64_t u = (j >> tlog) + 1; u * T <= j + c && u <= s.ns; u++) {
+        const uint64_t p = (w << 6) + select_in_word(x, (uint32_t)(u * T - j - 1));
+        s.pos[u - 1] = p;
+        if (u == 1) ends[2 * b] = p;
+        if (u == s.ns) ends[2 * b + 1] = p;
+      }
+      j += c;
+    }
+  }
 }
 
 }  // namespace
@@ -704,8 +766,7 @@ static int lsel_models(wtg_ctx* ctx, wtg_lsel* ls, const wtg_lsel_params& P, con
   WTG_CHECK_HIP(hipMemcpyAsync(d_act, act.data(), na * 4, hipMemcpyHostToDevice, ctx->stream));
   uint64_t wmax = 1;
   for (uint32_t a = 0; a < na; a++) wmax = std::max<uint64_t>(wmax, (nbits[act[a]] + 63) / 64);
-  const unsigned ge_x = (unsigned)std::max<uint64_t>(1, std::min<uint64_t>(wtg_ceil_div(wmax, 256),
-                                                                          std::max<uint64_t>(4096 / na, 32)));
+  const unsigned ge_x = (unsigned)std::max<uint64_t>(1, std::min<uint64_t>(wtg_ceil_div(wmax, 256 * LSEL_EPS_W), 256));
   auto measure = [&](std::vector<uint64_t>& eps, std::vector<double>& loss) -> int {
     WTG_TRY(wtg_fill_u64(ctx, d_eps, nvec, 0));
     WTG_LAUNCH(ctx, "lsel_fwd_bwd", k_lsel_fwd_bwd, dim3(gk), dim3(256), 0, d_tv, na, tb, nk_total);
@@ -808,26 +869,46 @@ extern "C" int wtg_lsel_build(wtg_ctx* ctx, uint32_t nvec, const uint64_t* const
   if (rc != WTG_OK) goto fail;
   if ((rc = wtg_malloc(ctx, (void**)&ls->d_desc, nvec * sizeof(LselDesc))) != WTG_OK) goto fail;
   if ((rc = lsel_upload_desc(ctx, ls)) != WTG_OK) goto fail;
-  // 2. training samples and the baseline line
-  for (uint32_t b = 0; b < nvec; b++) {
-    ns[b] = ls->desc[b].ones >> tlog;
-    if (ns[b] < 2 || ls->desc[b].ones >= (1ull << 32)) continue;  // too few ones (or u32 knots): plain select
-    if ((rc = alloc_block(ctx, ls, ns[b] * 8, (void**)&pos[b])) != WTG_OK) goto fail;
-    const unsigned g = (unsigned)std::min<uint64_t>(wtg_ceil_div(ns[b], 256), 4096);
-    WTG_LAUNCH(ctx, "lsel_samples", k_lsel_samples, dim3(g), dim3(256), 0, ls->d_desc, b, tlog, ns[b], pos[b]);
-    WTG_LAUNCH_CHECK();
-    uint64_t first = 0, last = 0;
-    if (hipMemcpyAsync(&first, pos[b], 8, hipMemcpyDeviceToHost, ctx->stream) != hipSuccess ||
-        hipMemcpyAsync(&last, pos[b] + ns[b] - 1, 8, hipMemcpyDeviceToHost, ctx->stream) != hipSuccess ||
-        hipStreamSynchronize(ctx->stream) != hipSuccess) {
-      wtg_set_error("wtg_lsel_build: sample copy failed");
-      rc = WTG_ERR_HIP;
-      goto fail;
+  // 2. training samples of every vector (one launch), then (one host read of
+  //    the first and last samples) the baseline lines
+  {
+    uint64_t* d_ends = nullptr;
+    if ((rc = alloc_block(ctx, ls, nvec * 16, (void**)&d_ends)) != WTG_OK) goto fail;
+    std::vector<SampleVec> sv;
+    uint64_t wmax = 1;
+    for (uint32_t b = 0; b < nvec; b++) {
+      ns[b] = ls->desc[b].ones >> tlog;
+      if (ns[b] < 2 || ls->desc[b].ones >= (1ull << 32)) continue;  // too few ones (or u32 knots): plain select
+      if ((rc = alloc_block(ctx, ls, ns[b] * 8, (void**)&pos[b])) != WTG_OK) goto fail;
+      act.push_back(b);
+      sv.push_back(SampleVec{pos[b], ns[b]});
+      wmax = std::max<uint64_t>(wmax, (nbits[b] + 63) / 64);
     }
-    const double j0 = (double)P.train_stride, j1 = (double)(ns[b] << tlog);
-    ls->desc[b].alpha = ((double)last - (double)first) / (j1 - j0);
-    ls->desc[b].beta = (double)first - ls->desc[b].alpha * j0;
-    act.push_back(b);
+    if (!act.empty()) {
+      SampleVec* d_sv = nullptr;
+      uint32_t* d_vids = nullptr;
+      if ((rc = alloc_block(ctx, ls, sv.size() * sizeof(SampleVec), (void**)&d_sv)) != WTG_OK) goto fail;
+      if ((rc = alloc_block(ctx, ls, act.size() * 4, (void**)&d_vids)) != WTG_OK) goto fail;
+      if (hipMemcpyAsync(d_sv, sv.data(), sv.size() * sizeof(SampleVec), hipMemcpyHostToDevice, ctx->stream) !=
+              hipSuccess ||
+          hipMemcpyAsync(d_vids, act.data(), act.size() * 4, hipMemcpyHostToDevice, ctx->stream) != hipSuccess) {
+        wtg_set_error("wtg_lsel_build: sample table copy failed");
+        rc = WTG_ERR_HIP;
+        goto fail;
+      }
+      const unsigned gx = (unsigned)std::max<uint64_t>(1, std::min<uint64_t>(wtg_ceil_div(wmax, 256 * LSEL_EPS_W), 256));
+      WTG_LAUNCH(ctx, "lsel_samples", k_lsel_samples, dim3(gx, (unsigned)act.size()), dim3(256), 0, ls->d_desc,
+                 d_vids, d_sv, tlog, d_ends);
+      WTG_LAUNCH_CHECK();
+    }
+    std::vector<uint64_t> ends(2 * nvec, 0);
+    if (!act.empty() && (rc = sync_copy(ctx, ends.data(), d_ends, nvec * 16, hipMemcpyDeviceToHost)) != WTG_OK)
+      goto fail;
+    for (const uint32_t b : act) {
+      const double j0 = (double)P.train_stride, j1 = (double)(ns[b] << tlog);
+      ls->desc[b].alpha = ((double)ends[2 * b + 1] - (double)ends[2 * b]) / (j1 - j0);
+      ls->desc[b].beta = (double)ends[2 * b] - ls->desc[b].alpha * j0;
+    }
   }
   if (!act.empty() && (rc = lsel_models(ctx, ls, P, nbits, pos, ns, act)) != WTG_OK) goto fail;
   if ((rc = wtg_finish(ctx)) != WTG_OK) goto fail;
