@@ -97,8 +97,9 @@ uint64_t wtg_levels_words(uint64_t n, uint32_t levels);
  *   children 2v+1 / 2v+2); reached nodes as in the reference, other slots
  *   {0, 0, WTG_NONE, WTG_NONE, WTG_NONE}.
  * Limits: 1 <= levels <= 64, n < 2^32; d_nodes only up to 24 levels (above, pass
- * NULL: WTG_ERR_UNSUPPORTED otherwise).  Above 24 levels nodes are numbered by
- * their rank within the level (linear work per level) instead of by prefix. */
+ * NULL: WTG_ERR_UNSUPPORTED otherwise).  Above 24 levels the top 24 come from the
+ * multi-level passes over the keys' top 24 bits and the rest level by level, with
+ * nodes numbered by their rank within the level (linear work per level). */
 int wtg_levels_build(wtg_ctx* ctx, const void* d_sym, int sym_bytes, uint64_t n,
                      uint32_t levels, uint64_t* d_words, wtg_node* d_nodes,
                      uint64_t node_slots);
