@@ -170,6 +170,44 @@ __device__ __forceinline__ U wave_incl_scan(U v) {
   return v;
 }
 
+// Inclusive scans over the 64 lanes with DPP moves (no LDS round trip): row_shr
+// 1, 2, 4, 8 scan each 16-lane row (lanes with no source in the row take the
+// identity), then row_bcast 15 combines row 0's last lane into row 1 and row 2's
+// into row 3, and row_bcast 31 the last lane of rows 0-1 into rows 2-3.
+template <int CTRL, int RM>
+__device__ __forceinline__ uint32_t dpp_u32(uint32_t ident, uint32_t v) {
+  return (uint32_t)__builtin_amdgcn_update_dpp((int)ident, (int)v, CTRL, RM, 0xf, false);
+}
+template <int CTRL, int RM>
+__device__ __forceinline__ double dpp_f64(double ident, double v) {
+  const uint64_t u = (uint64_t)__double_as_longlong(v), o = (uint64_t)__double_as_longlong(ident);
+  const uint32_t lo = dpp_u32<CTRL, RM>((uint32_t)o, (uint32_t)u);
+  const uint32_t hi = dpp_u32<CTRL, RM>((uint32_t)(o >> 32), (uint32_t)(u >> 32));
+  return __longlong_as_double((long long)(((uint64_t)hi << 32) | lo));
+}
+#define WTG_DPP_SCAN_STEPS(STEP) STEP(0x111, 0xf) STEP(0x112, 0xf) STEP(0x114, 0xf) STEP(0x118, 0xf) \
+  STEP(0x142, 0xa) STEP(0x143, 0xc)
+__device__ __forceinline__ uint32_t wave_incl_sum_dpp(uint32_t v) {
+#define WTG_STEP(C, R) v += dpp_u32<C, R>(0u, v);
+  WTG_DPP_SCAN_STEPS(WTG_STEP)
+#undef WTG_STEP
+  return v;
+}
+__device__ __forceinline__ int32_t wave_incl_max_dpp(int32_t v) {
+#define WTG_STEP(C, R) { const int32_t t = (int32_t)dpp_u32<C, R>(0x80000000u, (uint32_t)v); v = t > v ? t : v; }
+  WTG_DPP_SCAN_STEPS(WTG_STEP)
+#undef WTG_STEP
+  return v;
+}
+// running max of a and min of b over the lanes (t > a ? t : a: ties keep the
+// lane's own value, as a sequential walk does)
+__device__ __forceinline__ void wave_incl_maxmin_f64_dpp(double& a, double& b) {
+#define WTG_STEP(C, R) { const double ta = dpp_f64<C, R>(-1e300, a), tb = dpp_f64<C, R>(1e300, b); \
+    a = ta > a ? ta : a; b = tb < b ? tb : b; }
+  WTG_DPP_SCAN_STEPS(WTG_STEP)
+#undef WTG_STEP
+}
+
 // position (0..63) of the k-th (0-based) set bit of x; x must have > k ones
 __device__ __forceinline__ uint32_t select_in_word(uint64_t x, uint32_t k) {
   uint32_t pos = 0;

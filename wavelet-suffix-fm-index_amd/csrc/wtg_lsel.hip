@@ -253,12 +253,6 @@ __device__ __forceinline__ uint64_t shfl_u64(uint64_t v, int l) {
   const uint32_t hi = (uint32_t)__shfl((int)(uint32_t)(v >> 32), l);
   return ((uint64_t)hi << 32) | lo;
 }
-__device__ __forceinline__ double shfl_up_f64(double v, int d) {
-  const uint64_t u = __double_as_longlong(v);
-  const uint32_t lo = (uint32_t)__shfl_up((int)(uint32_t)u, d);
-  const uint32_t hi = (uint32_t)__shfl_up((int)(uint32_t)(u >> 32), d);
-  return __longlong_as_double((long long)(((uint64_t)hi << 32) | lo));
-}
 
 // Greedy knot placement over one chunk of arguments [j0, j1], one wave per
 // chunk.  The sequential walk it reproduces: from the last knot (x0, y0) keep
@@ -267,12 +261,14 @@ __device__ __forceinline__ double shfl_up_f64(double v, int d) {
 // (j - x0); the first argument j that empties it puts a knot at the previous
 // argument jp on the cone's middle slope, y0 + (lo + hi)/2 * (jp - x0), and
 // the walk restarts from there with j.  Here the ones come 64 at a time (a
-// window of 64 words: popcount scan, lane i finds its one by a binary search
-// over the scan), lane i takes argument jn + i, and a running max / min over
-// the lanes (a prefix scan: max and min are exact, so the values are the
+// window of 64 words: popcount scan; each non-empty word marks its first one's
+// index in a per-wave LDS row and a running max over that row gives every one
+// its word), lane i takes argument jn + i, and a running max / min over the
+// lanes (a prefix scan: max and min are exact, so the values are the
 // sequential ones) finds the first lane whose cone is empty; the knot goes in,
 // and the lanes from that one on are redone from the new knot.  Same double
-// arithmetic as the walk (no contraction), so the same knots.
+// arithmetic as the walk (no contraction), so the same knots.  The scans are
+// DPP moves (no LDS round trips).
 // Count pass (off == nullptr): cnt[flat] = the chunk's knots, and the first
 // LSEL_CAP of them to scratch[flat * LSEL_CAP ..].  Emit pass: a chunk with at
 // most LSEL_CAP knots copies them from scratch; a larger one walks again and
@@ -285,7 +281,10 @@ __global__ __launch_bounds__(256) void k_lsel_place(const PlaceVec* __restrict__
                                                     uint32_t* __restrict__ err) {
 #pragma clang fp contract(off)
   const uint32_t lane = (uint32_t)lane_id();
-  const uint64_t flat = (uint64_t)blockIdx.x * 4 + (uint32_t)__builtin_amdgcn_readfirstlane((int)(threadIdx.x >> 6));
+  const uint32_t wv = (uint32_t)__builtin_amdgcn_readfirstlane((int)(threadIdx.x >> 6));
+  __shared__ int32_t s_first[4][64];  // per wave: the window word whose first one is one i
+  int32_t* first = s_first[wv];
+  const uint64_t flat = (uint64_t)blockIdx.x * 4 + wv;
   if (flat >= nflat) return;
   const PlaceVec& v = pv[find_base(cbase, nv, flat)];
   const uint64_t c = flat - v.cbase;
@@ -316,7 +315,7 @@ __global__ __launch_bounds__(256) void k_lsel_place(const PlaceVec* __restrict__
       const uint64_t wl = w + lane;
       const uint64_t wd = lane == 0 ? x : (wl < v.nwords ? v.words[wl] : 0ull);
       const uint32_t pc = popc64(wd);
-      const uint32_t incl = wave_incl_scan(pc);
+      const uint32_t incl = wave_incl_sum_dpp(pc);
       const uint32_t T = (uint32_t)__builtin_amdgcn_readlane((int)incl, 63);
       const uint64_t need = j1 - jn + 1;
       const uint32_t m = (uint32_t)(need < (uint64_t)(T < 64u ? T : 64u) ? need : (T < 64u ? T : 64u));
@@ -329,13 +328,18 @@ __global__ __launch_bounds__(256) void k_lsel_place(const PlaceVec* __restrict__
         x = v.words[w];
         continue;
       }
-      // lane i (< m): the i-th one of the window, in word k = the first lane
-      // whose inclusive count exceeds i
-      uint32_t k = 0;
-#pragma unroll
-      for (uint32_t st = 32; st >= 1; st >>= 1)
-        if ((uint32_t)__shfl((int)incl, (int)(k + st - 1)) <= lane) k += st;
-      const uint32_t ks = k < 63 ? k : 63;
+      // lane i (< m): the i-th one of the window, in word k = the last
+      // non-empty word whose first one has index <= i (the first one of word
+      // l has index incl - pc)
+      first[lane] = -1;
+      __builtin_amdgcn_fence(__ATOMIC_RELEASE, "wavefront");
+      __builtin_amdgcn_wave_barrier();
+      if (pc && incl - pc < 64u) first[incl - pc] = (int32_t)lane;
+      __builtin_amdgcn_fence(__ATOMIC_RELEASE, "wavefront");
+      __builtin_amdgcn_wave_barrier();
+      __builtin_amdgcn_fence(__ATOMIC_ACQUIRE, "wavefront");
+      const int32_t kw = wave_incl_max_dpp(first[lane]);
+      const uint32_t ks = kw < 0 ? 0u : (uint32_t)kw;
       const uint32_t ex_k = (uint32_t)__shfl((int)incl, (int)ks) - (uint32_t)__shfl((int)pc, (int)ks);
       const uint64_t wk = shfl_u64(wd, (int)ks);
       const uint32_t bit = lane < m ? select_in_word(wk, lane - ex_k) : 0u;
@@ -355,14 +359,7 @@ __global__ __launch_bounds__(256) void k_lsel_place(const PlaceVec* __restrict__
           a = (P - tol - y0) / dx;
           b = (P + tol - y0) / dx;
         }
-#pragma unroll
-        for (int d = 1; d < 64; d <<= 1) {
-          const double ta = shfl_up_f64(a, d), tb = shfl_up_f64(b, d);
-          if ((int)lane >= d) {
-            a = ta > a ? ta : a;
-            b = tb < b ? tb : b;
-          }
-        }
+        wave_incl_maxmin_f64_dpp(a, b);
         a = lo > a ? lo : a;
         b = hi < b ? hi : b;
         const uint64_t fail = wave_ballot(act && a > b);
