@@ -158,17 +158,6 @@ __device__ __forceinline__ U wave_sum(U v) {
   for (int d = 1; d < 64; d <<= 1) v += __shfl_xor(v, d);
   return v;
 }
-// inclusive scan over the 64 lanes (Hillis-Steele with shuffles)
-template <typename U>
-__device__ __forceinline__ U wave_incl_scan(U v) {
-  const int l = lane_id();
-#pragma unroll
-  for (int d = 1; d < 64; d <<= 1) {
-    U t = __shfl_up(v, d);
-    if (l >= d) v += t;
-  }
-  return v;
-}
 
 // Inclusive scans over the 64 lanes with DPP moves (no LDS round trip): row_shr
 // 1, 2, 4, 8 scan each 16-lane row (lanes with no source in the row take the
@@ -198,6 +187,22 @@ __device__ __forceinline__ int32_t wave_incl_max_dpp(int32_t v) {
   WTG_DPP_SCAN_STEPS(WTG_STEP)
 #undef WTG_STEP
   return v;
+}
+// inclusive scan over the 64 lanes: 32-bit values by DPP moves, wider ones
+// Hillis-Steele with shuffles
+template <typename U>
+__device__ __forceinline__ U wave_incl_scan(U v) {
+  if constexpr (sizeof(U) == 4) {
+    return (U)wave_incl_sum_dpp((uint32_t)v);
+  } else {
+    const int l = lane_id();
+#pragma unroll
+    for (int d = 1; d < 64; d <<= 1) {
+      U t = __shfl_up(v, d);
+      if (l >= d) v += t;
+    }
+    return v;
+  }
 }
 // running max of a and min of b over the lanes (t > a ? t : a: ties keep the
 // lane's own value, as a sequential walk does)
