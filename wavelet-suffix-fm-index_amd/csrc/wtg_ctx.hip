@@ -44,6 +44,8 @@ extern "C" int wtg_ctx_create(int device, wtg_ctx** out) {
   WTG_CHECK_HIP(hipMemset(c->d_flag, 0, 256));
   WTG_CHECK_HIP(hipEventCreate(&c->t0));
   WTG_CHECK_HIP(hipEventCreate(&c->t1));
+  WTG_CHECK_HIP(hipEventCreateWithFlags(&c->h2d_ev, hipEventDisableTiming));
+  WTG_CHECK_HIP(hipHostMalloc((void**)&c->h_stage, WTG_STAGE_BYTES, hipHostMallocDefault));
   // library allocations come stream-ordered from the device's default pool,
   // which keeps freed memory for reuse (WTG_NO_POOL=1: plain hipMalloc)
   if (!std::getenv("WTG_NO_POOL")) {
@@ -75,6 +77,8 @@ extern "C" int wtg_ctx_destroy(wtg_ctx* ctx) {
   if (ctx->d_flag) (void)hipFree(ctx->d_flag);
   if (ctx->t0) (void)hipEventDestroy(ctx->t0);
   if (ctx->t1) (void)hipEventDestroy(ctx->t1);
+  if (ctx->h2d_ev) (void)hipEventDestroy(ctx->h2d_ev);
+  if (ctx->h_stage) (void)hipHostFree(ctx->h_stage);
   (void)hipStreamDestroy(ctx->stream);
   delete ctx;
   return WTG_OK;
@@ -83,9 +87,11 @@ extern "C" int wtg_ctx_destroy(wtg_ctx* ctx) {
 extern "C" int wtg_ctx_sync(wtg_ctx* ctx) {
   WTG_REQUIRE(ctx, "null ctx");
   WTG_TRY(wtg_set_device(ctx));
+  // the flag rides the stream: one async copy into pinned memory, one sync
+  volatile uint32_t* hf = reinterpret_cast<volatile uint32_t*>(ctx->h_stage);
+  WTG_CHECK_HIP(hipMemcpyAsync(ctx->h_stage, ctx->d_flag, 4, hipMemcpyDeviceToHost, ctx->stream));
   WTG_CHECK_HIP(hipStreamSynchronize(ctx->stream));
-  uint32_t f = 0;
-  WTG_CHECK_HIP(hipMemcpy(&f, ctx->d_flag, 4, hipMemcpyDeviceToHost));
+  const uint32_t f = *hf;
   if (f) {
     WTG_CHECK_HIP(hipMemset(ctx->d_flag, 0, 4));
     if (f & WTG_FLAG_BADSYM) {
@@ -103,6 +109,30 @@ extern "C" int wtg_ctx_sync(wtg_ctx* ctx) {
     wtg_set_error("device-side bounded wait expired (look-back scan); results invalid");
     return WTG_ERR_DEVICE;
   }
+  return WTG_OK;
+}
+
+int wtg_stage_d2h(wtg_ctx* ctx, void* dst, const void* d_src, size_t bytes) {
+  if (bytes > WTG_STAGE_D2H_MAX) {
+    WTG_CHECK_HIP(hipMemcpyAsync(dst, d_src, bytes, hipMemcpyDeviceToHost, ctx->stream));
+    WTG_CHECK_HIP(hipStreamSynchronize(ctx->stream));
+    return WTG_OK;
+  }
+  WTG_CHECK_HIP(hipMemcpyAsync(ctx->h_stage + WTG_STAGE_D2H, d_src, bytes, hipMemcpyDeviceToHost, ctx->stream));
+  WTG_CHECK_HIP(hipStreamSynchronize(ctx->stream));
+  std::memcpy(dst, ctx->h_stage + WTG_STAGE_D2H, bytes);
+  return WTG_OK;
+}
+
+int wtg_stage_h2d(wtg_ctx* ctx, void* d_dst, const void* src, size_t bytes) {
+  if (bytes > WTG_STAGE_H2D_MAX) {
+    WTG_CHECK_HIP(hipMemcpyAsync(d_dst, src, bytes, hipMemcpyHostToDevice, ctx->stream));
+    return WTG_OK;
+  }
+  WTG_CHECK_HIP(hipEventSynchronize(ctx->h2d_ev));  // the previous staged copy has read the buffer
+  std::memcpy(ctx->h_stage + WTG_STAGE_H2D, src, bytes);
+  WTG_CHECK_HIP(hipMemcpyAsync(d_dst, ctx->h_stage + WTG_STAGE_H2D, bytes, hipMemcpyHostToDevice, ctx->stream));
+  WTG_CHECK_HIP(hipEventRecord(ctx->h2d_ev, ctx->stream));
   return WTG_OK;
 }
 

@@ -1066,18 +1066,21 @@ static int huff_finish(wtg_ctx* ctx, wtg_huff* wt, const uint64_t* hist) {
   if (S.n_inner > 0) {
     WTG_LAUNCH(ctx, "huff_rbase", k_huff_rbase, dim3(1), dim3(256), 0, wt->d_tab, wt->dir, wt->words);
     WTG_LAUNCH_CHECK();
-  } else if (hipMemcpyAsync(wt->d_tab, &wt->tab, sizeof(HuffTab), hipMemcpyHostToDevice, ctx->stream) !=
-             hipSuccess) {
-    wtg_set_error("wt_huff: table copy failed");
-    return WTG_ERR_HIP;
+  } else {
+    WTG_TRY(wtg_stage_h2d(ctx, wt->d_tab, &wt->tab, sizeof(HuffTab)));
   }
-  if ((pend.active && hipMemcpyAsync(nl, pend.d_nl, 8, hipMemcpyDeviceToHost, ctx->stream) != hipSuccess) ||
+  // both reads into the pinned staging area, one sync
+  uint8_t* st = ctx->h_stage + WTG_STAGE_D2H;
+  static_assert(64 + sizeof(rb) <= WTG_STAGE_D2H_MAX, "staging area");
+  if ((pend.active && hipMemcpyAsync(st, pend.d_nl, 8, hipMemcpyDeviceToHost, ctx->stream) != hipSuccess) ||
       (S.n_inner > 0 &&
-       hipMemcpyAsync(rb, wt->d_tab->rbase, sizeof(rb), hipMemcpyDeviceToHost, ctx->stream) != hipSuccess) ||
+       hipMemcpyAsync(st + 64, wt->d_tab->rbase, sizeof(rb), hipMemcpyDeviceToHost, ctx->stream) != hipSuccess) ||
       hipStreamSynchronize(ctx->stream) != hipSuccess) {
     wtg_set_error("wt_huff: rank base copy failed");
     return WTG_ERR_HIP;
   }
+  if (pend.active) std::memcpy(nl, st, 8);
+  if (S.n_inner > 0) std::memcpy(rb, st + 64, sizeof(rb));
   WTG_TRY(wtg_rank_select_end(ctx, &pend, nl));
   if (S.n_inner > 0) {
     std::memcpy(wt->tab.rbase, rb, sizeof(rb));
@@ -1113,12 +1116,7 @@ extern "C" int wtg_huff_build(wtg_ctx* ctx, const uint8_t* d_text, uint64_t n, w
                (unsigned long long*)wt->d_hist, ((uintptr_t)d_text & 15) == 0);
     WTG_LAUNCH_CHECK();
   }
-  if (hipMemcpyAsync(hist, wt->d_hist, sizeof(hist), hipMemcpyDeviceToHost, ctx->stream) != hipSuccess ||
-      hipStreamSynchronize(ctx->stream) != hipSuccess) {
-    wtg_set_error("wtg_huff_build: histogram copy failed");
-    rc = WTG_ERR_HIP;
-    goto fail;
-  }
+  if ((rc = wtg_stage_d2h(ctx, hist, wt->d_hist, sizeof(hist))) != WTG_OK) goto fail;
   build_shape(hist, n, wt->shape, wt->tab);
   if (wt->shape.max_depth > 64) {
     wtg_set_error("wtg_huff_build: Huffman code longer than 64 bits");
@@ -1146,12 +1144,7 @@ extern "C" int wtg_huff_build(wtg_ctx* ctx, const uint8_t* d_text, uint64_t n, w
       if ((rc = wtg_ws_reserve(ctx, total)) != WTG_OK) goto fail;
       char* ws = (char*)ctx->ws;
       if ((rc = wtg_fill_bytes_zero(ctx, ws + o_st, total - o_st)) != WTG_OK) goto fail;
-      if (hipMemcpyAsync(wt->d_tab, &wt->tab, sizeof(HuffTab), hipMemcpyHostToDevice, ctx->stream) !=
-          hipSuccess) {
-        wtg_set_error("wtg_huff_build: table copy failed");
-        rc = WTG_ERR_HIP;
-        goto fail;
-      }
+      if ((rc = wtg_stage_h2d(ctx, wt->d_tab, &wt->tab, sizeof(HuffTab))) != WTG_OK) goto fail;
       SegArgs a{};
       a.text = d_text;
       a.n = n;

@@ -39,6 +39,12 @@ struct wtg_ctx {
   size_t ws_bytes = 0;
   // device error flag (bounded waits set it), host-visible copy on sync
   uint32_t* d_flag = nullptr;
+  // pinned host staging for small transfers (async copies need no bounce
+  // buffer): [0, 256) the error flag, [WTG_STAGE_D2H, +16 KiB) device-to-host
+  // reads (each followed by a stream sync before use), [WTG_STAGE_H2D, +32 KiB)
+  // host-to-device tables (h2d_ev marks the last such copy)
+  uint8_t* h_stage = nullptr;
+  hipEvent_t h2d_ev = nullptr;
   // stream timer
   hipEvent_t t0 = nullptr, t1 = nullptr;
   // per-kernel timing
@@ -52,6 +58,14 @@ struct wtg_ctx {
 
 // workspace of at least `bytes`, 256-byte aligned regions carved by the caller
 int wtg_ws_reserve(wtg_ctx* ctx, size_t bytes);
+constexpr size_t WTG_STAGE_D2H = 4096, WTG_STAGE_D2H_MAX = 16384;
+constexpr size_t WTG_STAGE_H2D = 32768, WTG_STAGE_H2D_MAX = 32768;
+constexpr size_t WTG_STAGE_BYTES = WTG_STAGE_H2D + WTG_STAGE_H2D_MAX;
+// copy device bytes to host dst through the pinned staging area (one stream sync)
+int wtg_stage_d2h(wtg_ctx* ctx, void* dst, const void* d_src, size_t bytes);
+// stream-ordered host-to-device copy of a small table through the staging area
+// (waits for the previous such copy before reusing the buffer)
+int wtg_stage_h2d(wtg_ctx* ctx, void* d_dst, const void* src, size_t bytes);
 // bits of the device error flag (ctx->d_flag), decoded by wtg_ctx_sync
 constexpr uint32_t WTG_FLAG_WAIT = 1u;      // a bounded look-back wait expired
 constexpr uint32_t WTG_FLAG_BADSYM = 2u;    // input symbol >= 2^levels / tables inconsistent with the keys

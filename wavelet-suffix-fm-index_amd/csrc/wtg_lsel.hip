@@ -646,6 +646,7 @@ static int alloc_block(wtg_ctx* ctx, wtg_lsel* ls, size_t bytes, void** p) {
 }
 
 static int sync_copy(wtg_ctx* ctx, void* dst, const void* src, size_t bytes, hipMemcpyKind kind) {
+  if (kind == hipMemcpyDeviceToHost) return wtg_stage_d2h(ctx, dst, src, bytes);
   WTG_CHECK_HIP(hipMemcpyAsync(dst, src, bytes, kind, ctx->stream));
   WTG_CHECK_HIP(hipStreamSynchronize(ctx->stream));
   return WTG_OK;

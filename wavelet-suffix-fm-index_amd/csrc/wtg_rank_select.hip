@@ -488,8 +488,7 @@ int wtg_rank_select_begin(wtg_ctx* ctx, const uint64_t* d_words, uint64_t nbits,
     m1 = known_ones;
   } else if (build) {
     unsigned long long t = 0;
-    WTG_CHECK_HIP(hipMemcpyAsync(&t, ones_total, 8, hipMemcpyDeviceToHost, ctx->stream));
-    WTG_CHECK_HIP(hipStreamSynchronize(ctx->stream));
+    WTG_TRY(wtg_stage_d2h(ctx, &t, ones_total, 8));
     m1 = t;
   } else {
     // ones = D[2S] + ones in the last entry
@@ -537,8 +536,7 @@ int wtg_rank_select_run(wtg_ctx* ctx, const uint64_t* d_words, uint64_t nbits, u
   WTG_TRY(wtg_rank_select_begin(ctx, d_words, nbits, d_dir, build, sel1, sel0, WTG_ONES_UNKNOWN, &pend));
   if (!pend.active) return WTG_OK;
   uint32_t nl[2] = {0, 0};
-  WTG_CHECK_HIP(hipMemcpyAsync(nl, pend.d_nl, 8, hipMemcpyDeviceToHost, ctx->stream));
-  WTG_CHECK_HIP(hipStreamSynchronize(ctx->stream));
+  WTG_TRY(wtg_stage_d2h(ctx, nl, pend.d_nl, 8));
   return wtg_rank_select_end(ctx, &pend, nl);
 }
 
