@@ -202,20 +202,25 @@ __device__ __forceinline__ int32_t wave_incl_max_dpp(int32_t v) {
 #undef WTG_STEP
   return v;
 }
-// inclusive scan over the 64 lanes: 32-bit values by DPP moves, wider ones
-// Hillis-Steele with shuffles
+// inclusive scan over the 64 lanes, Hillis-Steele with shuffles
+template <typename U>
+__device__ __forceinline__ U wave_incl_scan_shfl(U v) {
+  const int l = lane_id();
+#pragma unroll
+  for (int d = 1; d < 64; d <<= 1) {
+    U t = __shfl_up(v, d);
+    if (l >= d) v += t;
+  }
+  return v;
+}
+// inclusive scan over the 64 lanes: 32-bit values by DPP moves, wider ones by
+// shuffles
 template <typename U>
 __device__ __forceinline__ U wave_incl_scan(U v) {
   if constexpr (sizeof(U) == 4) {
     return (U)wave_incl_sum_dpp((uint32_t)v);
   } else {
-    const int l = lane_id();
-#pragma unroll
-    for (int d = 1; d < 64; d <<= 1) {
-      U t = __shfl_up(v, d);
-      if (l >= d) v += t;
-    }
-    return v;
+    return wave_incl_scan_shfl(v);
   }
 }
 // running max of a and min of b over the lanes (t > a ? t : a: ties keep the
@@ -309,11 +314,13 @@ __device__ __forceinline__ uint64_t lb_exclusive(uint64_t* states, uint64_t tile
 }
 
 // ------------------------------------------------------ block scan helper
-// exclusive scan of one u32 per thread over a 256-thread block
+// exclusive scan of one u32 per thread over a 256-thread block (the wave scans
+// by shuffles or, DPP, by DPP moves)
+template <bool DPP = false>
 __device__ __forceinline__ uint32_t block_excl_scan_u32(uint32_t v, uint32_t* s_wave,
                                                         uint32_t& total) {
   const int w = threadIdx.x >> 6, l = lane_id();
-  uint32_t incl = wave_incl_scan(v);
+  uint32_t incl = DPP ? wave_incl_sum_dpp(v) : wave_incl_scan_shfl(v);
   if (l == 63) s_wave[w] = incl;
   __syncthreads();
   uint32_t off = 0, tot = 0;

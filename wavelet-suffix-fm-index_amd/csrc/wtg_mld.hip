@@ -422,6 +422,10 @@ __device__ __forceinline__ uint64_t lds_bits64(const uint32_t* row, uint32_t p) 
 
 template <typename K, bool SCATTER>
 __global__ __launch_bounds__(256) __attribute__((amdgpu_waves_per_eu(8))) void k_mld_emit(MldArgs a) {
+  // block scans by DPP moves in the bits-only pass, by shuffles in the pass
+  // with key output (measured: the DPP form made the key-output pass 9%
+  // slower and the bits-only pass faster: C2 5.68 -> 5.59 ms, C4 9.22 -> 8.96 ms)
+  constexpr bool MLD_SCAN_DPP = !SCATTER;
   // LDS of the level pipeline: keys (ping-pong), tile prefix and staged bits
   // (under 20 KiB at 2048-key tiles)
   __shared__ __align__(16) uint32_t s_k[2][MLD_T];   // the tile's keys in its level order
@@ -463,9 +467,9 @@ __global__ __launch_bounds__(256) __attribute__((amdgpu_waves_per_eu(8))) void k
       const uint32_t p = (w + 4 * m) * 64 + lane;
       kr[m] = reinterpret_cast<const K*>(a.keys)[src + (p < len ? p : 0u)];
     }
-    cpre_d = block_excl_scan_u32(cv, s_wave, ctot);
+    cpre_d = block_excl_scan_u32<MLD_SCAN_DPP>(cv, s_wave, ctot);
     uint32_t tot;
-    const uint32_t te = block_excl_scan_u32(tv, s_wave, tot);
+    const uint32_t te = block_excl_scan_u32<MLD_SCAN_DPP>(tv, s_wave, tot);
     if (d < G) s_tpre[d] = (uint16_t)te;
     if (d == 0) s_tpre[G] = (uint16_t)tot;
     // positions past the tile hold the pad key ~0: a one at every level, in
@@ -493,7 +497,7 @@ __global__ __launch_bounds__(256) __attribute__((amdgpu_waves_per_eu(8))) void k
       zn = s_tpre[(2 * q + 1) << (sgj - 1)] - S;
     }
     uint32_t tot;
-    const uint32_t zx = block_excl_scan_u32(zn, s_wave, tot);  // over all entries; each level rebased below
+    const uint32_t zx = block_excl_scan_u32<MLD_SCAN_DPP>(zn, s_wave, tot);  // over all entries; each level rebased below
     if (r < G - 1 && r + 1 == (1u << j)) s_lvl0[j] = zx;
     __syncthreads();
     if (r < G - 1) {
@@ -598,7 +602,7 @@ __global__ __launch_bounds__(256) __attribute__((amdgpu_waves_per_eu(8))) void k
       s_rcnt[r] = cnt;
     }
     uint32_t tot;
-    const uint32_t ex = block_excl_scan_u32(nw, s_wave, tot);  // (its barriers order the s_cum reads above)
+    const uint32_t ex = block_excl_scan_u32<MLD_SCAN_DPP>(nw, s_wave, tot);  // (its barriers order the s_cum reads above)
     if (r < nruns) s_rw[r] = ex;
     if (r == 0) s_rw[nruns] = tot;
     if (SCATTER && r < G) s_lb[r] = cum_d + (s_cpre[r + 1] - s_cpre[r]) - s_tpre[r];

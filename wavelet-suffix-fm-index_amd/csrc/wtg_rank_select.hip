@@ -483,6 +483,7 @@ int wtg_rank_select_begin(wtg_ctx* ctx, const uint64_t* d_words, uint64_t nbits,
                ((uintptr_t)d_words & 15) == 0);
   }
   WTG_LAUNCH_CHECK();
+  if (!ra.want[0] && !ra.want[1]) return WTG_OK;  // rank9 alone: no count of ones needed
   uint64_t m1 = 0;
   if (build && known_ones != WTG_ONES_UNKNOWN) {
     m1 = known_ones;

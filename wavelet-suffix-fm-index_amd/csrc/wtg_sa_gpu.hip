@@ -330,12 +330,7 @@ extern "C" int wtg_suffix_array_gpu(wtg_ctx* ctx, const uint8_t* d_text, uint64_
     WTG_LAUNCH(ctx, "sa_present", k_sa_present, dim3((unsigned)wtg_ceil_div(n, 4096)), dim3(256), 0, d_text, n,
                present);
     SA_LAUNCH_CHECK();
-    if (hipMemcpyAsync(h_present, present, sizeof(h_present), hipMemcpyDeviceToHost, ctx->stream) != hipSuccess ||
-        hipStreamSynchronize(ctx->stream) != hipSuccess) {
-      wtg_set_error("wtg_suffix_array_gpu: alphabet copy failed");
-      rc = WTG_ERR_HIP;
-      goto out;
-    }
+    if ((rc = wtg_stage_d2h(ctx, h_present, present, sizeof(h_present))) != WTG_OK) goto out;
     uint32_t sigma = 0;
     for (int b = 0; b < 256; b++) h_code[b] = h_present[b] ? (uint16_t)++sigma : 0;
     uint32_t cbits = 1;
@@ -391,12 +386,7 @@ extern "C" int wtg_suffix_array_gpu(wtg_ctx* ctx, const uint8_t* d_text, uint64_
       WTG_LAUNCH(ctx, "sa_compact", k_sa_compact, dim3(grid), dim3(256), 0, v1, keep, pos, m, act);
       SA_LAUNCH_CHECK();
       uint32_t mnext = 0;
-      if (hipMemcpyAsync(&mnext, pos + m, 4, hipMemcpyDeviceToHost, ctx->stream) != hipSuccess ||
-          hipStreamSynchronize(ctx->stream) != hipSuccess) {
-        wtg_set_error("wtg_suffix_array_gpu: count copy failed");
-        rc = WTG_ERR_HIP;
-        goto out;
-      }
+      if ((rc = wtg_stage_d2h(ctx, &mnext, pos + m, 4)) != WTG_OK) goto out;
       m = mnext;
       h = h == 0 ? q : 2 * h;
       if (m > 0 && (round == 39 || h >= 2 * n)) {

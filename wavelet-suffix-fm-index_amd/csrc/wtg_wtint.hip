@@ -227,12 +227,7 @@ extern "C" int wtg_wtint_build(wtg_ctx* ctx, const void* d_sym, int sym_bytes, u
     goto fail;
   {
     unsigned long long dv = 0;
-    if (hipMemcpyAsync(&dv, d_distinct, 8, hipMemcpyDeviceToHost, ctx->stream) != hipSuccess ||
-        hipStreamSynchronize(ctx->stream) != hipSuccess) {
-      wtg_set_error("wtg_wtint_build: copy failed");
-      rc = WTG_ERR_HIP;
-      goto fail;
-    }
+    if ((rc = wtg_stage_d2h(ctx, &dv, d_distinct, 8)) != WTG_OK) goto fail;
     wt->sigma = dv;
   }
   if ((rc = wtg_rank_select_run(ctx, wt->words, wt->nbits, wt->dir, true, &wt->s1, &wt->s0)) != WTG_OK)
