@@ -175,7 +175,8 @@ __global__ __launch_bounds__(256) void k_mld_count(MldArgs a) {
   const uint32_t w = (uint32_t)__builtin_amdgcn_readfirstlane((int)(threadIdx.x >> 6));
   const uint32_t G = 1u << a.g, lane = (uint32_t)lane_id();
   uintptr_t b0[TPW], b1[TPW], w0[TPW];
-  uint32_t nw[TPW], tt[TPW];
+  uint64_t pbase[TPW];
+  uint32_t nw[TPW], tt[TPW], plen[TPW];
   bool ok[TPW];
   mld_u4 v[TPW][NW];
 #pragma unroll
@@ -186,6 +187,8 @@ __global__ __launch_bounds__(256) void k_mld_count(MldArgs a) {
     ok[u] = mld_tile(a, tt[u], seg, base, len, first);  // wave-uniform
     b0[u] = b1[u] = w0[u] = 0;
     nw[u] = 0;
+    pbase[u] = base;
+    plen[u] = len;
     if (ok[u]) {
       b0[u] = (uintptr_t)a.keys + mld_src(a, seg, base) * KW;
       b1[u] = b0[u] + (uintptr_t)len * KW;
@@ -200,6 +203,20 @@ __global__ __launch_bounds__(256) void k_mld_count(MldArgs a) {
     }
   }
   if (!ok[0]) return;
+  // clear the group's level words: a tile clears, at each of the group's
+  // levels, the words whose first bit falls on one of its positions (the
+  // positions of a level-s node keep their range at every later level), so
+  // every word is cleared once, by the group of its first bit's level, before
+  // that group's emit ORs into it
+#pragma unroll
+  for (uint32_t u = 0; u < TPW; u++) {
+    if (!ok[u]) break;
+    for (uint32_t j = 0; j < a.g; j++) {
+      const uint64_t lb = (uint64_t)(a.s + j - a.lvl0) * a.n;
+      const uint64_t wa = (lb + pbase[u] + 63) >> 6, wb = (lb + pbase[u] + plen[u] + 63) >> 6;
+      for (uint64_t k = wa + lane; k < wb; k += 64) a.words[k] = 0ull;
+    }
+  }
   const uint32_t tsh = a.L - a.s - a.g + a.kshift;  // the leaf's first bit in the key
   constexpr uint32_t B = 8 * KW;
   const uint32_t gw = KW == 8 ? a.g : (tsh >= B ? 0u : (B - tsh < a.g ? B - tsh : a.g));
@@ -742,7 +759,7 @@ int mld_build_t(wtg_ctx* ctx, const K* d_sym, uint64_t n, uint32_t L, uint32_t s
   uint64_t* scst = (uint64_t*)(ws + P.off_scst);
   uint32_t* ctr = (uint32_t*)(ws + P.off_ctr);
   uint32_t* kbuf[2] = {(uint32_t*)(ws + P.off_keys[0]), (uint32_t*)(ws + P.off_keys[1])};
-  WTG_TRY(wtg_fill_bytes_zero(ctx, d_words, wtg_levels_words(n, L - s0) * 8));
+  // (the level words are cleared by the leaf-count passes)
   if (d_distinct) WTG_TRY(wtg_fill_u64(ctx, d_distinct, 1, 0));
   const uint32_t ntiles0 = (uint32_t)wtg_ceil_div(n, MLD_T);
   const void* src = d_sym;
