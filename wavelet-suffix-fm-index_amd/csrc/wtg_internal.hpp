@@ -58,6 +58,7 @@ struct wtg_ctx {
 
 // workspace of at least `bytes`, 256-byte aligned regions carved by the caller
 int wtg_ws_reserve(wtg_ctx* ctx, size_t bytes);
+constexpr size_t WTG_STAGE_AUX = 256;  // [256, 4096): reads collected by a later sync of the call
 constexpr size_t WTG_STAGE_D2H = 4096, WTG_STAGE_D2H_MAX = 16384;
 constexpr size_t WTG_STAGE_H2D = 32768, WTG_STAGE_H2D_MAX = 32768;
 constexpr size_t WTG_STAGE_BYTES = WTG_STAGE_H2D + WTG_STAGE_H2D_MAX;

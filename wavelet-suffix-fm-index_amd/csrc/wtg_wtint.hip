@@ -225,14 +225,24 @@ extern "C" int wtg_wtint_build(wtg_ctx* ctx, const void* d_sym, int sym_bytes, u
   if ((rc = wtg_malloc(ctx, (void**)&wt->dir, wtg_rank9_words(wt->nbits) * 8)) != WTG_OK) goto fail;
   if ((rc = wtg_levels_build_async(ctx, d_sym, sym_bytes, n, L, wt->words, nullptr, 0, d_distinct)) != WTG_OK)
     goto fail;
-  {
-    unsigned long long dv = 0;
-    if ((rc = wtg_stage_d2h(ctx, &dv, d_distinct, 8)) != WTG_OK) goto fail;
-    wt->sigma = dv;
+  // the distinct count rides the stream into pinned memory; the support build
+  // right behind the tree reads its count of ones with a sync, after which the
+  // count is there (no sync of its own, no idle gap between tree and supports)
+  if (hipMemcpyAsync(ctx->h_stage + WTG_STAGE_AUX, d_distinct, 8, hipMemcpyDeviceToHost, ctx->stream) !=
+      hipSuccess) {
+    wtg_set_error("wtg_wtint_build: copy failed");
+    rc = WTG_ERR_HIP;
+    goto fail;
   }
   if ((rc = wtg_rank_select_run(ctx, wt->words, wt->nbits, wt->dir, true, &wt->s1, &wt->s0)) != WTG_OK)
     goto fail;
   if ((rc = wtg_finish(ctx)) != WTG_OK) goto fail;
+  if (hipStreamSynchronize(ctx->stream) != hipSuccess) {  // (a no-op wait: the supports synced)
+    wtg_set_error("wtg_wtint_build: sync failed");
+    rc = WTG_ERR_HIP;
+    goto fail;
+  }
+  wt->sigma = *reinterpret_cast<volatile unsigned long long*>(ctx->h_stage + WTG_STAGE_AUX);
   *out = wt;
   return WTG_OK;
 fail:
