@@ -15,7 +15,8 @@
 //      within `tol` bits of every one seen, so |pred(j) - select(j)| <= tol for
 //      every j.  A run of ones is one segment; the gap between two runs costs
 //      two knots.  Counted in one pass, written in a second.
-//   2. training: forward + backward kernel (thread per knot interval) and Adam
+//   2. training: forward + backward kernels (work items of at most 16 samples,
+//      then a sum per knot interval) and Adam
 //      on every knot, minimising the mean squared error on the training
 //      samples select(T), select(2T), ... that the plain method answers.
 //   3. the error window eps = max_j |round(pred(j)) - select(j)| is measured
@@ -435,45 +436,100 @@ struct TrainBufs {
   const uint64_t* kfirst;  // [nv] model first knots (for the vector search)
 };
 
-// forward + backward: samples j = T*u in [J_k, J_{k+1}) (the last interval
-// includes J_{K-1}); samples outside [J_0, J_{K-1}] are not seen.  Thread per
-// interval; the sample positions are loaded four at a time before they are
-// used (the sums run in sample order).
-__global__ __launch_bounds__(256) void k_lsel_fwd_bwd(const TrainVec* __restrict__ tv, uint32_t nv, TrainBufs tb,
-                                                      uint64_t nk_total) {
+// the samples j = T*u of interval k (knots a, b): u in [t1, t2] (none when t1 > t2)
+__device__ __forceinline__ void interval_samples(const TrainVec& v, const Knot& a, const Knot& b, uint64_t k,
+                                                 uint64_t& t1, uint64_t& t2) {
+  const uint64_t T = 1ull << v.tlog;
+  t1 = ((uint64_t)a.j + T - 1) >> v.tlog;
+  if (t1 < 1) t1 = 1;
+  t2 = (k + 2 == v.nknots ? (uint64_t)b.j : (uint64_t)b.j - 1) >> v.tlog;
+  if (t2 > v.ns) t2 = v.ns;
+}
+
+// Training in work items: interval g's samples are cut into items of at most
+// LSEL_ITEM consecutive samples (nitem[g] of them, the first at item_start[g]),
+// a thread per item sums its samples in order, and a thread per interval sums
+// its items' partial sums in order (the same sums every run; a wave's threads
+// do about equal work whatever the interval lengths).
+constexpr uint32_t LSEL_ITEM = 16;
+struct TrainItems {
+  uint32_t* model_of;     // [nk] the model of each interval
+  uint32_t* nitem;        // [nk] items per interval
+  uint32_t* item_start;   // [nk + 1] exclusive prefix of nitem (item_start[nk] = items)
+  uint32_t* item_g;       // [cap] the interval of each item
+  float *pa, *pc, *pe;    // [cap] partial sums per item
+  uint64_t cap;           // items upper bound (grid size)
+};
+
+__global__ __launch_bounds__(256) void k_lsel_items_count(const TrainVec* __restrict__ tv, uint32_t nv, TrainBufs tb,
+                                                          uint64_t nk_total, TrainItems it) {
   const uint64_t g = (uint64_t)blockIdx.x * blockDim.x + threadIdx.x;
   if (g >= nk_total) return;
-  const TrainVec& v = tv[find_base(tb.kfirst, nv, g)];
+  const uint32_t m = find_base(tb.kfirst, nv, g);
+  const TrainVec& v = tv[m];
   const uint64_t k = g - v.kfirst;
-  if (k + 1 >= v.nknots) {
-    tb.gl[g] = 0.f;
-    tb.gr[g] = 0.f;
-    tb.le[g] = 0.f;
-    return;
+  uint32_t c = 0;
+  if (k + 1 < v.nknots) {
+    uint64_t t1, t2;
+    interval_samples(v, tb.K[g], tb.K[g + 1], k, t1, t2);
+    if (t2 >= t1) c = (uint32_t)((t2 - t1 + LSEL_ITEM) / LSEL_ITEM);
   }
-  const Knot a = tb.K[g], b = tb.K[g + 1];
+  it.model_of[g] = m;
+  it.nitem[g] = c;
+}
+
+__global__ __launch_bounds__(256) void k_lsel_items_map(uint64_t nk_total, TrainItems it) {
+  const uint64_t g = (uint64_t)blockIdx.x * blockDim.x + threadIdx.x;
+  if (g >= nk_total) return;
+  const uint32_t s0 = it.item_start[g], c = it.nitem[g];
+  for (uint32_t q = 0; q < c; q++) it.item_g[s0 + q] = (uint32_t)g;
+}
+
+// forward + backward of one item: its samples' residuals, partial gradients, loss
+__global__ __launch_bounds__(256) void k_lsel_items(const TrainVec* __restrict__ tv, TrainBufs tb,
+                                                    uint64_t nk_total, TrainItems it) {
+  const uint64_t i = (uint64_t)blockIdx.x * blockDim.x + threadIdx.x;
+  if (i >= it.item_start[nk_total]) return;
+  const uint32_t g = it.item_g[i];
+  const TrainVec& v = tv[it.model_of[g]];
+  const uint64_t k = g - v.kfirst;
+  const Knot a = tb.K[g], b = tb.K[(uint64_t)g + 1];
   const uint64_t T = 1ull << v.tlog;
-  uint64_t t1 = ((uint64_t)a.j + T - 1) >> v.tlog;
-  if (t1 < 1) t1 = 1;
-  uint64_t t2 = (k + 2 == v.nknots ? (uint64_t)b.j : (uint64_t)b.j - 1) >> v.tlog;
-  if (t2 > v.ns) t2 = v.ns;
+  uint64_t t1, t2;
+  interval_samples(v, a, b, k, t1, t2);
+  const uint64_t u0 = t1 + (uint64_t)(i - it.item_start[g]) * LSEL_ITEM;
+  const uint64_t ue = u0 + LSEL_ITEM - 1 < t2 ? u0 + LSEL_ITEM - 1 : t2;
   const float span = (float)(b.j - a.j);
+  uint64_t pv[LSEL_ITEM];
+#pragma unroll
+  for (uint32_t q = 0; q < LSEL_ITEM; q++) pv[q] = u0 + q <= ue ? v.pos[u0 + q - 1] : 0ull;
   float sa = 0.f, sc = 0.f, e2 = 0.f;
-  for (uint64_t u0 = t1; u0 <= t2; u0 += 4) {
-    uint64_t pv[4];
 #pragma unroll
-    for (uint32_t q = 0; q < 4; q++) pv[q] = u0 + q <= t2 ? v.pos[u0 + q - 1] : 0ull;
-#pragma unroll
-    for (uint32_t q = 0; q < 4; q++) {
-      if (u0 + q > t2) break;
-      const uint64_t j = (u0 + q) * T;
-      const float f = (float)(j - a.j) / span;
-      const float target = (float)((double)pv[q] - lsel_line(v.alpha, v.beta, (double)j));
-      const float e = (1.0f - f) * a.y + f * b.y - target;
-      sa += e * (1.0f - f);
-      sc += e * f;
-      e2 += e * e;
-    }
+  for (uint32_t q = 0; q < LSEL_ITEM; q++) {
+    if (u0 + q > ue) break;
+    const uint64_t j = (u0 + q) * T;
+    const float f = (float)(j - a.j) / span;
+    const float target = (float)((double)pv[q] - lsel_line(v.alpha, v.beta, (double)j));
+    const float e = (1.0f - f) * a.y + f * b.y - target;
+    sa += e * (1.0f - f);
+    sc += e * f;
+    e2 += e * e;
+  }
+  it.pa[i] = sa;
+  it.pc[i] = sc;
+  it.pe[i] = e2;
+}
+
+// per interval: the sums of its items, in item order
+__global__ __launch_bounds__(256) void k_lsel_items_sum(TrainBufs tb, uint64_t nk_total, TrainItems it) {
+  const uint64_t g = (uint64_t)blockIdx.x * blockDim.x + threadIdx.x;
+  if (g >= nk_total) return;
+  const uint32_t s0 = it.item_start[g], s1 = it.item_start[g + 1];
+  float sa = 0.f, sc = 0.f, e2 = 0.f;
+  for (uint32_t q = s0; q < s1; q++) {
+    sa += it.pa[q];
+    sc += it.pc[q];
+    e2 += it.pe[q];
   }
   tb.gl[g] = sa;
   tb.gr[g] = sc;
@@ -652,6 +708,48 @@ static int sync_copy(wtg_ctx* ctx, void* dst, const void* src, size_t bytes, hip
   return WTG_OK;
 }
 
+// the work items of the training kernels (their intervals' sample counts do
+// not change: training moves the knots' y only); scratch from `mem`
+static size_t train_items_bytes(uint64_t nk, uint64_t cap) {
+  return wtg_align256(nk * 4) * 2 + wtg_align256((nk + 1) * 4) + wtg_align256(cap * 4) * 4 +
+         wtg_align256((wtg_ceil_div(nk + 1, 4096) + 2) * 8) + 256;
+}
+static int train_items_setup(wtg_ctx* ctx, const TrainVec* d_tv, uint32_t nv, const TrainBufs& tb, uint64_t nk,
+                             uint64_t cap, char* mem, TrainItems& it) {
+  char* p = mem;
+  auto take = [&](size_t bytes) { char* r = p; p += wtg_align256(bytes); return r; };
+  it.model_of = (uint32_t*)take(nk * 4);
+  it.nitem = (uint32_t*)take(nk * 4);
+  it.item_start = (uint32_t*)take((nk + 1) * 4);
+  it.item_g = (uint32_t*)take(cap * 4);
+  it.pa = (float*)take(cap * 4);
+  it.pc = (float*)take(cap * 4);
+  it.pe = (float*)take(cap * 4);
+  it.cap = cap;
+  const uint64_t tiles = wtg_ceil_div(nk + 1, 4096);
+  uint64_t* states = (uint64_t*)take((tiles + 2) * 8);
+  uint32_t* ctr = (uint32_t*)p;
+  WTG_TRY(wtg_fill_bytes_zero(ctx, states, (char*)ctr + 256 - (char*)states));
+  const unsigned gk = (unsigned)wtg_ceil_div(nk, 256);
+  WTG_LAUNCH(ctx, "lsel_items", k_lsel_items_count, dim3(gk), dim3(256), 0, d_tv, nv, tb, nk, it);
+  WTG_LAUNCH_CHECK();
+  WTG_LAUNCH(ctx, "lsel_items", k_scan_u32<uint32_t>, dim3((unsigned)wtg_ceil_div(nk, 4096)), dim3(256), 0,
+             it.nitem, it.item_start, nk, states, ctr, ctx->d_flag);
+  WTG_LAUNCH_CHECK();
+  WTG_LAUNCH(ctx, "lsel_items", k_lsel_items_map, dim3(gk), dim3(256), 0, nk, it);
+  WTG_LAUNCH_CHECK();
+  return WTG_OK;
+}
+// gl / gr / le of every interval
+static int train_grad(wtg_ctx* ctx, const TrainVec* d_tv, const TrainBufs& tb, uint64_t nk, const TrainItems& it) {
+  WTG_LAUNCH(ctx, "lsel_fwd_bwd", k_lsel_items, dim3((unsigned)wtg_ceil_div(it.cap, 256)), dim3(256), 0, d_tv, tb,
+             nk, it);
+  WTG_LAUNCH_CHECK();
+  WTG_LAUNCH(ctx, "lsel_fwd_bwd", k_lsel_items_sum, dim3((unsigned)wtg_ceil_div(nk, 256)), dim3(256), 0, tb, nk, it);
+  WTG_LAUNCH_CHECK();
+  return WTG_OK;
+}
+
 // knot placement, lookup tables, training and the eps check for the vectors
 // in `act` (all memory goes to ls->blocks, released by the caller on failure)
 static int lsel_models(wtg_ctx* ctx, wtg_lsel* ls, const wtg_lsel_params& P, const uint64_t* nbits,
@@ -758,6 +856,14 @@ static int lsel_models(wtg_ctx* ctx, wtg_lsel* ls, const wtg_lsel_params& P, con
   WTG_TRY(lsel_upload_desc(ctx, ls));
   TrainBufs tb{d_K, m1, m2, gl, gr, le, d_kfirst};
   const unsigned gk = (unsigned)wtg_ceil_div(nk_total, 256);
+  TrainItems items{};
+  {
+    uint64_t cap = nk_total + 1;
+    for (uint32_t a = 0; a < na; a++) cap += ns[act[a]] / LSEL_ITEM + 1;
+    char* mem = nullptr;
+    WTG_TRY(alloc_block(ctx, ls, train_items_bytes(nk_total, cap), (void**)&mem));
+    WTG_TRY(train_items_setup(ctx, d_tv, na, tb, nk_total, cap, mem, items));
+  }
   // eps (exhaustive) and training loss of the knots now in d_K
   uint32_t* d_act = nullptr;
   WTG_TRY(alloc_block(ctx, ls, na * 4 + 256, (void**)&d_act));
@@ -767,8 +873,7 @@ static int lsel_models(wtg_ctx* ctx, wtg_lsel* ls, const wtg_lsel_params& P, con
   const unsigned ge_x = (unsigned)std::max<uint64_t>(1, std::min<uint64_t>(wtg_ceil_div(wmax, 256 * LSEL_EPS_W), 256));
   auto measure = [&](std::vector<uint64_t>& eps, std::vector<double>& loss) -> int {
     WTG_TRY(wtg_fill_u64(ctx, d_eps, nvec, 0));
-    WTG_LAUNCH(ctx, "lsel_fwd_bwd", k_lsel_fwd_bwd, dim3(gk), dim3(256), 0, d_tv, na, tb, nk_total);
-    WTG_LAUNCH_CHECK();
+    WTG_TRY(train_grad(ctx, d_tv, tb, nk_total, items));
     // every model in one launch each: blocks of a vector stride over its words
     WTG_LAUNCH(ctx, "lsel_eps", k_lsel_eps, dim3(ge_x, na), dim3(256), 0, ls->d_desc, d_act, d_eps);
     WTG_LAUNCH_CHECK();
@@ -792,8 +897,7 @@ static int lsel_models(wtg_ctx* ctx, wtg_lsel* ls, const wtg_lsel_params& P, con
   const float b1 = 0.9f, b2 = 0.999f;
   float p1 = 1.f, p2 = 1.f;
   for (uint32_t e = 0; e < P.epochs; e++) {
-    WTG_LAUNCH(ctx, "lsel_fwd_bwd", k_lsel_fwd_bwd, dim3(gk), dim3(256), 0, d_tv, na, tb, nk_total);
-    WTG_LAUNCH_CHECK();
+    WTG_TRY(train_grad(ctx, d_tv, tb, nk_total, items));
     p1 *= b1;
     p2 *= b2;
     WTG_LAUNCH(ctx, "lsel_adam", k_lsel_adam, dim3(gk), dim3(256), 0, d_tv, na, tb, nk_total, P.lr, b1, b2,
@@ -989,10 +1093,12 @@ extern "C" int wtg_lsel_loss_grad(wtg_ctx* ctx, const void* d_knots, uint32_t nk
   WTG_REQUIRE(ctx && d_knots && d_pos && d_grad && loss && nknots >= 2 && pow2(stride),
               "wtg_lsel_loss_grad: bad argument");
   WTG_TRY(wtg_set_device(ctx));
-  // scratch: TrainVec | kfirst | gl | gr | le | loss
+  // scratch: TrainVec | kfirst | gl | gr | le | loss | work items
+  const uint64_t cap = (uint64_t)nknots + ns / LSEL_ITEM + 2;
   const size_t o_tv = 0, o_kf = wtg_align256(sizeof(TrainVec)), o_gl = o_kf + 256;
   const size_t o_gr = wtg_align256(o_gl + (size_t)nknots * 4), o_le = wtg_align256(o_gr + (size_t)nknots * 4);
-  const size_t o_l = wtg_align256(o_le + (size_t)nknots * 4), total = o_l + 256;
+  const size_t o_l = wtg_align256(o_le + (size_t)nknots * 4), o_it = o_l + 256;
+  const size_t total = o_it + train_items_bytes(nknots, cap);
   WTG_TRY(wtg_ws_reserve(ctx, total));
   char* ws = (char*)ctx->ws;
   const TrainVec v{d_pos, ns, 0, nknots, (uint32_t)__builtin_ctz(stride), alpha, beta};
@@ -1003,8 +1109,9 @@ extern "C" int wtg_lsel_loss_grad(wtg_ctx* ctx, const void* d_knots, uint32_t nk
   TrainBufs tb{(Knot*)const_cast<void*>(d_knots), nullptr, nullptr, (float*)(ws + o_gl), (float*)(ws + o_gr),
                (float*)(ws + o_le), (const uint64_t*)(ws + o_kf)};
   const unsigned g = (unsigned)wtg_ceil_div(nknots, 256);
-  WTG_LAUNCH(ctx, "lsel_fwd_bwd", k_lsel_fwd_bwd, dim3(g), dim3(256), 0, d_tv, 1u, tb, (uint64_t)nknots);
-  WTG_LAUNCH_CHECK();
+  TrainItems items{};
+  WTG_TRY(train_items_setup(ctx, d_tv, 1u, tb, nknots, cap, ws + o_it, items));
+  WTG_TRY(train_grad(ctx, d_tv, tb, nknots, items));
   WTG_LAUNCH(ctx, "lsel_grad", k_lsel_grad, dim3(g), dim3(256), 0, d_tv, tb, d_grad);
   WTG_LAUNCH_CHECK();
   WTG_LAUNCH(ctx, "lsel_loss", k_lsel_loss, dim3(1), dim3(256), 0, d_tv, tb, (double*)(ws + o_l));
