@@ -420,6 +420,7 @@ __global__ __launch_bounds__(256) void k_lsel_lut(const Knot* __restrict__ K, ui
 // starting at global knot g (none for a model's last knot).
 struct TrainVec {
   const uint64_t* pos;   // ns training samples: pos[t] = select(T*(t+1))
+  float* tgt;            // ns targets: pos[t] - line(T*(t+1)), rounded to float once
   uint64_t ns;
   uint64_t kfirst;       // global index of the model's first knot
   uint32_t nknots, tlog;
@@ -485,6 +486,15 @@ __global__ __launch_bounds__(256) void k_lsel_items_map(uint64_t nk_total, Train
   for (uint32_t q = 0; q < c; q++) it.item_g[s0 + q] = (uint32_t)g;
 }
 
+// the training targets of model blockIdx.y: the samples' offsets from the
+// model's baseline line (they do not change while the knots train)
+__global__ __launch_bounds__(256) void k_lsel_targets(const TrainVec* __restrict__ tv) {
+  const TrainVec v = tv[blockIdx.y];
+  const uint64_t T = 1ull << v.tlog;
+  for (uint64_t t = (uint64_t)blockIdx.x * blockDim.x + threadIdx.x; t < v.ns; t += (uint64_t)gridDim.x * blockDim.x)
+    v.tgt[t] = (float)((double)v.pos[t] - lsel_line(v.alpha, v.beta, (double)((t + 1) * T)));
+}
+
 // forward + backward of one item: its samples' residuals, partial gradients, loss
 __global__ __launch_bounds__(256) void k_lsel_items(const TrainVec* __restrict__ tv, TrainBufs tb,
                                                     uint64_t nk_total, TrainItems it) {
@@ -500,17 +510,16 @@ __global__ __launch_bounds__(256) void k_lsel_items(const TrainVec* __restrict__
   const uint64_t u0 = t1 + (uint64_t)(i - it.item_start[g]) * LSEL_ITEM;
   const uint64_t ue = u0 + LSEL_ITEM - 1 < t2 ? u0 + LSEL_ITEM - 1 : t2;
   const float span = (float)(b.j - a.j);
-  uint64_t pv[LSEL_ITEM];
+  float tg[LSEL_ITEM];
 #pragma unroll
-  for (uint32_t q = 0; q < LSEL_ITEM; q++) pv[q] = u0 + q <= ue ? v.pos[u0 + q - 1] : 0ull;
+  for (uint32_t q = 0; q < LSEL_ITEM; q++) tg[q] = u0 + q <= ue ? v.tgt[u0 + q - 1] : 0.f;
   float sa = 0.f, sc = 0.f, e2 = 0.f;
 #pragma unroll
   for (uint32_t q = 0; q < LSEL_ITEM; q++) {
     if (u0 + q > ue) break;
     const uint64_t j = (u0 + q) * T;
     const float f = (float)(j - a.j) / span;
-    const float target = (float)((double)pv[q] - lsel_line(v.alpha, v.beta, (double)j));
-    const float e = (1.0f - f) * a.y + f * b.y - target;
+    const float e = (1.0f - f) * a.y + f * b.y - tg[q];
     sa += e * (1.0f - f);
     sc += e * f;
     e2 += e * e;
@@ -545,12 +554,24 @@ __device__ __forceinline__ float knot_grad(const TrainVec& v, const TrainBufs& t
   return s * (2.0f / (float)(v.ns ? v.ns : 1));
 }
 
+// Adam step of knot g; its gradient from the item sums of the intervals on
+// either side (summed in item order, as k_lsel_items_sum does)
 __global__ __launch_bounds__(256) void k_lsel_adam(const TrainVec* __restrict__ tv, uint32_t nv, TrainBufs tb,
-                                                   uint64_t nk_total, float lr, float b1, float b2, float bc1,
-                                                   float bc2) {
+                                                   uint64_t nk_total, TrainItems it, float lr, float b1, float b2,
+                                                   float bc1, float bc2) {
   const uint64_t g = (uint64_t)blockIdx.x * blockDim.x + threadIdx.x;
   if (g >= nk_total) return;
-  const float gr = knot_grad(tv[find_base(tb.kfirst, nv, g)], tb, g);
+  const TrainVec& v = tv[find_base(tb.kfirst, nv, g)];
+  const uint64_t k = g - v.kfirst;
+  float sa = 0.f, sc = 0.f;
+  if (k + 1 < v.nknots)
+    for (uint32_t q = it.item_start[g], q1 = it.item_start[g + 1]; q < q1; q++) sa += it.pa[q];
+  if (k >= 1)
+    for (uint32_t q = it.item_start[g - 1], q1 = it.item_start[g]; q < q1; q++) sc += it.pc[q];
+  float sg = 0.f;
+  if (k + 1 < v.nknots) sg += sa;
+  if (k >= 1) sg += sc;
+  const float gr = sg * (2.0f / (float)(v.ns ? v.ns : 1));
   const float m = b1 * tb.m1[g] + (1.f - b1) * gr;
   const float s = b2 * tb.m2[g] + (1.f - b2) * gr * gr;
   tb.m1[g] = m;
@@ -667,7 +688,7 @@ struct wtg_lsel {
   uint32_t nvec = 0;
   std::vector<LselDesc> desc;          // host copy
   LselDesc* d_desc = nullptr;
-  std::vector<uint64_t*> dirs;         // owned rank directories
+  std::vector<uint64_t*> dirs;         // rank directories (in one of the blocks)
   std::vector<wtg_select> sel;         // owned select1 structures
   std::vector<wtg_lsel_vec_info> info;
   std::vector<void*> blocks;           // owned model / sample memory
@@ -676,7 +697,6 @@ struct wtg_lsel {
 static void lsel_release(wtg_ctx* ctx, wtg_lsel* ls) {
   if (!ls) return;
   wtg_free(ctx, ls->d_desc);
-  for (auto* d : ls->dirs) wtg_free(ctx, d);
   for (auto& s : ls->sel) wtg_select_free(ctx, &s);
   for (auto* p : ls->blocks) wtg_free(ctx, p);
   delete ls;
@@ -715,7 +735,7 @@ static size_t train_items_bytes(uint64_t nk, uint64_t cap) {
          wtg_align256((wtg_ceil_div(nk + 1, 4096) + 2) * 8) + 256;
 }
 static int train_items_setup(wtg_ctx* ctx, const TrainVec* d_tv, uint32_t nv, const TrainBufs& tb, uint64_t nk,
-                             uint64_t cap, char* mem, TrainItems& it) {
+                             uint64_t cap, uint64_t nsmax, char* mem, TrainItems& it) {
   char* p = mem;
   auto take = [&](size_t bytes) { char* r = p; p += wtg_align256(bytes); return r; };
   it.model_of = (uint32_t*)take(nk * 4);
@@ -731,6 +751,9 @@ static int train_items_setup(wtg_ctx* ctx, const TrainVec* d_tv, uint32_t nv, co
   uint32_t* ctr = (uint32_t*)p;
   WTG_TRY(wtg_fill_bytes_zero(ctx, states, (char*)ctr + 256 - (char*)states));
   const unsigned gk = (unsigned)wtg_ceil_div(nk, 256);
+  const unsigned gt = (unsigned)std::max<uint64_t>(1, std::min<uint64_t>(wtg_ceil_div(nsmax, 256), 1024));
+  WTG_LAUNCH(ctx, "lsel_targets", k_lsel_targets, dim3(gt, nv), dim3(256), 0, d_tv);
+  WTG_LAUNCH_CHECK();
   WTG_LAUNCH(ctx, "lsel_items", k_lsel_items_count, dim3(gk), dim3(256), 0, d_tv, nv, tb, nk, it);
   WTG_LAUNCH_CHECK();
   WTG_LAUNCH(ctx, "lsel_items", k_scan_u32<uint32_t>, dim3((unsigned)wtg_ceil_div(nk, 4096)), dim3(256), 0,
@@ -820,12 +843,20 @@ static int lsel_models(wtg_ctx* ctx, wtg_lsel* ls, const wtg_lsel_params& P, con
   // 4. per vector: knots, lookup table
   std::vector<TrainVec> tv(na);
   std::vector<uint64_t> kfirst(na + 1, 0);
+  uint64_t ns_total = 0, nsmax = 0;
+  for (uint32_t a = 0; a < na; a++) {
+    ns_total += wtg_align256(ns[act[a]] * 4) / 4;
+    nsmax = std::max<uint64_t>(nsmax, ns[act[a]]);
+  }
+  float* tgt = nullptr;
+  WTG_TRY(alloc_block(ctx, ls, ns_total * 4, (void**)&tgt));
   for (uint32_t a = 0; a < na; a++) {
     const uint32_t b = act[a];
     LselDesc& d = ls->desc[b];
     const uint64_t k0 = off[cbase[a]], k1 = off[cbase[a + 1]];
     kfirst[a] = k0;
-    tv[a] = TrainVec{pos[b], ns[b], k0, (uint32_t)(k1 - k0), tlog, d.alpha, d.beta};
+    tv[a] = TrainVec{pos[b], tgt, ns[b], k0, (uint32_t)(k1 - k0), tlog, d.alpha, d.beta};
+    tgt += wtg_align256(ns[b] * 4) / 4;
     d.knots = d_K + k0;
     d.nknots = (uint32_t)(k1 - k0);
     uint32_t lg = 0;
@@ -862,7 +893,7 @@ static int lsel_models(wtg_ctx* ctx, wtg_lsel* ls, const wtg_lsel_params& P, con
     for (uint32_t a = 0; a < na; a++) cap += ns[act[a]] / LSEL_ITEM + 1;
     char* mem = nullptr;
     WTG_TRY(alloc_block(ctx, ls, train_items_bytes(nk_total, cap), (void**)&mem));
-    WTG_TRY(train_items_setup(ctx, d_tv, na, tb, nk_total, cap, mem, items));
+    WTG_TRY(train_items_setup(ctx, d_tv, na, tb, nk_total, cap, nsmax, mem, items));
   }
   // eps (exhaustive) and training loss of the knots now in d_K
   uint32_t* d_act = nullptr;
@@ -897,10 +928,12 @@ static int lsel_models(wtg_ctx* ctx, wtg_lsel* ls, const wtg_lsel_params& P, con
   const float b1 = 0.9f, b2 = 0.999f;
   float p1 = 1.f, p2 = 1.f;
   for (uint32_t e = 0; e < P.epochs; e++) {
-    WTG_TRY(train_grad(ctx, d_tv, tb, nk_total, items));
+    WTG_LAUNCH(ctx, "lsel_fwd_bwd", k_lsel_items, dim3((unsigned)wtg_ceil_div(items.cap, 256)), dim3(256), 0, d_tv,
+               tb, nk_total, items);
+    WTG_LAUNCH_CHECK();
     p1 *= b1;
     p2 *= b2;
-    WTG_LAUNCH(ctx, "lsel_adam", k_lsel_adam, dim3(gk), dim3(256), 0, d_tv, na, tb, nk_total, P.lr, b1, b2,
+    WTG_LAUNCH(ctx, "lsel_adam", k_lsel_adam, dim3(gk), dim3(256), 0, d_tv, na, tb, nk_total, items, P.lr, b1, b2,
                1.f - p1, 1.f - p2);
     WTG_LAUNCH_CHECK();
   }
@@ -950,10 +983,19 @@ extern "C" int wtg_lsel_build(wtg_ctx* ctx, uint32_t nvec, const uint64_t* const
   std::vector<uint64_t*> pos(nvec, nullptr);
   std::vector<uint64_t> ns(nvec, 0);
   std::vector<uint32_t> act;  // vectors that get a model
-  // 1. the plain method: rank9 + select1 per vector
+  // 1. the plain method: rank9 + select1 per vector (the directories in one block)
+  for (uint32_t b = 0; b < nvec; b++) std::memset(&ls->sel[b], 0, sizeof(wtg_select));
+  {
+    uint64_t dw = 0;
+    for (uint32_t b = 0; b < nvec; b++) dw += wtg_align256(wtg_rank9_words(nbits[b]) * 8) / 8;
+    uint64_t* dirs = nullptr;
+    if ((rc = alloc_block(ctx, ls, dw * 8, (void**)&dirs)) != WTG_OK) goto fail;
+    for (uint32_t b = 0; b < nvec; b++) {
+      ls->dirs[b] = dirs;
+      dirs += wtg_align256(wtg_rank9_words(nbits[b]) * 8) / 8;
+    }
+  }
   for (uint32_t b = 0; b < nvec && rc == WTG_OK; b++) {
-    std::memset(&ls->sel[b], 0, sizeof(wtg_select));
-    if ((rc = wtg_malloc(ctx, (void**)&ls->dirs[b], wtg_rank9_words(nbits[b]) * 8)) != WTG_OK) break;
     if ((rc = wtg_rank_select_run(ctx, d_words[b], nbits[b], ls->dirs[b], true, &ls->sel[b], nullptr)) != WTG_OK)
       break;
     LselDesc& d = ls->desc[b];
@@ -1093,15 +1135,16 @@ extern "C" int wtg_lsel_loss_grad(wtg_ctx* ctx, const void* d_knots, uint32_t nk
   WTG_REQUIRE(ctx && d_knots && d_pos && d_grad && loss && nknots >= 2 && pow2(stride),
               "wtg_lsel_loss_grad: bad argument");
   WTG_TRY(wtg_set_device(ctx));
-  // scratch: TrainVec | kfirst | gl | gr | le | loss | work items
+  // scratch: TrainVec | kfirst | gl | gr | le | loss | targets | work items
   const uint64_t cap = (uint64_t)nknots + ns / LSEL_ITEM + 2;
   const size_t o_tv = 0, o_kf = wtg_align256(sizeof(TrainVec)), o_gl = o_kf + 256;
   const size_t o_gr = wtg_align256(o_gl + (size_t)nknots * 4), o_le = wtg_align256(o_gr + (size_t)nknots * 4);
-  const size_t o_l = wtg_align256(o_le + (size_t)nknots * 4), o_it = o_l + 256;
+  const size_t o_l = wtg_align256(o_le + (size_t)nknots * 4), o_tg = o_l + 256;
+  const size_t o_it = wtg_align256(o_tg + (size_t)ns * 4);
   const size_t total = o_it + train_items_bytes(nknots, cap);
   WTG_TRY(wtg_ws_reserve(ctx, total));
   char* ws = (char*)ctx->ws;
-  const TrainVec v{d_pos, ns, 0, nknots, (uint32_t)__builtin_ctz(stride), alpha, beta};
+  const TrainVec v{d_pos, (float*)(ws + o_tg), ns, 0, nknots, (uint32_t)__builtin_ctz(stride), alpha, beta};
   const uint64_t kf = 0;
   WTG_CHECK_HIP(hipMemcpyAsync(ws + o_tv, &v, sizeof(v), hipMemcpyHostToDevice, ctx->stream));
   WTG_CHECK_HIP(hipMemcpyAsync(ws + o_kf, &kf, 8, hipMemcpyHostToDevice, ctx->stream));
@@ -1110,7 +1153,7 @@ extern "C" int wtg_lsel_loss_grad(wtg_ctx* ctx, const void* d_knots, uint32_t nk
                (float*)(ws + o_le), (const uint64_t*)(ws + o_kf)};
   const unsigned g = (unsigned)wtg_ceil_div(nknots, 256);
   TrainItems items{};
-  WTG_TRY(train_items_setup(ctx, d_tv, 1u, tb, nknots, cap, ws + o_it, items));
+  WTG_TRY(train_items_setup(ctx, d_tv, 1u, tb, nknots, cap, ns, ws + o_it, items));
   WTG_TRY(train_grad(ctx, d_tv, tb, nknots, items));
   WTG_LAUNCH(ctx, "lsel_grad", k_lsel_grad, dim3(g), dim3(256), 0, d_tv, tb, d_grad);
   WTG_LAUNCH_CHECK();
