@@ -338,6 +338,12 @@ def bench_lsel(pkg, args, full):
     del w
     log("build + train")
     prm = pkg.LselParams(16, args.lsel_gap, args.lsel_tol, args.lsel_epochs, 4.0, 512)
+    # the first build in the process also loads the kernels and maps fresh
+    # memory (reported as cold); the one kept is the second
+    ctx.sync()
+    t0 = time.perf_counter()
+    pkg.LearnedSelect(ctx, dv, [nb] * nvec, prm).free()
+    build_cold_s = time.perf_counter() - t0
     ctx.sync()
     t0 = time.perf_counter()
     ls = pkg.LearnedSelect(ctx, dv, [nb] * nvec, prm)
@@ -382,7 +388,7 @@ def bench_lsel(pkg, args, full):
         a = dsec.get()[:ns]
         dsec.free()
         placed = {"Mq_s": round(nq / (ms0 / 1e3) / 1e6, 1), "sectors_per_query": round(float(a.mean()), 3),
-                  "build_s": round(build0_s, 3),
+                  "build_s": round(build0_s, 4),
                   "identical_answers": bool(np.array_equal(out0.get(), out_p.get()))}
         i0 = [ls0.info(b) for b in range(nvec)]
         placed["model_bytes"] = sum(i["nknots"] * 8 + ((i["ones"] >> i["lut_log"]) + 1) * 4 for i in i0 if i["learned"])
@@ -399,7 +405,8 @@ def bench_lsel(pkg, args, full):
            "roofline": {"bound": "hbm", "kernel": "k_lsel_query<true>", "achieved": round(achieved, 1),
                         "peak": HBM_PEAK_GBS, "unit": "GB/s", "frac": round(achieved / HBM_PEAK_GBS, 4),
                         "bytes_per_query": round(sec["learned"] * 64.0, 1)},
-           "build_and_train_s": round(build_s, 3), "learned_vectors": nl,
+           "build_and_train_s": round(build_s, 4), "build_and_train_cold_s": round(build_cold_s, 4),
+           "learned_vectors": nl,
            "trained_vectors": sum(1 for i in infos if i["learned"] and i["trained"]),
            "placed_only": placed, "model_bytes": model_bytes,
            "training": {"loss_placed_mean": round(float(np.mean([i["loss_init"] for i in infos if i["learned"]] or [0])), 1),

@@ -115,7 +115,21 @@ int wtg_rank_select_run(wtg_ctx* ctx, const uint64_t* d_words, uint64_t nbits, u
 // counts (at pend->d_nl; the workspace must stay untouched until end), end
 // allocates and fills the long pools.  known_ones (build only): the number of
 // ones when the caller knows it, which saves a host read.
+// For many vectors at once (one host read per half for all of them): give
+// each its own workspace (ws_ext, wtg_rank_select_ws_bytes) and an arena for
+// the select content, and pass known_ones = WTG_ONES_DEFER: begin then stops
+// after the rank pass, with the number of ones at ws_ext + WTG_RS_ONES_OFF
+// (uint64), and wtg_rank_select_count, given that number, enqueues the rest up
+// to the long counts (uint32 per structure, bit t at ws_ext + WTG_RS_NL_OFF + 4t).
 constexpr uint64_t WTG_ONES_UNKNOWN = ~0ull;
+constexpr uint64_t WTG_ONES_DEFER = ~0ull - 1;
+constexpr size_t WTG_RS_ONES_OFF = 64, WTG_RS_NL_OFF = 192;
+// bump allocation from one device block; what it holds is freed with the
+// block, never by wtg_select_free
+struct wtg_arena {
+  char* base;
+  size_t size, used;
+};
 struct wtg_rs_pending {
   bool active;
   const uint64_t* d_words;
@@ -125,10 +139,20 @@ struct wtg_rs_pending {
   const uint64_t* sbend[2];
   const unsigned long long* last[2];
   const char* fin[2];
+  wtg_arena* ar;
+  bool want[2];
 };
 int wtg_rank_select_begin(wtg_ctx* ctx, const uint64_t* d_words, uint64_t nbits, uint64_t* d_dir, bool build,
-                          wtg_select* sel1, wtg_select* sel0, uint64_t known_ones, wtg_rs_pending* pend);
+                          wtg_select* sel1, wtg_select* sel0, uint64_t known_ones, wtg_rs_pending* pend,
+                          char* ws_ext = nullptr, wtg_arena* ar = nullptr);
+int wtg_rank_select_count(wtg_ctx* ctx, wtg_rs_pending* pend, uint64_t ones);
 int wtg_rank_select_end(wtg_ctx* ctx, const wtg_rs_pending* pend, const uint32_t nl[2]);
+size_t wtg_rank_select_ws_bytes(uint64_t nbits);
+// arena bytes of one select structure: before the count (by nbits), after it
+// (by the number of ones), and its long pool (by the long count)
+size_t wtg_select_arena_worst(uint64_t nbits);
+size_t wtg_select_arena_count(uint64_t ones);
+size_t wtg_select_arena_pool(uint32_t n_long);
 // multi-level direct build (up to 8 levels per pass; n >= 1, L <= 24, n < 2^32)
 // and whether it is on (WTG_LEVEL_SPLIT=1 selects the level-by-level split)
 int wtg_mld_enabled();

@@ -403,15 +403,20 @@ __global__ __launch_bounds__(256) void k_lsel_place(const PlaceVec* __restrict__
 
 // lookup table: thread per knot interval k, entries i with i << lut_log in
 // the interval's arguments [J_k, J_{k+1}) (the first from 0, the last to ones)
-__global__ __launch_bounds__(256) void k_lsel_lut(const Knot* __restrict__ K, uint32_t nk, uint64_t ones,
-                                                  uint32_t lut_log, uint32_t* __restrict__ lut) {
-  const uint32_t k = blockIdx.x * blockDim.x + threadIdx.x;
-  if (k + 1 >= nk) return;
-  const uint64_t lo = k == 0 ? 0ull : K[k].j;
-  const uint64_t hi = k + 2 == nk ? ones + 1 : K[k + 1].j;
-  const uint64_t nlut = (ones >> lut_log) + 1;
-  for (uint64_t i = (lo + (1ull << lut_log) - 1) >> lut_log; (i << lut_log) < hi && i < nlut; i++)
-    lut[i] = k;
+// (every model in one launch: model vids[blockIdx.y], blocks stride over its knots)
+__global__ __launch_bounds__(256) void k_lsel_lut(const LselDesc* __restrict__ descs,
+                                                  const uint32_t* __restrict__ vids) {
+  const LselDesc& d = descs[vids[blockIdx.y]];
+  const Knot* K = d.knots;
+  const uint32_t nk = d.nknots, lut_log = d.lut_log;
+  const uint64_t ones = d.ones, nlut = (ones >> lut_log) + 1;
+  uint32_t* lut = const_cast<uint32_t*>(d.lut);  // the table being built
+  for (uint32_t k = blockIdx.x * blockDim.x + threadIdx.x; k + 1 < nk; k += gridDim.x * blockDim.x) {
+    const uint64_t lo = k == 0 ? 0ull : K[k].j;
+    const uint64_t hi = k + 2 == nk ? ones + 1 : K[k + 1].j;
+    for (uint64_t i = (lo + (1ull << lut_log) - 1) >> lut_log; (i << lut_log) < hi && i < nlut; i++)
+      lut[i] = k;
+  }
 }
 
 // ------------------------------------------------------------- training
@@ -682,6 +687,15 @@ __global__ __launch_bounds__(256) void k_lsel_samples(const LselDesc* __restrict
   }
 }
 
+// out[b] = the count at byte off of workspace b (uint64 or uint32)
+__global__ __launch_bounds__(256) void k_lsel_gather(const char* __restrict__ ws, uint64_t stride, uint64_t off,
+                                                     uint32_t n, uint32_t bytes, uint64_t* __restrict__ out) {
+  const uint32_t b = blockIdx.x * blockDim.x + threadIdx.x;
+  if (b >= n) return;
+  const char* p = ws + b * stride + off;
+  out[b] = bytes == 8 ? *(const uint64_t*)p : (uint64_t) * (const uint32_t*)p;
+}
+
 }  // namespace
 
 struct wtg_lsel {
@@ -689,7 +703,7 @@ struct wtg_lsel {
   std::vector<LselDesc> desc;          // host copy
   LselDesc* d_desc = nullptr;
   std::vector<uint64_t*> dirs;         // rank directories (in one of the blocks)
-  std::vector<wtg_select> sel;         // owned select1 structures
+  std::vector<wtg_select> sel;         // select1 structures (their content in the blocks)
   std::vector<wtg_lsel_vec_info> info;
   std::vector<void*> blocks;           // owned model / sample memory
 };
@@ -697,7 +711,6 @@ struct wtg_lsel {
 static void lsel_release(wtg_ctx* ctx, wtg_lsel* ls) {
   if (!ls) return;
   wtg_free(ctx, ls->d_desc);
-  for (auto& s : ls->sel) wtg_select_free(ctx, &s);
   for (auto* p : ls->blocks) wtg_free(ctx, p);
   delete ls;
 }
@@ -843,13 +856,26 @@ static int lsel_models(wtg_ctx* ctx, wtg_lsel* ls, const wtg_lsel_params& P, con
   // 4. per vector: knots, lookup table
   std::vector<TrainVec> tv(na);
   std::vector<uint64_t> kfirst(na + 1, 0);
-  uint64_t ns_total = 0, nsmax = 0;
+  uint64_t ns_total = 0, nsmax = 0, lut_total = 0;
+  uint32_t nkmax = 2;
+  std::vector<uint32_t> lut_logs(na);
   for (uint32_t a = 0; a < na; a++) {
     ns_total += wtg_align256(ns[act[a]] * 4) / 4;
     nsmax = std::max<uint64_t>(nsmax, ns[act[a]]);
+    const uint64_t nka = off[cbase[a + 1]] - off[cbase[a]];
+    const uint64_t ones = ls->desc[act[a]].ones;
+    const uint64_t avg = ones / (nka > 1 ? nka - 1 : 1);
+    uint32_t lg = 0;
+    while (lg < 20 && (2ull << lg) <= avg) lg++;
+    lut_logs[a] = lg;
+    lut_total += wtg_align256(((ones >> lg) + 1) * 4) / 4;
+    nkmax = std::max<uint32_t>(nkmax, (uint32_t)nka);
   }
   float* tgt = nullptr;
   WTG_TRY(alloc_block(ctx, ls, ns_total * 4, (void**)&tgt));
+  uint32_t* lut = nullptr;
+  WTG_TRY(alloc_block(ctx, ls, lut_total * 4, (void**)&lut));
+  WTG_TRY(wtg_fill_bytes_zero(ctx, lut, lut_total * 4));  // every entry is written below; 0 keeps reads in range
   for (uint32_t a = 0; a < na; a++) {
     const uint32_t b = act[a];
     LselDesc& d = ls->desc[b];
@@ -859,32 +885,31 @@ static int lsel_models(wtg_ctx* ctx, wtg_lsel* ls, const wtg_lsel_params& P, con
     tgt += wtg_align256(ns[b] * 4) / 4;
     d.knots = d_K + k0;
     d.nknots = (uint32_t)(k1 - k0);
-    uint32_t lg = 0;
-    const uint64_t avg = d.ones / (d.nknots > 1 ? d.nknots - 1 : 1);
-    while (lg < 20 && (2ull << lg) <= avg) lg++;
+    const uint32_t lg = lut_logs[a];
     d.lut_log = lg;
-    uint32_t* lut = nullptr;
-    const uint64_t nlut = (d.ones >> lg) + 1;
-    WTG_TRY(alloc_block(ctx, ls, (nlut + 2) * 4, (void**)&lut));
-    WTG_TRY(wtg_fill_u64(ctx, lut, (nlut + 1) / 2, 0));  // every entry is written below; 0 keeps reads in range
     d.lut = lut;
-    WTG_LAUNCH(ctx, "lsel_lut", k_lsel_lut, dim3((unsigned)wtg_ceil_div(d.nknots, 256)), dim3(256), 0, d.knots,
-               d.nknots, d.ones, lg, lut);
-    WTG_LAUNCH_CHECK();
+    lut += wtg_align256(((d.ones >> lg) + 1) * 4) / 4;
     d.learned = 1;
     ls->info[b].nknots = d.nknots;
     ls->info[b].lut_log = lg;
   }
   kfirst[na] = nk_total;
-  TrainVec* d_tv = nullptr;
-  unsigned long long* d_eps = nullptr;
-  double* d_loss = nullptr;
-  WTG_TRY(alloc_block(ctx, ls, na * sizeof(TrainVec), (void**)&d_tv));
-  WTG_TRY(alloc_block(ctx, ls, nvec * 8, (void**)&d_eps));
-  WTG_TRY(alloc_block(ctx, ls, na * 8, (void**)&d_loss));
+  // model tables: TrainVec[na] | eps[nvec] | loss[na] | act[na]
+  const size_t o_eps = wtg_align256(na * sizeof(TrainVec)), o_loss = o_eps + wtg_align256(nvec * 8);
+  const size_t o_act = o_loss + wtg_align256(na * 8);
+  char* tabs = nullptr;
+  WTG_TRY(alloc_block(ctx, ls, o_act + wtg_align256(na * 4), (void**)&tabs));
+  TrainVec* d_tv = (TrainVec*)tabs;
+  unsigned long long* d_eps = (unsigned long long*)(tabs + o_eps);
+  double* d_loss = (double*)(tabs + o_loss);
+  uint32_t* d_act = (uint32_t*)(tabs + o_act);
   WTG_CHECK_HIP(hipMemcpyAsync(d_tv, tv.data(), na * sizeof(TrainVec), hipMemcpyHostToDevice, ctx->stream));
   WTG_CHECK_HIP(hipMemcpyAsync(d_kfirst, kfirst.data(), (na + 1) * 8, hipMemcpyHostToDevice, ctx->stream));
+  WTG_CHECK_HIP(hipMemcpyAsync(d_act, act.data(), na * 4, hipMemcpyHostToDevice, ctx->stream));
   WTG_TRY(lsel_upload_desc(ctx, ls));
+  WTG_LAUNCH(ctx, "lsel_lut", k_lsel_lut, dim3((unsigned)std::min<uint64_t>(wtg_ceil_div(nkmax, 256), 1024), na),
+             dim3(256), 0, ls->d_desc, d_act);
+  WTG_LAUNCH_CHECK();
   TrainBufs tb{d_K, m1, m2, gl, gr, le, d_kfirst};
   const unsigned gk = (unsigned)wtg_ceil_div(nk_total, 256);
   TrainItems items{};
@@ -896,9 +921,6 @@ static int lsel_models(wtg_ctx* ctx, wtg_lsel* ls, const wtg_lsel_params& P, con
     WTG_TRY(train_items_setup(ctx, d_tv, na, tb, nk_total, cap, nsmax, mem, items));
   }
   // eps (exhaustive) and training loss of the knots now in d_K
-  uint32_t* d_act = nullptr;
-  WTG_TRY(alloc_block(ctx, ls, na * 4 + 256, (void**)&d_act));
-  WTG_CHECK_HIP(hipMemcpyAsync(d_act, act.data(), na * 4, hipMemcpyHostToDevice, ctx->stream));
   uint64_t wmax = 1;
   for (uint32_t a = 0; a < na; a++) wmax = std::max<uint64_t>(wmax, (nbits[act[a]] + 63) / 64);
   const unsigned ge_x = (unsigned)std::max<uint64_t>(1, std::min<uint64_t>(wtg_ceil_div(wmax, 256 * LSEL_EPS_W), 256));
@@ -958,6 +980,66 @@ static int lsel_models(wtg_ctx* ctx, wtg_lsel* ls, const wtg_lsel_params& P, con
   return WTG_OK;
 }
 
+// rank9 directories and select1 structures of all vectors, in three host
+// reads: none between the vectors' rank passes, one for the ones counts, one
+// for the long-superblock counts
+static int lsel_plain_build(wtg_ctx* ctx, wtg_lsel* ls, const uint64_t* const* d_words, const uint64_t* nbits) {
+  const uint32_t nvec = ls->nvec;
+  uint64_t dw = 0;
+  size_t stride = 0, worst = 0;
+  for (uint32_t b = 0; b < nvec; b++) {
+    dw += wtg_align256(wtg_rank9_words(nbits[b]) * 8) / 8;
+    stride = std::max(stride, wtg_align256(wtg_rank_select_ws_bytes(nbits[b])));
+    worst += wtg_select_arena_worst(nbits[b]);
+  }
+  uint64_t* dirs = nullptr;
+  WTG_TRY(alloc_block(ctx, ls, dw * 8, (void**)&dirs));
+  for (uint32_t b = 0; b < nvec; b++) {
+    ls->dirs[b] = dirs;
+    dirs += wtg_align256(wtg_rank9_words(nbits[b]) * 8) / 8;
+  }
+  // per-vector workspaces, then the gathered counts
+  WTG_TRY(wtg_ws_reserve(ctx, stride * nvec + (size_t)nvec * 8 + 256));
+  char* ws = (char*)ctx->ws;
+  uint64_t* d_cnt = (uint64_t*)(ws + stride * nvec);
+  wtg_arena ar[3]{};
+  WTG_TRY(alloc_block(ctx, ls, worst, (void**)&ar[0].base));
+  ar[0].size = worst;
+  std::vector<wtg_rs_pending> pend(nvec);
+  for (uint32_t b = 0; b < nvec; b++)
+    WTG_TRY(wtg_rank_select_begin(ctx, d_words[b], nbits[b], ls->dirs[b], true, &ls->sel[b], nullptr,
+                                  WTG_ONES_DEFER, &pend[b], ws + stride * b, &ar[0]));
+  std::vector<uint64_t> cnt(nvec);
+  auto gather = [&](size_t off, uint32_t bytes) -> int {
+    WTG_LAUNCH(ctx, "lsel_gather", k_lsel_gather, dim3((unsigned)wtg_ceil_div(nvec, 256)), dim3(256), 0, ws, stride,
+               off, nvec, bytes, d_cnt);
+    WTG_LAUNCH_CHECK();
+    return sync_copy(ctx, cnt.data(), d_cnt, (size_t)nvec * 8, hipMemcpyDeviceToHost);
+  };
+  WTG_TRY(gather(WTG_RS_ONES_OFF, 8));
+  size_t after = 0;
+  for (uint32_t b = 0; b < nvec; b++) after += wtg_select_arena_count(cnt[b]);
+  WTG_TRY(alloc_block(ctx, ls, after, (void**)&ar[1].base));
+  ar[1].size = after;
+  for (uint32_t b = 0; b < nvec; b++) {
+    pend[b].ar = &ar[1];
+    WTG_TRY(wtg_rank_select_count(ctx, &pend[b], cnt[b]));
+  }
+  WTG_TRY(gather(WTG_RS_NL_OFF + 4, 4));  // the select1 structure's long count
+  size_t pool = 0;
+  for (uint32_t b = 0; b < nvec; b++) pool += wtg_select_arena_pool((uint32_t)cnt[b]);
+  if (pool) {
+    WTG_TRY(alloc_block(ctx, ls, pool, (void**)&ar[2].base));
+    ar[2].size = pool;
+  }
+  for (uint32_t b = 0; b < nvec; b++) {
+    pend[b].ar = &ar[2];
+    const uint32_t nl[2] = {0, (uint32_t)cnt[b]};
+    WTG_TRY(wtg_rank_select_end(ctx, &pend[b], nl));
+  }
+  return WTG_OK;
+}
+
 extern "C" int wtg_lsel_build(wtg_ctx* ctx, uint32_t nvec, const uint64_t* const* d_words,
                               const uint64_t* nbits, const wtg_lsel_params* prm, wtg_lsel** out) {
   WTG_REQUIRE(ctx && out && d_words && nbits && nvec >= 1 && nvec <= 256, "wtg_lsel_build: bad argument (1 <= nvec <= 256)");
@@ -983,21 +1065,12 @@ extern "C" int wtg_lsel_build(wtg_ctx* ctx, uint32_t nvec, const uint64_t* const
   std::vector<uint64_t*> pos(nvec, nullptr);
   std::vector<uint64_t> ns(nvec, 0);
   std::vector<uint32_t> act;  // vectors that get a model
-  // 1. the plain method: rank9 + select1 per vector (the directories in one block)
+  // 1. the plain method: rank9 + select1 of every vector.  Each vector has its
+  //    own workspace, so one host read brings the ones counts of all vectors
+  //    and one the long counts; directories and select content sit in blocks.
   for (uint32_t b = 0; b < nvec; b++) std::memset(&ls->sel[b], 0, sizeof(wtg_select));
-  {
-    uint64_t dw = 0;
-    for (uint32_t b = 0; b < nvec; b++) dw += wtg_align256(wtg_rank9_words(nbits[b]) * 8) / 8;
-    uint64_t* dirs = nullptr;
-    if ((rc = alloc_block(ctx, ls, dw * 8, (void**)&dirs)) != WTG_OK) goto fail;
-    for (uint32_t b = 0; b < nvec; b++) {
-      ls->dirs[b] = dirs;
-      dirs += wtg_align256(wtg_rank9_words(nbits[b]) * 8) / 8;
-    }
-  }
-  for (uint32_t b = 0; b < nvec && rc == WTG_OK; b++) {
-    if ((rc = wtg_rank_select_run(ctx, d_words[b], nbits[b], ls->dirs[b], true, &ls->sel[b], nullptr)) != WTG_OK)
-      break;
+  if ((rc = lsel_plain_build(ctx, ls, d_words, nbits)) != WTG_OK) goto fail;
+  for (uint32_t b = 0; b < nvec; b++) {
     LselDesc& d = ls->desc[b];
     std::memset(&d, 0, sizeof(d));
     d.words = d_words[b];
@@ -1010,7 +1083,6 @@ extern "C" int wtg_lsel_build(wtg_ctx* ctx, uint32_t nvec, const uint64_t* const
     ls->info[b].nbits = nbits[b];
     ls->info[b].ones = s.arg_cnt;
   }
-  if (rc != WTG_OK) goto fail;
   if ((rc = wtg_malloc(ctx, (void**)&ls->d_desc, nvec * sizeof(LselDesc))) != WTG_OK) goto fail;
   if ((rc = lsel_upload_desc(ctx, ls)) != WTG_OK) goto fail;
   // 2. training samples of every vector (one launch), then (one host read of

@@ -324,13 +324,24 @@ struct SelTmp {
   unsigned long long* last = nullptr;
 };
 
-int sel_alloc_worst(wtg_ctx* ctx, uint64_t nbits, int bit, wtg_select* s) {
+// device memory of a select structure: its own allocation, or the next
+// 256-byte-aligned piece of the arena
+int sel_take(wtg_ctx* ctx, wtg_arena* ar, void** p, uint64_t bytes) {
+  if (!ar) return wtg_malloc(ctx, p, bytes);
+  const size_t a = wtg_align256(bytes);
+  WTG_REQUIRE(ar->used + a <= ar->size, "select arena exhausted");
+  *p = ar->base + ar->used;
+  ar->used += a;
+  return WTG_OK;
+}
+
+int sel_alloc_worst(wtg_ctx* ctx, uint64_t nbits, int bit, wtg_select* s, wtg_arena* ar) {
   std::memset(s, 0, sizeof(*s));
   s->nbits = nbits;
   s->bit = bit;
   const uint64_t sbmax = wtg_ceil_div(nbits, 4096) + 1;
-  WTG_TRY(wtg_malloc(ctx, (void**)&s->mini, sbmax * 64 * 4));
-  WTG_TRY(wtg_malloc(ctx, (void**)&s->superblock, sbmax * 8));
+  WTG_TRY(sel_take(ctx, ar, (void**)&s->mini, sbmax * 64 * 4));
+  WTG_TRY(sel_take(ctx, ar, (void**)&s->superblock, sbmax * 8));
   return WTG_OK;
 }
 
@@ -359,7 +370,7 @@ FinScratch fin_scratch(char* scratch, uint64_t sb) {
 // writes the number of long superblocks to *d_nl; sel_fill allocates and
 // fills the long pool once that number is on the host
 int sel_count(wtg_ctx* ctx, uint64_t nbits, wtg_select* s, uint64_t m, const uint64_t* sbend,
-              const unsigned long long* last, char* scratch, uint32_t* d_nl) {
+              const unsigned long long* last, char* scratch, uint32_t* d_nl, wtg_arena* ar) {
   const uint64_t cap = (nbits + 63) / 64 * 64;
   s->arg_cnt = m;
   s->logn = (uint32_t)wtg_hi_host(cap) + 1;
@@ -367,9 +378,9 @@ int sel_count(wtg_ctx* ctx, uint64_t nbits, wtg_select* s, uint64_t m, const uin
   s->sb = wtg_ceil_div(m, 4096);
   s->n_long = 0;
   const uint64_t sb = s->sb;
-  WTG_TRY(wtg_malloc(ctx, (void**)&s->mini_width, sb ? sb : 1));
-  WTG_TRY(wtg_malloc(ctx, (void**)&s->long_width, sb ? sb : 1));
-  WTG_TRY(wtg_malloc(ctx, (void**)&s->long_index, (sb ? sb : 1) * 8));
+  WTG_TRY(sel_take(ctx, ar, (void**)&s->mini_width, sb ? sb : 1));
+  WTG_TRY(sel_take(ctx, ar, (void**)&s->long_width, sb ? sb : 1));
+  WTG_TRY(sel_take(ctx, ar, (void**)&s->long_index, (sb ? sb : 1) * 8));
   if (sb == 0) return wtg_fill_bytes_zero(ctx, d_nl, 4);
   const FinScratch f = fin_scratch(scratch, sb);
   uint32_t* flag = f.flag;
@@ -394,12 +405,12 @@ int sel_count(wtg_ctx* ctx, uint64_t nbits, wtg_select* s, uint64_t m, const uin
 }
 
 int sel_fill(wtg_ctx* ctx, const uint64_t* d_words, uint64_t nbits, wtg_select* s, uint32_t nl,
-             const uint64_t* sbend, const unsigned long long* last, const char* scratch) {
+             const uint64_t* sbend, const unsigned long long* last, const char* scratch, wtg_arena* ar) {
   const uint64_t sb = s->sb;
   s->n_long = nl;
   if (sb == 0 || nl == 0) return WTG_OK;
   const FinScratch f = fin_scratch(const_cast<char*>(scratch), sb);
-  WTG_TRY(wtg_malloc(ctx, (void**)&s->long_pool, (uint64_t)nl * 4096 * 8));
+  WTG_TRY(sel_take(ctx, ar, (void**)&s->long_pool, (uint64_t)nl * 4096 * 8));
   WTG_TRY(wtg_fill_u64(ctx, s->long_pool, (uint64_t)nl * 4096, 0));
   WTG_LAUNCH(ctx, "select_longfill", k_long_fill, dim3(nl), dim3(LF_THREADS), 0, d_words, nbits, s->bit,
              s->superblock, sbend, last, f.list, sb, s->long_pool);
@@ -407,35 +418,64 @@ int sel_fill(wtg_ctx* ctx, const uint64_t* d_words, uint64_t nbits, wtg_select* 
   return WTG_OK;
 }
 
+// workspace: counter | ones_total | last[2] | long counts[2] (256 bytes, first)
+// | states | tile sums | tile prefixes | scan states | sbend[2] | finish scratch[2]
+struct RsLayout {
+  uint64_t W, S, tiles, wtiles, stiles, sbmax;
+  size_t o_ctr, o_states, o_tsum, o_tpre, o_scst, o_end0, o_end1, fin_bytes, o_fin, total;
+};
+RsLayout rs_layout(uint64_t nbits) {
+  RsLayout L{};
+  const uint64_t cap = (nbits + 63) / 64 * 64;
+  L.W = cap >> 6;
+  L.S = cap >> 9;
+  L.tiles = wtg_ceil_div(L.S + 1, 256);
+  L.sbmax = wtg_ceil_div(nbits, 4096) + 1;
+  // word tiles: every entry 0..S has its first word 8b inside them
+  L.wtiles = wtg_ceil_div(8 * (L.S + 1), RS_TILE_WORDS);
+  L.stiles = wtg_ceil_div(L.wtiles, 4096);
+  size_t o = 0;
+  L.o_ctr = o; o = wtg_align256(o + 256);
+  L.o_states = o; o = wtg_align256(o + (L.tiles + 1) * 8);
+  L.o_tsum = o; o = wtg_align256(o + (L.wtiles + 1) * 4);
+  L.o_tpre = o; o = wtg_align256(o + (L.wtiles + 1) * 8);
+  L.o_scst = o; o = wtg_align256(o + (L.stiles + 1) * 8);
+  L.o_end0 = o; o = wtg_align256(o + L.sbmax * 8);
+  L.o_end1 = o; o = wtg_align256(o + L.sbmax * 8);
+  L.fin_bytes = wtg_align256(4 * wtg_align256((L.sbmax + 1) * 8) + 4096);
+  L.o_fin = o; o = wtg_align256(o + 2 * L.fin_bytes);
+  L.total = o;
+  return L;
+}
+static_assert(WTG_RS_ONES_OFF == 64 && WTG_RS_NL_OFF == 192, "counter block layout");
+
 }  // namespace
 
-int wtg_rank_select_begin(wtg_ctx* ctx, const uint64_t* d_words, uint64_t nbits, uint64_t* d_dir, bool build,
-                          wtg_select* sel1, wtg_select* sel0, uint64_t known_ones, wtg_rs_pending* pend) {
-  *pend = wtg_rs_pending{};
-  const uint64_t cap = (nbits + 63) / 64 * 64;
-  const uint64_t W = cap >> 6, S = cap >> 9;
-  const uint64_t entries = S + 1;
-  const uint64_t tiles = wtg_ceil_div(entries, 256);
+size_t wtg_rank_select_ws_bytes(uint64_t nbits) { return rs_layout(nbits).total; }
+size_t wtg_select_arena_worst(uint64_t nbits) {
   const uint64_t sbmax = wtg_ceil_div(nbits, 4096) + 1;
-  // workspace: states | counter | ones_total | last[2] | long counts[2] | sbend[2] | finish scratch[2]
-  size_t o = 0;
-  const size_t o_states = o; o = wtg_align256(o + (tiles + 1) * 8);
-  // word tiles: every entry 0..S has its first word 8b inside them
-  const uint64_t wtiles = wtg_ceil_div(8 * (S + 1), RS_TILE_WORDS);
-  const uint64_t stiles = wtg_ceil_div(wtiles, 4096);
-  const size_t o_tsum = o; o = wtg_align256(o + (wtiles + 1) * 4);
-  const size_t o_tpre = o; o = wtg_align256(o + (wtiles + 1) * 8);
-  const size_t o_scst = o; o = wtg_align256(o + (stiles + 1) * 8);
-  const size_t o_ctr = o; o = wtg_align256(o + 256);
-  const size_t o_end0 = o; o = wtg_align256(o + sbmax * 8);
-  const size_t o_end1 = o; o = wtg_align256(o + sbmax * 8);
-  const size_t fin_bytes = wtg_align256(4 * wtg_align256((sbmax + 1) * 8) + 4096);
-  const size_t o_fin = o; o = wtg_align256(o + 2 * fin_bytes);
-  WTG_TRY(wtg_ws_reserve(ctx, o));
-  char* ws = (char*)ctx->ws;
+  return wtg_align256(sbmax * 64 * 4) + wtg_align256(sbmax * 8);
+}
+size_t wtg_select_arena_count(uint64_t ones) {
+  const uint64_t sb = std::max<uint64_t>(wtg_ceil_div(ones, 4096), 1);
+  return 2 * wtg_align256(sb) + wtg_align256(sb * 8);
+}
+size_t wtg_select_arena_pool(uint32_t n_long) { return wtg_align256((uint64_t)n_long * 4096 * 8); }
+
+int wtg_rank_select_begin(wtg_ctx* ctx, const uint64_t* d_words, uint64_t nbits, uint64_t* d_dir, bool build,
+                          wtg_select* sel1, wtg_select* sel0, uint64_t known_ones, wtg_rs_pending* pend,
+                          char* ws_ext, wtg_arena* ar) {
+  *pend = wtg_rs_pending{};
+  const RsLayout L = rs_layout(nbits);
+  const uint64_t W = L.W, S = L.S, wtiles = L.wtiles, stiles = L.stiles;
+  const size_t o_ctr = L.o_ctr, o_states = L.o_states, o_tsum = L.o_tsum, o_tpre = L.o_tpre;
+  const size_t o_scst = L.o_scst, o_end0 = L.o_end0, o_end1 = L.o_end1, o_fin = L.o_fin;
+  const size_t fin_bytes = L.fin_bytes;
+  if (!ws_ext) WTG_TRY(wtg_ws_reserve(ctx, L.total));
+  char* ws = ws_ext ? ws_ext : (char*)ctx->ws;
   WTG_TRY(wtg_fill_bytes_zero(ctx, ws, o_end0));
   uint32_t* ctr = (uint32_t*)(ws + o_ctr);
-  unsigned long long* ones_total = (unsigned long long*)(ws + o_ctr + 64);
+  unsigned long long* ones_total = (unsigned long long*)(ws + o_ctr + WTG_RS_ONES_OFF);
   unsigned long long* last = (unsigned long long*)(ws + o_ctr + 128);  // [0]=zeros, [1]=ones
   wtg_select* outs[2] = {sel0, sel1};
   RankSelArgs ra{};
@@ -445,7 +485,7 @@ int wtg_rank_select_begin(wtg_ctx* ctx, const uint64_t* d_words, uint64_t nbits,
   for (int t = 0; t < 2; t++) {
     ra.want[t] = outs[t] != nullptr && nbits > 0;
     if (ra.want[t]) {
-      WTG_TRY(sel_alloc_worst(ctx, nbits, t, outs[t]));
+      WTG_TRY(sel_alloc_worst(ctx, nbits, t, outs[t], ar));
       ra.sel[t].mini = outs[t]->mini;
       ra.sel[t].sbstart = outs[t]->superblock;
       ra.sel[t].sbend = (uint64_t*)(ws + (t ? o_end1 : o_end0));
@@ -459,9 +499,9 @@ int wtg_rank_select_begin(wtg_ctx* ctx, const uint64_t* d_words, uint64_t nbits,
     if (build) WTG_TRY(wtg_fill_u64(ctx, d_dir, 2, 0));
     for (int t = 0; t < 2; t++)
       if (outs[t]) {
-        WTG_TRY(wtg_malloc(ctx, (void**)&outs[t]->mini_width, 1));
-        WTG_TRY(wtg_malloc(ctx, (void**)&outs[t]->long_width, 1));
-        WTG_TRY(wtg_malloc(ctx, (void**)&outs[t]->long_index, 8));
+        WTG_TRY(sel_take(ctx, ar, (void**)&outs[t]->mini_width, 1));
+        WTG_TRY(sel_take(ctx, ar, (void**)&outs[t]->long_width, 1));
+        WTG_TRY(sel_take(ctx, ar, (void**)&outs[t]->long_index, 8));
         outs[t]->logn = 1;
         outs[t]->logn4 = 1;
       }
@@ -484,6 +524,20 @@ int wtg_rank_select_begin(wtg_ctx* ctx, const uint64_t* d_words, uint64_t nbits,
   }
   WTG_LAUNCH_CHECK();
   if (!ra.want[0] && !ra.want[1]) return WTG_OK;  // rank9 alone: no count of ones needed
+  pend->active = true;
+  pend->d_words = d_words;
+  pend->nbits = nbits;
+  pend->d_nl = (const uint32_t*)(ws + o_ctr + WTG_RS_NL_OFF);
+  pend->ar = ar;
+  for (int t = 0; t < 2; t++) {
+    pend->want[t] = ra.want[t];
+    pend->outs[t] = ra.want[t] ? outs[t] : nullptr;
+    pend->sbend[t] = ra.sel[t].sbend;
+    pend->last[t] = last + t;
+    pend->fin[t] = ws + o_fin + t * fin_bytes;
+  }
+  WTG_REQUIRE(known_ones != WTG_ONES_DEFER || (build && ws_ext), "rank/select: a deferred count needs its own workspace");
+  if (known_ones == WTG_ONES_DEFER) return WTG_OK;
   uint64_t m1 = 0;
   if (build && known_ones != WTG_ONES_UNKNOWN) {
     m1 = known_ones;
@@ -503,22 +557,17 @@ int wtg_rank_select_begin(wtg_ctx* ctx, const uint64_t* d_words, uint64_t nbits,
     m1 = e[0];
     for (uint64_t k = 0; k < nt; k++) m1 += __builtin_popcountll(tailw[k]);
   }
-  const uint64_t m[2] = {nbits - m1, m1};
-  uint32_t* d_nl = (uint32_t*)(ws + o_ctr + 192);
+  return wtg_rank_select_count(ctx, pend, m1);
+}
+
+int wtg_rank_select_count(wtg_ctx* ctx, wtg_rs_pending* pend, uint64_t ones) {
+  if (!pend->active) return WTG_OK;
+  const uint64_t m[2] = {pend->nbits - ones, ones};
+  uint32_t* d_nl = const_cast<uint32_t*>(pend->d_nl);
   for (int t = 0; t < 2; t++)
-    if (ra.want[t])
-      WTG_TRY(sel_count(ctx, nbits, outs[t], m[t], ra.sel[t].sbend, last + t, ws + o_fin + t * fin_bytes, d_nl + t));
-  if (!ra.want[0] && !ra.want[1]) return WTG_OK;
-  pend->active = true;
-  pend->d_words = d_words;
-  pend->nbits = nbits;
-  pend->d_nl = d_nl;
-  for (int t = 0; t < 2; t++) {
-    pend->outs[t] = ra.want[t] ? outs[t] : nullptr;
-    pend->sbend[t] = ra.sel[t].sbend;
-    pend->last[t] = last + t;
-    pend->fin[t] = ws + o_fin + t * fin_bytes;
-  }
+    if (pend->want[t])
+      WTG_TRY(sel_count(ctx, pend->nbits, pend->outs[t], m[t], pend->sbend[t], pend->last[t],
+                        const_cast<char*>(pend->fin[t]), d_nl + t, pend->ar));
   return WTG_OK;
 }
 
@@ -527,7 +576,7 @@ int wtg_rank_select_end(wtg_ctx* ctx, const wtg_rs_pending* pend, const uint32_t
   for (int t = 0; t < 2; t++)
     if (pend->outs[t])
       WTG_TRY(sel_fill(ctx, pend->d_words, pend->nbits, pend->outs[t], nl[t], pend->sbend[t], pend->last[t],
-                       pend->fin[t]));
+                       pend->fin[t], pend->ar));
   return WTG_OK;
 }
 
