@@ -608,48 +608,94 @@ __global__ __launch_bounds__(256) void k_lsel_loss(const TrainVec* __restrict__ 
   if (threadIdx.x == 0) out[b] = s[0];
 }
 
-// exhaustive error window of vector vids[blockIdx.y]: thread per run of
-// LSEL_EPS_W consecutive words, every one of them.  The ones of a run have
-// consecutive arguments, so the run looks its knot interval up once (at its
-// first one) and moves it forward from there: the interval lsel_predict finds
-// for each argument, and the same arithmetic (lsel_eval).
+// exact error window eps = max over every one j of |pred(j) - select(j)| of
+// vector vids[blockIdx.y], in two passes over its words (thread per run of
+// LSEL_EPS_W consecutive words; the run looks its knot interval up once and
+// moves it forward, the interval lsel_predict finds for each argument, and
+// the same arithmetic, lsel_eval).
+// Bound pass (EXACT = false): the exact errors of every word's first and last
+// one; their maximum (a lower bound of eps) goes to eps[b].
+// Exact pass: inside one knot interval the prediction is linear in j (up to
+// the rounding of its float and double steps, within `margin`), so a word's
+// predictions lie between those of its first and last one and its positions
+// between theirs: no error of the word exceeds
+// max(qmax - pfirst, plast - qmin) + margin.  Only a word whose bound exceeds
+// the first pass's maximum (or whose ones span two intervals) can raise it;
+// its ones are evaluated one by one.
 constexpr uint32_t LSEL_EPS_W = 16;
+template <bool EXACT>
 __global__ __launch_bounds__(256) void k_lsel_eps(const LselDesc* __restrict__ descs,
                                                   const uint32_t* __restrict__ vids,
                                                   unsigned long long* __restrict__ eps) {
   const uint32_t b = vids[blockIdx.y];
   const LselDesc d = descs[b];
   const uint64_t W = (d.nbits + 63) >> 6;
+  const uint64_t lb = EXACT ? (uint64_t)eps[b] : 0ull;
   uint64_t mx = 0;
+  auto dist = [](uint64_t q, uint64_t p) { return q > p ? q - p : p - q; };
   for (uint64_t w0 = ((uint64_t)blockIdx.x * blockDim.x + threadIdx.x) * LSEL_EPS_W; w0 < W;
        w0 += (uint64_t)gridDim.x * blockDim.x * LSEL_EPS_W) {
-    const uint64_t w1 = w0 + LSEL_EPS_W < W ? w0 + LSEL_EPS_W : W;
+    uint64_t xs[LSEL_EPS_W];
+#pragma unroll
+    for (uint32_t q = 0; q < LSEL_EPS_W; q++) xs[q] = w0 + q < W ? d.words[w0 + q] : 0ull;
     uint64_t j = rank9_dev(d.dir, d.words, w0 << 6);  // ones before the run
     uint32_t k = 0;
     Knot ka{0u, 0.f}, kb{0u, 0.f};
     bool have = false;
-    for (uint64_t w = w0; w < w1; w++) {
-      uint64_t x = d.words[w];
-      while (x) {
-        const uint32_t o = (uint32_t)__builtin_ctzll(x);
-        x &= x - 1;
-        j++;
-        if (!have) {
-          k = d.lut[j >> d.lut_log];
-          ka = d.knots[k];
-          kb = d.knots[k + 1];
-          have = true;
-        }
-        lsel_advance(d, j, k, ka, kb);
-        const uint64_t p = (w << 6) + o;
-        const uint64_t q = lsel_eval(d, ka, kb, j);
-        const uint64_t e = q > p ? q - p : p - q;
-        mx = e > mx ? e : mx;
+#pragma unroll
+    for (uint32_t q = 0; q < LSEL_EPS_W; q++) {
+      const uint64_t x = xs[q];
+      if (!x) continue;
+      const uint64_t w = w0 + q;
+      const uint32_t c = popc64(x);
+      const uint64_t jf = j + 1, jl = j + c;
+      if (!have) {
+        k = d.lut[jf >> d.lut_log];
+        ka = d.knots[k];
+        kb = d.knots[k + 1];
+        have = true;
       }
+      lsel_advance(d, jf, k, ka, kb);
+      const uint32_t kf = k;
+      const Knot fa = ka, fb = kb;
+      const uint64_t pf = (w << 6) + (uint64_t)__builtin_ctzll(x);
+      const uint64_t pl = (w << 6) + 63 - (uint64_t)__builtin_clzll(x);
+      const uint64_t qf = lsel_eval(d, ka, kb, jf);
+      uint64_t ql = qf;
+      if (c > 1) {
+        lsel_advance(d, jl, k, ka, kb);
+        ql = lsel_eval(d, ka, kb, jl);
+      }
+      if (!EXACT) {
+        const uint64_t e = dist(qf, pf) > dist(ql, pl) ? dist(qf, pf) : dist(ql, pl);
+        mx = e > mx ? e : mx;
+      } else {
+        bool cand = k != kf;
+        if (!cand && c > 2) {
+          const uint64_t qmax = qf > ql ? qf : ql, qmin = qf < ql ? qf : ql;
+          const uint64_t margin = 2 + (uint64_t)((fabsf(fa.y) + fabsf(fb.y)) * 0x1p-19f);
+          const uint64_t u1 = qmax > pf ? qmax - pf : 0, u2 = pl > qmin ? pl - qmin : 0;
+          cand = (u1 > u2 ? u1 : u2) + margin > lb;
+        }
+        if (cand) {
+          uint32_t k2 = kf;
+          Knot a2 = fa, b2 = fb;
+          uint64_t xx = x, jj = j;
+          while (xx) {
+            const uint32_t o = (uint32_t)__builtin_ctzll(xx);
+            xx &= xx - 1;
+            jj++;
+            lsel_advance(d, jj, k2, a2, b2);
+            const uint64_t e = dist(lsel_eval(d, a2, b2, jj), (w << 6) + o);
+            mx = e > mx ? e : mx;
+          }
+        }
+      }
+      j += c;
     }
   }
   mx = wave_max_u64(mx);
-  if (lane_id() == 0 && mx) atomicMax(&eps[b], (unsigned long long)mx);
+  if (lane_id() == 0 && mx > lb) atomicMax(&eps[b], (unsigned long long)mx);
 }
 
 // training samples of every vector in one launch: pos[u-1] = select1(u*T)
@@ -928,7 +974,9 @@ static int lsel_models(wtg_ctx* ctx, wtg_lsel* ls, const wtg_lsel_params& P, con
     WTG_TRY(wtg_fill_u64(ctx, d_eps, nvec, 0));
     WTG_TRY(train_grad(ctx, d_tv, tb, nk_total, items));
     // every model in one launch each: blocks of a vector stride over its words
-    WTG_LAUNCH(ctx, "lsel_eps", k_lsel_eps, dim3(ge_x, na), dim3(256), 0, ls->d_desc, d_act, d_eps);
+    WTG_LAUNCH(ctx, "lsel_eps", k_lsel_eps<false>, dim3(ge_x, na), dim3(256), 0, ls->d_desc, d_act, d_eps);
+    WTG_LAUNCH_CHECK();
+    WTG_LAUNCH(ctx, "lsel_eps", k_lsel_eps<true>, dim3(ge_x, na), dim3(256), 0, ls->d_desc, d_act, d_eps);
     WTG_LAUNCH_CHECK();
     WTG_LAUNCH(ctx, "lsel_loss", k_lsel_loss, dim3(na), dim3(256), 0, d_tv, tb, d_loss);
     WTG_LAUNCH_CHECK();
