@@ -169,6 +169,21 @@ int wtg_select_free(wtg_ctx* ctx, wtg_select* s);
 int wtg_select_batch(wtg_ctx* ctx, const wtg_select* s, const uint64_t* d_words,
                      const uint64_t* d_i, uint64_t nq, uint64_t* d_out);
 
+/* ------------------------------------------------ (b'') two-bit patterns */
+/* The argument words of sdsl's two-bit patterns, for rank_support_v<t_b,2>
+ * (rank/rank_support_v_par.hpp:54-58, :84-105) and select_support_mcl<t_b,2>
+ * (select/select_support_mcl_par.hpp:72-77, :227-276): bit i of the output is 1
+ * when bits i-1 and i of the vector read `10` (pattern = 10) or `01`
+ * (pattern = 1, the template argument 01); before bit 0 sits the bit that
+ * makes position 0 no occurrence (0 for `10`, 1 for `01`), each word's carry
+ * is the previous word's top bit.  The rank9 directory and the select content
+ * of the output words are those of the pattern: rank_support_v counts whole
+ * words (clip = 0: an occurrence in the padding bits past nbits counts, as the
+ * reference's block pass does), select_support_mcl counts positions below
+ * nbits (clip = 1).  d_out: ceil(nbits/64) words. */
+int wtg_pattern_words(wtg_ctx* ctx, const uint64_t* d_words, uint64_t nbits, uint32_t pattern, int clip,
+                      uint64_t* d_out);
+
 /* ------------------------------------------------ wt_int (integer wavelet tree) */
 typedef struct wtg_wtint wtg_wtint;
 /* sdsl::wt_int<>(int_vector<w>& buf, size, max_level)  (wt_int/wt_int_par.hpp:368-406):

@@ -62,6 +62,7 @@ def lib():
         L.orc_rank9_entries.restype = u64
         L.orc_rank9_entries.argtypes = [u64]
         L.orc_rank9_build.argtypes = [vp, u64, vp]
+        L.orc_pattern_words.argtypes = [vp, u64, u32, i32, vp]
         L.orc_rank9_rank.restype = u64
         L.orc_rank9_rank.argtypes = [vp, vp, u64]
         L.orc_select_plan.argtypes = [vp, u64, i32, C.POINTER(OrcSelect)]
@@ -166,6 +167,14 @@ def rank9_build(words, nbits):
     w = padded(words)
     lib().orc_rank9_build(_p(w), int(nbits), _p(D))
     return D
+
+
+def pattern_words(words, nbits, pattern, clip):
+    """argument words of the two-bit pattern `10` (pattern=10) or `01` (pattern=1)"""
+    w = np.ascontiguousarray(words, dtype=np.uint64)
+    out = np.zeros(max(1, (int(nbits) + 63) // 64), np.uint64)
+    lib().orc_pattern_words(_p(w), int(nbits), int(pattern), int(bool(clip)), _p(out))
+    return out
 
 
 def rank9(D, words, idx, nbits=None):

@@ -15,6 +15,9 @@
  *   orc_select_*       -- brute-force position of the i-th argument; structure
  *                         content cross-checked against an independent Python
  *                         restatement of init_slow in the tests.
+ *   orc_pattern_words  -- brute force on the bits: position i > 0 with bits
+ *                         (i-1, i) equal to the pattern, every word boundary
+ *                         and both clip modes (tests).
  *   orc_wtint_*        -- brute force on the text (count / i-th occurrence / value;
  *                         lex counts: values below / above c in [i, j); range
  *                         count: positions in [lb, rb] with values in [vlb, vrb]).
@@ -251,6 +254,23 @@ uint64_t orc_rank9_rank(const uint64_t* D, const uint64_t* words, uint64_t idx) 
 void orc_rank9_rank_batch(const uint64_t* D, const uint64_t* words, uint64_t nbits, const uint64_t* idx,
                           uint64_t nq, uint64_t* out) {
   for (uint64_t q = 0; q < nq; q++) out[q] = idx[q] <= nbits ? orc_rank9_rank(D, words, idx[q]) : ORC_NONE;
+}
+
+/* ------------------------------------------------------ two-bit patterns */
+void orc_pattern_words(const uint64_t* words, uint64_t nbits, uint32_t pattern, int clip, uint64_t* out) {
+  /* sdsl-lite's rank_support_trait / select_support_trait <10,2> and <01,2>
+   * (bits::map10 / map01 with the carry), as the reference's block pass calls
+   * them (rank/rank_support_v_par.hpp:91-105: carry = the previous word's top
+   * bit, init_carry before word 0) and as init_slow's found_arg scan
+   * (select/select_support_mcl_par.hpp:240-245) counts them below size. */
+  uint64_t W = (nbits + 63) / 64, k, carry = pattern == 10 ? 0 : 1;
+  for (k = 0; k < W; k++) {
+    uint64_t x = words[k], prev = (x << 1) | carry, r;
+    r = pattern == 10 ? (x ^ prev) & ~x : (x ^ prev) & x; /* map10 / map01 */
+    if (clip && k + 1 == W && (nbits & 63)) r &= (1ull << (nbits & 63)) - 1;
+    out[k] = r;
+    carry = x >> 63;
+  }
 }
 
 /* ----------------------------------------------------------------- select */

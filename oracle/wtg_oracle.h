@@ -48,6 +48,11 @@ uint64_t orc_rank9_rank(const uint64_t* D, const uint64_t* words, uint64_t idx);
 void orc_rank9_rank_batch(const uint64_t* D, const uint64_t* words, uint64_t nbits, const uint64_t* idx,
                           uint64_t nq, uint64_t* out);
 
+/* argument words of sdsl's two-bit patterns `10` (pattern 10) and `01` (pattern 1):
+ * bit i = (bit i-1, bit i) == the pattern, position 0 never; clip: none at or past
+ * nbits (select's count) instead of whole words (rank's block pass) */
+void orc_pattern_words(const uint64_t* words, uint64_t nbits, uint32_t pattern, int clip, uint64_t* out);
+
 /* select_support_mcl<b,1> content, select/select_support_mcl_par.hpp:227-276 (init_slow)
  * and :483-561 (init_fast) produce the same content. */
 typedef struct {

@@ -68,6 +68,33 @@ int main(int argc, char** argv) {
     dump(out + "/sel1.bin", a.data(), a.size() * 8);
     dump(out + "/sel0.bin", b.data(), b.size() * 8);
   }
+  // ---- two-bit patterns: rank_support_v<10,2> / <01,2>, select_support_mcl<10,2> / <01,2>
+  {
+    const uint64_t nbits = 200003, nw = (nbits + 63) / 64;
+    std::vector<uint64_t> w(nw);
+    wtg::check(wtg_gen_uniform(14, nw, UINT64_MAX, 8, w.data()));
+    w[nw - 1] &= (1ull << (nbits & 63)) - 1;
+    w[nw - 1] |= 1ull << ((nbits - 1) & 63);  // last bit 1: a `10` in the padding
+    sdsl::bit_vector bv(nbits);
+    std::memcpy(bv.data(), w.data(), nw * 8);
+    sdsl::rank_support_v<10, 2> r10(&bv);
+    sdsl::rank_support_v<01, 2> r01(&bv);
+    std::vector<uint64_t> r;
+    for (uint64_t i = 0; i <= nbits; i += 3) { r.push_back(r10(i)); r.push_back(r01.rank(i)); }
+    r.push_back(r10(nbits));
+    r.push_back(r01(nbits));
+    dump(out + "/rank_pat.bin", r.data(), r.size() * 8);
+    dump(out + "/rank10_dir.bin", r10.basic_block().data(), r10.basic_block().size() * 8);
+    sdsl::select_support_mcl<10, 2> s10(&bv);
+    sdsl::select_support_mcl<01, 2> s01(&bv);
+    std::vector<uint64_t> a, b;
+    a.push_back(s10.arg_cnt());
+    b.push_back(s01.arg_cnt());
+    for (uint64_t i = 1; i <= s10.arg_cnt(); i += 2) a.push_back(s10(i));
+    for (uint64_t i = 1; i <= s01.arg_cnt(); i += 2) b.push_back(s01.select(i));
+    dump(out + "/sel10.bin", a.data(), a.size() * 8);
+    dump(out + "/sel01.bin", b.data(), b.size() * 8);
+  }
   // ---- wt_int<>
   {
     const uint64_t n = 50000;

@@ -56,6 +56,27 @@ def test_dropin_driver(wsfm, orc, variant, dt):
         p0 = np.nonzero(bits == 0)[0]
         assert np.array_equal(load(d + "/sel1.bin"), p1[::5].astype(np.uint64))
         assert np.array_equal(load(d + "/sel0.bin"), p0[::5].astype(np.uint64))
+        # two-bit patterns `10` / `01`: rank over whole words, select below size
+        from test_oracle_rank_select import pattern_occ
+        nbits = 200003
+        nw = (nbits + 63) // 64
+        w = wsfm.gen_uniform(14, nw, (1 << 64) - 1, np.uint64)
+        w[-1] &= np.uint64((1 << (nbits & 63)) - 1)
+        w[-1] |= np.uint64(1 << ((nbits - 1) & 63))
+        idx = np.concatenate([np.arange(0, nbits + 1, 3), [nbits]])
+        rp = load(d + "/rank_pat.bin")
+        got = {10: np.concatenate([rp[0:-2:2], rp[-2:-1]]), 1: np.concatenate([rp[1:-2:2], rp[-1:]])}
+        for pat, name in ((10, "sel10"), (1, "sel01")):
+            occ = pattern_occ(w, nw * 64, pat)
+            pref = np.concatenate([[0], np.cumsum(occ)])
+            assert np.array_equal(got[pat], pref[idx].astype(np.uint64)), pat
+            pos = np.nonzero(occ[:nbits])[0]
+            sel = load(d + "/" + name + ".bin")
+            assert sel[0] == pos.size and np.array_equal(sel[1:], pos[::2].astype(np.uint64)), pat
+        occ10 = pattern_occ(w, nw * 64, 10)
+        assert occ10[nbits] == 1  # counted by the whole-word directory, not by select
+        pw = orc.pattern_words(w, nbits, 10, clip=False)
+        assert np.array_equal(load(d + "/rank10_dir.bin"), orc.rank9_build(pw, nbits))
         # wt_int
         m = 50000
         sv = wsfm.gen_uniform(13, m, 5000, np.uint32)

@@ -109,3 +109,17 @@ def test_rank_select_large_properties(ctx, wsfm):
             a.free()
     finally:
         D.free(); s1.free(); s0.free(); dw.free()
+
+
+@pytest.mark.parametrize("nbits", [1, 63, 64, 65, 4099, 100003, (1 << 22) + 77])
+@pytest.mark.parametrize("pattern", [10, 1])
+def test_pattern_words_vs_oracle(ctx, orc, nbits, pattern):
+    """wtg_pattern_words (two-bit patterns) = the oracle's map10 / map01 words,
+    both clip modes, with the last bit set (a `10` in the padding)."""
+    from test_oracle_rank_select import pattern_case
+    say("pattern_words", nbits, pattern)
+    w = pattern_case(nbits, nbits * 7 + pattern, True)
+    dw = ctx.to_device(w)
+    for clip in (False, True):
+        got = ctx.pattern_words(dw, nbits, pattern, clip).get()
+        assert np.array_equal(got, orc.pattern_words(w, nbits, pattern, clip)), clip

@@ -156,3 +156,52 @@ def test_long_superblocks_present(orc):
     w = rand_words(rng, 1 << 20, 0.01)
     S = orc.Select(w, 1 << 20, 1)
     assert S.n_long > 0 and S.n_long <= S.sb
+
+
+# ---- two-bit patterns `10` / `01` (rank_support_v<t_b,2>, select_support_mcl<t_b,2>)
+def pattern_occ(words, nbits_total, pattern):
+    """position i > 0 is an occurrence when bits (i-1, i) read the pattern"""
+    b = np.unpackbits(words.view(np.uint8), bitorder="little")[:nbits_total].astype(np.int64)
+    first, second = (1, 0) if pattern == 10 else (0, 1)
+    occ = np.zeros(nbits_total, np.int64)
+    occ[1:] = (b[:-1] == first) & (b[1:] == second)
+    return occ
+
+
+def pattern_case(nbits, seed, last_one):
+    rng = np.random.default_rng(seed)
+    nw = (nbits + 63) // 64
+    w = rng.integers(0, np.iinfo(np.uint64).max, nw, dtype=np.uint64, endpoint=True)
+    if nbits & 63:
+        w[-1] &= np.uint64((1 << (nbits & 63)) - 1)
+    if last_one:  # bit nbits-1 set: `10` then has an occurrence in the padding
+        w[-1] |= np.uint64(1 << ((nbits - 1) & 63))
+    return w
+
+
+@pytest.mark.parametrize("nbits", [1, 2, 63, 64, 65, 127, 128, 1000, 4099, 100003])
+@pytest.mark.parametrize("pattern", [10, 1])
+@pytest.mark.parametrize("last_one", [False, True])
+def test_pattern_words_vs_bits(orc, nbits, pattern, last_one):
+    w = pattern_case(nbits, nbits * 100 + pattern + last_one, last_one)
+    cap = w.size * 64
+    occ = pattern_occ(w, cap, pattern)  # over whole words (padding bits are 0)
+    pw0 = orc.pattern_words(w, nbits, pattern, clip=False)
+    pw1 = orc.pattern_words(w, nbits, pattern, clip=True)
+    assert np.array_equal(np.unpackbits(pw0.view(np.uint8), bitorder="little")[:cap], occ)
+    clipped = occ.copy()
+    clipped[nbits:] = 0
+    assert np.array_equal(np.unpackbits(pw1.view(np.uint8), bitorder="little")[:cap], clipped)
+    if pattern == 10 and last_one and nbits & 63:
+        assert occ[nbits] == 1  # the padding occurrence the whole-word count includes
+    # rank over whole words: rank(i) = occurrences before i, for every i <= nbits
+    D = orc.rank9_build(pw0, nbits)
+    pref = np.concatenate([[0], np.cumsum(occ)])
+    idx = np.arange(nbits + 1, dtype=np.uint64)
+    assert np.array_equal(orc.rank9(D, pw0, idx, nbits), pref[:nbits + 1].astype(np.uint64))
+    # select below nbits: the i-th occurrence
+    S = orc.Select(pw1, nbits, 1)
+    pos = np.nonzero(occ[:nbits])[0]
+    assert S.arg_cnt == pos.size
+    if pos.size:
+        assert np.array_equal(S.queries(np.arange(1, pos.size + 1)), pos.astype(np.uint64))

@@ -102,6 +102,7 @@ SIGNATURES = {
     "wtg_select_build": (i32, [vp, vp, vp, u64, i32, C.POINTER(Select)]),
     "wtg_rank9_select_build": (i32, [vp, vp, u64, vp, C.POINTER(Select), C.POINTER(Select)]),
     "wtg_select_free": (i32, [vp, C.POINTER(Select)]),
+    "wtg_pattern_words": (i32, [vp, vp, u64, u32, i32, vp]),
     "wtg_select_batch": (i32, [vp, C.POINTER(Select), vp, vp, u64, vp]),
     "wtg_wtint_build": (i32, [vp, vp, i32, u64, u32, C.POINTER(vp)]),
     "wtg_wtint_destroy": (i32, [vp, vp]),
@@ -495,6 +496,12 @@ class Context:
         nq = d_idx.size
         out = d_out if d_out is not None else self.empty(nq, np.uint64)
         _check(lib().wtg_rank9_batch(self.h, d_words.ptr, D.ptr, int(nbits), d_idx.ptr, nq, out.ptr))
+        return out
+
+    # ---- (b'') two-bit patterns: argument words of `10` (pattern 10) or `01` (1)
+    def pattern_words(self, d_words, nbits, pattern, clip):
+        out = self.empty(max(1, (int(nbits) + 63) // 64), np.uint64)
+        _check(lib().wtg_pattern_words(self.h, d_words.ptr, int(nbits), int(pattern), int(bool(clip)), out.ptr))
         return out
 
     # ---- (b') select

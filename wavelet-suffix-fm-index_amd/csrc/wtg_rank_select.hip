@@ -615,6 +615,37 @@ extern "C" int wtg_select_build(wtg_ctx* ctx, const uint64_t* d_words, const uin
   return wtg_finish(ctx);
 }
 
+namespace {
+// argument words of the two-bit patterns `10` (pattern 10) and `01` (pattern 1):
+// bit i = (bit i-1, bit i) == the pattern; the previous word's top bit is the
+// carry, and before the vector sits the bit that rules out position 0
+__global__ __launch_bounds__(256) void k_pattern_words(const uint64_t* __restrict__ w, uint64_t W, uint32_t pattern,
+                                                       uint64_t tail_mask, uint64_t* __restrict__ out) {
+  for (uint64_t i = (uint64_t)blockIdx.x * blockDim.x + threadIdx.x; i < W; i += (uint64_t)gridDim.x * blockDim.x) {
+    const uint64_t x = w[i];
+    const uint64_t c = i ? w[i - 1] >> 63 : (pattern == 10 ? 0ull : 1ull);
+    const uint64_t prev = (x << 1) | c;
+    uint64_t r = pattern == 10 ? prev & ~x : ~prev & x;
+    if (i + 1 == W) r &= tail_mask;
+    out[i] = r;
+  }
+}
+}  // namespace
+
+extern "C" int wtg_pattern_words(wtg_ctx* ctx, const uint64_t* d_words, uint64_t nbits, uint32_t pattern, int clip,
+                                 uint64_t* d_out) {
+  WTG_REQUIRE(ctx && (nbits == 0 || (d_words && d_out)), "wtg_pattern_words: null argument");
+  WTG_REQUIRE(pattern == 10 || pattern == 1, "wtg_pattern_words: pattern must be 10 or 1 (01)");
+  if (nbits == 0) return WTG_OK;
+  WTG_TRY(wtg_set_device(ctx));
+  const uint64_t W = (nbits + 63) / 64;
+  const uint64_t tail = (clip && (nbits & 63)) ? (1ull << (nbits & 63)) - 1 : ~0ull;
+  const unsigned g = (unsigned)std::min<uint64_t>(wtg_ceil_div(W, 256), 65536);
+  WTG_LAUNCH(ctx, "pattern_words", k_pattern_words, dim3(g), dim3(256), 0, d_words, W, pattern, tail, d_out);
+  WTG_LAUNCH_CHECK();
+  return wtg_finish(ctx);
+}
+
 extern "C" int wtg_select_free(wtg_ctx* ctx, wtg_select* s) {
   WTG_REQUIRE(ctx && s, "null argument");
   wtg_free(ctx, s->superblock);

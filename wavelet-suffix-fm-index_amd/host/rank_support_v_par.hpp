@@ -7,10 +7,14 @@
 // host call exactly as in the reference; rank_batch() answers many queries on
 // the device.
 //
-// Bit patterns: `1` (rank_1_type) and `0` (bit_vector::rank_0_type, counting
-// zeros over whole words, so the directory is 64*words - the ones' directory).
-// The two-bit patterns `10` and `01` of the reference (:57-58) are not built
-// here; instantiating them stops compilation with a message.
+// Bit patterns: `1` (rank_1_type), `0` (bit_vector::rank_0_type, counting
+// zeros over whole words, so the directory is 64*words - the ones' directory)
+// and the two-bit patterns `10` and `01` of the reference (:57-58, template
+// arguments <10,2> and <01,2>): an occurrence sits at position i when bits
+// i-1, i read the pattern (sdsl's rank_support_trait: the previous word's top
+// bit carries into the next word, and position 0 is never one).  Their
+// directory is the ones' directory of the argument words (wtg_pattern_words),
+// counted over whole words as the reference's block pass (:84-105) does.
 #ifndef INCLUDED_SDSL_RANK_SUPPORT_V
 #define INCLUDED_SDSL_RANK_SUPPORT_V
 #include <cstdint>
@@ -26,9 +30,9 @@ namespace sdsl {
 
 template <uint8_t t_b = 1, uint8_t t_pat_len = 1>
 class rank_support_v {
-  static_assert((t_b == 1 || t_b == 0) && t_pat_len == 1,
-                "rank_support_v: the GPU directory is built for the bit patterns `1` and `0`; "
-                "the patterns `10` and `01` are not supported by this drop-in");
+  static_assert((t_pat_len == 1 && (t_b == 1 || t_b == 0)) || (t_pat_len == 2 && (t_b == 10 || t_b == 1)),
+                "rank_support_v: bit pattern must be `0`, `1`, `10` or `01`");
+  static constexpr bool kPair = t_pat_len == 2;
 
  public:
   typedef uint64_t size_type;
@@ -40,9 +44,16 @@ class rank_support_v {
   const bit_vector* m_v = nullptr;
   std::vector<uint64_t> m_basic_block;
 
-  // pattern `0`: arguments are the zeros of whole words (bits past size()
-  // included, as the reference's trait counts ~word)
-  static uint64_t argword(uint64_t w) { return t_b ? w : ~w; }
+  // the arguments of word k: pattern `0`, the zeros of whole words (bits past
+  // size() included, as the reference's trait counts ~word); two-bit
+  // patterns, bit i = (bit i-1, bit i) == the pattern, the carry being the
+  // previous word's top bit (before word 0: 0 for `10`, 1 for `01`)
+  uint64_t argword(uint64_t k) const {
+    const uint64_t* d = m_v->data();
+    if (!kPair) return t_b ? d[k] : ~d[k];
+    const uint64_t x = d[k], prev = (x << 1) | (k ? d[k - 1] >> 63 : (t_b == 10 ? 0ull : 1ull));
+    return t_b == 10 ? prev & ~x : ~prev & x;
+  }
 
   // the ones' directory (GPU) -> the zeros' directory: every count c over j
   // whole words becomes 64*j - c
@@ -82,13 +93,19 @@ class rank_support_v {
   // directory of words already resident on the device
   void build_from_device(const uint64_t* d_words, uint64_t nbits) {
     const uint64_t nd = wtg_rank9_words(nbits);
-    wtg::dbuf dd(nd * 8);
+    wtg::dbuf dd(nd * 8), dp;
+    if (kPair) {  // the argument words, whole words counted
+      dp.alloc((nbits + 63) / 64 * 8);
+      wtg::check(wtg_pattern_words(wtg::default_context(), d_words, nbits, t_b, 0, dp.as<uint64_t>()));
+      d_words = dp.as<uint64_t>();
+    }
     wtg::check(wtg_rank9_build(wtg::default_context(), d_words, nbits, dd.as<uint64_t>()));
     m_basic_block.resize(nd);
     dd.download(m_basic_block.data(), nd * 8);
     to_pattern(nbits);
   }
-  // the ones' directory of nbits bits, already built on the device
+  // the ones' directory of nbits bits, already built on the device (of the
+  // argument words, for a two-bit pattern)
   void assign_directory(const uint64_t* d_dir, uint64_t nbits) {
     const uint64_t nd = wtg_rank9_words(nbits);
     m_basic_block.resize(nd);
@@ -99,7 +116,7 @@ class rank_support_v {
   size_type rank(size_type idx) const {  // :131-142
     const uint64_t* p = m_basic_block.data() + ((idx >> 8) & 0xFFFFFFFFFFFFFFFEULL);
     size_type r = *p + ((*(p + 1) >> (63 - 9 * ((idx & 0x1FF) >> 6))) & 0x1FF);
-    if (idx & 0x3F) r += __builtin_popcountll(argword(m_v->data()[idx >> 6]) & ((1ULL << (idx & 63)) - 1));
+    if (idx & 0x3F) r += __builtin_popcountll(argword(idx >> 6) & ((1ULL << (idx & 63)) - 1));
     return r;
   }
   size_type operator()(size_type idx) const { return rank(idx); }
@@ -134,10 +151,15 @@ class rank_support_v {
         ones_dir[2 * b + 1] = G;
       }
     }
-    wtg::dbuf dd(ones_dir.size() * 8);
+    wtg::dbuf dd(ones_dir.size() * 8), dp;
     dd.upload(ones_dir.data(), ones_dir.size() * 8);
-    wtg::check(wtg_rank9_batch_bit(wtg::default_context(), d_words, dd.as<uint64_t>(), nbits, t_b, d_idx, nq,
-                                   d_out));
+    if (kPair && nbits) {  // the argument words' ones
+      dp.alloc((nbits + 63) / 64 * 8);
+      wtg::check(wtg_pattern_words(wtg::default_context(), d_words, nbits, t_b, 0, dp.as<uint64_t>()));
+      d_words = dp.as<uint64_t>();
+    }
+    wtg::check(wtg_rank9_batch_bit(wtg::default_context(), d_words, dd.as<uint64_t>(), nbits, kPair ? 1 : t_b, d_idx,
+                                   nq, d_out));
   }
 };
 

@@ -5,10 +5,13 @@
 // GPU (wtg_rank9_select_build); the content (superblock starts, 64 mini offsets
 // or 4096 long positions per 4096 arguments, logn^4 long/mini rule, and each
 // block's bit width) is kept on the host so select() is the reference's host
-// algorithm (:564-611); select_batch() answers many queries on the device.
+// algorithm (:564-611).
 //
-// Bit patterns `1` and `0`; the reference's `10` and `01` (:76-77) are not
-// built here and stop compilation with a message.
+// Bit patterns `1`, `0` and the two-bit patterns `10` and `01` of the
+// reference (:76-77, template arguments <10,2> and <01,2>): an occurrence sits
+// at position i < size() when bits i-1, i read the pattern (position 0 never;
+// init_slow's found_arg scan, :240-245).  Their content is the ones' content
+// of the argument words (wtg_pattern_words, clipped at size()).
 #ifndef INCLUDED_SDSL_SELECT_SUPPORT_MCL
 #define INCLUDED_SDSL_SELECT_SUPPORT_MCL
 #include <cassert>
@@ -25,9 +28,9 @@ namespace sdsl {
 
 template <uint8_t t_b = 1, uint8_t t_pat_len = 1>
 class select_support_mcl {
-  static_assert((t_b == 1 || t_b == 0) && t_pat_len == 1,
-                "select_support_mcl: the GPU select samples are built for the bit patterns `1` and `0`; "
-                "the patterns `10` and `01` are not supported by this drop-in");
+  static_assert((t_pat_len == 1 && (t_b == 1 || t_b == 0)) || (t_pat_len == 2 && (t_b == 10 || t_b == 1)),
+                "select_support_mcl: bit pattern must be `0`, `1`, `10` or `01`");
+  static constexpr bool kPair = t_pat_len == 2;
 
  public:
   typedef uint64_t size_type;
@@ -46,7 +49,14 @@ class select_support_mcl {
   std::vector<uint64_t> m_long_pool;    // n_long*4096
   size_type m_arg_cnt = 0;
 
-  static uint64_t argword(uint64_t w) { return t_b ? w : ~w; }
+  // the arguments of word k (for a two-bit pattern: bit i = (bit i-1, bit i)
+  // == the pattern, the previous word's top bit carried in)
+  uint64_t argword(uint64_t k) const {
+    const uint64_t* d = m_v->data();
+    if (!kPair) return t_b ? d[k] : ~d[k];
+    const uint64_t x = d[k], prev = (x << 1) | (k ? d[k - 1] >> 63 : (t_b == 10 ? 0ull : 1ull));
+    return t_b == 10 ? prev & ~x : ~prev & x;
+  }
   static uint32_t sel_in_word(uint64_t x, uint64_t i) {  // 1-based i-th set bit
     while (--i) x &= x - 1;
     return (uint32_t)__builtin_ctzll(x);
@@ -81,10 +91,16 @@ class select_support_mcl {
   void init_fast(const bit_vector* v = nullptr) { init_slow(v); }
 
   void build_from_device(const uint64_t* d_words, uint64_t nbits) {
-    wtg::dbuf dd(wtg_rank9_words(nbits) * 8);
+    wtg::dbuf dd(wtg_rank9_words(nbits) * 8), dp;
+    if (kPair && nbits) {  // the argument words, positions below nbits
+      dp.alloc((nbits + 63) / 64 * 8);
+      wtg::check(wtg_pattern_words(wtg::default_context(), d_words, nbits, t_b, 1, dp.as<uint64_t>()));
+      d_words = dp.as<uint64_t>();
+    }
     wtg_select s{};
+    const bool ones = kPair || t_b;
     wtg::check(wtg_rank9_select_build(wtg::default_context(), d_words, nbits, dd.as<uint64_t>(),
-                                      t_b ? &s : nullptr, t_b ? nullptr : &s));
+                                      ones ? &s : nullptr, ones ? nullptr : &s));
     assign(s);
     wtg_select_free(wtg::default_context(), &s);
   }
@@ -121,13 +137,12 @@ class select_support_mcl {
     const size_type pos = m_superblock[sb_idx] + m_mini[sb_idx * 64 + (offset >> 6)] + 1;
     size_type word_pos = pos >> 6;
     const size_type word_off = pos & 0x3F;
-    const uint64_t* data = m_v->data();
-    uint64_t x = argword(data[word_pos]) >> word_off;
+    uint64_t x = argword(word_pos) >> word_off;
     size_type args = __builtin_popcountll(x);
     if (args >= i) return (word_pos << 6) + word_off + sel_in_word(x, i);
     size_type sum = args;
     for (;;) {
-      x = argword(data[++word_pos]);
+      x = argword(++word_pos);
       args = __builtin_popcountll(x);
       if (sum + args >= i) return (word_pos << 6) + sel_in_word(x, i - sum);
       sum += args;
