@@ -83,6 +83,25 @@ int main(int argc, char** argv) {
     for (uint64_t i = 0; i <= nbits; i += 3) { r.push_back(r10(i)); r.push_back(r01.rank(i)); }
     r.push_back(r10(nbits));
     r.push_back(r01(nbits));
+    // the same ranks answered on the device (rank_batch over the argument words)
+    {
+      std::vector<uint64_t> idx;
+      for (uint64_t i = 0; i <= nbits; i += 3) idx.push_back(i);
+      idx.push_back(nbits);
+      wtg::dbuf dw(nw * 8), di(idx.size() * 8), d10(idx.size() * 8), d01(idx.size() * 8);
+      dw.upload(w.data(), nw * 8);
+      di.upload(idx.data(), idx.size() * 8);
+      r10.rank_batch(dw.as<uint64_t>(), di.as<uint64_t>(), idx.size(), d10.as<uint64_t>());
+      r01.rank_batch(dw.as<uint64_t>(), di.as<uint64_t>(), idx.size(), d01.as<uint64_t>());
+      std::vector<uint64_t> a10(idx.size()), a01(idx.size());
+      d10.download(a10.data(), idx.size() * 8);
+      d01.download(a01.data(), idx.size() * 8);
+      for (size_t k = 0; k < idx.size(); k++)
+        if (a10[k] != r[2 * k] || a01[k] != r[2 * k + 1]) {
+          std::fprintf(stderr, "rank_batch pattern mismatch at %zu\n", k);
+          return 3;
+        }
+    }
     dump(out + "/rank_pat.bin", r.data(), r.size() * 8);
     dump(out + "/rank10_dir.bin", r10.basic_block().data(), r10.basic_block().size() * 8);
     sdsl::select_support_mcl<10, 2> s10(&bv);
