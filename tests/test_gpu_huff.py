@@ -32,13 +32,15 @@ def texts(wsfm):
     for n in (1, 2, 63, 64, 65, 1023, 1024, 1025, 4097):
         yield "tiny%d" % n, rng.integers(0, 4, n).astype(np.uint8) * 3 + 60
     # alphabet sizes around the count kernel's per-lane counter widths
-    # (8, 16 and 32 inner nodes), skewed so the trees are unbalanced
-    for sig, n in ((9, 20011), (10, 3000), (17, 50007), (33, 65536 + 129), (34, 30001)):
+    # (8, 16 and 32 inner nodes) and the emit's spread counts (up to 6 inner
+    # nodes), skewed so the trees are unbalanced
+    for sig, n in ((9, 20011), (10, 3000), (17, 50007), (33, 65536 + 129), (34, 30001),
+                   (5, 40009), (6, 4099), (7, 70001), (8, 9001)):
         p = 1.0 / np.arange(1, sig + 1) ** 0.7
         yield "alpha%d" % sig, (rng.choice(sig, n, p=p / p.sum()) * 7 + 1).astype(np.uint8)
 
 
-NTEXTS = 20
+NTEXTS = 24
 
 
 def check_shape(wt, oh):

@@ -519,9 +519,16 @@ __global__ __launch_bounds__(HF_THREADS) void k_huff_emit(SegArgs a) {
 // ORs per node and lane).  The runs are then written out in one pass over all
 // the words they touch (lane k: the k-th word, its node found by a search over
 // the per-node word prefix).
+// With at most HF_SPREAD inner nodes the pass bits are kept spread, node j's
+// in bits 5j..5j+4 of the low word: one add per byte then advances the count
+// of every node at once (16 bytes per lane, so a field never carries).
+constexpr uint32_t HF_SPREAD = 6;
 __global__ __launch_bounds__(HF_THREADS) void k_huff_emit_lane(SegArgs a) {
-  __shared__ uint64_t s_pb[256];  // per symbol: bit j = passes inner node j, bit 32+j = branch 1 there
+  // per symbol: bit 32+j = branch 1 at inner node j; low word: bit j = passes
+  // inner node j (or 1 << 5j, the spread form for small trees)
+  __shared__ uint64_t s_pb[256];
   extern __shared__ uint64_t s_dyn[];
+  const bool spread = a.n_inner <= HF_SPREAD;
   for (int c = threadIdx.x; c < 256; c += HF_THREADS) {
     uint64_t e = 0;
     const uint32_t len = a.tab->len[c];
@@ -529,7 +536,7 @@ __global__ __launch_bounds__(HF_THREADS) void k_huff_emit_lane(SegArgs a) {
     uint32_t node = 0;
     for (uint32_t l = 0; l < len; l++) {
       const uint32_t b = (uint32_t)(code >> l) & 1u;
-      e |= (1ull << node) | ((uint64_t)b << (32 + node));
+      e |= (spread ? 1ull << (5 * node) : 1ull << node) | ((uint64_t)b << (32 + node));
       node = a.tab->child[node][b];
     }
     s_pb[c] = e;
@@ -582,15 +589,8 @@ __global__ __launch_bounds__(HF_THREADS) void k_huff_emit_lane(SegArgs a) {
     const uint32_t used = (total + 63) / 64 + 2;
     for (uint32_t k = lane; k < used; k += 64) stg[k] = 0;
     wave_sync_lds();
-    for (uint32_t j = 0; j < ni; j += 2) {
-      uint32_t buf0 = 0, buf1 = 0, n0 = 0, n1 = 0;
-#pragma unroll
-      for (int b = 0; b < 16; b++) {
-        buf0 |= ((bm[b] >> j) & 1u) << n0;
-        n0 += (pm[b] >> j) & 1u;
-        buf1 |= ((bm[b] >> (j + 1)) & 1u) << n1;
-        n1 += (pm[b] >> (j + 1)) & 1u;
-      }
+    // nodes j and j+1: their gathered bits (n0, n1 of them) into their runs
+    auto place = [&](uint32_t j, uint32_t buf0, uint32_t n0, uint32_t buf1, uint32_t n1) {
       const uint32_t pk = n0 | (n1 << 16);
       const uint32_t ex2 = wave_incl_scan(pk) - pk;
 #pragma unroll
@@ -604,6 +604,32 @@ __global__ __launch_bounds__(HF_THREADS) void k_huff_emit_lane(SegArgs a) {
           if (sh + cntj > 64)
             atomicOr(reinterpret_cast<unsigned long long*>(&stg[wi + 1]), (unsigned long long)bj >> (64 - sh));
         }
+      }
+    };
+    if (spread) {
+      // every node at once: field j of cnt counts the earlier bytes that pass
+      // node j, which is where this byte's branch bit goes in node j's bits
+      uint32_t buf[HF_SPREAD] = {}, cnt = 0;
+#pragma unroll
+      for (int b = 0; b < 16; b++) {
+#pragma unroll
+        for (uint32_t j = 0; j < HF_SPREAD; j++) buf[j] |= ((bm[b] >> j) & 1u) << ((cnt >> (5 * j)) & 31u);
+        cnt += pm[b];
+      }
+#pragma unroll
+      for (uint32_t j = 0; j < HF_SPREAD; j += 2)
+        if (j < ni) place(j, buf[j], (cnt >> (5 * j)) & 31u, buf[j + 1], (cnt >> (5 * j + 5)) & 31u);
+    } else {
+      for (uint32_t j = 0; j < ni; j += 2) {
+        uint32_t buf0 = 0, buf1 = 0, n0 = 0, n1 = 0;
+#pragma unroll
+        for (int b = 0; b < 16; b++) {
+          buf0 |= ((bm[b] >> j) & 1u) << n0;
+          n0 += (pm[b] >> j) & 1u;
+          buf1 |= ((bm[b] >> (j + 1)) & 1u) << n1;
+          n1 += (pm[b] >> (j + 1)) & 1u;
+        }
+        place(j, buf0, n0, buf1, n1);
       }
     }
     wave_sync_lds();
