@@ -156,10 +156,11 @@ __device__ __forceinline__ uint32_t word_leaf(const mld_u4& q, uint32_t e, uint3
 // slot and leaf one ballot of the lanes holding it, the counts in scalar
 // registers (a few leaves would otherwise put every lane's atomic on one of a
 // few LDS words).
-// tiles per wave of the count kernel (byte keys: one, so a 2^24-key tree
-// still has 8192 waves in flight; 23 vs 31 us with four)
+// tiles per wave of the count kernel: one for every key width, so small
+// trees keep enough waves in flight (2^24 byte keys: 23 vs 31 us with four
+// tiles per wave; the batched 16-bit keys: 239 vs 236 Gsym/s with two)
 template <typename K>
-constexpr uint32_t mld_cnt_tpw() { return sizeof(K) == 2 ? 2u : 1u; }
+constexpr uint32_t mld_cnt_tpw() { return 1u; }
 
 __device__ __forceinline__ void wave_sync_lds_mld() {
   __builtin_amdgcn_fence(__ATOMIC_RELEASE, "wavefront");
