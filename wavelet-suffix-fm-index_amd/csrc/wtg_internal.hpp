@@ -248,11 +248,12 @@ __device__ __forceinline__ U wave_incl_scan(U v) {
     return wave_incl_scan_shfl(v);
   }
 }
-// running max of a and min of b over the lanes (t > a ? t : a: ties keep the
-// lane's own value, as a sequential walk does)
+// running max of a and min of b over the lanes (no NaNs, no -0: max and min
+// are exact and equal the sequential walk's); a lane with no source in a step
+// reads its own value
 __device__ __forceinline__ void wave_incl_maxmin_f64_dpp(double& a, double& b) {
-#define WTG_STEP(C, R) { const double ta = dpp_f64<C, R>(-1e300, a), tb = dpp_f64<C, R>(1e300, b); \
-    a = ta > a ? ta : a; b = tb < b ? tb : b; }
+#define WTG_STEP(C, R) { const double ta = dpp_f64<C, R>(a, a), tb = dpp_f64<C, R>(b, b); \
+    a = __builtin_fmax(ta, a); b = __builtin_fmin(tb, b); }
   WTG_DPP_SCAN_STEPS(WTG_STEP)
 #undef WTG_STEP
 }
