@@ -274,7 +274,7 @@ __device__ __forceinline__ uint64_t shfl_u64(uint64_t v, int l) {
 // LSEL_CAP of them to scratch[flat * LSEL_CAP ..].  Emit pass: a chunk with at
 // most LSEL_CAP knots copies them from scratch; a larger one walks again and
 // writes knots[off[flat] ..] (a different count sets *err).
-constexpr uint32_t LSEL_CAP = 16;
+constexpr uint32_t LSEL_CAP = 64;
 __global__ __launch_bounds__(256) void k_lsel_place(const PlaceVec* __restrict__ pv, const uint64_t* __restrict__ cbase,
                                                     uint32_t nv, uint64_t nflat, uint32_t gap_log, double tol,
                                                     uint32_t* __restrict__ cnt, const uint32_t* __restrict__ off,
@@ -293,7 +293,7 @@ __global__ __launch_bounds__(256) void k_lsel_place(const PlaceVec* __restrict__
   const uint64_t j1 = ((c + 1) << gap_log) < v.ones ? ((c + 1) << gap_log) : v.ones;
   const uint32_t lim = off ? off[flat + 1] - off[flat] : 0u;
   if (off && lim <= LSEL_CAP) {  // the count pass kept them
-    if (lane < lim) knots[off[flat] + lane] = scratch[flat * LSEL_CAP + lane];
+    for (uint32_t i = lane; i < lim; i += 64) knots[off[flat] + i] = scratch[flat * LSEL_CAP + i];
     return;
   }
   Knot* out = off ? knots + off[flat] : scratch + flat * LSEL_CAP;
