@@ -616,12 +616,12 @@ __global__ __launch_bounds__(256) void k_lsel_loss(const TrainVec* __restrict__ 
 // Bound pass (EXACT = false): the exact errors of every word's first and last
 // one; their maximum (a lower bound of eps) goes to eps[b].
 // Exact pass: inside one knot interval the prediction is linear in j (up to
-// the rounding of its float and double steps, within `margin`), so a word's
-// predictions lie between those of its first and last one and its positions
-// between theirs: no error of the word exceeds
-// max(qmax - pfirst, plast - qmin) + margin.  Only a word whose bound exceeds
-// the first pass's maximum (or whose ones span two intervals) can raise it;
-// its ones are evaluated one by one.
+// the rounding of its float and double steps, within `margin`), and the i-th
+// of its c ones lies in [pfirst + i, plast - (c-1-i)]: no error of the word
+// exceeds the larger of its end errors, qlast - pfirst - (c-1) and
+// plast - qfirst - (c-1), plus the margin.  Only a word whose bound exceeds the
+// first pass's maximum (or whose ones span two intervals) can raise it; its
+// ones are evaluated one by one.
 constexpr uint32_t LSEL_EPS_W = 16;
 template <bool EXACT>
 __global__ __launch_bounds__(256) void k_lsel_eps(const LselDesc* __restrict__ descs,
@@ -672,10 +672,15 @@ __global__ __launch_bounds__(256) void k_lsel_eps(const LselDesc* __restrict__ d
       } else {
         bool cand = k != kf;
         if (!cand && c > 2) {
-          const uint64_t qmax = qf > ql ? qf : ql, qmin = qf < ql ? qf : ql;
-          const uint64_t margin = 2 + (uint64_t)((fabsf(fa.y) + fabsf(fb.y)) * 0x1p-19f);
-          const uint64_t u1 = qmax > pf ? qmax - pf : 0, u2 = pl > qmin ? pl - qmin : 0;
-          cand = (u1 > u2 ? u1 : u2) + margin > lb;
+          // the i-th one (i = 0..c-1) sits in [pf + i, pl - (c-1-i)]: its error
+          // q_i - p_i is at most linear in i, so at most qf - pf or
+          // ql - pf - (c-1); p_i - q_i at most pl - ql or pl - (c-1) - qf
+          const int64_t margin = 2 + (int64_t)((fabsf(fa.y) + fabsf(fb.y)) * 0x1p-19f);
+          const int64_t cm = (int64_t)c - 1;
+          const int64_t u1 = (int64_t)ql - (int64_t)pf - cm, u2 = (int64_t)pl - (int64_t)qf - cm;
+          const int64_t u0 = (int64_t)(dist(qf, pf) > dist(ql, pl) ? dist(qf, pf) : dist(ql, pl));
+          const int64_t ub = (u0 > u1 ? (u0 > u2 ? u0 : u2) : (u1 > u2 ? u1 : u2)) + margin;
+          cand = ub > (int64_t)lb;
         }
         if (cand) {
           uint32_t k2 = kf;
