@@ -452,8 +452,9 @@ __device__ __forceinline__ void interval_samples(const TrainVec& v, const Knot& 
   if (t2 > v.ns) t2 = v.ns;
 }
 
-// Training in work items: interval g's samples are cut into items of at most
-// LSEL_ITEM consecutive samples (nitem[g] of them, the first at item_start[g]),
+// Training in work items: interval g's samples are cut at the multiples of
+// LSEL_ITEM of their target index into items of at most LSEL_ITEM samples
+// (nitem[g] of them, the first at item_start[g]; each reads one aligned block),
 // a thread per item sums its samples in order, and a thread per interval sums
 // its items' partial sums in order (the same sums every run; a wave's threads
 // do about equal work whatever the interval lengths).
@@ -478,7 +479,7 @@ __global__ __launch_bounds__(256) void k_lsel_items_count(const TrainVec* __rest
   if (k + 1 < v.nknots) {
     uint64_t t1, t2;
     interval_samples(v, tb.K[g], tb.K[g + 1], k, t1, t2);
-    if (t2 >= t1) c = (uint32_t)((t2 - t1 + LSEL_ITEM) / LSEL_ITEM);
+    if (t2 >= t1) c = (uint32_t)((t2 - 1) / LSEL_ITEM - (t1 - 1) / LSEL_ITEM + 1);
   }
   it.model_of[g] = m;
   it.nitem[g] = c;
@@ -512,22 +513,35 @@ __global__ __launch_bounds__(256) void k_lsel_items(const TrainVec* __restrict__
   const uint64_t T = 1ull << v.tlog;
   uint64_t t1, t2;
   interval_samples(v, a, b, k, t1, t2);
-  const uint64_t u0 = t1 + (uint64_t)(i - it.item_start[g]) * LSEL_ITEM;
-  const uint64_t ue = u0 + LSEL_ITEM - 1 < t2 ? u0 + LSEL_ITEM - 1 : t2;
+  // the item's aligned block of LSEL_ITEM targets (indices x = u - 1), clipped to [t1, t2]
+  const uint64_t xb = ((t1 - 1) / LSEL_ITEM + (uint64_t)(i - it.item_start[g])) * LSEL_ITEM;
+  const uint64_t u0 = xb + 1 > t1 ? xb + 1 : t1;
+  const uint64_t ue = xb + LSEL_ITEM < t2 ? xb + LSEL_ITEM : t2;
   const float span = (float)(b.j - a.j);
+  // four aligned 16-byte loads (each model's targets start 256-byte aligned
+  // and are padded to a multiple of 64)
   float tg[LSEL_ITEM];
+  const float4* tb4 = reinterpret_cast<const float4*>(v.tgt + xb);
 #pragma unroll
-  for (uint32_t q = 0; q < LSEL_ITEM; q++) tg[q] = u0 + q <= ue ? v.tgt[u0 + q - 1] : 0.f;
+  for (uint32_t r = 0; r < LSEL_ITEM / 4; r++) {
+    const float4 f4 = tb4[r];
+    tg[4 * r] = f4.x;
+    tg[4 * r + 1] = f4.y;
+    tg[4 * r + 2] = f4.z;
+    tg[4 * r + 3] = f4.w;
+  }
   float sa = 0.f, sc = 0.f, e2 = 0.f;
 #pragma unroll
   for (uint32_t q = 0; q < LSEL_ITEM; q++) {
-    if (u0 + q > ue) break;
-    const uint64_t j = (u0 + q) * T;
-    const float f = (float)(j - a.j) / span;
-    const float e = (1.0f - f) * a.y + f * b.y - tg[q];
-    sa += e * (1.0f - f);
-    sc += e * f;
-    e2 += e * e;
+    const uint64_t u = xb + 1 + q;  // sample u, target tg[q]
+    if (u >= u0 && u <= ue) {
+      const uint64_t j = u * T;
+      const float f = (float)(j - a.j) / span;
+      const float e = (1.0f - f) * a.y + f * b.y - tg[q];
+      sa += e * (1.0f - f);
+      sc += e * f;
+      e2 += e * e;
+    }
   }
   it.pa[i] = sa;
   it.pc[i] = sc;
@@ -566,7 +580,7 @@ __global__ __launch_bounds__(256) void k_lsel_adam(const TrainVec* __restrict__ 
                                                    float bc1, float bc2) {
   const uint64_t g = (uint64_t)blockIdx.x * blockDim.x + threadIdx.x;
   if (g >= nk_total) return;
-  const TrainVec& v = tv[find_base(tb.kfirst, nv, g)];
+  const TrainVec& v = tv[it.model_of[g]];
   const uint64_t k = g - v.kfirst;
   float sa = 0.f, sc = 0.f;
   if (k + 1 < v.nknots)
@@ -966,6 +980,7 @@ static int lsel_models(wtg_ctx* ctx, wtg_lsel* ls, const wtg_lsel_params& P, con
   TrainItems items{};
   {
     uint64_t cap = nk_total + 1;
+    cap += nk_total;  // an interval's aligned items: at most one more than its samples / LSEL_ITEM + 1
     for (uint32_t a = 0; a < na; a++) cap += ns[act[a]] / LSEL_ITEM + 1;
     char* mem = nullptr;
     WTG_TRY(alloc_block(ctx, ls, train_items_bytes(nk_total, cap), (void**)&mem));
@@ -1261,7 +1276,7 @@ extern "C" int wtg_lsel_loss_grad(wtg_ctx* ctx, const void* d_knots, uint32_t nk
               "wtg_lsel_loss_grad: bad argument");
   WTG_TRY(wtg_set_device(ctx));
   // scratch: TrainVec | kfirst | gl | gr | le | loss | targets | work items
-  const uint64_t cap = (uint64_t)nknots + ns / LSEL_ITEM + 2;
+  const uint64_t cap = 2 * (uint64_t)nknots + ns / LSEL_ITEM + 2;
   const size_t o_tv = 0, o_kf = wtg_align256(sizeof(TrainVec)), o_gl = o_kf + 256;
   const size_t o_gr = wtg_align256(o_gl + (size_t)nknots * 4), o_le = wtg_align256(o_gr + (size_t)nknots * 4);
   const size_t o_l = wtg_align256(o_le + (size_t)nknots * 4), o_tg = o_l + 256;
