@@ -645,76 +645,96 @@ __global__ __launch_bounds__(256) void k_lsel_eps(const LselDesc* __restrict__ d
   const LselDesc d = descs[b];
   const uint64_t W = (d.nbits + 63) >> 6;
   const uint64_t lb = EXACT ? (uint64_t)eps[b] : 0ull;
+  const uint32_t lane = (uint32_t)lane_id();
   uint64_t mx = 0;
   auto dist = [](uint64_t q, uint64_t p) { return q > p ? q - p : p - q; };
-  for (uint64_t w0 = ((uint64_t)blockIdx.x * blockDim.x + threadIdx.x) * LSEL_EPS_W; w0 < W;
+  auto rl64 = [](uint64_t v, int l) {
+    return ((uint64_t)(uint32_t)__builtin_amdgcn_readlane((int)(uint32_t)(v >> 32), l) << 32) |
+           (uint32_t)__builtin_amdgcn_readlane((int)(uint32_t)v, l);
+  };
+  // wave-uniform loop (the candidate words below are evaluated by the whole wave)
+  for (uint64_t w0 = ((uint64_t)blockIdx.x * blockDim.x + threadIdx.x) * LSEL_EPS_W;;
        w0 += (uint64_t)gridDim.x * blockDim.x * LSEL_EPS_W) {
+    const bool live = w0 < W;
+    if (!wave_ballot(live)) break;
     uint64_t xs[LSEL_EPS_W];
 #pragma unroll
     for (uint32_t q = 0; q < LSEL_EPS_W; q++) xs[q] = w0 + q < W ? d.words[w0 + q] : 0ull;
-    uint64_t j = rank9_dev(d.dir, d.words, w0 << 6);  // ones before the run
+    uint64_t j = live ? rank9_dev(d.dir, d.words, w0 << 6) : 0ull;  // ones before the run
     uint32_t k = 0;
     Knot ka{0u, 0.f}, kb{0u, 0.f};
     bool have = false;
 #pragma unroll
     for (uint32_t q = 0; q < LSEL_EPS_W; q++) {
       const uint64_t x = xs[q];
-      if (!x) continue;
-      const uint64_t w = w0 + q;
-      const uint32_t c = popc64(x);
-      const uint64_t jf = j + 1, jl = j + c;
-      if (!have) {
-        k = d.lut[jf >> d.lut_log];
-        ka = d.knots[k];
-        kb = d.knots[k + 1];
-        have = true;
-      }
-      lsel_advance(d, jf, k, ka, kb);
-      const uint32_t kf = k;
-      const Knot fa = ka, fb = kb;
-      const uint64_t pf = (w << 6) + (uint64_t)__builtin_ctzll(x);
-      const uint64_t pl = (w << 6) + 63 - (uint64_t)__builtin_clzll(x);
-      const uint64_t qf = lsel_eval(d, ka, kb, jf);
-      uint64_t ql = qf;
-      if (c > 1) {
-        lsel_advance(d, jl, k, ka, kb);
-        ql = lsel_eval(d, ka, kb, jl);
-      }
-      if (!EXACT) {
-        const uint64_t e = dist(qf, pf) > dist(ql, pl) ? dist(qf, pf) : dist(ql, pl);
-        mx = e > mx ? e : mx;
-      } else {
-        bool cand = k != kf;
-        if (!cand && c > 2) {
-          // the i-th one (i = 0..c-1) sits in [pf + i, pl - (c-1-i)]: its error
-          // q_i - p_i is at most linear in i, so at most qf - pf or
-          // ql - pf - (c-1); p_i - q_i at most pl - ql or pl - (c-1) - qf
-          const int64_t margin = 2 + (int64_t)((fabsf(fa.y) + fabsf(fb.y)) * 0x1p-19f);
-          const int64_t cm = (int64_t)c - 1;
-          const int64_t u1 = (int64_t)ql - (int64_t)pf - cm, u2 = (int64_t)pl - (int64_t)qf - cm;
-          const int64_t u0 = (int64_t)(dist(qf, pf) > dist(ql, pl) ? dist(qf, pf) : dist(ql, pl));
-          const int64_t ub = (u0 > u1 ? (u0 > u2 ? u0 : u2) : (u1 > u2 ? u1 : u2)) + margin;
-          cand = ub > (int64_t)lb;
+      const uint64_t w = w0 + q, jb = j;
+      bool cand = false;
+      uint32_t kf = 0;
+      Knot fa{0u, 0.f}, fb{0u, 0.f};
+      if (x) {
+        const uint32_t c = popc64(x);
+        const uint64_t jf = j + 1, jl = j + c;
+        if (!have) {
+          k = d.lut[jf >> d.lut_log];
+          ka = d.knots[k];
+          kb = d.knots[k + 1];
+          have = true;
         }
-        if (cand) {
-          uint32_t k2 = kf;
-          Knot a2 = fa, b2 = fb;
-          uint64_t xx = x, jj = j;
-          while (xx) {
-            const uint32_t o = (uint32_t)__builtin_ctzll(xx);
-            xx &= xx - 1;
-            jj++;
+        lsel_advance(d, jf, k, ka, kb);
+        kf = k;
+        fa = ka;
+        fb = kb;
+        const uint64_t pf = (w << 6) + (uint64_t)__builtin_ctzll(x);
+        const uint64_t pl = (w << 6) + 63 - (uint64_t)__builtin_clzll(x);
+        const uint64_t qf = lsel_eval(d, ka, kb, jf);
+        uint64_t ql = qf;
+        if (c > 1) {
+          lsel_advance(d, jl, k, ka, kb);
+          ql = lsel_eval(d, ka, kb, jl);
+        }
+        if (!EXACT) {
+          const uint64_t e = dist(qf, pf) > dist(ql, pl) ? dist(qf, pf) : dist(ql, pl);
+          mx = e > mx ? e : mx;
+        } else {
+          cand = k != kf;
+          if (!cand && c > 2) {
+            // the i-th one (i = 0..c-1) sits in [pf + i, pl - (c-1-i)]: its error
+            // q_i - p_i is at most linear in i, so at most qf - pf or
+            // ql - pf - (c-1); p_i - q_i at most pl - ql or pl - (c-1) - qf
+            const int64_t margin = 2 + (int64_t)((fabsf(fa.y) + fabsf(fb.y)) * 0x1p-19f);
+            const int64_t cm = (int64_t)c - 1;
+            const int64_t u1 = (int64_t)ql - (int64_t)pf - cm, u2 = (int64_t)pl - (int64_t)qf - cm;
+            const int64_t u0 = (int64_t)(dist(qf, pf) > dist(ql, pl) ? dist(qf, pf) : dist(ql, pl));
+            const int64_t ub = (u0 > u1 ? (u0 > u2 ? u0 : u2) : (u1 > u2 ? u1 : u2)) + margin;
+            cand = ub > (int64_t)lb;
+          }
+        }
+        j += c;
+      }
+      if (EXACT) {
+        // each candidate word by the whole wave: lane i takes the word's bit i
+        // (its rank from the word's rank base, its interval moved forward
+        // from the one of the word's first one: the walk's arithmetic)
+        for (uint64_t cmask = wave_ballot(cand); cmask; cmask &= cmask - 1) {
+          const int l = (int)__builtin_ctzll(cmask);
+          const uint64_t xw = rl64(x, l), jw = rl64(jb, l), ww = rl64(w, l);
+          uint32_t k2 = (uint32_t)__builtin_amdgcn_readlane((int)kf, l);
+          Knot a2{(uint32_t)__builtin_amdgcn_readlane((int)fa.j, l),
+                  __int_as_float(__builtin_amdgcn_readlane(__float_as_int(fa.y), l))};
+          Knot b2{(uint32_t)__builtin_amdgcn_readlane((int)fb.j, l),
+                  __int_as_float(__builtin_amdgcn_readlane(__float_as_int(fb.y), l))};
+          if ((xw >> lane) & 1ull) {
+            const uint64_t jj = jw + popc64(xw & ((1ull << lane) - 1ull)) + 1;
             lsel_advance(d, jj, k2, a2, b2);
-            const uint64_t e = dist(lsel_eval(d, a2, b2, jj), (w << 6) + o);
+            const uint64_t e = dist(lsel_eval(d, a2, b2, jj), (ww << 6) + lane);
             mx = e > mx ? e : mx;
           }
         }
       }
-      j += c;
     }
   }
   mx = wave_max_u64(mx);
-  if (lane_id() == 0 && mx > lb) atomicMax(&eps[b], (unsigned long long)mx);
+  if (lane == 0 && mx > lb) atomicMax(&eps[b], (unsigned long long)mx);
 }
 
 // training samples of every vector in one launch: pos[u-1] = select1(u*T)
