@@ -755,10 +755,13 @@ __global__ __launch_bounds__(256) void k_lsel_samples(const LselDesc* __restrict
   const uint64_t W = (d.nbits + 63) >> 6, T = 1ull << tlog;
   for (uint64_t w0 = ((uint64_t)blockIdx.x * blockDim.x + threadIdx.x) * LSEL_EPS_W; w0 < W;
        w0 += (uint64_t)gridDim.x * blockDim.x * LSEL_EPS_W) {
-    const uint64_t w1 = w0 + LSEL_EPS_W < W ? w0 + LSEL_EPS_W : W;
+    uint64_t xs[LSEL_EPS_W];  // the run's words, all loads in flight together
+#pragma unroll
+    for (uint32_t q = 0; q < LSEL_EPS_W; q++) xs[q] = w0 + q < W ? d.words[w0 + q] : 0ull;
     uint64_t j = rank9_dev(d.dir, d.words, w0 << 6);  // ones before the word
-    for (uint64_t w = w0; w < w1; w++) {
-      const uint64_t x = d.words[w];
+#pragma unroll
+    for (uint32_t q = 0; q < LSEL_EPS_W; q++) {
+      const uint64_t w = w0 + q, x = xs[q];
       const uint64_t c = (uint64_t)__builtin_popcountll(x);
       // arguments u*T in (j, j + c]
       for (uint64_t u = (j >> tlog) + 1; u * T <= j + c && u <= s.ns; u++) {
