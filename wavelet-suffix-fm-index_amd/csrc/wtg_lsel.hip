@@ -623,28 +623,54 @@ __global__ __launch_bounds__(256) void k_lsel_loss(const TrainVec* __restrict__ 
 }
 
 // exact error window eps = max over every one j of |pred(j) - select(j)| of
-// vector vids[blockIdx.y], in two passes over its words (thread per run of
-// LSEL_EPS_W consecutive words; the run looks its knot interval up once and
-// moves it forward, the interval lsel_predict finds for each argument, and
-// the same arithmetic, lsel_eval).
-// Bound pass (EXACT = false): the exact errors of every word's first and last
-// one; their maximum (a lower bound of eps) goes to eps[b].
-// Exact pass: inside one knot interval the prediction is linear in j (up to
-// the rounding of its float and double steps, within `margin`), and the i-th
-// of its c ones lies in [pfirst + i, plast - (c-1-i)]: no error of the word
-// exceeds the larger of its end errors, qlast - pfirst - (c-1) and
-// plast - qfirst - (c-1), plus the margin.  Only a word whose bound exceeds the
-// first pass's maximum (or whose ones span two intervals) can raise it; its
-// ones are evaluated one by one.
+// model vids[blockIdx.y], in two passes.
+// k_lsel_eps_lb: the exact errors at the model's training samples (each is a
+// one: select(T*(t+1)) at pos[t]); their maximum, a lower bound of eps, goes
+// to eps[b].
+// k_lsel_eps: a thread per run of LSEL_EPS_W consecutive words (the run looks
+// its knot interval up once and moves it forward, the interval lsel_predict
+// finds for each argument, and the same arithmetic, lsel_eval).  Inside one
+// knot interval the prediction is linear in j (up to the rounding of its float
+// and double steps, within `margin`), and the i-th of a word's c ones lies in
+// [pfirst + i, plast - (c-1-i)]: no error of the word exceeds the larger of its
+// end errors, qlast - pfirst - (c-1) and plast - qfirst - (c-1), plus the
+// margin.  Every word's end errors are exact and kept; a word of more than two
+// ones whose bound exceeds the lower bound (or whose ones span two intervals)
+// can raise it: the whole wave evaluates its ones.
+__global__ __launch_bounds__(256) void k_lsel_eps_lb(const LselDesc* __restrict__ descs,
+                                                     const uint32_t* __restrict__ vids,
+                                                     const TrainVec* __restrict__ tv,
+                                                     unsigned long long* __restrict__ eps) {
+  const uint32_t b = vids[blockIdx.y];
+  const LselDesc d = descs[b];
+  const TrainVec v = tv[blockIdx.y];
+  uint64_t mx = 0;
+  for (uint64_t t = (uint64_t)blockIdx.x * blockDim.x + threadIdx.x; t < v.ns; t += (uint64_t)gridDim.x * blockDim.x) {
+    const uint64_t q = lsel_predict(d, (t + 1) << v.tlog), p = v.pos[t];
+    const uint64_t e = q > p ? q - p : p - q;
+    mx = e > mx ? e : mx;
+  }
+  // one atomic per block (eps[b] only grows: a block whose maximum is not
+  // above the value it reads cannot raise it)
+  __shared__ uint64_t s_mx[4];
+  mx = wave_max_u64(mx);
+  if (lane_id() == 0) s_mx[threadIdx.x >> 6] = mx;
+  __syncthreads();
+  if (threadIdx.x == 0) {
+    uint64_t m = s_mx[0];
+    for (uint32_t k = 1; k < blockDim.x / 64; k++) m = s_mx[k] > m ? s_mx[k] : m;
+    if (m > __atomic_load_n(&eps[b], __ATOMIC_RELAXED)) atomicMax(&eps[b], (unsigned long long)m);
+  }
+}
+
 constexpr uint32_t LSEL_EPS_W = 16;
-template <bool EXACT>
 __global__ __launch_bounds__(256) void k_lsel_eps(const LselDesc* __restrict__ descs,
                                                   const uint32_t* __restrict__ vids,
                                                   unsigned long long* __restrict__ eps) {
   const uint32_t b = vids[blockIdx.y];
   const LselDesc d = descs[b];
   const uint64_t W = (d.nbits + 63) >> 6;
-  const uint64_t lb = EXACT ? (uint64_t)eps[b] : 0ull;
+  const uint64_t lb = (uint64_t)eps[b];
   const uint32_t lane = (uint32_t)lane_id();
   uint64_t mx = 0;
   auto dist = [](uint64_t q, uint64_t p) { return q > p ? q - p : p - q; };
@@ -692,11 +718,12 @@ __global__ __launch_bounds__(256) void k_lsel_eps(const LselDesc* __restrict__ d
           lsel_advance(d, jl, k, ka, kb);
           ql = lsel_eval(d, ka, kb, jl);
         }
-        if (!EXACT) {
-          const uint64_t e = dist(qf, pf) > dist(ql, pl) ? dist(qf, pf) : dist(ql, pl);
-          mx = e > mx ? e : mx;
-        } else {
-          cand = k != kf;
+        {
+          // the word's end ones: exact errors, kept directly (a word of one or
+          // two ones is then done)
+          const uint64_t e0 = dist(qf, pf) > dist(ql, pl) ? dist(qf, pf) : dist(ql, pl);
+          mx = e0 > mx ? e0 : mx;
+          cand = k != kf && c > 2;
           if (!cand && c > 2) {
             // the i-th one (i = 0..c-1) sits in [pf + i, pl - (c-1-i)]: its error
             // q_i - p_i is at most linear in i, so at most qf - pf or
@@ -704,14 +731,14 @@ __global__ __launch_bounds__(256) void k_lsel_eps(const LselDesc* __restrict__ d
             const int64_t margin = 2 + (int64_t)((fabsf(fa.y) + fabsf(fb.y)) * 0x1p-19f);
             const int64_t cm = (int64_t)c - 1;
             const int64_t u1 = (int64_t)ql - (int64_t)pf - cm, u2 = (int64_t)pl - (int64_t)qf - cm;
-            const int64_t u0 = (int64_t)(dist(qf, pf) > dist(ql, pl) ? dist(qf, pf) : dist(ql, pl));
+            const int64_t u0 = (int64_t)e0;
             const int64_t ub = (u0 > u1 ? (u0 > u2 ? u0 : u2) : (u1 > u2 ? u1 : u2)) + margin;
             cand = ub > (int64_t)lb;
           }
         }
         j += c;
       }
-      if (EXACT) {
+      {
         // each candidate word by the whole wave: lane i takes the word's bit i
         // (its rank from the word's rank base, its interval moved forward
         // from the one of the word's first one: the walk's arithmetic)
@@ -1013,13 +1040,14 @@ static int lsel_models(wtg_ctx* ctx, wtg_lsel* ls, const wtg_lsel_params& P, con
   uint64_t wmax = 1;
   for (uint32_t a = 0; a < na; a++) wmax = std::max<uint64_t>(wmax, (nbits[act[a]] + 63) / 64);
   const unsigned ge_x = (unsigned)std::max<uint64_t>(1, std::min<uint64_t>(wtg_ceil_div(wmax, 256 * LSEL_EPS_W), 256));
+  const unsigned ge_s = (unsigned)std::max<uint64_t>(1, std::min<uint64_t>(wtg_ceil_div(nsmax, 256), 256));
   auto measure = [&](std::vector<uint64_t>& eps, std::vector<double>& loss) -> int {
     WTG_TRY(wtg_fill_u64(ctx, d_eps, nvec, 0));
     WTG_TRY(train_grad(ctx, d_tv, tb, nk_total, items));
     // every model in one launch each: blocks of a vector stride over its words
-    WTG_LAUNCH(ctx, "lsel_eps", k_lsel_eps<false>, dim3(ge_x, na), dim3(256), 0, ls->d_desc, d_act, d_eps);
+    WTG_LAUNCH(ctx, "lsel_eps", k_lsel_eps_lb, dim3(ge_s, na), dim3(256), 0, ls->d_desc, d_act, d_tv, d_eps);
     WTG_LAUNCH_CHECK();
-    WTG_LAUNCH(ctx, "lsel_eps", k_lsel_eps<true>, dim3(ge_x, na), dim3(256), 0, ls->d_desc, d_act, d_eps);
+    WTG_LAUNCH(ctx, "lsel_eps", k_lsel_eps, dim3(ge_x, na), dim3(256), 0, ls->d_desc, d_act, d_eps);
     WTG_LAUNCH_CHECK();
     WTG_LAUNCH(ctx, "lsel_loss", k_lsel_loss, dim3(na), dim3(256), 0, d_tv, tb, d_loss);
     WTG_LAUNCH_CHECK();
