@@ -1078,14 +1078,8 @@ static int huff_finish(wtg_ctx* ctx, wtg_huff* wt, const uint64_t* hist) {
   for (int c = 0; c < 256; c++) ones += hist[c] * (uint64_t)__builtin_popcountll(wt->tab.code[c]);
   wtg_rs_pending pend;
   WTG_TRY(wtg_rank_select_begin(ctx, wt->words, S.bv_size, wt->dir, true, &wt->s1, &wt->s0, ones, &pend));
-  // the query blocks (rank9 directory done)
-  if (S.bv_size > 0) {
-    const uint64_t nblocks = S.bv_size / FMR_BITS + 1;
-    WTG_TRY(wtg_malloc(ctx, (void**)&wt->fmr, nblocks * 64));
-    WTG_LAUNCH(ctx, "huff_fmr", k_huff_fmr, dim3((unsigned)wtg_ceil_div(nblocks * 8, 256)), dim3(256), 0,
-               wt->dir, wt->words, nw, nblocks, wt->fmr);
-    WTG_LAUNCH_CHECK();
-  }
+  const uint64_t nblocks = S.bv_size / FMR_BITS + 1;
+  if (S.bv_size > 0) WTG_TRY(wtg_malloc(ctx, (void**)&wt->fmr, nblocks * 64));
   // one host read for the long-superblock counts and the per-node rank bases
   uint32_t nl[2] = {0, 0};
   uint64_t rb[256];
@@ -1104,6 +1098,13 @@ static int huff_finish(wtg_ctx* ctx, wtg_huff* wt, const uint64_t* hist) {
       hipStreamSynchronize(ctx->stream) != hipSuccess) {
     wtg_set_error("wt_huff: rank base copy failed");
     return WTG_ERR_HIP;
+  }
+  // the query blocks (rank9 directory done): queued after the read, so the
+  // device has them to run while the host takes the counts
+  if (S.bv_size > 0) {
+    WTG_LAUNCH(ctx, "huff_fmr", k_huff_fmr, dim3((unsigned)wtg_ceil_div(nblocks * 8, 256)), dim3(256), 0,
+               wt->dir, wt->words, nw, nblocks, wt->fmr);
+    WTG_LAUNCH_CHECK();
   }
   if (pend.active) std::memcpy(nl, st, 8);
   if (S.n_inner > 0) std::memcpy(rb, st + 64, sizeof(rb));
