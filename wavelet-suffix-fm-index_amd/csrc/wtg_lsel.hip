@@ -31,6 +31,7 @@
 // predicts: outside that window the plain select answers.  The plain sampled
 // select stays available (wtg_lsel_plain_batch) as the judge.
 #include "wtg_internal.hpp"
+#include <functional>
 #include <algorithm>
 #include <cmath>
 #include <cstring>
@@ -1101,8 +1102,10 @@ static int lsel_models(wtg_ctx* ctx, wtg_lsel* ls, const wtg_lsel_params& P, con
 
 // rank9 directories and select1 structures of all vectors, in three host
 // reads: none between the vectors' rank passes, one for the ones counts, one
-// for the long-superblock counts
-static int lsel_plain_build(wtg_ctx* ctx, wtg_lsel* ls, const uint64_t* const* d_words, const uint64_t* nbits) {
+// for the long-superblock counts (after_ones runs between the first two, its
+// device work queued ahead of the select counts)
+static int lsel_plain_build(wtg_ctx* ctx, wtg_lsel* ls, const uint64_t* const* d_words, const uint64_t* nbits,
+                            const std::function<int(const std::vector<uint64_t>&)>& after_ones) {
   const uint32_t nvec = ls->nvec;
   uint64_t dw = 0;
   size_t stride = 0, worst = 0;
@@ -1136,6 +1139,7 @@ static int lsel_plain_build(wtg_ctx* ctx, wtg_lsel* ls, const uint64_t* const* d
     return sync_copy(ctx, cnt.data(), d_cnt, (size_t)nvec * 8, hipMemcpyDeviceToHost);
   };
   WTG_TRY(gather(WTG_RS_ONES_OFF, 8));
+  WTG_TRY(after_ones(cnt));  // work that needs only the directories and the counts
   size_t after = 0;
   for (uint32_t b = 0; b < nvec; b++) after += wtg_select_arena_count(cnt[b]);
   WTG_TRY(alloc_block(ctx, ls, after, (void**)&ar[1].base));
@@ -1184,58 +1188,63 @@ extern "C" int wtg_lsel_build(wtg_ctx* ctx, uint32_t nvec, const uint64_t* const
   std::vector<uint64_t*> pos(nvec, nullptr);
   std::vector<uint64_t> ns(nvec, 0);
   std::vector<uint32_t> act;  // vectors that get a model
-  // 1. the plain method: rank9 + select1 of every vector.  Each vector has its
-  //    own workspace, so one host read brings the ones counts of all vectors
-  //    and one the long counts; directories and select content sit in blocks.
-  for (uint32_t b = 0; b < nvec; b++) std::memset(&ls->sel[b], 0, sizeof(wtg_select));
-  if ((rc = lsel_plain_build(ctx, ls, d_words, nbits)) != WTG_OK) goto fail;
-  for (uint32_t b = 0; b < nvec; b++) {
-    LselDesc& d = ls->desc[b];
-    std::memset(&d, 0, sizeof(d));
-    d.words = d_words[b];
-    d.dir = ls->dirs[b];
-    const wtg_select& s = ls->sel[b];
-    d.s1 = SelQ{s.superblock, s.mini, s.long_index, s.long_pool, d_words[b], (nbits[b] + 63) / 64 + 1, 1};
-    d.nbits = nbits[b];
-    d.ones = s.arg_cnt;
-    d.nblk = ((nbits[b] + 63) / 64) >> 3;
-    ls->info[b].nbits = nbits[b];
-    ls->info[b].ones = s.arg_cnt;
-  }
-  if ((rc = wtg_malloc(ctx, (void**)&ls->d_desc, nvec * sizeof(LselDesc))) != WTG_OK) goto fail;
-  if ((rc = lsel_upload_desc(ctx, ls)) != WTG_OK) goto fail;
-  // 2. training samples of every vector (one launch), then (one host read of
-  //    the first and last samples) the baseline lines
-  {
-    uint64_t* d_ends = nullptr;
-    if ((rc = alloc_block(ctx, ls, nvec * 16, (void**)&d_ends)) != WTG_OK) goto fail;
+  // 2. (queued inside 1, once the ones counts are known) the training samples
+  //    of every vector, one launch; after 1, one host read of the first and
+  //    last samples gives the baseline lines
+  uint64_t* d_ends = nullptr;
+  std::vector<LselDesc> desc0;  // the first upload's source (alive past the plain build's next host read)
+  auto start_samples = [&](const std::vector<uint64_t>& ones) -> int {
+    for (uint32_t b = 0; b < nvec; b++) {
+      LselDesc& d = ls->desc[b];
+      std::memset(&d, 0, sizeof(d));
+      d.words = d_words[b];
+      d.dir = ls->dirs[b];
+      d.nbits = nbits[b];
+      d.ones = ones[b];
+      d.nblk = ((nbits[b] + 63) / 64) >> 3;
+    }
+    WTG_TRY(wtg_malloc(ctx, (void**)&ls->d_desc, nvec * sizeof(LselDesc)));
+    desc0 = ls->desc;  // (the select fields follow after 1)
+    WTG_CHECK_HIP(hipMemcpyAsync(ls->d_desc, desc0.data(), nvec * sizeof(LselDesc), hipMemcpyHostToDevice,
+                                 ctx->stream));
+    WTG_TRY(alloc_block(ctx, ls, nvec * 16, (void**)&d_ends));
     std::vector<SampleVec> sv;
     uint64_t wmax = 1;
     for (uint32_t b = 0; b < nvec; b++) {
-      ns[b] = ls->desc[b].ones >> tlog;
-      if (ns[b] < 2 || ls->desc[b].ones >= (1ull << 32)) continue;  // too few ones (or u32 knots): plain select
-      if ((rc = alloc_block(ctx, ls, ns[b] * 8, (void**)&pos[b])) != WTG_OK) goto fail;
+      ns[b] = ones[b] >> tlog;
+      if (ns[b] < 2 || ones[b] >= (1ull << 32)) continue;  // too few ones (or u32 knots): plain select
+      WTG_TRY(alloc_block(ctx, ls, ns[b] * 8, (void**)&pos[b]));
       act.push_back(b);
       sv.push_back(SampleVec{pos[b], ns[b]});
       wmax = std::max<uint64_t>(wmax, (nbits[b] + 63) / 64);
     }
-    if (!act.empty()) {
-      SampleVec* d_sv = nullptr;
-      uint32_t* d_vids = nullptr;
-      if ((rc = alloc_block(ctx, ls, sv.size() * sizeof(SampleVec), (void**)&d_sv)) != WTG_OK) goto fail;
-      if ((rc = alloc_block(ctx, ls, act.size() * 4, (void**)&d_vids)) != WTG_OK) goto fail;
-      if (hipMemcpyAsync(d_sv, sv.data(), sv.size() * sizeof(SampleVec), hipMemcpyHostToDevice, ctx->stream) !=
-              hipSuccess ||
-          hipMemcpyAsync(d_vids, act.data(), act.size() * 4, hipMemcpyHostToDevice, ctx->stream) != hipSuccess) {
-        wtg_set_error("wtg_lsel_build: sample table copy failed");
-        rc = WTG_ERR_HIP;
-        goto fail;
-      }
-      const unsigned gx = (unsigned)std::max<uint64_t>(1, std::min<uint64_t>(wtg_ceil_div(wmax, 256 * LSEL_EPS_W), 256));
-      WTG_LAUNCH(ctx, "lsel_samples", k_lsel_samples, dim3(gx, (unsigned)act.size()), dim3(256), 0, ls->d_desc,
-                 d_vids, d_sv, tlog, d_ends);
-      WTG_LAUNCH_CHECK();
-    }
+    if (act.empty()) return WTG_OK;
+    SampleVec* d_sv = nullptr;
+    uint32_t* d_vids = nullptr;
+    WTG_TRY(alloc_block(ctx, ls, sv.size() * sizeof(SampleVec), (void**)&d_sv));
+    WTG_TRY(alloc_block(ctx, ls, act.size() * 4, (void**)&d_vids));
+    WTG_CHECK_HIP(hipMemcpyAsync(d_sv, sv.data(), sv.size() * sizeof(SampleVec), hipMemcpyHostToDevice, ctx->stream));
+    WTG_CHECK_HIP(hipMemcpyAsync(d_vids, act.data(), act.size() * 4, hipMemcpyHostToDevice, ctx->stream));
+    const unsigned gx = (unsigned)std::max<uint64_t>(1, std::min<uint64_t>(wtg_ceil_div(wmax, 256 * LSEL_EPS_W), 256));
+    WTG_LAUNCH(ctx, "lsel_samples", k_lsel_samples, dim3(gx, (unsigned)act.size()), dim3(256), 0, ls->d_desc, d_vids,
+               d_sv, tlog, d_ends);
+    WTG_LAUNCH_CHECK();
+    return WTG_OK;
+  };
+  // 1. the plain method: rank9 + select1 of every vector.  Each vector has its
+  //    own workspace, so one host read brings the ones counts of all vectors
+  //    and one the long counts; directories and select content sit in blocks.
+  for (uint32_t b = 0; b < nvec; b++) std::memset(&ls->sel[b], 0, sizeof(wtg_select));
+  if ((rc = lsel_plain_build(ctx, ls, d_words, nbits, start_samples)) != WTG_OK) goto fail;
+  for (uint32_t b = 0; b < nvec; b++) {
+    LselDesc& d = ls->desc[b];
+    const wtg_select& s = ls->sel[b];
+    d.s1 = SelQ{s.superblock, s.mini, s.long_index, s.long_pool, d_words[b], (nbits[b] + 63) / 64 + 1, 1};
+    ls->info[b].nbits = nbits[b];
+    ls->info[b].ones = s.arg_cnt;
+  }
+  if ((rc = lsel_upload_desc(ctx, ls)) != WTG_OK) goto fail;
+  {
     std::vector<uint64_t> ends(2 * nvec, 0);
     if (!act.empty() && (rc = sync_copy(ctx, ends.data(), d_ends, nvec * 16, hipMemcpyDeviceToHost)) != WTG_OK)
       goto fail;
