@@ -26,7 +26,7 @@ extern "C" {
 #define WTG_ERR_ARG (-1)          /* invalid argument (null pointer, size, width) */
 #define WTG_ERR_HIP (-2)          /* a HIP runtime call failed */
 #define WTG_ERR_NOMEM (-3)        /* device allocation failed */
-#define WTG_ERR_UNSUPPORTED (-4)  /* valid input outside what this build handles */
+#define WTG_ERR_UNSUPPORTED (-4)  /* valid input outside what the library handles */
 #define WTG_ERR_ABSENT (-5)       /* select(i,c): c does not occur i times */
 #define WTG_ERR_DEVICE (-6)       /* a kernel's bounded wait expired (device-side error flag) */
 
